@@ -1,0 +1,318 @@
+// Fused cell-loop assembly: gather -> CellValues reinit! -> quadrature loop ->
+// Ke/fe -> deterministic colored scatter into CSC nzval and f.
+//
+// Replaces, per colour, the body of the threaded colored loop
+// (docs/src/literate-howto/threaded_assembly.jl:232-265):
+//   reinit!(cellcache, i)           iterators.jl:72-88     (gather coords/dofs)
+//   reinit!(cellvalues, cellcache)  CellValues.jl:122-140  (J, detJ, J⁻¹, ∇N)
+//   assemble_element!               heat_equation.jl:143-164
+//   assemble!(assembler, dofs,...)  assembler.jl:322-457   (scatter)
+//
+// Kernel shape (one launch per colour): a block owns BLOCK consecutive colour
+// slots.  Phase 1 maps one lane to one cell and keeps the whole element
+// computation in registers (no redundant geometry work, ~36 wave-instructions
+// per cell for Q1 hex).  The upper triangle of Ke is staged in LDS.  Phase 2
+// remaps lanes to (cell, entry) so that consecutive lanes write consecutive
+// rows of one column of K: one wave instruction touches the 1-2 cache lines of
+// 8 columns instead of 64 scattered lines.  Cells of one colour share no dof,
+// so the read-modify-write of nzval/f needs no atomics and the sum order per
+// entry is the colour order: results are bitwise reproducible run to run.
+#include "fem_internal.h"
+
+namespace fem {
+
+// Packed reference tables: N[NQ][NB], dN[NQ][NB][DIM], dM[NQ][NG][DIM], w[NQ].
+template <int DIM, int NG, int NB, int NQ>
+struct TabLayout {
+  static constexpr int oN = 0;
+  static constexpr int odN = oN + NQ * NB;
+  static constexpr int odM = odN + NQ * NB * DIM;
+  static constexpr int ow = odM + NQ * NG * DIM;
+  static constexpr int size = ow + NQ;
+};
+
+template <typename T, int DIM>
+__device__ __forceinline__ T det_inv(const T (&J)[DIM][DIM], T (&Ji)[DIM][DIM]) {
+  if constexpr (DIM == 2) {
+    const T d = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const T id = T(1) / d;
+    Ji[0][0] = J[1][1] * id;  Ji[0][1] = -J[0][1] * id;
+    Ji[1][0] = -J[1][0] * id; Ji[1][1] = J[0][0] * id;
+    return d;
+  } else {
+    const T a00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const T a01 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    const T a02 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    const T d = J[0][0] * a00 + J[1][0] * a01 + J[2][0] * a02;
+    const T id = T(1) / d;
+    Ji[0][0] = a00 * id; Ji[0][1] = a01 * id; Ji[0][2] = a02 * id;
+    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    return d;
+  }
+}
+
+constexpr __host__ __device__ int tri_idx(int lo, int hi, int nb) {
+  return lo * (2 * nb - lo + 1) / 2 + (hi - lo);
+}
+
+__global__ void k_fill_zero(double* __restrict__ a, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0.0;
+}
+
+__global__ void k_reset_err(DevError* e) {
+  e->detj_cell = INT_MAX;
+  e->pattern_cell = INT_MAX;
+  e->pattern_entry = -1;
+}
+
+// Scalar Poisson/heat element, Q1 geometry == function interpolation.
+template <typename T, int DIM, int NG, int NB, int NQ, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_poisson(
+    int64_t s0, int64_t s1, const double* __restrict__ X0, const double* __restrict__ X1,
+    const double* __restrict__ X2, const int32_t* __restrict__ cnodes, const int32_t* __restrict__ cdofs,
+    const int32_t* __restrict__ cnz, const int32_t* __restrict__ ccell, const double* __restrict__ cparam,
+    const T* __restrict__ tab, T src, double* __restrict__ nzval, double* __restrict__ f,
+    DevError* __restrict__ err) {
+  using L = TabLayout<DIM, NG, NB, NQ>;
+  constexpr int TRI = NB * (NB + 1) / 2;
+  __shared__ T s_ke[BLOCK * TRI];
+  __shared__ T s_fe[BLOCK * NB];
+
+  const int64_t sb = s0 + (int64_t)blockIdx.x * BLOCK;
+  const int ncell = (int)min<int64_t>(BLOCK, s1 - sb);
+  const int lc = threadIdx.x;
+
+  if (lc < ncell) {
+    const int64_t s = sb + lc;
+    T x[NG][DIM];
+#pragma unroll
+    for (int a = 0; a < NG; ++a) {
+      const int32_t nd = cnodes[s * NG + a];
+      x[a][0] = (T)X0[nd];
+      x[a][1] = (T)X1[nd];
+      if constexpr (DIM == 3) x[a][2] = (T)X2[nd];
+    }
+    const T k = (T)cparam[s];
+    T ke[TRI];
+    T fe[NB];
+#pragma unroll
+    for (int i = 0; i < TRI; ++i) ke[i] = T(0);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) fe[i] = T(0);
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      // J = Σ_j x_j ⊗ dMdξ_j   (GeometryMapping.jl:124-130)
+      T J[DIM][DIM];
+#pragma unroll
+      for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) J[i][j] = T(0);
+#pragma unroll
+      for (int a = 0; a < NG; ++a)
+#pragma unroll
+        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+          for (int j = 0; j < DIM; ++j) J[i][j] += x[a][i] * tab[L::odM + (q * NG + a) * DIM + j];
+      T Ji[DIM][DIM];
+      const T detJ = det_inv<T, DIM>(J, Ji);
+      bad |= !(detJ > T(0));
+      const T dV = detJ * tab[L::ow + q];
+      // dNdx = dNdξ ⋅ J⁻¹  (FunctionValues.jl:186-192)
+      T g[NB][DIM];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) {
+          T acc = T(0);
+#pragma unroll
+          for (int i = 0; i < DIM; ++i) acc += tab[L::odN + (q * NB + b) * DIM + i] * Ji[i][j];
+          g[b][j] = acc;
+        }
+      const T kdV = k * dV;
+      const T sdV = src * dV;
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        fe[a] += tab[L::oN + q * NB + a] * sdV;
+#pragma unroll
+        for (int b = a; b < NB; ++b) {
+          T dot = g[a][0] * g[b][0];
+#pragma unroll
+          for (int j = 1; j < DIM; ++j) dot += g[a][j] * g[b][j];
+          ke[tri_idx(a, b, NB)] += dot * kdV;
+        }
+      }
+    }
+    if (bad) atomicMin(&err->detj_cell, ccell[s]);
+#pragma unroll
+    for (int i = 0; i < TRI; ++i) s_ke[lc * TRI + i] = ke[i];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) s_fe[lc * NB + i] = fe[i];
+  }
+  __syncthreads();
+
+  // Phase 2: lanes = (cell, column b, row a); entry t = cell*NB*NB + b*NB + a.
+  const int32_t* __restrict__ nzt = cnz + sb * (NB * NB);
+  const int tot = ncell * NB * NB;
+  for (int t = threadIdx.x; t < tot; t += BLOCK) {
+    const int lcell = t / (NB * NB);
+    const int e = t - lcell * (NB * NB);
+    const int b = e / NB, a = e - b * NB;
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    const T v = s_ke[lcell * TRI + tri_idx(lo, hi, NB)];
+    const int32_t p = nzt[t];
+    if (p >= 0) {
+      if (v != T(0)) nzval[p] += (double)v;
+    } else if (v != T(0)) {
+      atomicMin(&err->pattern_cell, ccell[sb + lcell]);
+    }
+  }
+  const int32_t* __restrict__ dt = cdofs + sb * NB;
+  for (int t = threadIdx.x; t < ncell * NB; t += BLOCK) f[dt[t]] += (double)s_fe[t];
+}
+
+template <typename T, int DIM, int NG, int NB, int NQ>
+static void pack_tables(const RefTables& r, std::vector<T>& out) {
+  using L = TabLayout<DIM, NG, NB, NQ>;
+  out.assign(L::size, T(0));
+  for (int q = 0; q < NQ; ++q) {
+    out[L::ow + q] = (T)r.w[q];
+    for (int b = 0; b < NB; ++b) {
+      out[L::oN + q * NB + b] = (T)r.N[q][b];
+      for (int d = 0; d < DIM; ++d) out[L::odN + (q * NB + b) * DIM + d] = (T)r.dN[q][b][d];
+    }
+    for (int a = 0; a < NG; ++a)
+      for (int d = 0; d < DIM; ++d) out[L::odM + (q * NG + a) * DIM + d] = (T)r.dM[q][a][d];
+  }
+}
+
+template <int DIM, int NG, int NB, int NQ>
+static int upload_tables(fem_ctx* c) {
+  if (!c->tab_dirty) return FEM_OK;
+  std::vector<double> td;
+  std::vector<float> tf;
+  pack_tables<double, DIM, NG, NB, NQ>(c->ref, td);
+  pack_tables<float, DIM, NG, NB, NQ>(c->ref, tf);
+  int rc;
+  if ((rc = dev_alloc(c, &c->d_tab, td.size())) || (rc = dev_alloc(c, &c->d_tabf, tf.size()))) return rc;
+  FEM_HIP(c, hipMemcpyAsync(c->d_tab, td.data(), td.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipMemcpyAsync(c->d_tabf, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  c->tab_dirty = false;
+  return FEM_OK;
+}
+
+template <typename T, int DIM, int NG, int NB, int NQ, int BLOCK>
+static int launch_poisson(fem_ctx* c, double src) {
+  int rc = upload_tables<DIM, NG, NB, NQ>(c);
+  if (rc) return rc;
+  const T* tab = nullptr;
+  if constexpr (sizeof(T) == 8) tab = c->d_tab; else tab = c->d_tabf;
+  FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+  for (int col = 0; col < c->ncolors; ++col) {
+    const int64_t s0 = c->color_ptr[col], s1 = c->color_ptr[col + 1];
+    if (s1 <= s0) continue;
+    const int64_t nb = (s1 - s0 + BLOCK - 1) / BLOCK;
+    k_poisson<T, DIM, NG, NB, NQ, BLOCK><<<dim3((unsigned)nb), dim3(BLOCK), 0, c->stream>>>(
+        s0, s1, c->d_x[0], c->d_x[1], c->d_x[2], c->d_cnodes, c->d_cdofs, c->d_cnz, c->d_ccell,
+        c->d_cparam, tab, (T)src, c->d_nzval, c->d_f, c->d_err);
+    c->timings.assemble_launches++;
+  }
+  FEM_HIP(c, hipGetLastError());
+  FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
+  c->timings.assemble_kernel_ms = event_ms(c);
+  return FEM_OK;
+}
+
+// After a launch sequence: translate the device error word into Ferrite's errors.
+static int check_assembly_errors(fem_ctx* c) {
+  DevError e{};
+  FEM_HIP(c, hipMemcpyAsync(&e, c->d_err, sizeof(DevError), hipMemcpyDeviceToHost, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  if (e.detj_cell != INT_MAX) {
+    c->err_cell = (int64_t)e.detj_cell + 1;
+    char buf[160];
+    snprintf(buf, sizeof buf, "det(J) is not positive in cell %lld", (long long)c->err_cell);
+    return set_error(c, FEM_ERR_DETJ, buf);
+  }
+  if (e.pattern_cell != INT_MAX) {
+    c->err_cell = (int64_t)e.pattern_cell + 1;
+    // first missing entry of that cell, found on the host from the cell's dofs
+    const int nb = c->ndpc;
+    const int32_t* cd = &c->h_cell_dofs[(int64_t)e.pattern_cell * nb];
+    std::vector<int32_t> cp(c->n + 1);
+    FEM_HIP(c, hipMemcpy(cp.data(), c->d_colptr, sizeof(int32_t) * (c->n + 1), hipMemcpyDeviceToHost));
+    for (int b = 0; b < nb && c->err_row < 0; ++b) {
+      const int32_t col = cd[b];
+      std::vector<int32_t> rows(cp[col + 1] - cp[col]);
+      FEM_HIP(c, hipMemcpy(rows.data(), c->d_rowval + cp[col], sizeof(int32_t) * rows.size(), hipMemcpyDeviceToHost));
+      for (int a = 0; a < nb; ++a) {
+        bool found = false;
+        for (int32_t r : rows) found |= (r == cd[a]);
+        if (!found) { c->err_row = cd[a] + 1; c->err_col = col + 1; break; }
+      }
+    }
+    char buf[512];
+    snprintf(buf, sizeof buf,
+             "You are trying to assemble values in to K[%lld, %lld], but K[%lld, %lld] is missing in the "
+             "sparsity pattern. Make sure you have called `K = allocate_matrix(dh)` or `K = "
+             "allocate_matrix(dh, ch)` if you have affine constraints.",
+             (long long)c->err_row, (long long)c->err_col, (long long)c->err_row, (long long)c->err_col);
+    return set_error(c, FEM_ERR_PATTERN, buf);
+  }
+  return FEM_OK;
+}
+
+int assemble_elasticity(fem_ctx* c, const fem_assemble_cfg* cfg);
+
+}  // namespace fem
+
+using namespace fem;
+
+extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
+  if (!c || !cfg) return FEM_ERR_ARG;
+  if (!c->have_cv) return set_error(c, FEM_ERR_STATE, "upload CellValues tables first");
+  if (!c->have_param) return set_error(c, FEM_ERR_STATE, "upload the per-cell parameter first");
+  int rc = build_tables(c);
+  if (rc) return rc;
+  FEM_HIP(c, hipSetDevice(c->device));
+  c->err_cell = c->err_row = c->err_col = -1;
+  c->timings.assemble_launches = 0;
+  k_reset_err<<<1, 1, 0, c->stream>>>(c->d_err);
+  if (cfg->fillzero) {
+    FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    k_fill_zero<<<2048, 256, 0, c->stream>>>(c->d_nzval, c->nnz);
+    k_fill_zero<<<512, 256, 0, c->stream>>>(c->d_f, c->ndofs);
+    FEM_HIP(c, hipGetLastError());
+    FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
+    c->timings.fill_ms = event_ms(c);
+  }
+  const bool mixed = cfg->precision == FEM_MIXED;
+  if (cfg->kernel == FEM_KERNEL_POISSON) {
+    if (c->vdim != 1 || c->ndpc != c->nbf) return set_error(c, FEM_ERR_ARG, "Poisson kernel needs a scalar field");
+    const int key = c->rdim * 1000 + c->ngeo * 100 + c->nbf * 10 + (c->nqp == 8 ? 8 : c->nqp);
+    if (c->rdim == 3 && c->ngeo == 8 && c->nbf == 8 && c->nqp == 8 && c->npc == 8 && c->sdim == 3) {
+      rc = mixed ? launch_poisson<float, 3, 8, 8, 8, 128>(c, cfg->source)
+                 : launch_poisson<double, 3, 8, 8, 8, 64>(c, cfg->source);
+    } else if (c->rdim == 2 && c->ngeo == 4 && c->nbf == 4 && c->nqp == 4 && c->npc == 4 && c->sdim == 2) {
+      rc = mixed ? launch_poisson<float, 2, 4, 4, 4, 128>(c, cfg->source)
+                 : launch_poisson<double, 2, 4, 4, 4, 128>(c, cfg->source);
+    } else {
+      (void)key;
+      return set_error(c, FEM_ERR_ARG, "no Poisson kernel for this element/quadrature combination "
+                                       "(supported: Q1 quad 2x2 Gauss, Q1 hex 2x2x2 Gauss)");
+    }
+  } else if (cfg->kernel == FEM_KERNEL_ELASTICITY) {
+    rc = assemble_elasticity(c, cfg);
+  } else {
+    return set_error(c, FEM_ERR_ARG, "unknown kernel id");
+  }
+  if (rc) return rc;
+  return check_assembly_errors(c);
+}

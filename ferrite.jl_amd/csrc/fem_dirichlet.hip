@@ -1,0 +1,180 @@
+// Dirichlet apply!(K, f, ch) on the device, deterministic and atomic-free.
+//
+// Follows src/Dofs/ConstraintHandler.jl:710-747 step by step:
+//   m = meandiag(K)                  :952-958  (before any zeroing)
+//   f -= K * g   (only if some g≠0)  :755-789  add_inhomogeneities_csc!
+//   zero_out_columns!, zero_out_rows! :931-950
+//   K[d,d] = m ; f[d] = g_d * m (0 for apply_zero!)
+// The reference's GPU version (ext/FerriteKAExt/constraints.jl:51-61) subtracts
+// the inhomogeneities with KA.@atomic; here each row r gathers its own
+// contributions K[r,d]·g_d in ascending d, which is the order the serial CPU
+// loop applies them in, with the multiply and subtract rounded separately:
+// f is bitwise identical to the reference loop's for the same K.
+#include "fem_internal.h"
+
+namespace fem {
+
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < BLOCK / 64; ++w) r += sh[w];
+  return r;
+}
+
+// Σ|K_jj| over fixed block partitions (fixed order -> deterministic).
+__global__ __launch_bounds__(256) void k_absdiag_partial(int64_t n, const int32_t* __restrict__ diag,
+                                                         const double* __restrict__ nzval,
+                                                         double* __restrict__ partial) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (int64_t j = blockIdx.x * 256LL + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256) {
+    const int32_t p = diag[j];
+    if (p >= 0) acc += fabs(nzval[p]);
+  }
+  const double s = block_sum<256>(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void k_finish_mean(int nparts, const double* __restrict__ partial, int64_t n,
+                              double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < nparts; ++i) s += partial[i];
+    out[0] = s / (double)n;
+  }
+}
+
+// f[r] -= g_d K[r,d] for prescribed d (ascending d), rows found through the
+// structurally symmetric pattern: row r's prescribed neighbours d are the rows
+// of column r; K[r,d] is located in column d by binary search.
+__global__ void k_add_inhom(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
+                            const double* __restrict__ nzval, const uint8_t* __restrict__ isc,
+                            const double* __restrict__ gfull, double* __restrict__ f) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double fr = f[r];
+    bool touched = false;
+    for (int32_t p = colptr[r]; p < colptr[r + 1]; ++p) {
+      const int32_t d = rowval[p];
+      if (!isc[d]) continue;
+      const double v = gfull[d];
+      if (v == 0.0) continue;
+      int32_t lo = colptr[d], hi = colptr[d + 1];
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (rowval[mid] < (int32_t)r) lo = mid + 1; else hi = mid;
+      }
+      if (lo < colptr[d + 1] && rowval[lo] == (int32_t)r) {
+        fr = __dsub_rn(fr, __dmul_rn(v, nzval[lo]));
+        touched = true;
+      }
+    }
+    if (touched) f[r] = fr;
+  }
+}
+
+// Zero every stored entry in a prescribed column or a prescribed row.
+__global__ void k_zero_rows_cols(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
+                                 const uint8_t* __restrict__ isc, double* __restrict__ nzval) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const bool cj = isc[j];
+    for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p)
+      if (cj || isc[rowval[p]]) nzval[p] = 0.0;
+  }
+}
+
+__global__ void k_set_diag(int64_t np, const int32_t* __restrict__ pres, const int32_t* __restrict__ diag,
+                           const double* __restrict__ g, const double* __restrict__ m, int applyzero,
+                           double* __restrict__ nzval, double* __restrict__ f) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t d = pres[i];
+    const double mm = m[0];
+    const int32_t pd = diag[d];
+    if (pd >= 0) nzval[pd] = mm;
+    f[d] = applyzero ? 0.0 * mm : __dmul_rn(g[i], mm);
+  }
+}
+
+__global__ void k_scatter_g(int64_t np, const int32_t* __restrict__ pres, const double* __restrict__ g,
+                            double* __restrict__ gfull) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x)
+    gfull[pres[i]] = g[i];
+}
+
+static int grid_of(int64_t n, int block = 256) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > 8192) g = 8192;
+  return (int)g;
+}
+
+}  // namespace fem
+
+using namespace fem;
+
+extern "C" {
+
+int32_t fem_constraints_upload(fem_ctx* c, int64_t np, const int64_t* prescribed, const double* g) {
+  if (!c || np < 0 || (np > 0 && (!prescribed || !g)) || c->ndofs == 0)
+    return set_error(c, FEM_ERR_ARG, "fem_constraints_upload: invalid arguments (upload dofs first)");
+  std::vector<int32_t> p(np > 0 ? np : 1);
+  std::vector<uint8_t> isc(c->ndofs, 0);
+  bool inhom = false;
+  for (int64_t i = 0; i < np; ++i) {
+    const int64_t d = prescribed[i];
+    if (d < 1 || d > c->ndofs) return set_error(c, FEM_ERR_ARG, "prescribed dof out of range");
+    if (i > 0 && d <= prescribed[i - 1]) return set_error(c, FEM_ERR_ARG, "prescribed dofs must be sorted and unique");
+    p[i] = (int32_t)(d - 1);
+    isc[d - 1] = 1;
+    inhom |= (g[i] != 0.0);
+  }
+  int rc;
+  if ((rc = dev_alloc(c, &c->d_pres, np)) || (rc = dev_alloc(c, &c->d_g, np)) || (rc = dev_alloc(c, &c->d_isc, c->ndofs)))
+    return rc;
+  if (np > 0) {
+    FEM_HIP(c, hipMemcpyAsync(c->d_pres, p.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, c->stream));
+    FEM_HIP(c, hipMemcpyAsync(c->d_g, g, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+  }
+  FEM_HIP(c, hipMemcpyAsync(c->d_isc, isc.data(), c->ndofs, hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  c->np = np;
+  c->any_inhom = inhom;
+  return FEM_OK;
+}
+
+int32_t fem_apply_dirichlet(fem_ctx* c, int32_t applyzero, double* m_out) {
+  if (!c || c->nnz == 0) return set_error(c, FEM_ERR_STATE, "no matrix to apply constraints to");
+  if (!c->d_isc) return set_error(c, FEM_ERR_STATE, "upload constraints first");
+  FEM_HIP(c, hipSetDevice(c->device));
+  const int64_t n = c->n;
+  FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+  const int nparts = 1024;
+  k_absdiag_partial<<<nparts, 256, 0, c->stream>>>(n, c->d_diag, c->d_nzval, c->d_partials);
+  k_finish_mean<<<1, 64, 0, c->stream>>>(nparts, c->d_partials, n, c->d_scalar);
+  if (!applyzero && c->any_inhom) {
+    double* gfull = nullptr;
+    int rc = dev_alloc(c, &gfull, n);
+    if (rc) return rc;
+    FEM_HIP(c, hipMemsetAsync(gfull, 0, sizeof(double) * n, c->stream));
+    k_scatter_g<<<grid_of(c->np), 256, 0, c->stream>>>(c->np, c->d_pres, c->d_g, gfull);
+    k_add_inhom<<<grid_of(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_isc, gfull, c->d_f);
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    dev_free(&gfull);
+  }
+  k_zero_rows_cols<<<grid_of(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, c->d_isc, c->d_nzval);
+  if (c->np > 0)
+    k_set_diag<<<grid_of(c->np), 256, 0, c->stream>>>(c->np, c->d_pres, c->d_diag, c->d_g, c->d_scalar, applyzero,
+                                                        c->d_nzval, c->d_f);
+  FEM_HIP(c, hipGetLastError());
+  FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
+  c->timings.apply_ms = event_ms(c);
+  if (m_out) FEM_HIP(c, hipMemcpy(m_out, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost));
+  dev_free(&c->d_nzval32);  // the FP32 copy must be rebuilt from the constrained K
+  return FEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// Internal declarations shared by the libferrite_mi355x translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "ferrite_mi355x.h"
+
+namespace fem {
+
+constexpr int kMaxQP = 27;     // 3x3x3 Gauss
+constexpr int kMaxNB = 27;     // scalar base functions (Q2 hex)
+constexpr int kMaxNG = 8;      // geometric nodes (Q1 geometry)
+constexpr int kMaxNFQ = 9;     // facet quadrature points (3x3)
+
+// Reference tables of CellValues, tabulated on the host at the quadrature points.
+// Kept in __constant__ memory; every lane reads the same entry, so the loads are
+// scalar (s_load) and cost no VGPRs.
+struct RefTables {
+  double N[kMaxQP][kMaxNB];
+  double dN[kMaxQP][kMaxNB][3];
+  double dM[kMaxQP][kMaxNG][3];
+  double w[kMaxQP];
+};
+struct RefTablesF {
+  float N[kMaxQP][kMaxNB];
+  float dN[kMaxQP][kMaxNB][3];
+  float dM[kMaxQP][kMaxNG][3];
+  float w[kMaxQP];
+};
+
+// Device error word: atomicMin of the first offending cell (in color order) per kind.
+struct DevError {
+  int detj_cell;     // cell id (0-based) with det(J) <= 0, INT_MAX if none
+  int pattern_cell;  // cell id with a nonzero value for a missing pattern entry
+  int pattern_entry; // local entry (col * nb + row) of that cell (valid when pattern_cell set)
+  int pad;
+};
+
+struct SolverWork;
+
+}  // namespace fem
+
+struct fem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int64_t err_cell = -1, err_row = -1, err_col = -1;
+
+  // mesh
+  int sdim = 0, npc = 0;
+  int64_t nnodes = 0, ncells = 0;
+  double* d_x[3] = {nullptr, nullptr, nullptr};
+  std::vector<int32_t> h_cell_nodes;   // 0-based, cell-id order
+
+  // dofs
+  int64_t ndofs = 0;
+  int ndpc = 0;
+  std::vector<int32_t> h_cell_dofs;    // 0-based, cell-id order
+
+  // pattern (CSC, 0-based, int32 indices; nnz < 2^31)
+  int64_t n = 0, nnz = 0;
+  int32_t* d_colptr = nullptr;
+  int32_t* d_rowval = nullptr;
+  int32_t* d_diag = nullptr;           // position of K[j,j] in column j (-1 if absent)
+  double* d_nzval = nullptr;
+  float* d_nzval32 = nullptr;          // FP32 copy used by the mixed solver
+
+  // colors: cells permuted into color-major order on the device
+  int ncolors = 0;
+  std::vector<int64_t> color_ptr;      // ncolors + 1
+  std::vector<int32_t> h_color_cells;  // 0-based cell id per color slot
+  int32_t* d_ccell = nullptr;          // [ncells] cell id of color slot
+  int32_t* d_cnodes = nullptr;         // [ncells][npc]
+  int32_t* d_cdofs = nullptr;          // [ncells][ndpc]
+  int32_t* d_cnz = nullptr;            // [ncells][ndpc*ndpc] nzval position, -1 = missing
+  bool tables_ready = false;
+
+  // per-cell parameter, color order
+  double* d_cparam = nullptr;
+  bool have_param = false;
+
+  // cellvalues
+  int rdim = 0, nqp = 0, nbf = 0, ngeo = 0, vdim = 1;
+  fem::RefTables ref{};
+  bool have_cv = false;
+  bool tab_dirty = true;
+  double* d_tab = nullptr;             // packed N, dN, dM, w for the element kernels (FP64)
+  float* d_tabf = nullptr;             // the same in FP32
+
+  // vectors
+  double* d_f = nullptr;
+  double* d_u = nullptr;
+
+  // constraints
+  int64_t np = 0;
+  int32_t* d_pres = nullptr;
+  double* d_g = nullptr;
+  uint8_t* d_isc = nullptr;
+  bool any_inhom = false;
+  double* d_scalar = nullptr;          // small device scratch for reductions (meandiag etc.)
+  double* d_partials = nullptr;
+  int npartials = 0;
+
+  fem::DevError* d_err = nullptr;
+  fem::SolverWork* solver = nullptr;
+
+  fem_timings timings{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace fem {
+
+int set_error(fem_ctx* c, int code, const std::string& msg);
+int check_hip(fem_ctx* c, hipError_t e, const char* what);
+
+#define FEM_HIP(ctx, expr)                                   \
+  do {                                                       \
+    hipError_t _e = (expr);                                  \
+    if (_e != hipSuccess) return fem::check_hip(ctx, _e, #expr); \
+  } while (0)
+
+template <class T>
+int dev_alloc(fem_ctx* c, T** p, size_t n) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(c, FEM_ERR_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  }
+  return FEM_OK;
+}
+
+template <class T>
+void dev_free(T** p) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+}
+
+// Elapsed time between two events recorded on the context stream (stream synchronized).
+float event_ms(fem_ctx* c);
+
+int build_tables(fem_ctx* c);
+void free_solver(fem_ctx* c);
+
+constexpr int kReduceBlocks = 1024;   // fixed partial count -> deterministic reductions
+
+}  // namespace fem

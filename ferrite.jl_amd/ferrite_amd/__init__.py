@@ -1,0 +1,27 @@
+"""ferrite_amd: Ferrite.jl's cell-loop assembly path on AMD MI355X (gfx950).
+
+Python host interface over the C ABI of libferrite_mi355x.so (include/ferrite_mi355x.h).
+Names follow Ferrite.jl 1.6.0; mutating Julia functions drop the `!`
+(`add!` -> `add`, `close!` -> `close`, `apply!` -> `apply`).
+"""
+from ._lib import (FemArgumentError, FemConvergenceError, FemError, FemPatternError, LIB_PATH,
+                   exported_symbols)
+from .constraints import ConstraintHandler, Dirichlet
+from .device import (MI355XCSC, Context, ElasticityElement, PoissonElement, allocate_matrix, allocate_vector,
+                     apply, apply_zero, assemble_global, set_cell_parameters, solve, spmv_bench, start_assemble,
+                     timings)
+from .dofs import DofHandler, add, close, create_coloring, create_sparsity_pattern
+from .mesh import Grid, Hexahedron, Quadrilateral, generate_grid
+from .refel import (FacetQuadratureRule, Lagrange, QuadratureRule, RefHexahedron, RefLine, RefQuadrilateral,
+                    VectorizedInterpolation)
+from .values import CellValues, FacetValues
+
+__all__ = [
+    "FemArgumentError", "FemConvergenceError", "FemError", "FemPatternError", "LIB_PATH", "exported_symbols",
+    "ConstraintHandler", "Dirichlet", "MI355XCSC", "Context", "ElasticityElement", "PoissonElement",
+    "allocate_matrix", "allocate_vector", "apply", "apply_zero", "assemble_global", "set_cell_parameters",
+    "solve", "spmv_bench", "start_assemble", "timings", "DofHandler", "add", "close", "create_coloring",
+    "create_sparsity_pattern", "Grid", "Hexahedron", "Quadrilateral", "generate_grid", "FacetQuadratureRule",
+    "Lagrange", "QuadratureRule", "RefHexahedron", "RefLine", "RefQuadrilateral", "VectorizedInterpolation",
+    "CellValues", "FacetValues",
+]

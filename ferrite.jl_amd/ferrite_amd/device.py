@@ -1,0 +1,243 @@
+"""Device-resident system and the drop-in assembly API.
+
+The counterparts of Ferrite's GPU path (docs/src/literate-howto/gpu_assembly.jl,
+ext/FerriteCudaExt.jl, ext/FerriteKAExt/*):
+
+  allocate_matrix(MI355XCSC, dh)   allocate_matrix(CuSparseMatrixCSC, dh)  FerriteCudaExt.jl:95-101
+  start_assemble(K, f)             start_assemble(K, f; fillzero)          assembler.jl:287-291
+  assemble_global(A, cv, colors, element)
+                                   the colored threaded cell loop           threaded_assembly.jl:232-265
+                                   (reinit!, element routine, assemble! fused on the device)
+  apply(K, f, ch)                  apply!(K, f, ch)                         ConstraintHandler.jl:710-747
+  solve(K, f, ...)                 u = K \\ f                               heat_equation.jl:218
+
+Every call goes through the C ABI of libferrite_mi355x.so; nothing here computes
+on the host beyond argument marshalling.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .dofs import create_sparsity_pattern
+
+
+class MI355XCSC:
+    """Marker type for allocate_matrix(MI355XCSC, dh)."""
+
+
+class Context:
+    """Owns one fem_ctx (one HIP stream on one device)."""
+
+    def __init__(self, device=0):
+        self._lib = L.lib()
+        h = C.c_void_p()
+        rc = self._lib.fem_ctx_create(int(device), C.byref(h))
+        L.raise_for(rc, what=f"fem_ctx_create(device={device}) failed (is a GPU visible?)")
+        self.h = h
+
+    def call(self, name, *args):
+        rc = getattr(self._lib, name)(self.h, *args)
+        L.raise_for(rc, self.h, name)
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.fem_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceMatrix:
+    """K on the device: Ferrite's pattern (colptr, rowval; 1-based) plus nzval in HBM."""
+
+    def __init__(self, dh, ctx, colptr, rowval):
+        self.dh = dh
+        self.ctx = ctx
+        self.colptr = colptr
+        self.rowval = rowval
+        self.n = dh.ndofs
+        self.nnz = len(rowval)
+        self._colors = None
+        self._cv = None
+        self._param = None
+
+    @property
+    def shape(self):
+        return (self.n, self.n)
+
+    def nzval(self):
+        out = np.empty(self.nnz)
+        self.ctx.call("fem_matrix_download", L.ptr(out, C.c_double), None)
+        return out
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csc_matrix((self.nzval(), self.rowval - 1, self.colptr - 1), shape=self.shape)
+
+
+class DeviceVector:
+    """f (or u) resident in HBM next to its matrix."""
+
+    def __init__(self, K, which):
+        self.K, self.which = K, which
+
+    def __len__(self):
+        return self.K.n
+
+    def to_numpy(self):
+        out = np.empty(self.K.n)
+        if self.which == 1:
+            self.K.ctx.call("fem_matrix_download", None, L.ptr(out, C.c_double))
+        else:
+            self.K.ctx.call("fem_solution_download", L.ptr(out, C.c_double))
+        return out
+
+
+def allocate_matrix(mtype, dh, device=0, ctx=None):
+    """allocate_matrix(MI355XCSC, dh): upload mesh, dofs and the pattern; nzval zeroed in HBM."""
+    if mtype is not MI355XCSC:
+        raise TypeError("only MI355XCSC is provided by this package")
+    colptr, rowval = create_sparsity_pattern(dh)
+    ctx = ctx or Context(device)
+    g = dh.grid
+    xyz = L.as_f64(g.nodes)
+    cells = L.as_i64(g.cells)
+    ctx.call("fem_mesh_upload", g.sdim, g.nnodes, L.ptr(xyz, C.c_double), g.ncells, cells.shape[1],
+             L.ptr(cells, C.c_int64))
+    cd = L.as_i64(dh.cell_dofs)
+    ctx.call("fem_dofs_upload", dh.ndofs, dh.ndofs_per_cell, L.ptr(cd, C.c_int64))
+    ctx.call("fem_pattern_upload", dh.ndofs, L.ptr(colptr, C.c_int64), L.ptr(rowval, C.c_int64))
+    return DeviceMatrix(dh, ctx, colptr, rowval)
+
+
+def allocate_vector(K):
+    """The device f paired with K (zeros(ndofs(dh)) in the reference)."""
+    return DeviceVector(K, 1)
+
+
+class PoissonElement:
+    """Ke += k_e (∇Nᵢ·∇Nⱼ) dΩ ; fe += s Nᵢ dΩ  (heat_equation.jl:143-164 with per-cell k_e)."""
+    kernel = L.FEM_KERNEL_POISSON
+
+    def __init__(self, k=None, source=1.0):
+        self.k, self.source = k, float(source)
+
+    def param(self, ncells):
+        return np.ones(ncells) if self.k is None else L.as_f64(self.k)
+
+
+class ElasticityElement:
+    """Ke += E_e (εᵢ : C(ν) : εⱼ) dΩ with isotropic C (threaded_assembly.jl:105-132)."""
+    kernel = L.FEM_KERNEL_ELASTICITY
+
+    def __init__(self, E=None, nu=0.3):
+        self.E, self.nu, self.source = E, float(nu), 0.0
+
+    def param(self, ncells):
+        return np.ones(ncells) if self.E is None else L.as_f64(self.E)
+
+
+class Assembler:
+    def __init__(self, K, f, fillzero):
+        self.K, self.f, self.fillzero = K, f, fillzero
+
+
+def start_assemble(K, f=None, fillzero=True):
+    if f is not None and f.K is not K:
+        raise ValueError("f must be the device vector allocated with K")
+    return Assembler(K, f or allocate_vector(K), fillzero)
+
+
+def _upload_colors(K, colors):
+    if K._colors is colors:
+        return
+    sizes = np.array([len(c) for c in colors], dtype=np.int64)
+    ptr = np.zeros(len(colors) + 1, dtype=np.int64)
+    ptr[1:] = np.cumsum(sizes)
+    cells = L.as_i64(np.concatenate([np.asarray(c, dtype=np.int64) for c in colors]))
+    K.ctx.call("fem_colors_upload", len(colors), L.ptr(ptr, C.c_int64), L.ptr(cells, C.c_int64))
+    K._colors = colors
+    K._param = None
+
+
+def _upload_cellvalues(K, cv):
+    if K._cv is cv:
+        return
+    rdim, nqp, nbf, ngeo, vdim, N, dN, dM, w = cv.device_tables()
+    K.ctx.call("fem_cellvalues_upload", rdim, nqp, nbf, ngeo, vdim, L.ptr(N, C.c_double), L.ptr(dN, C.c_double),
+               L.ptr(dM, C.c_double), L.ptr(w, C.c_double))
+    K._cv = cv
+
+
+def set_cell_parameters(K, values):
+    p = L.as_f64(values)
+    K.ctx.call("fem_cellparam_upload", len(p), L.ptr(p, C.c_double))
+    K._param = values
+
+
+def assemble_global(assembler, cv, colors, element, precision="fp64"):
+    """The fused colored cell loop: K[dofs, dofs] += Ke, f[dofs] += fe for every cell."""
+    K = assembler.K
+    _upload_colors(K, colors)
+    _upload_cellvalues(K, cv)
+    vals = element.param(K.dh.grid.ncells)
+    if K._param is None or K._param is not vals and not np.array_equal(np.asarray(K._param), vals):
+        set_cell_parameters(K, vals)
+    cfg = L.AssembleCfg(kernel=element.kernel,
+                        precision=L.FEM_MIXED if precision == "mixed" else L.FEM_FP64,
+                        fillzero=1 if assembler.fillzero else 0, reserved=0,
+                        source=float(getattr(element, "source", 0.0)), nu=float(getattr(element, "nu", 0.0)))
+    K.ctx.call("fem_assemble", C.byref(cfg))
+    assembler.fillzero = False
+    return K, assembler.f
+
+
+def apply(K, f, ch, applyzero=False):
+    """apply!(K, f, ch) on the device; returns meandiag(K)."""
+    p = L.as_i64(ch.prescribed_dofs)
+    g = L.as_f64(ch.inhomogeneities)
+    K.ctx.call("fem_constraints_upload", len(p), L.ptr(p, C.c_int64), L.ptr(g, C.c_double))
+    m = C.c_double(0.0)
+    K.ctx.call("fem_apply_dirichlet", 1 if applyzero else 0, C.byref(m))
+    return m.value
+
+
+def apply_zero(K, f, ch):
+    return apply(K, f, ch, applyzero=True)
+
+
+class SolveResult:
+    def __init__(self, info, u):
+        self.iterations = info.iterations
+        self.outer_iterations = info.outer_iterations
+        self.relres = info.relres
+        self.u = u
+
+
+def solve(K, f, mode="cg", rtol=1e-10, maxit=0, precond="jacobi", inner_rtol=0.0, download=True):
+    """u = K \\ f on the device: FP64 PCG (mode='cg') or FP64-refined FP32 PCG (mode='mixed')."""
+    cfg = L.SolveCfg(mode=L.FEM_SOLVE_IR_MIXED if mode == "mixed" else L.FEM_SOLVE_CG_FP64,
+                     maxit=int(maxit), rtol=float(rtol), inner_rtol=float(inner_rtol),
+                     precond=1 if precond == "jacobi" else 0, reserved=0)
+    info = L.SolveInfo()
+    K.ctx.call("fem_solve", C.byref(cfg), C.byref(info))
+    u = DeviceVector(K, 2).to_numpy() if download else None
+    return SolveResult(info, u)
+
+
+def timings(K):
+    t = L.Timings()
+    K.ctx.call("fem_get_timings", C.byref(t))
+    return {k: getattr(t, k) for k, _ in L.Timings._fields_}
+
+
+def spmv_bench(K, mixed=False, reps=20):
+    ms = C.c_double(0.0)
+    K.ctx.call("fem_spmv_bench", 1 if mixed else 0, int(reps), C.byref(ms))
+    return ms.value

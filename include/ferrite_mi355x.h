@@ -1,0 +1,167 @@
+/*
+ * ferrite_mi355x.h -- C ABI of libferrite_mi355x.so, the MI355X (gfx950)
+ * finite-element assembly engine that sits behind Ferrite.jl's cell-loop API.
+ *
+ * The reference (Ferrite.jl 1.6.0) is pure Julia; a Julia package extension
+ * would bind these entry points with `ccall` (INTEGRATION.md shows the stub).
+ * Every entry point replaces one step of the reference's assembly path; the
+ * reference interface each one stands in for is cited beside it.
+ *
+ * Conventions
+ *   - status codes: 0 = OK, otherwise a FEM_ERR_* code; fem_last_error()
+ *     returns the message (Ferrite's exception text where one exists).
+ *   - integer arrays coming from Julia are Int64 and 1-based, in the memory
+ *     layout of Ferrite's own containers (Vector{Node}, Vector{Hexahedron},
+ *     dh.cell_dofs, SparseMatrixCSC colptr/rowval).  The library copies them
+ *     to 0-based int32 device tables; it never keeps a host pointer.
+ *   - the library owns all device memory behind the opaque context; K.nzval,
+ *     f and u stay resident in HBM between calls and are copied out only by
+ *     the *_download calls (or read in place through fem_device_pointer).
+ *   - one context = one HIP stream on one device; a context is not thread
+ *     safe.  Every call returns after its work on the stream has completed.
+ */
+#ifndef FERRITE_MI355X_H
+#define FERRITE_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fem_ctx fem_ctx;
+
+enum fem_status {
+  FEM_OK = 0,
+  FEM_ERR_ARG = 1,      /* ArgumentError (length/shape mismatch, common_values.jl:19-31) */
+  FEM_ERR_HIP = 2,      /* HIP runtime error */
+  FEM_ERR_PATTERN = 3,  /* entry missing in the sparsity pattern (assembler.jl:459-467) */
+  FEM_ERR_DETJ = 4,     /* det(J) <= 0 in reinit! (common_values.jl:5, CellValues.jl:110-116) */
+  FEM_ERR_STATE = 5,    /* call out of order (e.g. assemble before the pattern is uploaded) */
+  FEM_ERR_NOCONV = 6,   /* CG did not reach the tolerance within maxit */
+  FEM_ERR_OOM = 7       /* device allocation failed */
+};
+
+enum fem_kernel {
+  FEM_KERNEL_POISSON = 1,    /* Ke += k (∇Nᵢ·∇Nⱼ) dΩ, fe += s Nᵢ dΩ  (heat_equation.jl:143-164) */
+  FEM_KERNEL_ELASTICITY = 2  /* Ke += E (εᵢ : C(ν) : εⱼ) dΩ (threaded_assembly.jl:105-132) */
+};
+
+enum fem_precision {
+  FEM_FP64 = 0,   /* element math and accumulation in FP64 */
+  FEM_MIXED = 1   /* element math in FP32, accumulation into nzval/f in FP64 */
+};
+
+/* ---- context ------------------------------------------------------------ */
+int32_t fem_ctx_create(int32_t device, fem_ctx** out);
+void fem_ctx_destroy(fem_ctx* ctx);
+const char* fem_last_error(const fem_ctx* ctx);
+/* Error details of the last failed assemble: cell id (1-based), row, col. */
+int32_t fem_last_error_info(const fem_ctx* ctx, int64_t* cell, int64_t* row, int64_t* col);
+
+/* ---- device mesh and dof layout (replaces adapt(backend, dh), KAExt/dof_handler.jl:56-86) ---- */
+/* Grid.nodes (Vector{Node{sdim,Float64}}, AoS) and Grid.cells (NTuple{npc,Int}, 1-based),
+ * src/Grid/grid.jl:12-14, :385-393.  Stored on device as SoA coordinates + int32 tables. */
+int32_t fem_mesh_upload(fem_ctx* ctx, int32_t sdim, int64_t nnodes, const double* xyz,
+                        int64_t ncells, int32_t nodes_per_cell, const int64_t* cell_nodes);
+/* dh.cell_dofs (DofHandler.jl:121-166, celldofs! :248-275), constant ndofs per cell. */
+int32_t fem_dofs_upload(fem_ctx* ctx, int64_t ndofs, int32_t ndofs_per_cell, const int64_t* cell_dofs);
+/* allocate_matrix(dh) pattern (sparsity_pattern.jl:628-645): SparseMatrixCSC colptr/rowval.
+ * Allocates K.nzval (FP64) on the device. */
+int32_t fem_pattern_upload(fem_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t* rowval);
+/* create_coloring(grid) (coloring.jl:308-317) flattened: color c holds
+ * color_cells[color_ptr[c] .. color_ptr[c+1]-1] (0-based offsets, 1-based cell ids). */
+int32_t fem_colors_upload(fem_ctx* ctx, int32_t ncolors, const int64_t* color_ptr, const int64_t* color_cells);
+/* CellValues(qr, ip) tables (CellValues.jl:66-90, FunctionValues.jl:62-98,
+ * GeometryMapping.jl:68-98): N[q][i], dNdxi[q][i][d] of the scalar base interpolation,
+ * dMdxi[q][j][d] of the geometric interpolation, weights[q].  vdim = number of
+ * vectorized components (1 = scalar field, rdim = ip^rdim). */
+int32_t fem_cellvalues_upload(fem_ctx* ctx, int32_t rdim, int32_t nqp, int32_t nbf, int32_t ngeo,
+                              int32_t vdim, const double* N, const double* dNdxi,
+                              const double* dMdxi, const double* weights);
+/* Per-cell material parameter in cell-id order (k_e for Poisson, E_e for elasticity). */
+int32_t fem_cellparam_upload(fem_ctx* ctx, int64_t ncells, const double* param);
+
+/* ---- assembly (replaces the colored threaded cell loop, threaded_assembly.jl:232-265,
+ *      i.e. start_assemble assembler.jl:287-291 + per cell reinit! CellValues.jl:122-140
+ *      + element routine + assemble! assembler.jl:322-457) ---- */
+typedef struct {
+  int32_t kernel;      /* fem_kernel */
+  int32_t precision;   /* fem_precision */
+  int32_t fillzero;    /* start_assemble(K, f; fillzero) */
+  int32_t reserved;
+  double source;       /* Poisson: constant body source s */
+  double nu;           /* elasticity: Poisson's ratio */
+} fem_assemble_cfg;
+
+int32_t fem_assemble(fem_ctx* ctx, const fem_assemble_cfg* cfg);
+
+/* FacetValues traction load f[dofs] += t·N dΓ on a facet set (FacetValues.jl:95-156,
+ * topology_optimization.jl:369-381).  facets = (cell, facet) pairs, 1-based;
+ * fN[q][i], fdMdxi[q][j][d], fweights[q] are the facet tables of EVERY facet, facet-major. */
+int32_t fem_facet_traction(fem_ctx* ctx, int64_t nfacets, const int64_t* facets,
+                           int32_t nfq, int32_t nfacet_types, const double* fN,
+                           const double* fdMdxi, const double* fweights, const double* traction);
+/* Point load f[dof] += value (nodal force), 1-based dofs. */
+int32_t fem_point_load(fem_ctx* ctx, int64_t n, const int64_t* dofs, const double* values);
+
+/* ---- Dirichlet apply! (ConstraintHandler.jl:710-747; GPU counterpart KAExt/constraints.jl:115-148) ---- */
+/* ch.prescribed_dofs (sorted, 1-based) and ch.inhomogeneities. */
+int32_t fem_constraints_upload(fem_ctx* ctx, int64_t np, const int64_t* prescribed, const double* g);
+/* apply!(K, f, ch) (applyzero = 0) / apply_zero!(K, f, ch) (applyzero = 1).  Deterministic,
+ * no atomics.  Returns meandiag(K) (ConstraintHandler.jl:952-958) in *m_out if non-NULL. */
+int32_t fem_apply_dirichlet(fem_ctx* ctx, int32_t applyzero, double* m_out);
+
+/* ---- solve (replaces u = K \ f, heat_equation.jl:218; the reference has no CG) ---- */
+enum fem_solve_mode {
+  FEM_SOLVE_CG_FP64 = 0,  /* CG, FP64 SpMV and vectors */
+  FEM_SOLVE_IR_MIXED = 1  /* FP64 iterative refinement around an FP32-SpMV inner CG */
+};
+typedef struct {
+  int32_t mode;        /* fem_solve_mode */
+  int32_t maxit;       /* total CG iterations allowed */
+  double rtol;         /* ‖f − K u‖₂ / ‖f‖₂ target (FP64 residual) */
+  double inner_rtol;   /* IR inner solve reduction (0 -> default 1e-5) */
+  int32_t precond;     /* 0 = none, 1 = Jacobi */
+  int32_t reserved;
+} fem_solve_cfg;
+
+typedef struct {
+  int32_t iterations;        /* total CG iterations */
+  int32_t outer_iterations;  /* IR refinement steps (1 for plain CG) */
+  double relres;             /* final true relative residual ‖f − K u‖/‖f‖ in FP64 */
+} fem_solve_info;
+
+int32_t fem_solve(fem_ctx* ctx, const fem_solve_cfg* cfg, fem_solve_info* info);
+
+/* ---- results ---- */
+int32_t fem_matrix_download(fem_ctx* ctx, double* nzval, double* f);
+int32_t fem_solution_download(fem_ctx* ctx, double* u);
+int32_t fem_solution_upload(fem_ctx* ctx, const double* u);
+/* Device address of a resident array (0 = nzval, 1 = f, 2 = u) for zero-copy wrapping. */
+int32_t fem_device_pointer(fem_ctx* ctx, int32_t which, void** ptr, int64_t* nelem);
+
+/* ---- timing: HIP-event durations of the last call's kernels on the context stream ---- */
+typedef struct {
+  double assemble_kernel_ms;   /* element + scatter kernels, summed over colors */
+  double fill_ms;              /* start_assemble zero fill */
+  double apply_ms;             /* all apply! kernels */
+  double spmv_ms;              /* SpMV kernels of the last solve, summed */
+  int32_t spmv_launches;
+  int32_t assemble_launches;
+} fem_timings;
+int32_t fem_get_timings(const fem_ctx* ctx, fem_timings* t);
+/* Average duration (HIP events) of `reps` back-to-back SpMV launches y = K x of the solver,
+ * FP64 (mixed = 0) or FP32 values and vectors (mixed = 1).  The solver's dominant kernel. */
+int32_t fem_spmv_bench(fem_ctx* ctx, int32_t mixed, int32_t reps, double* avg_ms);
+
+/* ---- compliance adjoint (no counterpart in the reference; BASELINE differentiation config) ----
+ * J = fᵀu, dJ/dθ_e = −p θ_e^{p−1}(1−E_min) u_eᵀ K⁰_e u_e, dJ/dX by recomputing Ke per cell.
+ * Requires a solved system (fem_solve) assembled with FEM_KERNEL_ELASTICITY. */
+int32_t fem_adjoint_compliance(fem_ctx* ctx, const double* theta, double emin, double penal,
+                               double nu, double* J_out, double* dJdtheta, double* dJdX);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FERRITE_MI355X_H */

@@ -1,0 +1,119 @@
+"""Host-side setup of the product vs the oracle (CPU only): bit-exact grid, dofs, pattern, colors, constraints."""
+import numpy as np
+import pytest
+
+import ferrite_amd as fa
+from oracle import constraints as OC
+from oracle.coloring import create_coloring as o_coloring
+from oracle.dofs import DofHandler as ODofHandler
+from oracle.grid import generate_grid as o_generate_grid
+from oracle.refel import Lagrange as OLagrange
+from oracle.refel import QuadratureRule as OQuadratureRule
+from oracle.fem import CellValues as OCellValues
+from oracle.sparsity import create_pattern
+
+CASES = [
+    ("Quadrilateral", (7, 5), 1, 1, (-1.0, -1.0), (1.0, 1.0)),
+    ("Quadrilateral", (6, 4), 1, 2, (0.0, 0.0), (2.0, 1.0)),
+    ("Quadrilateral", (3, 3), 2, 1, (0.1, 0.2), (0.7, 0.9)),
+    ("Hexahedron", (3, 4, 2), 1, 1, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)),
+    ("Hexahedron", (3, 2, 2), 2, 3, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)),
+    ("Hexahedron", (2, 2, 3), 1, 3, (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)),
+]
+
+
+def _pair(ct, nel, order, vdim, left, right):
+    g = fa.generate_grid(ct, nel, left, right)
+    og = o_generate_grid(ct, nel, left, right)
+    ip = fa.Lagrange(g.refshape, order)
+    oip = OLagrange(og.refshape, order)
+    if vdim > 1:
+        ip, oip = ip ** vdim, oip ** vdim
+    return g, og, ip, oip
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_grid_dofs_pattern_bitexact(case):
+    g, og, ip, oip = _pair(*case)
+    assert np.array_equal(g.nodes, og.nodes)          # bitwise: same blend, same operation order
+    assert np.array_equal(g.cells, np.array(og.cells))
+    assert g.facetsets == og.facetsets
+    dh = fa.DofHandler(g).add("u", ip).close()
+    odh = ODofHandler(og).add("u", oip).close()
+    assert dh.ndofs == odh.ndofs
+    assert np.array_equal(dh.cell_dofs.ravel(), np.array(odh.cell_dofs))
+    cp, rv = fa.create_sparsity_pattern(dh)
+    ocp, orv = create_pattern(odh)
+    assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("alg", ["greedy", "workstream"])
+def test_coloring_matches_reference_algorithm(case, alg):
+    g, og, _, _ = _pair(*case)
+    c1 = fa.create_coloring(g, alg)
+    c2 = o_coloring(og, alg)
+    assert len(c1) == len(c2)
+    for a, b in zip(c1, c2):
+        assert np.array_equal(a, np.array(b))
+
+
+def test_two_field_dofs_kat():
+    """test_dofs.jl:235-258 via the product's native distribution."""
+    g = fa.generate_grid("Quadrilateral", (2, 1))
+    dh = fa.DofHandler(g).add("v", fa.Lagrange("RefQuadrilateral", 1) ** 2).add(
+        "s", fa.Lagrange("RefQuadrilateral", 1)).close()
+    assert dh.celldofs(1).tolist() == list(range(1, 13))
+    assert dh.celldofs(2).tolist() == [3, 4, 13, 14, 15, 16, 5, 6, 10, 17, 18, 11]
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.Dirichlet("v", g.facetsets["left"], lambda x, t: 0.0, [2]))
+    ch.add(fa.Dirichlet("s", g.facetsets["left"], lambda x, t: 0.0))
+    ch.close()
+    assert ch.prescribed_dofs.tolist() == [2, 8, 9, 12]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_prescribed_dofs_and_inhomogeneities(case):
+    g, og, ip, oip = _pair(*case)
+    dh = fa.DofHandler(g).add("u", ip).close()
+    odh = ODofHandler(og).add("u", oip).close()
+    names = sorted(g.facetsets)[:2]
+    gf = lambda x, t: 1.0 + 2.0 * x[0] - x[1]  # noqa: E731
+    ch = fa.ConstraintHandler(dh)
+    och = OC.ConstraintHandler(odh)
+    for nm in names:
+        ch.add(fa.Dirichlet("u", g.facetsets[nm], gf))
+        och.add(OC.Dirichlet("u", og.facetsets[nm], gf))
+    ch.close()
+    och.close()
+    assert np.array_equal(ch.prescribed_dofs, och.prescribed_dofs)
+    assert np.allclose(ch.inhomogeneities, och.inhomogeneities, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("shape,order,q", [("RefQuadrilateral", 1, 2), ("RefHexahedron", 1, 2),
+                                           ("RefHexahedron", 2, 3), ("RefQuadrilateral", 2, 3)])
+def test_cellvalues_tables_match(shape, order, q):
+    cv = fa.CellValues(fa.QuadratureRule(shape, q), fa.Lagrange(shape, order))
+    ocv = OCellValues(OQuadratureRule(shape, q), OLagrange(shape, order))
+    assert np.allclose(cv.N, ocv.N.T, rtol=0, atol=4e-16)
+    assert np.allclose(cv.dNdxi, ocv.dNdxi.transpose(1, 0, 2), rtol=0, atol=8e-16)
+    assert np.allclose(cv.dMdxi, ocv.dMdxi.transpose(1, 0, 2), rtol=0, atol=8e-16)
+    assert np.allclose(cv.weights, ocv.qr.weights, rtol=0, atol=2e-15)
+
+
+def test_c2_scale_setup_properties():
+    """Hex 128^3 Q1: sizes from the 3D Poisson configuration; size-independent pattern invariants."""
+    g = fa.generate_grid("Hexahedron", (128, 128, 128), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    dh = fa.DofHandler(g).add("u", fa.Lagrange("RefHexahedron", 1)).close()
+    assert g.ncells == 2097152 and dh.ndofs == 2146689
+    cp, rv = fa.create_sparsity_pattern(dh)
+    assert len(rv) == 57066625 == 385 ** 3
+    counts = np.diff(cp)
+    assert counts.min() == 8 and counts.max() == 27
+    colors = fa.create_coloring(g)
+    allc = np.sort(np.concatenate(colors))
+    assert np.array_equal(allc, np.arange(1, g.ncells + 1))
+    # no two cells of one colour share a node (checked on a sample of colours)
+    for col in colors[:3]:
+        nodes = g.cells[col - 1].ravel()
+        assert len(np.unique(nodes)) == nodes.size
