@@ -157,24 +157,56 @@ __global__ __launch_bounds__(BLOCK) void k_poisson(
   }
   __syncthreads();
 
-  // Phase 2: lanes = (cell, column b, row a); entry t = cell*NB*NB + b*NB + a.
-  const int32_t* __restrict__ nzt = cnz + sb * (NB * NB);
-  const int tot = ncell * NB * NB;
-  for (int t = threadIdx.x; t < tot; t += BLOCK) {
-    const int lcell = t / (NB * NB);
-    const int e = t - lcell * (NB * NB);
-    const int b = e / NB, a = e - b * NB;
-    const int lo = a < b ? a : b, hi = a < b ? b : a;
-    const T v = s_ke[lcell * TRI + tri_idx(lo, hi, NB)];
-    const int32_t p = nzt[t];
-    if (p >= 0) {
-      if (v != T(0)) nzval[p] += (double)v;
-    } else if (v != T(0)) {
-      atomicMin(&err->pattern_cell, ccell[sb + lcell]);
+  // Phase 2: lanes = (cell, column b, row a); entry t = cell*NB*NB + b*NB + a.  Each thread
+  // first loads CH table entries, then the CH nzval values they address (all independent),
+  // then writes the CH sums back, so CH read-modify-writes are in flight per lane.
+  constexpr int E = NB * NB;
+  constexpr int CH = 16;
+  const int32_t* __restrict__ nzt = cnz + sb * E;
+  const int tot = ncell * E;
+  for (int base = 0; base < tot; base += BLOCK * CH) {
+    int32_t pp[CH];
+    double old[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int t = base + k * BLOCK + threadIdx.x;
+      pp[k] = t < tot ? nzt[t] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) old[k] = pp[k] >= 0 ? nzval[pp[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int t = base + k * BLOCK + threadIdx.x;
+      if (t < tot) {
+        const int lcell = t / E;
+        const int e = t - lcell * E;
+        const int b = e / NB, a = e - b * NB;
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        const T v = s_ke[lcell * TRI + tri_idx(lo, hi, NB)];
+        if (pp[k] >= 0) {
+          if (v != T(0)) nzval[pp[k]] = old[k] + (double)v;
+        } else if (v != T(0)) {
+          atomicMin(&err->pattern_cell, ccell[sb + lcell]);
+        }
+      }
     }
   }
+  constexpr int CF = (BLOCK * NB + BLOCK - 1) / BLOCK;
   const int32_t* __restrict__ dt = cdofs + sb * NB;
-  for (int t = threadIdx.x; t < ncell * NB; t += BLOCK) f[dt[t]] += (double)s_fe[t];
+  int32_t dd[CF];
+  double fo[CF];
+#pragma unroll
+  for (int k = 0; k < CF; ++k) {
+    const int t = k * BLOCK + threadIdx.x;
+    dd[k] = t < ncell * NB ? dt[t] : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < CF; ++k) fo[k] = dd[k] >= 0 ? f[dd[k]] : 0.0;
+#pragma unroll
+  for (int k = 0; k < CF; ++k) {
+    const int t = k * BLOCK + threadIdx.x;
+    if (dd[k] >= 0) f[dd[k]] = fo[k] + (double)s_fe[t];
+  }
 }
 
 template <typename T, int DIM, int NG, int NB, int NQ>

@@ -55,6 +55,7 @@ __global__ void k_finish_mean(int nparts, const double* __restrict__ partial, in
 __global__ void k_add_inhom(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
                             const double* __restrict__ nzval, const uint8_t* __restrict__ isc,
                             const double* __restrict__ gfull, double* __restrict__ f) {
+#pragma clang fp contract(off)
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     double fr = f[r];
     bool touched = false;
@@ -69,7 +70,7 @@ __global__ void k_add_inhom(int64_t n, const int32_t* __restrict__ colptr, const
         if (rowval[mid] < (int32_t)r) lo = mid + 1; else hi = mid;
       }
       if (lo < colptr[d + 1] && rowval[lo] == (int32_t)r) {
-        fr = __dsub_rn(fr, __dmul_rn(v, nzval[lo]));
+        fr = fr - v * nzval[lo];
         touched = true;
       }
     }
@@ -90,12 +91,13 @@ __global__ void k_zero_rows_cols(int64_t n, const int32_t* __restrict__ colptr, 
 __global__ void k_set_diag(int64_t np, const int32_t* __restrict__ pres, const int32_t* __restrict__ diag,
                            const double* __restrict__ g, const double* __restrict__ m, int applyzero,
                            double* __restrict__ nzval, double* __restrict__ f) {
+#pragma clang fp contract(off)
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t d = pres[i];
     const double mm = m[0];
     const int32_t pd = diag[d];
     if (pd >= 0) nzval[pd] = mm;
-    f[d] = applyzero ? 0.0 * mm : __dmul_rn(g[i], mm);
+    f[d] = applyzero ? 0.0 * mm : g[i] * mm;
   }
 }
 

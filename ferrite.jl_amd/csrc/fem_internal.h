@@ -18,9 +18,8 @@ constexpr int kMaxNB = 27;     // scalar base functions (Q2 hex)
 constexpr int kMaxNG = 8;      // geometric nodes (Q1 geometry)
 constexpr int kMaxNFQ = 9;     // facet quadrature points (3x3)
 
-// Reference tables of CellValues, tabulated on the host at the quadrature points.
-// Kept in __constant__ memory; every lane reads the same entry, so the loads are
-// scalar (s_load) and cost no VGPRs.
+// Reference tables of CellValues as uploaded (host copy); the element kernels read a
+// packed copy from device memory with lane-uniform indices (scalar loads).
 struct RefTables {
   double N[kMaxQP][kMaxNB];
   double dN[kMaxQP][kMaxNB][3];
@@ -65,6 +64,7 @@ struct fem_ctx {
 
   // pattern (CSC, 0-based, int32 indices; nnz < 2^31)
   int64_t n = 0, nnz = 0;
+  int max_row_nnz = 0;                 // longest column of the pattern
   int32_t* d_colptr = nullptr;
   int32_t* d_rowval = nullptr;
   int32_t* d_diag = nullptr;           // position of K[j,j] in column j (-1 if absent)

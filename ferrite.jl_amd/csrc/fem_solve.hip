@@ -7,63 +7,86 @@
 //                        FP64-accumulated dot products, u += d in FP64.
 //
 // K is symmetric after apply! (rows and columns of prescribed dofs are zeroed
-// together), so the CSC columns double as CSR rows: y = K x is a row-gather
-// y[j] = Σ_p nzval[p] x[rowval[p]] over column j, with no atomics.
-// Every reduction is a fixed-shape two-level tree (per-block partials, then the
-// last block to finish sums them in a fixed order), so results are bitwise
-// reproducible; the CG scalars (α, β) never leave the device.  The host looks
-// at the residual only every few iterations; kernels launched past
-// convergence return immediately on a device flag.
+// together), so the CSC columns double as CSR rows: y = K x is a row gather
+// y[j] = Σ_p nzval[p] x[rowval[p]] over column j.  The SpMV is CSR-stream: a
+// block owns R consecutive rows whose entries it streams once, fully coalesced,
+// into LDS products, then one lane per row sums its products in storage order.
+//
+// Reductions are two-level and fence-free: each block writes one partial, a
+// one-block kernel sums the partials in a fixed order and updates the CG
+// scalars (α, β, convergence flag) in device memory, so results are bitwise
+// reproducible and the host never sits in the iteration.  A batch of
+// iterations is captured once into a hipGraph and replayed; kernels launched
+// past convergence return at once on the device flag.
 #include <cmath>
+#include <cstring>
 
 #include "fem_internal.h"
 
 namespace fem {
 
 constexpr int kBlock = 256;
-constexpr int kParts = 1024;      // fixed number of blocks of every reducing kernel
+constexpr int kTile = 4 * kBlock;    // elements per block per pass of the vector kernels
+constexpr int kMaxVecBlocks = 2048;
+constexpr int kBatch = 16;           // CG iterations per graph replay
 
-// Device-side solver state (one cache line apart to avoid false sharing).
 struct Scal {
-  double rz, rz_new, pq, rr, alpha, beta, bnorm2, pad0;
-  unsigned int counter[4];
-  int done;
-  int iters;
-  int pad1[2];
+  double rz, pq, rr, alpha, beta, bnorm2;
+  int done, iters;
+};
+
+struct SpmvShape {
+  int R = 0;          // rows per block
+  int blocks = 0;
+};
+
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  const void* key[8] = {};
 };
 
 struct SolverWork {
   int64_t n = 0;
   double *r = nullptr, *p = nullptr, *q = nullptr, *invd = nullptr, *res = nullptr;
   float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr, *invd32 = nullptr;
-  double* part = nullptr;     // [4][kParts]
+  double* part_s = nullptr;   // SpMV partials, one per row block
+  double* part_v = nullptr;   // vector-kernel partials, 2 per block
   Scal* s = nullptr;
+  SpmvShape sh64, sh32;
+  int vblocks = 0;
+  GraphCache g64, g32;
 };
+
+static void destroy_graph(GraphCache& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g = GraphCache{};
+}
 
 void free_solver(fem_ctx* c) {
   if (!c->solver) return;
   SolverWork* w = c->solver;
+  destroy_graph(w->g64);
+  destroy_graph(w->g32);
   dev_free(&w->r); dev_free(&w->p); dev_free(&w->q); dev_free(&w->invd); dev_free(&w->res);
   dev_free(&w->r32); dev_free(&w->p32); dev_free(&w->q32); dev_free(&w->x32); dev_free(&w->invd32);
-  dev_free(&w->part); dev_free(&w->s);
+  dev_free(&w->part_s); dev_free(&w->part_v); dev_free(&w->s);
   delete w;
   c->solver = nullptr;
 }
 
 // --------------------------------------------------------------------------
-// reductions
+// reductions (no fences: partials are consumed by the next kernel)
 // --------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   return v;
 }
 
-// Block sum; result valid in thread 0.
+// Block sum, valid in thread 0; `sh` holds kBlock/64 doubles.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __syncthreads();
-  if (lane == 0) sh[wid] = v;
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
   double r = 0.0;
   if (threadIdx.x == 0)
@@ -71,126 +94,173 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
-// Sum of part[0..kParts) in a fixed order by one block; result in thread 0.
-__device__ __forceinline__ double sum_parts(const double* part, double* sh) {
+// Fixed-order sum of part[0..m) (stride apart) by one block; valid in thread 0.
+__device__ __forceinline__ double sum_parts(const double* __restrict__ part, int m, int stride, double* sh) {
   double v = 0.0;
-  for (int i = threadIdx.x; i < kParts; i += kBlock) v += part[i];
+  for (int i = threadIdx.x; i < m; i += kBlock) v += part[(int64_t)i * stride];
   return block_sum(v, sh);
 }
 
-// Returns true in every thread of the block that finishes last.
-__device__ __forceinline__ bool last_block(unsigned int* counter) {
-  __shared__ bool last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int t = atomicAdd(counter, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  return last;
-}
-
 // --------------------------------------------------------------------------
-// SpMV  y = K x (+ optional fused dots), G lanes per row
+// CSR-stream SpMV
+//   MODE 0: y = K x, partial Σ x·y   (CG: q = K p, p·q)
+//   MODE 1: y = f − K x, partial Σ y² (FP64 residual)
 // --------------------------------------------------------------------------
-template <int G, typename V, typename X, typename A>
-__device__ __forceinline__ A row_dot(int64_t j, const int32_t* __restrict__ colptr,
-                                     const int32_t* __restrict__ rowval, const V* __restrict__ val,
-                                     const X* __restrict__ x, int sub) {
+template <typename V, typename X, int MODE, int CAP>
+__global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, int R, const int32_t* __restrict__ colptr,
+                                                 const int32_t* __restrict__ rowval, const V* __restrict__ val,
+                                                 const X* __restrict__ x, const double* __restrict__ f,
+                                                 X* __restrict__ y, double* __restrict__ part,
+                                                 const Scal* __restrict__ s) {
+  using A = typename std::conditional<sizeof(X) == 8, double, float>::type;
+  if (MODE == 0 && s->done) return;
+  __shared__ A prod[CAP];
+  __shared__ int32_t sptr[kBlock + 1];
+  __shared__ double sh[kBlock / 64];
+  const int tid = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * R;
+  const int nr = (int)min<int64_t>(R, n - r0);
+  for (int t = tid; t <= nr; t += kBlock) sptr[t] = colptr[r0 + t];
+  __syncthreads();
+  const int32_t e0 = sptr[0], e1 = sptr[nr];
   A acc = A(0);
-  const int32_t p0 = colptr[j], p1 = colptr[j + 1];
-  for (int32_t p = p0 + sub; p < p1; p += G) acc += (A)val[p] * (A)x[rowval[p]];
+  for (int32_t c0 = e0; c0 < e1; c0 += CAP) {
+    const int cn = min(CAP, e1 - c0);
+    constexpr int PER = CAP / kBlock;
+    V v[PER];
+    int32_t ri[PER];
 #pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, G);
-  return acc;
+    for (int k = 0; k < PER; ++k) {
+      const int idx = k * kBlock + tid;
+      const bool ok = idx < cn;
+      v[k] = ok ? val[c0 + idx] : V(0);
+      ri[k] = ok ? rowval[c0 + idx] : 0;
+    }
+    X xv[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) xv[k] = x[ri[k]];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int idx = k * kBlock + tid;
+      if (idx < cn) prod[idx] = (A)v[k] * (A)xv[k];
+    }
+    __syncthreads();
+    if (tid < nr) {
+      const int32_t a = max(sptr[tid], c0), b = min(sptr[tid + 1], c0 + cn);
+      for (int32_t e = a; e < b; ++e) acc += prod[e - c0];
+    }
+    __syncthreads();
+  }
+  double d = 0.0;
+  if (tid < nr) {
+    const int64_t j = r0 + tid;
+    if (MODE == 0) {
+      y[j] = (X)acc;
+      d = (double)x[j] * (double)(X)acc;
+    } else {
+      const double rj = f[j] - (double)acc;
+      y[j] = (X)rj;
+      d = rj * rj;
+    }
+  }
+  const double b = block_sum(d, sh);
+  if (tid == 0) part[blockIdx.x] = b;
 }
 
-// q = K p ; part0 = Σ p·q.  Last block computes α = rz / pq.
-template <int G, typename V, typename X>
-__global__ __launch_bounds__(kBlock) void k_spmv_pq(int64_t n, const int32_t* __restrict__ colptr,
-                                                    const int32_t* __restrict__ rowval,
-                                                    const V* __restrict__ val, const X* __restrict__ p,
-                                                    X* __restrict__ q, double* __restrict__ part,
-                                                    Scal* __restrict__ s) {
+// α = rz / (p·q)
+__global__ __launch_bounds__(kBlock) void k_alpha(int m, const double* __restrict__ part, Scal* __restrict__ s) {
   if (s->done) return;
   __shared__ double sh[kBlock / 64];
-  using A = typename std::conditional<sizeof(X) == 8, double, float>::type;
-  const int sub = threadIdx.x % G;
-  const int64_t rows_per_pass = (int64_t)gridDim.x * (kBlock / G);
-  double dot = 0.0;
-  for (int64_t j = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G; j < n; j += rows_per_pass) {
-    const A y = row_dot<G, V, X, A>(j, colptr, rowval, val, p, sub);
-    if (sub == 0) {
-      q[j] = (X)y;
-      dot += (double)p[j] * (double)(X)y;
-    }
-  }
-  const double b = block_sum(dot, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = b;
-  if (last_block(&s->counter[0])) {
-    const double pq = sum_parts(part, sh);
-    if (threadIdx.x == 0) {
-      s->pq = pq;
-      s->alpha = s->rz / pq;
-      s->counter[0] = 0;
-    }
+  const double pq = sum_parts(part, m, 1, sh);
+  if (threadIdx.x == 0) {
+    s->pq = pq;
+    s->alpha = s->rz / pq;
   }
 }
 
-// x += α p ; r -= α q ; z = D⁻¹ r ; partials of r·z and r·r.  Last block: β, convergence flag.
+// x += α p ; r -= α q ; z = D⁻¹ r (into q) ; partials r·z, r·r
 template <typename X>
-__global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, X* __restrict__ x, X* __restrict__ r,
-                                                      const X* __restrict__ p, const X* q,
-                                                      const X* __restrict__ invd, X* z_out,
-                                                      double* __restrict__ part, Scal* __restrict__ s,
-                                                      double tol2) {
+__global__ __launch_bounds__(kBlock) void k_update(int64_t n, X* __restrict__ x, X* __restrict__ r,
+                                                   const X* __restrict__ p, X* q, const X* __restrict__ invd,
+                                                   double* __restrict__ part, const Scal* __restrict__ s) {
   if (s->done) return;
   __shared__ double sh[kBlock / 64];
   const X alpha = (X)s->alpha;
   double rz = 0.0, rr = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    x[i] += alpha * p[i];
-    const X ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    const X zi = ri * invd[i];
-    z_out[i] = zi;
-    rz += (double)ri * (double)zi;
-    rr += (double)ri * (double)ri;
+  for (int64_t base = (int64_t)blockIdx.x * kTile; base < n; base += (int64_t)gridDim.x * kTile) {
+    X pv[4], qv[4], xv[4], rv[4], dv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = base + k * kBlock + threadIdx.x;
+      const bool ok = i < n;
+      pv[k] = ok ? p[i] : X(0);
+      qv[k] = ok ? q[i] : X(0);
+      xv[k] = ok ? x[i] : X(0);
+      rv[k] = ok ? r[i] : X(0);
+      dv[k] = ok ? invd[i] : X(0);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = base + k * kBlock + threadIdx.x;
+      if (i < n) {
+        const X ri = rv[k] - alpha * qv[k];
+        const X zi = ri * dv[k];
+        x[i] = xv[k] + alpha * pv[k];
+        r[i] = ri;
+        q[i] = zi;
+        rz += (double)ri * (double)zi;
+        rr += (double)ri * (double)ri;
+      }
+    }
   }
   double b = block_sum(rz, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = b;
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = b;
   b = block_sum(rr, sh);
-  if (threadIdx.x == 0) part[kParts + blockIdx.x] = b;
-  if (last_block(&s->counter[1])) {
-    const double rzn = sum_parts(part, sh);
-    const double rrn = sum_parts(part + kParts, sh);
-    if (threadIdx.x == 0) {
-      s->beta = rzn / s->rz;
-      s->rz = rzn;
-      s->rr = rrn;
-      s->iters += 1;
-      if (rrn <= tol2) s->done = 1;
-      s->counter[1] = 0;
+  if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = b;
+}
+
+// β = rz_new / rz ; convergence flag
+__global__ __launch_bounds__(kBlock) void k_beta(int m, const double* __restrict__ part, Scal* __restrict__ s,
+                                                 double tol2) {
+  if (s->done) return;
+  __shared__ double sh[kBlock / 64];
+  const double rzn = sum_parts(part, m, 2, sh);
+  const double rrn = sum_parts(part + 1, m, 2, sh);
+  if (threadIdx.x == 0) {
+    s->beta = rzn / s->rz;
+    s->rz = rzn;
+    s->rr = rrn;
+    s->iters += 1;
+    if (rrn <= tol2) s->done = 1;
+  }
+}
+
+// p = z + β p
+template <typename X>
+__global__ __launch_bounds__(kBlock) void k_pupd(int64_t n, X* __restrict__ p, const X* __restrict__ z,
+                                                 const Scal* __restrict__ s) {
+  if (s->done) return;
+  const X beta = (X)s->beta;
+  for (int64_t base = (int64_t)blockIdx.x * kTile; base < n; base += (int64_t)gridDim.x * kTile) {
+    X zv[4], pv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = base + k * kBlock + threadIdx.x;
+      zv[k] = i < n ? z[i] : X(0);
+      pv[k] = i < n ? p[i] : X(0);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = base + k * kBlock + threadIdx.x;
+      if (i < n) p[i] = zv[k] + beta * pv[k];
     }
   }
 }
 
-// p = z + β p   (z stored in q's slot by the update kernel's caller)
+// z = D⁻¹ r, p = z ; partials r·z, r·r
 template <typename X>
-__global__ __launch_bounds__(kBlock) void k_p_update(int64_t n, X* __restrict__ p, const X* __restrict__ z,
-                                                     const Scal* __restrict__ s) {
-  if (s->done) return;
-  const X beta = (X)s->beta;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    p[i] = z[i] + beta * p[i];
-}
-
-// Start: z = D⁻¹ r, p = z, rz = r·z, rr = r·r, done = (rr <= tol2).
-template <typename X>
-__global__ __launch_bounds__(kBlock) void k_cg_init(int64_t n, const X* __restrict__ r, const X* __restrict__ invd,
-                                                    X* __restrict__ p, double* __restrict__ part,
-                                                    Scal* __restrict__ s, double tol2) {
+__global__ __launch_bounds__(kBlock) void k_init(int64_t n, const X* __restrict__ r, const X* __restrict__ invd,
+                                                 X* __restrict__ p, double* __restrict__ part) {
   __shared__ double sh[kBlock / 64];
   double rz = 0.0, rr = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
@@ -201,65 +271,41 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int64_t n, const X* __restri
     rr += (double)ri * (double)ri;
   }
   double b = block_sum(rz, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = b;
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = b;
   b = block_sum(rr, sh);
-  if (threadIdx.x == 0) part[kParts + blockIdx.x] = b;
-  if (last_block(&s->counter[2])) {
-    const double rzn = sum_parts(part, sh);
-    const double rrn = sum_parts(part + kParts, sh);
-    if (threadIdx.x == 0) {
-      s->rz = rzn;
-      s->rr = rrn;
-      s->done = (rrn <= tol2) ? 1 : 0;
-      s->counter[2] = 0;
-    }
-  }
+  if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = b;
 }
 
-// res = f − K u (FP64) ; part = Σ res² ; optionally scaled FP32 copy r32 = res * scale is done separately.
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_residual(int64_t n, const int32_t* __restrict__ colptr,
-                                                     const int32_t* __restrict__ rowval,
-                                                     const double* __restrict__ val, const double* __restrict__ u,
-                                                     const double* __restrict__ f, double* __restrict__ res,
-                                                     double* __restrict__ part, Scal* __restrict__ s) {
+__global__ __launch_bounds__(kBlock) void k_init_scal(int m, const double* __restrict__ part, Scal* __restrict__ s,
+                                                      double tol2) {
   __shared__ double sh[kBlock / 64];
-  const int sub = threadIdx.x % G;
-  const int64_t rows_per_pass = (int64_t)gridDim.x * (kBlock / G);
-  double acc = 0.0;
-  for (int64_t j = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G; j < n; j += rows_per_pass) {
-    const double y = row_dot<G, double, double, double>(j, colptr, rowval, val, u, sub);
-    if (sub == 0) {
-      const double rj = f[j] - y;
-      res[j] = rj;
-      acc += rj * rj;
-    }
-  }
-  const double b = block_sum(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = b;
-  if (last_block(&s->counter[3])) {
-    const double rr = sum_parts(part, sh);
-    if (threadIdx.x == 0) {
-      s->rr = rr;
-      s->counter[3] = 0;
-    }
+  const double rz = sum_parts(part, m, 2, sh);
+  const double rr = sum_parts(part + 1, m, 2, sh);
+  if (threadIdx.x == 0) {
+    s->rz = rz;
+    s->rr = rr;
+    s->iters = 0;
+    s->done = (rr <= tol2) ? 1 : 0;
   }
 }
 
-__global__ void k_sumsq(int64_t n, const double* __restrict__ v, double* __restrict__ part, Scal* __restrict__ s) {
+// s->rr (which = 0) or s->bnorm2 (which = 1) = Σ part, fixed order
+__global__ __launch_bounds__(kBlock) void k_sum_to(int m, const double* __restrict__ part, Scal* __restrict__ s,
+                                                   int which) {
+  __shared__ double sh[kBlock / 64];
+  const double t = sum_parts(part, m, 1, sh);
+  if (threadIdx.x == 0) {
+    if (which == 0) s->rr = t; else s->bnorm2 = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sumsq(int64_t n, const double* __restrict__ v, double* __restrict__ part) {
   __shared__ double sh[kBlock / 64];
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     acc += v[i] * v[i];
   const double b = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
-  if (last_block(&s->counter[3])) {
-    const double t = sum_parts(part, sh);
-    if (threadIdx.x == 0) {
-      s->bnorm2 = t;
-      s->counter[3] = 0;
-    }
-  }
 }
 
 template <typename X>
@@ -277,20 +323,49 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
     b[i] = (float)a[i];
 }
 
-// r32 = res / ‖res‖ ; x32 = 0
-__global__ void k_scale_to_f32(int64_t n, const double* __restrict__ res, const Scal* __restrict__ s,
-                               float* __restrict__ r32, float* __restrict__ x32) {
-  const double inv = 1.0 / sqrt(s->rr);
+// Reliable update, step 1: fold the FP32 correction into the FP64 iterate: u += x32, x32 = 0.
+__global__ void k_fold(int64_t n, float* __restrict__ x32, double* __restrict__ u) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    r32[i] = (float)(res[i] * inv);
+    u[i] += (double)x32[i];
     x32[i] = 0.0f;
   }
 }
 
-// u += ‖res‖ d
-__global__ void k_correct(int64_t n, const float* __restrict__ d, double scale, double* __restrict__ u) {
+// Reliable update, step 2: r32 = res (the FP64 true residual), partials of r32·D r32.
+__global__ __launch_bounds__(kBlock) void k_reseed(int64_t n, const double* __restrict__ res,
+                                                   const float* __restrict__ invd, float* __restrict__ r32,
+                                                   double* __restrict__ part) {
+  __shared__ double sh[kBlock / 64];
+  double rz = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const float ri = (float)res[i];
+    r32[i] = ri;
+    rz += (double)ri * (double)(ri * invd[i]);
+  }
+  const double b = block_sum(rz, sh);
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = b;
+  if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = 0.0;
+}
+
+// Reliable update, step 3: rz from the reseeded residual; the search direction p is kept.
+__global__ __launch_bounds__(kBlock) void k_reseed_scal(int m, const double* __restrict__ part,
+                                                        Scal* __restrict__ s) {
+  __shared__ double sh[kBlock / 64];
+  const double rz = sum_parts(part, m, 2, sh);
+  if (threadIdx.x == 0) {
+    s->rz = rz;
+    s->done = 0;
+  }
+}
+
+__global__ void k_f32_from(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    u[i] += scale * (double)d[i];
+    b[i] = (float)a[i];
+}
+
+__global__ void k_zero32(int64_t n, float* __restrict__ a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0.0f;
 }
 
 __global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
@@ -303,44 +378,26 @@ __global__ void k_zero(int64_t n, double* __restrict__ a) {
     a[i] = 0.0;
 }
 
-__global__ void k_reset_scal(Scal* s) {
-  s->rz = s->rz_new = s->pq = s->rr = s->alpha = s->beta = 0.0;
-  for (int i = 0; i < 4; ++i) s->counter[i] = 0;
-  s->done = 0;
-  s->iters = 0;
-}
-
 // --------------------------------------------------------------------------
 // host driver
 // --------------------------------------------------------------------------
-static int lanes_per_row(const fem_ctx* c) {
-  const double avg = (double)c->nnz / (double)c->n;
-  if (avg <= 6) return 4;
-  if (avg <= 14) return 8;
-  if (avg <= 40) return 16;
-  if (avg <= 90) return 32;
-  return 64;
+constexpr int kCap64 = 2048;   // LDS products per chunk (16 KB FP64)
+constexpr int kCap32 = 4096;   // (16 KB FP32)
+
+static SpmvShape spmv_shape(const fem_ctx* c, int cap) {
+  SpmvShape s;
+  const int longest = std::max(1, c->max_row_nnz);
+  s.R = std::max(1, std::min(kBlock, cap / longest));
+  s.blocks = (int)((c->n + s.R - 1) / s.R);
+  return s;
 }
 
-template <typename V, typename X>
-static void launch_spmv_pq(fem_ctx* c, int G, const V* val, const X* p, X* q, SolverWork* w) {
-  switch (G) {
-    case 4: k_spmv_pq<4, V, X><<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_colptr, c->d_rowval, val, p, q, w->part, w->s); break;
-    case 8: k_spmv_pq<8, V, X><<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_colptr, c->d_rowval, val, p, q, w->part, w->s); break;
-    case 16: k_spmv_pq<16, V, X><<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_colptr, c->d_rowval, val, p, q, w->part, w->s); break;
-    case 32: k_spmv_pq<32, V, X><<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_colptr, c->d_rowval, val, p, q, w->part, w->s); break;
-    default: k_spmv_pq<64, V, X><<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_colptr, c->d_rowval, val, p, q, w->part, w->s); break;
-  }
-}
-
-static void launch_residual(fem_ctx* c, int G, SolverWork* w) {
-  switch (G) {
-    case 4: k_residual<4><<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_u, c->d_f, w->res, w->part, w->s); break;
-    case 8: k_residual<8><<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_u, c->d_f, w->res, w->part, w->s); break;
-    case 16: k_residual<16><<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_u, c->d_f, w->res, w->part, w->s); break;
-    case 32: k_residual<32><<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_u, c->d_f, w->res, w->part, w->s); break;
-    default: k_residual<64><<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_u, c->d_f, w->res, w->part, w->s); break;
-  }
+template <typename V, typename X, int MODE>
+static void launch_spmv(fem_ctx* c, const SpmvShape& sh, const V* val, const X* x, const double* f, X* y,
+                        double* part, const Scal* s) {
+  constexpr int CAP = sizeof(X) == 8 ? kCap64 : kCap32;
+  k_spmv<V, X, MODE, CAP><<<sh.blocks, kBlock, 0, c->stream>>>(c->n, sh.R, c->d_colptr, c->d_rowval, val, x, f, y,
+                                                                 part, s);
 }
 
 static int ensure_work(fem_ctx* c, bool mixed) {
@@ -349,9 +406,16 @@ static int ensure_work(fem_ctx* c, bool mixed) {
   const int64_t n = c->n;
   int rc;
   if (w->n != n || !w->r) {
+    destroy_graph(w->g64);
+    destroy_graph(w->g32);
+    w->sh64 = spmv_shape(c, kCap64);
+    w->sh32 = spmv_shape(c, kCap32);
+    w->vblocks = (int)std::min<int64_t>(kMaxVecBlocks, (n + kTile - 1) / kTile);
+    const int ps = std::max(w->sh64.blocks, w->sh32.blocks);
     if ((rc = dev_alloc(c, &w->r, n)) || (rc = dev_alloc(c, &w->p, n)) || (rc = dev_alloc(c, &w->q, n)) ||
-        (rc = dev_alloc(c, &w->invd, n)) || (rc = dev_alloc(c, &w->res, n)) || (rc = dev_alloc(c, &w->part, 4 * kParts)) ||
-        (rc = dev_alloc(c, &w->s, 1)))
+        (rc = dev_alloc(c, &w->invd, n)) || (rc = dev_alloc(c, &w->res, n)) ||
+        (rc = dev_alloc(c, &w->part_s, std::max(ps, kMaxVecBlocks))) ||
+        (rc = dev_alloc(c, &w->part_v, 2 * kMaxVecBlocks)) || (rc = dev_alloc(c, &w->s, 1)))
       return rc;
     dev_free(&w->r32);
     w->n = n;
@@ -370,42 +434,68 @@ static int read_scal(fem_ctx* c, SolverWork* w, Scal* h) {
   return FEM_OK;
 }
 
-// Preconditioned CG on K (FP64: V = X = double; mixed inner: V = X = float).
-// Solves K x = r starting from x = 0 (r overwritten with the residual).
+// One CG iteration's launches (q = K p, α, update, β, p update).
+template <typename V, typename X>
+static void enqueue_iteration(fem_ctx* c, SolverWork* w, const SpmvShape& sh, const V* val, X* x, X* r, X* p, X* q,
+                              const X* invd, double tol2) {
+  launch_spmv<V, X, 0>(c, sh, val, p, nullptr, q, w->part_s, w->s);
+  k_alpha<<<1, kBlock, 0, c->stream>>>(sh.blocks, w->part_s, w->s);
+  k_update<X><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, x, r, p, q, invd, w->part_v, w->s);
+  k_beta<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2);
+  k_pupd<X><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, p, q, w->s);
+}
+
+// The captured batch of kBatch iterations (rebuilt only when the buffers or the tolerance change).
+template <typename V, typename X>
+static int batch_graph(fem_ctx* c, SolverWork* w, const V* val, X* x, X* r, X* p, X* q, const X* invd, double tol2,
+                       GraphCache& gc) {
+  const SpmvShape& sh = sizeof(X) == 8 ? w->sh64 : w->sh32;
+  double tkey = tol2;
+  const void* key[8] = {val, x, r, p, q, invd, w->s, nullptr};
+  std::memcpy(&key[7], &tkey, sizeof(double));
+  bool same = gc.exec != nullptr;
+  for (int i = 0; i < 8 && same; ++i) same = gc.key[i] == key[i];
+  if (same) return FEM_OK;
+  destroy_graph(gc);
+  hipGraph_t graph = nullptr;
+  FEM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < kBatch; ++k) enqueue_iteration<V, X>(c, w, sh, val, x, r, p, q, invd, tol2);
+  FEM_HIP(c, hipStreamEndCapture(c->stream, &graph));
+  hipError_t e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return check_hip(c, e, "hipGraphInstantiate");
+  for (int i = 0; i < 8; ++i) gc.key[i] = key[i];
+  return FEM_OK;
+}
+
+// Preconditioned CG on K x = r from x = 0 (r is overwritten with the residual).
 template <typename V, typename X>
 static int pcg(fem_ctx* c, SolverWork* w, const V* val, X* x, X* r, X* p, X* q, const X* invd, double tol2,
-               int maxit, int* iters, double* spmv_ms, int* spmv_count) {
-  const int G = lanes_per_row(c);
-  const int64_t n = c->n;
-  const int vgrid = kParts;
-  k_reset_scal<<<1, 1, 0, c->stream>>>(w->s);
-  k_cg_init<X><<<vgrid, kBlock, 0, c->stream>>>(n, r, invd, p, w->part, w->s, tol2);
+               int maxit, GraphCache& gc, int* iters, double* ms) {
+  k_init<X><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, r, invd, p, w->part_v);
+  k_init_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2);
   FEM_HIP(c, hipGetLastError());
-  const int check_every = 16;
-  int it = 0;
+  int rc = batch_graph<V, X>(c, w, val, x, r, p, q, invd, tol2, gc);
+  if (rc) return rc;
   Scal h{};
-  float ms_total = 0.f;
-  while (it < maxit) {
-    const int batch = std::min(check_every, maxit - it);
-    FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    for (int k = 0; k < batch; ++k) {
-      launch_spmv_pq<V, X>(c, G, val, p, q, w);
-      k_cg_update<X><<<vgrid, kBlock, 0, c->stream>>>(n, x, r, p, q, invd, q, w->part, w->s, tol2);
-      k_p_update<X><<<vgrid, kBlock, 0, c->stream>>>(n, p, q, w->s);
-    }
-    FEM_HIP(c, hipGetLastError());
-    FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
-    ms_total += event_ms(c);
-    int rc = read_scal(c, w, &h);
+  int launched = 0;
+  FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+  while (launched < maxit) {
+    FEM_HIP(c, hipGraphLaunch(gc.exec, c->stream));
+    launched += kBatch;
+    rc = read_scal(c, w, &h);
     if (rc) return rc;
-    it += batch;
     if (h.done) break;
   }
-  // iterations actually performed: the device stops on the first converged update
+  FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
+  *ms += event_ms(c);
   *iters = h.iters;
-  *spmv_ms += ms_total;
-  *spmv_count += h.iters;
   return h.done ? FEM_OK : FEM_ERR_NOCONV;
+}
+
+static void residual(fem_ctx* c, SolverWork* w) {
+  launch_spmv<double, double, 1>(c, w->sh64, c->d_nzval, c->d_u, c->d_f, w->res, w->part_s, w->s);
+  k_sum_to<<<1, kBlock, 0, c->stream>>>(w->sh64.blocks, w->part_s, w->s, 0);
 }
 
 }  // namespace fem
@@ -421,21 +511,17 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   if (rc) return rc;
   SolverWork* w = c->solver;
   const int64_t n = c->n;
-  const int grid = kParts;
   const int maxit = cfg->maxit > 0 ? cfg->maxit : 100000;
   fem_solve_info out{};
-  c->timings.spmv_ms = 0.0;
-  c->timings.spmv_launches = 0;
-  double spmv_ms = 0.0;
-  int spmv_count = 0;
+  double cg_ms = 0.0;
 
-  k_reset_scal<<<1, 1, 0, c->stream>>>(w->s);
-  k_sumsq<<<grid, kBlock, 0, c->stream>>>(n, c->d_f, w->part, w->s);
+  k_sumsq<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->part_s);
+  k_sum_to<<<1, kBlock, 0, c->stream>>>(kMaxVecBlocks, w->part_s, w->s, 1);
   Scal h{};
   if ((rc = read_scal(c, w, &h))) return rc;
   const double bnorm2 = h.bnorm2;
   if (bnorm2 == 0.0) {
-    k_zero<<<grid, kBlock, 0, c->stream>>>(n, c->d_u);
+    k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_u);
     FEM_HIP(c, hipStreamSynchronize(c->stream));
     out.outer_iterations = 1;
     if (info) *info = out;
@@ -444,53 +530,68 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   const double tol2 = cfg->rtol * cfg->rtol * bnorm2;
 
   if (!mixed) {
-    k_invdiag<double><<<grid, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd);
-    k_zero<<<grid, kBlock, 0, c->stream>>>(n, c->d_u);
-    k_copy<<<grid, kBlock, 0, c->stream>>>(n, c->d_f, w->r);
+    k_invdiag<double><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd);
+    k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_u);
+    k_copy<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->r);
     int iters = 0;
-    rc = pcg<double, double>(c, w, c->d_nzval, c->d_u, w->r, w->p, w->q, w->invd, tol2, maxit, &iters, &spmv_ms,
-                             &spmv_count);
+    rc = pcg<double, double>(c, w, c->d_nzval, c->d_u, w->r, w->p, w->q, w->invd, tol2, maxit, w->g64, &iters,
+                             &cg_ms);
     out.iterations = iters;
     out.outer_iterations = 1;
     if (rc && rc != FEM_ERR_NOCONV) return rc;
   } else {
     if (!c->d_nzval32) {
       if ((rc = dev_alloc(c, &c->d_nzval32, c->nnz))) return rc;
-      k_to_f32<<<4096, kBlock, 0, c->stream>>>(c->nnz, c->d_nzval, c->d_nzval32);
+      k_to_f32<<<8192, kBlock, 0, c->stream>>>(c->nnz, c->d_nzval, c->d_nzval32);
     }
-    k_invdiag<float><<<grid, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd32);
-    k_zero<<<grid, kBlock, 0, c->stream>>>(n, c->d_u);
-    const double inner = cfg->inner_rtol > 0 ? cfg->inner_rtol : 1e-5;
-    const int G = lanes_per_row(c);
-    int total = 0, outer = 0;
-    rc = FEM_ERR_NOCONV;
-    while (total < maxit && outer < 100) {
-      k_reset_scal<<<1, 1, 0, c->stream>>>(w->s);
-      launch_residual(c, G, w);
+    k_invdiag<float><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd32);
+    // Mixed CG with reliable updates: one FP32 Krylov sequence (FP32 SpMV and vectors,
+    // FP64 dot products); whenever its recursive residual has dropped by δ since the last
+    // refinement (or claims convergence), the FP32 iterate is folded into the FP64 u, the
+    // true residual f − K u is recomputed with the FP64 K and fed back as the FP32
+    // residual, keeping the search direction.
+    const double delta = cfg->inner_rtol > 0 ? cfg->inner_rtol : 0.1;
+    k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_u);
+    k_zero32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, w->x32);
+    k_f32_from<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->r32);
+    k_init<float><<<w->vblocks, kBlock, 0, c->stream>>>(n, w->r32, w->invd32, w->p32, w->part_v);
+    k_init_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2);
+    if ((rc = batch_graph<float, float>(c, w, c->d_nzval32, w->x32, w->r32, w->p32, w->q32, w->invd32, tol2, w->g32)))
+      return rc;
+    double maxrr = bnorm2;
+    int launched = 0, refinements = 0;
+    bool converged = false;
+    FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    while (launched < maxit && !converged) {
+      FEM_HIP(c, hipGraphLaunch(w->g32.exec, c->stream));
+      launched += kBatch;
       Scal hs{};
-      int r2 = read_scal(c, w, &hs);
-      if (r2) return r2;
-      if (hs.rr <= tol2) { rc = FEM_OK; break; }
-      ++outer;
-      k_scale_to_f32<<<grid, kBlock, 0, c->stream>>>(n, w->res, w->s, w->r32, w->x32);
-      int iters = 0;
-      int r3 = pcg<float, float>(c, w, c->d_nzval32, w->x32, w->r32, w->p32, w->q32, w->invd32, inner * inner,
-                                 maxit - total, &iters, &spmv_ms, &spmv_count);
-      if (r3 && r3 != FEM_ERR_NOCONV) return r3;
-      total += iters;
-      k_correct<<<grid, kBlock, 0, c->stream>>>(n, w->x32, std::sqrt(hs.rr), c->d_u);
+      if ((rc = read_scal(c, w, &hs))) return rc;
+      if (hs.done || hs.rr < delta * delta * maxrr) {
+        ++refinements;
+        k_fold<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, w->x32, c->d_u);
+        residual(c, w);
+        Scal ht{};
+        if ((rc = read_scal(c, w, &ht))) return rc;
+        if (ht.rr <= tol2) { converged = true; break; }
+        maxrr = ht.rr;
+        k_reseed<<<w->vblocks, kBlock, 0, c->stream>>>(n, w->res, w->invd32, w->r32, w->part_v);
+        k_reseed_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s);
+      }
     }
-    out.iterations = total;
-    out.outer_iterations = outer;
+    FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
+    cg_ms += event_ms(c);
+    Scal hf{};
+    if ((rc = read_scal(c, w, &hf))) return rc;
+    out.iterations = hf.iters;
+    out.outer_iterations = refinements;
   }
-  // true FP64 residual of the returned u
-  k_reset_scal<<<1, 1, 0, c->stream>>>(w->s);
-  launch_residual(c, lanes_per_row(c), w);
+  residual(c, w);
   FEM_HIP(c, hipGetLastError());
   if ((rc = read_scal(c, w, &h))) return rc;
   out.relres = std::sqrt(h.rr / bnorm2);
-  c->timings.spmv_ms = spmv_ms;
-  c->timings.spmv_launches = spmv_count;
+  c->timings.spmv_ms = cg_ms;
+  c->timings.spmv_launches = out.iterations;
   if (info) *info = out;
   if (out.relres > cfg->rtol * 1.0000001) {
     char buf[160];
@@ -500,9 +601,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   return FEM_OK;
 }
 
-// Times `reps` SpMV launches y = K x (FP64 values/vectors, or FP32 with mixed != 0) on the
-// context stream with HIP events; returns the average launch duration.  Used for the
-// roofline figure of the solver's dominant kernel.
+// Average duration (HIP events) of `reps` back-to-back SpMV launches q = K p of the solver.
 extern "C" int32_t fem_spmv_bench(fem_ctx* c, int32_t mixed, int32_t reps, double* avg_ms) {
   if (!c || reps <= 0 || !avg_ms) return FEM_ERR_ARG;
   if (c->nnz == 0) return set_error(c, FEM_ERR_STATE, "no matrix");
@@ -510,22 +609,22 @@ extern "C" int32_t fem_spmv_bench(fem_ctx* c, int32_t mixed, int32_t reps, doubl
   int rc = ensure_work(c, mixed != 0);
   if (rc) return rc;
   SolverWork* w = c->solver;
-  const int G = lanes_per_row(c);
-  k_reset_scal<<<1, 1, 0, c->stream>>>(w->s);
+  Scal zero{};
+  FEM_HIP(c, hipMemcpyAsync(w->s, &zero, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
   if (mixed) {
     if (!c->d_nzval32) {
       if ((rc = dev_alloc(c, &c->d_nzval32, c->nnz))) return rc;
-      k_to_f32<<<4096, kBlock, 0, c->stream>>>(c->nnz, c->d_nzval, c->d_nzval32);
+      k_to_f32<<<8192, kBlock, 0, c->stream>>>(c->nnz, c->d_nzval, c->d_nzval32);
     }
-    k_to_f32<<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p32);
+    k_to_f32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p32);
   } else {
-    k_copy<<<kParts, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p);
+    k_copy<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p);
   }
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < reps; ++i) {
-    if (mixed) launch_spmv_pq<float, float>(c, G, c->d_nzval32, w->p32, w->q32, w);
-    else launch_spmv_pq<double, double>(c, G, c->d_nzval, w->p, w->q, w);
+    if (mixed) launch_spmv<float, float, 0>(c, w->sh32, c->d_nzval32, w->p32, nullptr, w->q32, w->part_s, w->s);
+    else launch_spmv<double, double, 0>(c, w->sh64, c->d_nzval, w->p, nullptr, w->q, w->part_s, w->s);
   }
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));

@@ -1,0 +1,64 @@
+"""Kernel-level profiling driver for rocprofv3 (counters and per-kernel durations).
+
+    python -m ferrite_amd.profile_kernels [--n 128] [--reps 5]
+
+Builds the 3D Poisson Q1 hex n³ system of the headline benchmark, then runs, in order:
+  1. a plain device copy of a known size (torch), the calibration for FETCH_SIZE/WRITE_SIZE;
+  2. `reps` fused assemblies (FP64 element kernel), apply!;
+  3. `reps` FP64 and FP32 solver SpMVs.
+Each phase prints a line, so a counter run can be matched to its kernels by name.
+"""
+import argparse
+import os
+import sys
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    args = ap.parse_args()
+    import numpy as np
+    import torch
+
+    import ferrite_amd as fa
+
+    n = args.n
+    g = fa.generate_grid("Hexahedron", (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    ip = fa.Lagrange("RefHexahedron", 1)
+    cv = fa.CellValues(fa.QuadratureRule("RefHexahedron", 2), ip)
+    dh = fa.DofHandler(g).add("u", ip).close()
+    K = fa.allocate_matrix(fa.MI355XCSC, dh)
+    f = fa.allocate_vector(K)
+    colors = fa.create_coloring(g)
+    k = np.exp(np.random.default_rng(0).normal(0.0, 0.5, size=g.ncells))
+    el = fa.PoissonElement(k=k)
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.Dirichlet("u", g.facetsets["left"] + g.facetsets["right"], lambda x, t: 0.0)).close()
+
+    nbytes = args.copy_mib << 20
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"phase copy: {nbytes} bytes read + {nbytes} bytes written", flush=True)
+    del a, b
+
+    for _ in range(args.reps):
+        fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision="fp64")
+    t = fa.timings(K)
+    print(f"phase assemble: {args.reps} x {g.ncells} cells, last {t['assemble_kernel_ms']:.3f} ms "
+          f"over {t['assemble_launches']} colour launches", flush=True)
+    fa.apply(K, f, ch)
+    ms64 = fa.spmv_bench(K, mixed=False, reps=args.reps)
+    ms32 = fa.spmv_bench(K, mixed=True, reps=args.reps)
+    print(f"phase spmv: fp64 {ms64:.4f} ms, fp32 {ms32:.4f} ms per launch (nnz {K.nnz}, n {dh.ndofs})",
+          flush=True)
+
+
+if __name__ == "__main__":
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    main()

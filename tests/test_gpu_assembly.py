@@ -84,7 +84,7 @@ def test_c1_quad_100_parity():
     assert np.array_equal(ch.prescribed_dofs, och.prescribed_dofs)
     m = fa.apply(K, f, ch)
     om = OC.apply(OK, of, och)
-    assert abs(m - om) <= 1e-14 * abs(om)
+    assert abs(m - om) <= 1e-12 * abs(om)  # meandiag: sequential vs tree summation order
     _check_K(K, f, OK, of, 1e-12)
     res = fa.solve(K, f, mode="cg", rtol=1e-12)
     u_ref = OC.solve(OK, of)
