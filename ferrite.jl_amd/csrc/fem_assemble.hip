@@ -329,9 +329,17 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
   if (cfg->kernel == FEM_KERNEL_POISSON) {
     if (c->vdim != 1 || c->ndpc != c->nbf) return set_error(c, FEM_ERR_ARG, "Poisson kernel needs a scalar field");
     const int key = c->rdim * 1000 + c->ngeo * 100 + c->nbf * 10 + (c->nqp == 8 ? 8 : c->nqp);
+    const int bs = c->tune.asm_block;
     if (c->rdim == 3 && c->ngeo == 8 && c->nbf == 8 && c->nqp == 8 && c->npc == 8 && c->sdim == 3) {
-      rc = mixed ? launch_poisson<float, 3, 8, 8, 8, 128>(c, cfg->source)
-                 : launch_poisson<double, 3, 8, 8, 8, 64>(c, cfg->source);
+      if (mixed) {
+        rc = bs == 64 ? launch_poisson<float, 3, 8, 8, 8, 64>(c, cfg->source)
+           : bs == 256 ? launch_poisson<float, 3, 8, 8, 8, 256>(c, cfg->source)
+                       : launch_poisson<float, 3, 8, 8, 8, 128>(c, cfg->source);
+      } else {
+        rc = bs == 128 ? launch_poisson<double, 3, 8, 8, 8, 128>(c, cfg->source)
+           : bs == 256 ? launch_poisson<double, 3, 8, 8, 8, 256>(c, cfg->source)
+                       : launch_poisson<double, 3, 8, 8, 8, 64>(c, cfg->source);
+      }
     } else if (c->rdim == 2 && c->ngeo == 4 && c->nbf == 4 && c->nqp == 4 && c->npc == 4 && c->sdim == 2) {
       rc = mixed ? launch_poisson<float, 2, 4, 4, 4, 128>(c, cfg->source)
                  : launch_poisson<double, 2, 4, 4, 4, 128>(c, cfg->source);
@@ -346,5 +354,6 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
     return set_error(c, FEM_ERR_ARG, "unknown kernel id");
   }
   if (rc) return rc;
+  c->nzval32_dirty = true;
   return check_assembly_errors(c);
 }

@@ -175,7 +175,7 @@ int32_t fem_apply_dirichlet(fem_ctx* c, int32_t applyzero, double* m_out) {
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
   c->timings.apply_ms = event_ms(c);
   if (m_out) FEM_HIP(c, hipMemcpy(m_out, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost));
-  dev_free(&c->d_nzval32);  // the FP32 copy must be rebuilt from the constrained K
+  c->nzval32_dirty = true;  // the FP32 copy must be rebuilt from the constrained K
   return FEM_OK;
 }
 

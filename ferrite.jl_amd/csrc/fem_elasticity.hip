@@ -1,11 +1,430 @@
-// Linear elasticity element kernels (vector Lagrange) and load entry points.
+// Linear elasticity element kernels (vector Lagrange, small strain, isotropic C):
+//   Ke[(a,i),(b,j)] += E_e ∫ ε(N_a e_i) : C(ν) : ε(N_b e_j) dΩ
+//                    = E_e Σ_q dV_q (λ g_a,i g_b,j + μ δ_ij g_a·g_b + μ g_a,j g_b,i)
+// with g = ∇N at the quadrature point, λ = ν/((1+ν)(1−2ν)), μ = 1/(2(1+ν)) for E = 1
+// (docs/src/literate-howto/threaded_assembly.jl:105-132 element routine and material,
+// plane strain in 2D).  Local dofs are node-major, component-minor (ip^dim,
+// src/interpolations.jl:1815-1825).  Scatter: same colored, atomic-free CSC
+// read-modify-write as the Poisson kernel.
+//
+// Q2 hex (27 nodes, 81 dofs, 3x3x3 Gauss): one wavefront per cell.  The 81x81 Ke is
+// assembled from six 27x27 products P_ij = A_i B_j (i <= j) with A_i[a][q] = g_q[a,i]
+// and B_j[q][b] = dV_q g_q[b,j]:
+//   Ke_ij = E (λ P_ij + μ P_ji + μ δ_ij (P_00 + P_11 + P_22)),   P_ji = P_ijᵀ.  That is 6·27³ multiply-adds per cell instead of the 81·81·162 of a
+// dense BᵀDB.  FP32 products run on MFMA (v_mfma_f32_16x16x4f32, 27 padded to 32/28);
+// the FP64 path uses register-blocked VALU FMAs.
 #include "fem_internal.h"
 
 namespace fem {
 
+template <int DIM, int NG, int NB, int NQ>
+struct TabL {
+  static constexpr int oN = 0;
+  static constexpr int odN = oN + NQ * NB;
+  static constexpr int odM = odN + NQ * NB * DIM;
+  static constexpr int ow = odM + NQ * NG * DIM;
+  static constexpr int size = ow + NQ;
+};
+
+template <typename T, int DIM>
+__device__ __forceinline__ T det_inv_e(const T (&J)[DIM][DIM], T (&Ji)[DIM][DIM]) {
+  if constexpr (DIM == 2) {
+    const T d = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const T id = T(1) / d;
+    Ji[0][0] = J[1][1] * id;  Ji[0][1] = -J[0][1] * id;
+    Ji[1][0] = -J[1][0] * id; Ji[1][1] = J[0][0] * id;
+    return d;
+  } else {
+    const T a00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const T a01 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    const T a02 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    const T d = J[0][0] * a00 + J[1][0] * a01 + J[2][0] * a02;
+    const T id = T(1) / d;
+    Ji[0][0] = a00 * id; Ji[0][1] = a01 * id; Ji[0][2] = a02 * id;
+    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    return d;
+  }
+}
+
+constexpr __host__ __device__ int tri_e(int lo, int hi, int nb) { return lo * (2 * nb - lo + 1) / 2 + (hi - lo); }
+
+// ---------------------------------------------------------------------------
+// Small vector elements (Q1 quad^2: 8 dofs): one lane per cell, Ke in registers.
+// ---------------------------------------------------------------------------
+template <typename T, int DIM, int NG, int NQ, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_elast_small(
+    int64_t s0, int64_t s1, const double* __restrict__ X0, const double* __restrict__ X1,
+    const double* __restrict__ X2, const int32_t* __restrict__ cnodes, const int32_t* __restrict__ cdofs,
+    const int32_t* __restrict__ cnz, const int32_t* __restrict__ ccell, const double* __restrict__ cparam,
+    const T* __restrict__ tab, T lam, T mu, double* __restrict__ nzval, DevError* __restrict__ err) {
+  constexpr int NB = NG * DIM;  // dofs (function ip == geometry ip, vectorised)
+  constexpr int TRI = NB * (NB + 1) / 2;
+  using L = TabL<DIM, NG, NG, NQ>;
+  __shared__ T s_ke[BLOCK * TRI];
+  const int64_t sb = s0 + (int64_t)blockIdx.x * BLOCK;
+  const int ncell = (int)min<int64_t>(BLOCK, s1 - sb);
+  const int lc = threadIdx.x;
+  if (lc < ncell) {
+    const int64_t s = sb + lc;
+    T x[NG][DIM];
+#pragma unroll
+    for (int a = 0; a < NG; ++a) {
+      const int32_t nd = cnodes[s * NG + a];
+      x[a][0] = (T)X0[nd];
+      x[a][1] = (T)X1[nd];
+      if constexpr (DIM == 3) x[a][2] = (T)X2[nd];
+    }
+    const T E = (T)cparam[s];
+    T ke[TRI];
+#pragma unroll
+    for (int i = 0; i < TRI; ++i) ke[i] = T(0);
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      T J[DIM][DIM];
+#pragma unroll
+      for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) J[i][j] = T(0);
+#pragma unroll
+      for (int a = 0; a < NG; ++a)
+#pragma unroll
+        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+          for (int j = 0; j < DIM; ++j) J[i][j] += x[a][i] * tab[L::odM + (q * NG + a) * DIM + j];
+      T Ji[DIM][DIM];
+      const T detJ = det_inv_e<T, DIM>(J, Ji);
+      bad |= !(detJ > T(0));
+      const T dV = detJ * tab[L::ow + q] * E;
+      T g[NG][DIM];
+#pragma unroll
+      for (int b = 0; b < NG; ++b)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) {
+          T acc = T(0);
+#pragma unroll
+          for (int i = 0; i < DIM; ++i) acc += tab[L::odN + (q * NG + b) * DIM + i] * Ji[i][j];
+          g[b][j] = acc;
+        }
+#pragma unroll
+      for (int I = 0; I < NB; ++I) {
+#pragma unroll
+        for (int J2 = I; J2 < NB; ++J2) {
+          const int a = I / DIM, i = I % DIM, b = J2 / DIM, j = J2 % DIM;
+          T gg = g[a][0] * g[b][0];
+#pragma unroll
+          for (int k = 1; k < DIM; ++k) gg += g[a][k] * g[b][k];
+          T v = lam * g[a][i] * g[b][j] + mu * g[a][j] * g[b][i];
+          if (i == j) v += mu * gg;
+          ke[tri_e(I, J2, NB)] += v * dV;
+        }
+      }
+    }
+    if (bad) atomicMin(&err->detj_cell, ccell[s]);
+#pragma unroll
+    for (int i = 0; i < TRI; ++i) s_ke[lc * TRI + i] = ke[i];
+  }
+  __syncthreads();
+  constexpr int E2 = NB * NB;
+  constexpr int CH = 16;
+  const int32_t* __restrict__ nzt = cnz + sb * E2;
+  const int tot = ncell * E2;
+  for (int base = 0; base < tot; base += BLOCK * CH) {
+    int32_t pp[CH];
+    double old[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int t = base + k * BLOCK + threadIdx.x;
+      pp[k] = t < tot ? nzt[t] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) old[k] = pp[k] >= 0 ? nzval[pp[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int t = base + k * BLOCK + threadIdx.x;
+      if (t < tot) {
+        const int lcell = t / E2;
+        const int e = t - lcell * E2;
+        const int b = e / NB, a = e - b * NB;
+        const T v = s_ke[lcell * TRI + tri_e(a < b ? a : b, a < b ? b : a, NB)];
+        if (pp[k] >= 0) {
+          if (v != T(0)) nzval[pp[k]] = old[k] + (double)v;
+        } else if (v != T(0)) {
+          atomicMin(&err->pattern_cell, ccell[sb + lcell]);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Q2 hex^3: one wavefront (64 lanes) per cell.
+// ---------------------------------------------------------------------------
+constexpr int kQN = 27;   // nodes / scalar basis functions
+constexpr int kQQ = 27;   // quadrature points
+constexpr int kQD = 81;   // dofs
+constexpr int kKP = 28;   // padded K (quadrature) dimension, multiple of 4
+constexpr int kMP = 32;   // padded M/N dimension
+constexpr int kPS = 28;   // row stride of the stored products
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Q2Smem {
+  // A_i[a][q] and B_j[q][b]; the products P_ij alias them in the FP32 (MFMA) path, whose
+  // accumulators stay in registers until every product is done, and follow them in the
+  // FP64 path, which writes each product as soon as it is formed.
+  static constexpr int nA = 3 * kMP * kKP;
+  static constexpr int nB = 3 * kKP * kMP;
+  static constexpr int nP = 6 * kQN * kPS;
+  static constexpr int oP = sizeof(T) == 4 ? 0 : nA + nB;
+  static constexpr int n = sizeof(T) == 4 ? ((nA + nB) > nP ? (nA + nB) : nP) : nA + nB + nP;
+};
+
+__device__ __forceinline__ int pidx(int i, int j) {  // (i <= j) -> 0..5
+  return i == 0 ? j : (i == 1 ? 2 + j : 5);
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_elast_q2hex(
+    int64_t s0, const double* __restrict__ X0, const double* __restrict__ X1, const double* __restrict__ X2,
+    const int32_t* __restrict__ cnodes, const int32_t* __restrict__ cnz, const int32_t* __restrict__ ccell,
+    const double* __restrict__ cparam, const T* __restrict__ tab, T lam, T mu, double* __restrict__ nzval,
+    DevError* __restrict__ err) {
+  using L = TabL<3, 8, kQN, kQQ>;
+  __shared__ T sm[Q2Smem<T>::n];
+  T* sA = sm;                        // [3][kMP][kKP]
+  T* sB = sm + Q2Smem<T>::nA;        // [3][kKP][kMP]
+  const int lane = threadIdx.x;
+  const int64_t s = s0 + blockIdx.x;
+
+  for (int t = lane; t < Q2Smem<T>::nA + Q2Smem<T>::nB; t += 64) sm[t] = T(0);
+  T x[8][3];
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int32_t nd = cnodes[s * 8 + a];
+    x[a][0] = (T)X0[nd];
+    x[a][1] = (T)X1[nd];
+    x[a][2] = (T)X2[nd];
+  }
+  __syncthreads();
+  if (lane < kQQ) {
+    const int q = lane;
+    T J[3][3] = {{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) J[i][j] += x[a][i] * tab[L::odM + (q * 8 + a) * 3 + j];
+    T Ji[3][3];
+    const T detJ = det_inv_e<T, 3>(J, Ji);
+    if (!(detJ > T(0))) atomicMin(&err->detj_cell, ccell[s]);
+    const T dV = detJ * tab[L::ow + q];
+    for (int a = 0; a < kQN; ++a) {
+      const T* dn = tab + L::odN + (q * kQN + a) * 3;
+      const T d0 = dn[0], d1 = dn[1], d2 = dn[2];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const T gi = d0 * Ji[0][i] + d1 * Ji[1][i] + d2 * Ji[2][i];
+        sA[(i * kMP + a) * kKP + q] = gi;
+        sB[(i * kKP + q) * kMP + a] = dV * gi;
+      }
+    }
+  }
+  __syncthreads();
+
+  // P_ij = A_i B_j for the six (i <= j) pairs; results go to registers, then to LDS.
+  if constexpr (sizeof(T) == 4) {
+    floatx4 acc[6][2][2];
+    const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int pi = 0; pi < 6; ++pi) {
+      const int i = pi < 3 ? 0 : (pi < 5 ? 1 : 2);
+      const int j = pi < 3 ? pi : (pi < 5 ? pi - 2 : 2);
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+          floatx4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kk = 0; kk < kKP / 4; ++kk) {
+            const float av = sA[(i * kMP + 16 * ti + r) * kKP + 4 * kk + kq];
+            const float bv = sB[(j * kKP + 4 * kk + kq) * kMP + 16 * tj + r];
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, c, 0, 0, 0);
+          }
+          acc[pi][ti][tj] = c;
+        }
+    }
+    __syncthreads();
+    T* sP = sm;
+#pragma unroll
+    for (int pi = 0; pi < 6; ++pi)
+#pragma unroll
+      for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = 16 * ti + 4 * kq + e, col = 16 * tj + r;
+            if (row < kQN && col < kQN) sP[(pi * kQN + row) * kPS + col] = acc[pi][ti][tj][e];
+          }
+  } else {
+    // FP64: lane owns a 4x4 output tile of the padded 32x32 product (8x8 lanes).
+    const int tr = (lane >> 3) * 4, tc = (lane & 7) * 4;
+    T* sP = sm + Q2Smem<T>::oP;
+    for (int pi = 0; pi < 6; ++pi) {
+      const int i = pi < 3 ? 0 : (pi < 5 ? 1 : 2);
+      const int j = pi < 3 ? pi : (pi < 5 ? pi - 2 : 2);
+      T acc[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = T(0);
+      for (int q = 0; q < kQQ; ++q) {
+        T av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) av[u] = sA[(i * kMP + tr + u) * kKP + q];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) bv[v] = sB[(j * kKP + q) * kMP + tc + v];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = tr + u, col = tc + v;
+          if (row < kQN && col < kQN) sP[(pi * kQN + row) * kPS + col] = acc[u][v];
+        }
+    }
+  }
+  __syncthreads();
+
+  // Ke entries in CSC order (entry e = J*81 + I: column J, row I) and the scatter.
+  const T* sP = sm + Q2Smem<T>::oP;
+  const T E = (T)cparam[s];
+  const T lE = lam * E, mE = mu * E;
+  const int32_t* __restrict__ nzt = cnz + s * (kQD * kQD);
+  constexpr int CH = 8;
+  constexpr int TOT = kQD * kQD;
+  for (int base = 0; base < TOT; base += 64 * CH) {
+    int32_t pp[CH];
+    double old[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int t = base + k * 64 + lane;
+      pp[k] = t < TOT ? nzt[t] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < CH; ++k) old[k] = pp[k] >= 0 ? nzval[pp[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int t = base + k * 64 + lane;
+      if (t >= TOT) continue;
+      const int Jc = t / kQD, Ir = t - Jc * kQD;
+      const int a = Ir / 3, i = Ir - 3 * a, b = Jc / 3, j = Jc - 3 * b;
+      const T pij = i <= j ? sP[(pidx(i, j) * kQN + a) * kPS + b] : sP[(pidx(j, i) * kQN + b) * kPS + a];
+      const T pji = j <= i ? sP[(pidx(j, i) * kQN + a) * kPS + b] : sP[(pidx(i, j) * kQN + b) * kPS + a];
+      T v = lE * pij + mE * pji;
+      if (i == j) v += mE * (sP[(0 * kQN + a) * kPS + b] + sP[(3 * kQN + a) * kPS + b] + sP[(5 * kQN + a) * kPS + b]);
+      if (pp[k] >= 0) {
+        if (v != T(0)) nzval[pp[k]] = old[k] + (double)v;
+      } else if (v != T(0)) {
+        atomicMin(&err->pattern_cell, ccell[s]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <typename T, int DIM, int NG, int NB, int NQ>
+static void pack_tab(const RefTables& r, std::vector<T>& out) {
+  using L = TabL<DIM, NG, NB, NQ>;
+  out.assign(L::size, T(0));
+  for (int q = 0; q < NQ; ++q) {
+    out[L::ow + q] = (T)r.w[q];
+    for (int b = 0; b < NB; ++b) {
+      out[L::oN + q * NB + b] = (T)r.N[q][b];
+      for (int d = 0; d < DIM; ++d) out[L::odN + (q * NB + b) * DIM + d] = (T)r.dN[q][b][d];
+    }
+    for (int a = 0; a < NG; ++a)
+      for (int d = 0; d < DIM; ++d) out[L::odM + (q * NG + a) * DIM + d] = (T)r.dM[q][a][d];
+  }
+}
+
+template <int DIM, int NG, int NB, int NQ>
+static int upload_tab(fem_ctx* c) {
+  std::vector<double> td;
+  std::vector<float> tf;
+  pack_tab<double, DIM, NG, NB, NQ>(c->ref, td);
+  pack_tab<float, DIM, NG, NB, NQ>(c->ref, tf);
+  int rc;
+  if ((rc = dev_alloc(c, &c->d_tab, td.size())) || (rc = dev_alloc(c, &c->d_tabf, tf.size()))) return rc;
+  FEM_HIP(c, hipMemcpyAsync(c->d_tab, td.data(), td.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipMemcpyAsync(c->d_tabf, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  c->tab_dirty = true;  // packed for this kernel family; the Poisson path repacks
+  return FEM_OK;
+}
+
 int assemble_elasticity(fem_ctx* c, const fem_assemble_cfg* cfg) {
-  (void)cfg;
-  return set_error(c, FEM_ERR_ARG, "elasticity kernels are not available in this build");
+  const double nu = cfg->nu;
+  if (!(nu > -1.0 && nu < 0.5)) return set_error(c, FEM_ERR_ARG, "Poisson's ratio must be in (-1, 0.5)");
+  const double lam = nu / ((1 + nu) * (1 - 2 * nu)), mu = 1.0 / (2 * (1 + nu));
+  const bool mixed = cfg->precision == FEM_MIXED;
+  if (c->vdim != c->rdim || c->sdim != c->rdim || c->ndpc != c->nbf * c->vdim)
+    return set_error(c, FEM_ERR_ARG, "elasticity needs a vector field ip^dim on a dim-dimensional grid");
+  int rc;
+  FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+  if (c->rdim == 2 && c->ngeo == 4 && c->nbf == 4 && c->nqp == 4 && c->npc == 4) {
+    if ((rc = upload_tab<2, 4, 4, 4>(c))) return rc;
+    FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    for (int col = 0; col < c->ncolors; ++col) {
+      const int64_t s0 = c->color_ptr[col], s1 = c->color_ptr[col + 1];
+      if (s1 <= s0) continue;
+      constexpr int B = 128;
+      const unsigned nb = (unsigned)((s1 - s0 + B - 1) / B);
+      if (mixed)
+        k_elast_small<float, 2, 4, 4, B><<<nb, B, 0, c->stream>>>(s0, s1, c->d_x[0], c->d_x[1], c->d_x[2], c->d_cnodes,
+            c->d_cdofs, c->d_cnz, c->d_ccell, c->d_cparam, c->d_tabf, (float)lam, (float)mu, c->d_nzval, c->d_err);
+      else
+        k_elast_small<double, 2, 4, 4, B><<<nb, B, 0, c->stream>>>(s0, s1, c->d_x[0], c->d_x[1], c->d_x[2], c->d_cnodes,
+            c->d_cdofs, c->d_cnz, c->d_ccell, c->d_cparam, c->d_tab, lam, mu, c->d_nzval, c->d_err);
+      c->timings.assemble_launches++;
+    }
+  } else if (c->rdim == 3 && c->ngeo == 8 && c->nbf == 27 && c->nqp == 27 && c->npc == 8) {
+    if ((rc = upload_tab<3, 8, 27, 27>(c))) return rc;
+    FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+    for (int col = 0; col < c->ncolors; ++col) {
+      const int64_t s0 = c->color_ptr[col], s1 = c->color_ptr[col + 1];
+      if (s1 <= s0) continue;
+      const unsigned nb = (unsigned)(s1 - s0);
+      if (mixed)
+        k_elast_q2hex<float><<<nb, 64, 0, c->stream>>>(s0, c->d_x[0], c->d_x[1], c->d_x[2], c->d_cnodes, c->d_cnz,
+            c->d_ccell, c->d_cparam, c->d_tabf, (float)lam, (float)mu, c->d_nzval, c->d_err);
+      else
+        k_elast_q2hex<double><<<nb, 64, 0, c->stream>>>(s0, c->d_x[0], c->d_x[1], c->d_x[2], c->d_cnodes, c->d_cnz,
+            c->d_ccell, c->d_cparam, c->d_tab, lam, mu, c->d_nzval, c->d_err);
+      c->timings.assemble_launches++;
+    }
+  } else {
+    return set_error(c, FEM_ERR_ARG, "no elasticity kernel for this element/quadrature combination "
+                                     "(supported: Q1 quad^2 with 2x2 Gauss, Q2 hex^3 with 3x3x3 Gauss)");
+  }
+  FEM_HIP(c, hipGetLastError());
+  FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
+  c->timings.assemble_kernel_ms = event_ms(c);
+  return FEM_OK;
 }
 
 __global__ void k_point_load(int64_t n, const int32_t* __restrict__ dofs, const double* __restrict__ v,
@@ -19,9 +438,7 @@ __global__ void k_point_load(int64_t n, const int32_t* __restrict__ dofs, const 
 
 using namespace fem;
 
-extern "C" {
-
-int32_t fem_point_load(fem_ctx* c, int64_t n, const int64_t* dofs, const double* values) {
+extern "C" int32_t fem_point_load(fem_ctx* c, int64_t n, const int64_t* dofs, const double* values) {
   if (!c || n < 0 || (n > 0 && (!dofs || !values)) || c->ndofs == 0)
     return set_error(c, FEM_ERR_ARG, "fem_point_load: invalid arguments");
   if (n == 0) return FEM_OK;
@@ -41,16 +458,6 @@ int32_t fem_point_load(fem_ctx* c, int64_t n, const int64_t* dofs, const double*
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   dev_free(&dd);
   dev_free(&dv);
+  c->nzval32_dirty = true;
   return FEM_OK;
 }
-
-int32_t fem_facet_traction(fem_ctx* c, int64_t, const int64_t*, int32_t, int32_t, const double*, const double*,
-                           const double*, const double*) {
-  return set_error(c, FEM_ERR_STATE, "facet traction is not available in this build");
-}
-
-int32_t fem_adjoint_compliance(fem_ctx* c, const double*, double, double, double, double*, double*, double*) {
-  return set_error(c, FEM_ERR_STATE, "the compliance adjoint is not available in this build");
-}
-
-}  // extern "C"

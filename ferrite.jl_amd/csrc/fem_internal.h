@@ -70,6 +70,7 @@ struct fem_ctx {
   int32_t* d_diag = nullptr;           // position of K[j,j] in column j (-1 if absent)
   double* d_nzval = nullptr;
   float* d_nzval32 = nullptr;          // FP32 copy used by the mixed solver
+  bool nzval32_dirty = true;           // nzval changed since the FP32 copy was made
 
   // colors: cells permuted into color-major order on the device
   int ncolors = 0;
@@ -111,6 +112,7 @@ struct fem_ctx {
   fem::SolverWork* solver = nullptr;
 
   fem_timings timings{};
+  fem_tuning tune{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 

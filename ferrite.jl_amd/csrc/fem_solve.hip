@@ -38,6 +38,8 @@ struct Scal {
 struct SpmvShape {
   int R = 0;          // rows per block
   int blocks = 0;
+  int cap = 2048;     // LDS products per chunk
+  int L = 1;          // lanes per row in the reduction phase
 };
 
 struct GraphCache {
@@ -55,6 +57,9 @@ struct SolverWork {
   SpmvShape sh64, sh32;
   int vblocks = 0;
   GraphCache g64, g32;
+  uint16_t* roff32 = nullptr;  // FP32 operator: 16-bit column offsets (nullptr: use rowval)
+  int32_t* bbase32 = nullptr;
+  bool comp_checked = false;
 };
 
 static void destroy_graph(GraphCache& g) {
@@ -70,6 +75,7 @@ void free_solver(fem_ctx* c) {
   dev_free(&w->r); dev_free(&w->p); dev_free(&w->q); dev_free(&w->invd); dev_free(&w->res);
   dev_free(&w->r32); dev_free(&w->p32); dev_free(&w->q32); dev_free(&w->x32); dev_free(&w->invd32);
   dev_free(&w->part_s); dev_free(&w->part_v); dev_free(&w->s);
+  dev_free(&w->roff32); dev_free(&w->bbase32);
   delete w;
   c->solver = nullptr;
 }
@@ -101,14 +107,59 @@ __device__ __forceinline__ double sum_parts(const double* __restrict__ part, int
   return block_sum(v, sh);
 }
 
+// Wide variant for the scalar kernels (kWide threads, 8 loads in flight per thread).
+constexpr int kWide = 1024;
+__device__ __forceinline__ double wide_sum(const double* __restrict__ part, int m, int stride, double* sh) {
+  double v = 0.0;
+  for (int i0 = 0; i0 < m; i0 += 8 * kWide) {
+    double t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = i0 + k * kWide + threadIdx.x;
+      t[k] = i < m ? part[(int64_t)i * stride] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v += t[k];
+  }
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kWide / 64; ++i) r += sh[i];
+  return r;
+}
+
 // --------------------------------------------------------------------------
 // CSR-stream SpMV
 //   MODE 0: y = K x, partial Σ x·y   (CG: q = K p, p·q)
 //   MODE 1: y = f − K x, partial Σ y² (FP64 residual)
+// Column indices are either the pattern's int32 rowval (COMP = false) or, for the
+// FP32 operator, 16-bit offsets from a per-block base row (COMP = true), which
+// cuts the operator stream from 8 to 6 bytes per nonzero.  In the reduction
+// phase L lanes share a row: lane k sums the row's products k, k+L, ... in order
+// and the L partial sums are combined by a fixed butterfly (deterministic).
 // --------------------------------------------------------------------------
-template <typename V, typename X, int MODE, int CAP>
+// Loads N consecutive elements with 16-byte vector loads (the address is N-element aligned).
+template <typename T, int N>
+__device__ __forceinline__ void vec_load(const T* __restrict__ p, T (&out)[N]) {
+  constexpr int VB = 16 / sizeof(T);       // elements per 16-byte load
+  static_assert(N % VB == 0, "vector width");
+#pragma unroll
+  for (int k = 0; k < N / VB; ++k) {
+    const uint4 u = *reinterpret_cast<const uint4*>(p + k * VB);
+    const T* e = reinterpret_cast<const T*>(&u);
+#pragma unroll
+    for (int j = 0; j < VB; ++j) out[k * VB + j] = e[j];
+  }
+}
+
+template <typename V, typename X, int MODE, int CAP, int L, bool COMP>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, int R, const int32_t* __restrict__ colptr,
-                                                 const int32_t* __restrict__ rowval, const V* __restrict__ val,
+                                                 const int32_t* __restrict__ rowval,
+                                                 const uint16_t* __restrict__ roff,
+                                                 const int32_t* __restrict__ bbase, const V* __restrict__ val,
                                                  const X* __restrict__ x, const double* __restrict__ f,
                                                  X* __restrict__ y, double* __restrict__ part,
                                                  const Scal* __restrict__ s) {
@@ -121,39 +172,57 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, int R, const int32_t
   const int64_t r0 = (int64_t)blockIdx.x * R;
   const int nr = (int)min<int64_t>(R, n - r0);
   for (int t = tid; t <= nr; t += kBlock) sptr[t] = colptr[r0 + t];
+  int32_t base = 0;
+  if constexpr (COMP) base = bbase[blockIdx.x];
   __syncthreads();
   const int32_t e0 = sptr[0], e1 = sptr[nr];
+  const int row = tid / L, sub = tid - (tid / L) * L;
   A acc = A(0);
-  for (int32_t c0 = e0; c0 < e1; c0 += CAP) {
-    const int cn = min(CAP, e1 - c0);
-    constexpr int PER = CAP / kBlock;
+  // Entries are streamed in windows of CAP starting at an 8-aligned index; thread t owns the
+  // PER = CAP/kBlock consecutive entries [w0 + PER t, w0 + PER t + PER), loaded as whole vectors.
+  constexpr int PER = CAP / kBlock;
+  static_assert(PER % 8 == 0, "window per thread");
+  for (int32_t w0 = e0 & ~7; w0 < e1; w0 += CAP) {
+    const int32_t t0 = w0 + PER * tid;
     V v[PER];
     int32_t ri[PER];
+    if (t0 + PER <= e1 && t0 >= e0) {
+      vec_load<V, PER>(val + t0, v);
+      if constexpr (COMP) {
+        uint16_t o[PER];
+        vec_load<uint16_t, PER>(roff + t0, o);
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int idx = k * kBlock + tid;
-      const bool ok = idx < cn;
-      v[k] = ok ? val[c0 + idx] : V(0);
-      ri[k] = ok ? rowval[c0 + idx] : 0;
+        for (int k = 0; k < PER; ++k) ri[k] = base + (int32_t)o[k];
+      } else {
+        vec_load<int32_t, PER>(rowval + t0, ri);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int32_t e = t0 + k;
+        const bool ok = e >= e0 && e < e1;
+        v[k] = ok ? val[e] : V(0);
+        if constexpr (COMP) ri[k] = ok ? base + (int32_t)roff[e] : 0;
+        else ri[k] = ok ? rowval[e] : 0;
+      }
     }
     X xv[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) xv[k] = x[ri[k]];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int idx = k * kBlock + tid;
-      if (idx < cn) prod[idx] = (A)v[k] * (A)xv[k];
-    }
+    for (int k = 0; k < PER; ++k) prod[PER * tid + k] = (A)v[k] * (A)xv[k];
     __syncthreads();
-    if (tid < nr) {
-      const int32_t a = max(sptr[tid], c0), b = min(sptr[tid + 1], c0 + cn);
-      for (int32_t e = a; e < b; ++e) acc += prod[e - c0];
+    if (row < nr) {
+      const int32_t a = max(sptr[row], w0), b = min(sptr[row + 1], w0 + CAP);
+      for (int32_t e = a + sub; e < b; e += L) acc += prod[e - w0];
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, L);
   double d = 0.0;
-  if (tid < nr) {
-    const int64_t j = r0 + tid;
+  if (row < nr && sub == 0) {
+    const int64_t j = r0 + row;
     if (MODE == 0) {
       y[j] = (X)acc;
       d = (double)x[j] * (double)(X)acc;
@@ -167,11 +236,41 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, int R, const int32_t
   if (tid == 0) part[blockIdx.x] = b;
 }
 
+// Per row block of the FP32 operator: base = smallest column index, offsets = index − base.
+// span_out receives the largest (max − min) over all blocks (the host checks it < 65536).
+__global__ __launch_bounds__(kBlock) void k_compress_rows(int64_t n, int R, const int32_t* __restrict__ colptr,
+                                                          const int32_t* __restrict__ rowval,
+                                                          uint16_t* __restrict__ roff, int32_t* __restrict__ bbase,
+                                                          int* __restrict__ span_out) {
+  __shared__ int32_t smin[kBlock / 64], smax[kBlock / 64];
+  const int64_t r0 = (int64_t)blockIdx.x * R;
+  const int nr = (int)min<int64_t>(R, n - r0);
+  const int32_t e0 = colptr[r0], e1 = colptr[r0 + nr];
+  int32_t mn = INT32_MAX, mx = INT32_MIN;
+  for (int32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+    mn = min(mn, rowval[e]);
+    mx = max(mx, rowval[e]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = min(mn, __shfl_down(mn, o, 64));
+    mx = max(mx, __shfl_down(mx, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) { smin[threadIdx.x >> 6] = mn; smax[threadIdx.x >> 6] = mx; }
+  __syncthreads();
+  mn = smin[0]; mx = smax[0];
+  for (int i = 1; i < kBlock / 64; ++i) { mn = min(mn, smin[i]); mx = max(mx, smax[i]); }
+  if (threadIdx.x == 0) {
+    bbase[blockIdx.x] = mn;
+    atomicMax(span_out, mx - mn);
+  }
+  for (int32_t e = e0 + threadIdx.x; e < e1; e += kBlock) roff[e] = (uint16_t)min(rowval[e] - mn, 65535);
+}
+
 // α = rz / (p·q)
-__global__ __launch_bounds__(kBlock) void k_alpha(int m, const double* __restrict__ part, Scal* __restrict__ s) {
+__global__ __launch_bounds__(kWide) void k_alpha(int m, const double* __restrict__ part, Scal* __restrict__ s) {
   if (s->done) return;
-  __shared__ double sh[kBlock / 64];
-  const double pq = sum_parts(part, m, 1, sh);
+  __shared__ double sh[kWide / 64];
+  const double pq = wide_sum(part, m, 1, sh);
   if (threadIdx.x == 0) {
     s->pq = pq;
     s->alpha = s->rz / pq;
@@ -220,12 +319,12 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t n, X* __restrict__ x,
 }
 
 // β = rz_new / rz ; convergence flag
-__global__ __launch_bounds__(kBlock) void k_beta(int m, const double* __restrict__ part, Scal* __restrict__ s,
-                                                 double tol2) {
+__global__ __launch_bounds__(kWide) void k_beta(int m, const double* __restrict__ part, Scal* __restrict__ s,
+                                                double tol2) {
   if (s->done) return;
-  __shared__ double sh[kBlock / 64];
-  const double rzn = sum_parts(part, m, 2, sh);
-  const double rrn = sum_parts(part + 1, m, 2, sh);
+  __shared__ double sh[kWide / 64];
+  const double rzn = wide_sum(part, m, 2, sh);
+  const double rrn = wide_sum(part + 1, m, 2, sh);
   if (threadIdx.x == 0) {
     s->beta = rzn / s->rz;
     s->rz = rzn;
@@ -290,10 +389,10 @@ __global__ __launch_bounds__(kBlock) void k_init_scal(int m, const double* __res
 }
 
 // s->rr (which = 0) or s->bnorm2 (which = 1) = Σ part, fixed order
-__global__ __launch_bounds__(kBlock) void k_sum_to(int m, const double* __restrict__ part, Scal* __restrict__ s,
-                                                   int which) {
-  __shared__ double sh[kBlock / 64];
-  const double t = sum_parts(part, m, 1, sh);
+__global__ __launch_bounds__(kWide) void k_sum_to(int m, const double* __restrict__ part, Scal* __restrict__ s,
+                                                  int which) {
+  __shared__ double sh[kWide / 64];
+  const double t = wide_sum(part, m, 1, sh);
   if (threadIdx.x == 0) {
     if (which == 0) s->rr = t; else s->bnorm2 = t;
   }
@@ -381,23 +480,39 @@ __global__ void k_zero(int64_t n, double* __restrict__ a) {
 // --------------------------------------------------------------------------
 // host driver
 // --------------------------------------------------------------------------
-constexpr int kCap64 = 2048;   // LDS products per chunk (16 KB FP64)
-constexpr int kCap32 = 4096;   // (16 KB FP32)
-
-static SpmvShape spmv_shape(const fem_ctx* c, int cap) {
+static SpmvShape spmv_shape(const fem_ctx* c, int cap, int lanes) {
   SpmvShape s;
   const int longest = std::max(1, c->max_row_nnz);
-  s.R = std::max(1, std::min(kBlock, cap / longest));
+  s.cap = cap;
+  s.L = lanes;
+  s.R = std::max(1, std::min(kBlock / lanes, cap / longest));
   s.blocks = (int)((c->n + s.R - 1) / s.R);
   return s;
 }
 
+template <typename V, typename X, int MODE, int CAP, int L>
+static void launch_spmv_t(fem_ctx* c, const SpmvShape& sh, const V* val, const X* x, const double* f, X* y,
+                          double* part, const Scal* s, const uint16_t* roff, const int32_t* bbase) {
+  if (roff)
+    k_spmv<V, X, MODE, CAP, L, true><<<sh.blocks, kBlock, 0, c->stream>>>(c->n, sh.R, c->d_colptr, nullptr, roff,
+                                                                          bbase, val, x, f, y, part, s);
+  else
+    k_spmv<V, X, MODE, CAP, L, false><<<sh.blocks, kBlock, 0, c->stream>>>(c->n, sh.R, c->d_colptr, c->d_rowval,
+                                                                           nullptr, nullptr, val, x, f, y, part, s);
+}
+
 template <typename V, typename X, int MODE>
 static void launch_spmv(fem_ctx* c, const SpmvShape& sh, const V* val, const X* x, const double* f, X* y,
-                        double* part, const Scal* s) {
-  constexpr int CAP = sizeof(X) == 8 ? kCap64 : kCap32;
-  k_spmv<V, X, MODE, CAP><<<sh.blocks, kBlock, 0, c->stream>>>(c->n, sh.R, c->d_colptr, c->d_rowval, val, x, f, y,
-                                                                 part, s);
+                        double* part, const Scal* s, const uint16_t* roff = nullptr, const int32_t* bbase = nullptr) {
+#define FEM_SPMV_CASE(CAPV, LV)                                                               \
+  if (sh.cap == CAPV && sh.L == LV) {                                                        \
+    launch_spmv_t<V, X, MODE, CAPV, LV>(c, sh, val, x, f, y, part, s, roff, bbase);          \
+    return;                                                                                  \
+  }
+  FEM_SPMV_CASE(2048, 1) FEM_SPMV_CASE(2048, 2) FEM_SPMV_CASE(2048, 4)
+  FEM_SPMV_CASE(4096, 1) FEM_SPMV_CASE(4096, 2) FEM_SPMV_CASE(4096, 4)
+#undef FEM_SPMV_CASE
+  launch_spmv_t<V, X, MODE, 2048, 1>(c, sh, val, x, f, y, part, s, roff, bbase);
 }
 
 static int ensure_work(fem_ctx* c, bool mixed) {
@@ -408,8 +523,11 @@ static int ensure_work(fem_ctx* c, bool mixed) {
   if (w->n != n || !w->r) {
     destroy_graph(w->g64);
     destroy_graph(w->g32);
-    w->sh64 = spmv_shape(c, kCap64);
-    w->sh32 = spmv_shape(c, kCap32);
+    w->sh64 = spmv_shape(c, c->tune.spmv_cap64 ? c->tune.spmv_cap64 : 2048, c->tune.spmv_lanes64 ? c->tune.spmv_lanes64 : 2);
+    w->sh32 = spmv_shape(c, c->tune.spmv_cap32 ? c->tune.spmv_cap32 : 2048, c->tune.spmv_lanes32 ? c->tune.spmv_lanes32 : 2);
+    dev_free(&w->roff32);
+    dev_free(&w->bbase32);
+    w->comp_checked = false;
     w->vblocks = (int)std::min<int64_t>(kMaxVecBlocks, (n + kTile - 1) / kTile);
     const int ps = std::max(w->sh64.blocks, w->sh32.blocks);
     if ((rc = dev_alloc(c, &w->r, n)) || (rc = dev_alloc(c, &w->p, n)) || (rc = dev_alloc(c, &w->q, n)) ||
@@ -428,6 +546,39 @@ static int ensure_work(fem_ctx* c, bool mixed) {
   return FEM_OK;
 }
 
+// FP32 operator of the mixed solver: values (nzval32) and, when every row block spans fewer
+// than 65536 columns, 16-bit block-relative column offsets.
+static int ensure_operator32(fem_ctx* c, SolverWork* w) {
+  int rc;
+  if (!c->d_nzval32) {
+    if ((rc = dev_alloc(c, &c->d_nzval32, c->nnz))) return rc;
+    c->nzval32_dirty = true;
+  }
+  if (c->nzval32_dirty) {
+    k_to_f32<<<8192, kBlock, 0, c->stream>>>(c->nnz, c->d_nzval, c->d_nzval32);
+    c->nzval32_dirty = false;
+  }
+  if (w->comp_checked) return FEM_OK;
+  w->comp_checked = true;
+  if (c->tune.spmv_compress32 < 0) return FEM_OK;
+  int* dspan = nullptr;
+  if ((rc = dev_alloc(c, &w->roff32, c->nnz)) || (rc = dev_alloc(c, &w->bbase32, w->sh32.blocks)) ||
+      (rc = dev_alloc(c, &dspan, 1)))
+    return rc;
+  FEM_HIP(c, hipMemsetAsync(dspan, 0, sizeof(int), c->stream));
+  k_compress_rows<<<w->sh32.blocks, kBlock, 0, c->stream>>>(c->n, w->sh32.R, c->d_colptr, c->d_rowval, w->roff32,
+                                                            w->bbase32, dspan);
+  int span = 0;
+  FEM_HIP(c, hipMemcpyAsync(&span, dspan, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  dev_free(&dspan);
+  if (span >= 65536) {
+    dev_free(&w->roff32);
+    dev_free(&w->bbase32);
+  }
+  return FEM_OK;
+}
+
 static int read_scal(fem_ctx* c, SolverWork* w, Scal* h) {
   FEM_HIP(c, hipMemcpyAsync(h, w->s, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
   FEM_HIP(c, hipStreamSynchronize(c->stream));
@@ -438,10 +589,11 @@ static int read_scal(fem_ctx* c, SolverWork* w, Scal* h) {
 template <typename V, typename X>
 static void enqueue_iteration(fem_ctx* c, SolverWork* w, const SpmvShape& sh, const V* val, X* x, X* r, X* p, X* q,
                               const X* invd, double tol2) {
-  launch_spmv<V, X, 0>(c, sh, val, p, nullptr, q, w->part_s, w->s);
-  k_alpha<<<1, kBlock, 0, c->stream>>>(sh.blocks, w->part_s, w->s);
+  if constexpr (sizeof(X) == 4) launch_spmv<V, X, 0>(c, sh, val, p, nullptr, q, w->part_s, w->s, w->roff32, w->bbase32);
+  else launch_spmv<V, X, 0>(c, sh, val, p, nullptr, q, w->part_s, w->s);
+  k_alpha<<<1, kWide, 0, c->stream>>>(sh.blocks, w->part_s, w->s);
   k_update<X><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, x, r, p, q, invd, w->part_v, w->s);
-  k_beta<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2);
+  k_beta<<<1, kWide, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2);
   k_pupd<X><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, p, q, w->s);
 }
 
@@ -495,7 +647,7 @@ static int pcg(fem_ctx* c, SolverWork* w, const V* val, X* x, X* r, X* p, X* q, 
 
 static void residual(fem_ctx* c, SolverWork* w) {
   launch_spmv<double, double, 1>(c, w->sh64, c->d_nzval, c->d_u, c->d_f, w->res, w->part_s, w->s);
-  k_sum_to<<<1, kBlock, 0, c->stream>>>(w->sh64.blocks, w->part_s, w->s, 0);
+  k_sum_to<<<1, kWide, 0, c->stream>>>(w->sh64.blocks, w->part_s, w->s, 0);
 }
 
 }  // namespace fem
@@ -516,7 +668,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   double cg_ms = 0.0;
 
   k_sumsq<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->part_s);
-  k_sum_to<<<1, kBlock, 0, c->stream>>>(kMaxVecBlocks, w->part_s, w->s, 1);
+  k_sum_to<<<1, kWide, 0, c->stream>>>(kMaxVecBlocks, w->part_s, w->s, 1);
   Scal h{};
   if ((rc = read_scal(c, w, &h))) return rc;
   const double bnorm2 = h.bnorm2;
@@ -540,10 +692,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
     out.outer_iterations = 1;
     if (rc && rc != FEM_ERR_NOCONV) return rc;
   } else {
-    if (!c->d_nzval32) {
-      if ((rc = dev_alloc(c, &c->d_nzval32, c->nnz))) return rc;
-      k_to_f32<<<8192, kBlock, 0, c->stream>>>(c->nnz, c->d_nzval, c->d_nzval32);
-    }
+    if ((rc = ensure_operator32(c, w))) return rc;
     k_invdiag<float><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd32);
     // Mixed CG with reliable updates: one FP32 Krylov sequence (FP32 SpMV and vectors,
     // FP64 dot products); whenever its recursive residual has dropped by δ since the last
@@ -612,10 +761,7 @@ extern "C" int32_t fem_spmv_bench(fem_ctx* c, int32_t mixed, int32_t reps, doubl
   Scal zero{};
   FEM_HIP(c, hipMemcpyAsync(w->s, &zero, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
   if (mixed) {
-    if (!c->d_nzval32) {
-      if ((rc = dev_alloc(c, &c->d_nzval32, c->nnz))) return rc;
-      k_to_f32<<<8192, kBlock, 0, c->stream>>>(c->nnz, c->d_nzval, c->d_nzval32);
-    }
+    if ((rc = ensure_operator32(c, w))) return rc;
     k_to_f32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p32);
   } else {
     k_copy<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p);
@@ -623,11 +769,31 @@ extern "C" int32_t fem_spmv_bench(fem_ctx* c, int32_t mixed, int32_t reps, doubl
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < reps; ++i) {
-    if (mixed) launch_spmv<float, float, 0>(c, w->sh32, c->d_nzval32, w->p32, nullptr, w->q32, w->part_s, w->s);
+    if (mixed) launch_spmv<float, float, 0>(c, w->sh32, c->d_nzval32, w->p32, nullptr, w->q32, w->part_s, w->s,
+                                            w->roff32, w->bbase32);
     else launch_spmv<double, double, 0>(c, w->sh64, c->d_nzval, w->p, nullptr, w->q, w->part_s, w->s);
   }
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
   *avg_ms = event_ms(c) / reps;
+  return FEM_OK;
+}
+
+extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
+  if (!c || !t) return FEM_ERR_ARG;
+  auto ok_cap = [](int v) { return v == 0 || v == 2048 || v == 4096; };
+  auto ok_l = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4; };
+  if (!ok_cap(t->spmv_cap64) || !ok_cap(t->spmv_cap32) || !ok_l(t->spmv_lanes64) || !ok_l(t->spmv_lanes32) ||
+      !(t->asm_block == 0 || t->asm_block == 64 || t->asm_block == 128 || t->asm_block == 256))
+    return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
+  (void)hipStreamSynchronize(c->stream);
+  c->tune = *t;
+  free_solver(c);
+  return FEM_OK;
+}
+
+extern "C" int32_t fem_get_tuning(const fem_ctx* c, fem_tuning* t) {
+  if (!c || !t) return FEM_ERR_ARG;
+  *t = c->tune;
   return FEM_OK;
 }

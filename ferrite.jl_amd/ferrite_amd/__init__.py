@@ -7,9 +7,9 @@ Names follow Ferrite.jl 1.6.0; mutating Julia functions drop the `!`
 from ._lib import (FemArgumentError, FemConvergenceError, FemError, FemPatternError, LIB_PATH,
                    exported_symbols)
 from .constraints import ConstraintHandler, Dirichlet
-from .device import (MI355XCSC, Context, ElasticityElement, PoissonElement, allocate_matrix, allocate_vector,
-                     apply, apply_zero, assemble_global, set_cell_parameters, solve, spmv_bench, start_assemble,
-                     timings)
+from .device import (MI355XCSC, Context, ElasticityElement, PoissonElement, add_point_load, allocate_matrix,
+                     allocate_vector, apply, apply_traction, apply_zero, compliance_gradient, simp_modulus, assemble_global, get_tuning, set_cell_parameters, set_tuning, solve,
+                     spmv_bench, start_assemble, timings)
 from .dofs import DofHandler, add, close, create_coloring, create_sparsity_pattern
 from .mesh import Grid, Hexahedron, Quadrilateral, generate_grid
 from .refel import (FacetQuadratureRule, Lagrange, QuadratureRule, RefHexahedron, RefLine, RefQuadrilateral,
@@ -19,8 +19,9 @@ from .values import CellValues, FacetValues
 __all__ = [
     "FemArgumentError", "FemConvergenceError", "FemError", "FemPatternError", "LIB_PATH", "exported_symbols",
     "ConstraintHandler", "Dirichlet", "MI355XCSC", "Context", "ElasticityElement", "PoissonElement",
-    "allocate_matrix", "allocate_vector", "apply", "apply_zero", "assemble_global", "set_cell_parameters",
-    "solve", "spmv_bench", "start_assemble", "timings", "DofHandler", "add", "close", "create_coloring",
+    "allocate_matrix", "allocate_vector", "apply", "apply_traction", "add_point_load", "compliance_gradient",
+    "simp_modulus", "apply_zero", "assemble_global", "set_cell_parameters",
+    "solve", "spmv_bench", "set_tuning", "get_tuning", "start_assemble", "timings", "DofHandler", "add", "close", "create_coloring",
     "create_sparsity_pattern", "Grid", "Hexahedron", "Quadrilateral", "generate_grid", "FacetQuadratureRule",
     "Lagrange", "QuadratureRule", "RefHexahedron", "RefLine", "RefQuadrilateral", "VectorizedInterpolation",
     "CellValues", "FacetValues",

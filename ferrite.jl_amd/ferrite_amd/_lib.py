@@ -54,6 +54,12 @@ class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("outer_iterations", C.c_int32), ("relres", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("spmv_cap64", C.c_int32), ("spmv_lanes64", C.c_int32), ("spmv_cap32", C.c_int32),
+                ("spmv_lanes32", C.c_int32), ("spmv_compress32", C.c_int32), ("asm_block", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
+
+
 class Timings(C.Structure):
     _fields_ = [("assemble_kernel_ms", C.c_double), ("fill_ms", C.c_double), ("apply_ms", C.c_double),
                 ("spmv_ms", C.c_double), ("spmv_launches", C.c_int32), ("assemble_launches", C.c_int32)]
@@ -87,6 +93,8 @@ SIGNATURES = {
     "fem_device_pointer": (I32, [P, I32, C.POINTER(P), PI64]),
     "fem_get_timings": (I32, [P, C.POINTER(Timings)]),
     "fem_spmv_bench": (I32, [P, I32, I32, PD]),
+    "fem_set_tuning": (I32, [P, C.POINTER(Tuning)]),
+    "fem_get_tuning": (I32, [P, C.POINTER(Tuning)]),
     "fem_adjoint_compliance": (I32, [P, PD, D, D, D, PD, PD, PD]),
     "fem_setup_distribute_dofs": (I32, [I32, I64, I32, PI64, I64, I32, PI32, PI32, PI64, PI32, PI64]),
     "fem_setup_pattern": (I32, [I64, I32, PI64, I64, PI64, PI64, I64, PI64]),

@@ -241,3 +241,54 @@ def spmv_bench(K, mixed=False, reps=20):
     ms = C.c_double(0.0)
     K.ctx.call("fem_spmv_bench", 1 if mixed else 0, int(reps), C.byref(ms))
     return ms.value
+
+
+def set_tuning(K, **kw):
+    """Launch-shape knobs of the context (see fem_tuning in include/ferrite_mi355x.h)."""
+    t = L.Tuning()
+    K.ctx.call("fem_get_tuning", C.byref(t))
+    for k, v in kw.items():
+        setattr(t, k, int(v))
+    K.ctx.call("fem_set_tuning", C.byref(t))
+
+
+def get_tuning(K):
+    t = L.Tuning()
+    K.ctx.call("fem_get_tuning", C.byref(t))
+    return {k: getattr(t, k) for k, _ in L.Tuning._fields_ if k != "reserved"}
+
+
+def apply_traction(K, f, fv, facets, traction):
+    """f[dofs] += ∫ t·N dΓ over `facets` ((cell, facet) pairs, 1-based), the FacetValues
+    traction term of an element routine (topology_optimization.jl:369-381)."""
+    nfq, nft, N, dM, w = fv.device_tables()
+    fc = L.as_i64(np.asarray(facets, dtype=np.int64).reshape(-1, 2))
+    t = L.as_f64(np.resize(np.asarray(traction, dtype=np.float64), 3))
+    K.ctx.call("fem_facet_traction", len(fc), L.ptr(fc, C.c_int64), nfq, nft, L.ptr(N, C.c_double),
+               L.ptr(dM, C.c_double), L.ptr(w, C.c_double), L.ptr(t, C.c_double))
+
+
+def add_point_load(K, f, dofs, values):
+    """f[dofs] += values (nodal forces), 1-based dofs."""
+    d = L.as_i64(np.atleast_1d(dofs))
+    v = L.as_f64(np.atleast_1d(values))
+    K.ctx.call("fem_point_load", len(d), L.ptr(d, C.c_int64), L.ptr(v, C.c_double))
+
+
+def compliance_gradient(K, theta, emin=1e-3, penal=3.0, nu=0.3):
+    """J = fᵀu and its adjoint gradients for E_e = E_min + θ_e^p (1 − E_min):
+    returns (J, dJ/dθ (ncells,), dJ/dX (nnodes, dim)).  Needs the solved system in K."""
+    g = K.dh.grid
+    th = L.as_f64(theta)
+    dth = np.empty(g.ncells)
+    dx = np.empty((g.nnodes, g.sdim))
+    J = C.c_double(0.0)
+    K.ctx.call("fem_adjoint_compliance", L.ptr(th, C.c_double), float(emin), float(penal), float(nu), C.byref(J),
+               L.ptr(dth, C.c_double), L.ptr(dx, C.c_double))
+    return J.value, dth, dx
+
+
+def simp_modulus(theta, emin=1e-3, penal=3.0):
+    """E_e = E_min + θ_e^p (1 − E_min)."""
+    theta = np.asarray(theta, dtype=np.float64)
+    return emin + theta ** penal * (1.0 - emin)

@@ -18,6 +18,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--sweep", action="store_true", help="time SpMV / assembly launch shapes")
+    ap.add_argument("--solve", action="store_true", help="also run one FP64 and one mixed solve")
     args = ap.parse_args()
     import numpy as np
     import torch
@@ -57,6 +59,34 @@ def main():
     ms32 = fa.spmv_bench(K, mixed=True, reps=args.reps)
     print(f"phase spmv: fp64 {ms64:.4f} ms, fp32 {ms32:.4f} ms per launch (nnz {K.nnz}, n {dh.ndofs})",
           flush=True)
+    if args.solve:
+        for mode in ("cg", "mixed"):
+            r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
+            print(f"phase solve {mode}: {r.iterations} iterations, relres {r.relres:.2e}", flush=True)
+    if args.sweep:
+        for cap in (2048, 4096):
+            for lanes in (1, 2, 4):
+                fa.set_tuning(K, spmv_cap64=cap, spmv_lanes64=lanes, spmv_cap32=cap, spmv_lanes32=lanes,
+                              spmv_compress32=0)
+                a64 = fa.spmv_bench(K, mixed=False, reps=20)
+                a32 = fa.spmv_bench(K, mixed=True, reps=20)
+                fa.set_tuning(K, spmv_compress32=-1)
+                a32u = fa.spmv_bench(K, mixed=True, reps=20)
+                print(f"sweep spmv cap {cap} lanes {lanes}: fp64 {a64:.4f} ms, fp32+u16 {a32:.4f} ms, "
+                      f"fp32+i32 {a32u:.4f} ms", flush=True)
+        for bs in (64, 128, 256):
+            fa.set_tuning(K, asm_block=bs)
+            ts = []
+            for prec in ("fp64", "mixed"):
+                fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=prec)
+                ts.append(fa.timings(K)["assemble_kernel_ms"])
+            print(f"sweep assembly block {bs}: fp64 {ts[0]:.3f} ms, fp32-element {ts[1]:.3f} ms", flush=True)
+        for alg in ("greedy",):
+            cols = fa.create_coloring(g, alg)
+            fa.set_tuning(K, asm_block=0)
+            fa.assemble_global(fa.start_assemble(K, f), cv, cols, el, precision="fp64")
+            print(f"sweep coloring {alg} ({len(cols)} colors): fp64 {fa.timings(K)['assemble_kernel_ms']:.3f} ms",
+                  flush=True)
 
 
 if __name__ == "__main__":

@@ -64,5 +64,6 @@ class FacetValues:
         return len(self.weights[0])
 
     def device_tables(self):
+        """(nfq, nfacets, N[f][q][i], dMdξ[f][q][j][d], w[f][q]) contiguous."""
         return (self.nqp, self.nfacets, np.ascontiguousarray(np.stack(self.N)),
                 np.ascontiguousarray(np.stack(self.dMdxi)), np.ascontiguousarray(np.stack(self.weights)))

@@ -141,6 +141,20 @@ int32_t fem_solution_upload(fem_ctx* ctx, const double* u);
 /* Device address of a resident array (0 = nzval, 1 = f, 2 = u) for zero-copy wrapping. */
 int32_t fem_device_pointer(fem_ctx* ctx, int32_t which, void** ptr, int64_t* nelem);
 
+/* ---- launch-shape tuning (autotuner interface; no counterpart in the reference) ---- */
+typedef struct {
+  int32_t spmv_cap64, spmv_lanes64;  /* FP64 solver SpMV: LDS products per chunk (2048/4096) and
+                                        lanes per row in its reduction (1/2/4); 0 = default */
+  int32_t spmv_cap32, spmv_lanes32;  /* FP32 operator of the mixed solver, same meaning */
+  int32_t spmv_compress32;           /* 16-bit block-relative column offsets for the FP32 operator:
+                                        0 = default (on when every row block spans < 65536 columns),
+                                        -1 = off */
+  int32_t asm_block;                 /* cells per workgroup of the element kernels (0 = default) */
+  int32_t reserved[2];
+} fem_tuning;
+int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
+int32_t fem_get_tuning(const fem_ctx* ctx, fem_tuning* t);
+
 /* ---- timing: HIP-event durations of the last call's kernels on the context stream ---- */
 typedef struct {
   double assemble_kernel_ms;   /* element + scatter kernels, summed over colors */

@@ -65,3 +65,37 @@ def poisson_conductivity(nel, seed, sigma=0.5):
 
 __all__ = ["setup_poisson", "assemble_poisson", "heat_equation", "poisson_conductivity",
            "create_coloring"]
+
+
+def assemble_elasticity(grid, cv, dh, K, f, E_cell, nu):
+    """Serial loop with the linear-elastic element (threaded_assembly.jl:105-132)."""
+    from .fem import elasticity_element
+    A = start_assemble(K, f)
+    for ci in range(1, grid.ncells + 1):
+        Ke = elasticity_element(cv, grid.getcoordinates(ci), float(E_cell[ci - 1]), nu)
+        assemble(A, dh.celldofs(ci), Ke)
+    return K, f
+
+
+def assemble_traction(grid, fv, dh, f, facets, t):
+    """f[celldofs] += ∫ t·N dΓ over the facet set (FacetIterator loop, topology_optimization.jl:369-381)."""
+    from .fem import traction_element
+    for (ci, fc) in facets:
+        fe = traction_element(fv, grid.getcoordinates(ci), fc, np.asarray(t, dtype=np.float64))
+        for i, d in enumerate(dh.celldofs(ci)):
+            f[d - 1] += fe[i]
+    return f
+
+
+def setup_vector(celltype, nel, left, right, order, qorder):
+    from .fem import CSC
+    grid = generate_grid(celltype, nel, left, right)
+    shape = grid.refshape
+    dim = len(nel)
+    ip = Lagrange(shape, order) ** dim
+    cv = CellValues(QuadratureRule(shape, qorder), ip)
+    dh = DofHandler(grid).add("u", ip).close()
+    colptr, rowval = create_pattern(dh)
+    K = CSC(dh.ndofs, colptr, rowval)
+    f = np.zeros(dh.ndofs)
+    return grid, cv, dh, K, f
