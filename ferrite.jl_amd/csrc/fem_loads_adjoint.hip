@@ -226,6 +226,8 @@ int32_t fem_facet_traction(fem_ctx* c, int64_t nfacets, const int64_t* facets, i
       nft < 1 || nft > 6 || nfq > kMaxNFQ)
     return set_error(c, FEM_ERR_ARG, "fem_facet_traction: invalid arguments");
   if (!c->have_cv || c->ndpc == 0 || c->ncells == 0) return set_error(c, FEM_ERR_STATE, "upload CellValues and dofs first");
+  if (c->rdim != c->sdim || c->ndpc != c->nbf * c->vdim || c->ngeo > 8)
+    return set_error(c, FEM_ERR_ARG, "facet traction needs rdim == sdim and ndofs_per_cell == nbf * vdim");
   if (nfacets == 0) return FEM_OK;
   const int nbf = c->nbf, ngeo = c->ngeo, sdim = c->sdim, vdim = c->vdim, ndpc = c->ndpc;
   std::vector<int32_t> fc(nfacets), fnum(nfacets);
@@ -289,7 +291,8 @@ int32_t fem_adjoint_compliance(fem_ctx* c, const double* theta, double emin, dou
                                double* dJdtheta, double* dJdX) {
   if (!c || !theta) return set_error(c, FEM_ERR_ARG, "fem_adjoint_compliance: invalid arguments");
   if (!c->tables_ready || !c->have_cv) return set_error(c, FEM_ERR_STATE, "assemble and solve the system first");
-  if (!(c->vdim == c->rdim && c->nbf == c->ngeo && ((c->rdim == 2 && c->nqp == 4) || (c->rdim == 3 && c->nqp == 8))))
+  if (!(c->vdim == c->rdim && c->sdim == c->rdim && c->nbf == c->ngeo && c->ndpc == c->nbf * c->vdim &&
+        ((c->rdim == 2 && c->nqp == 4) || (c->rdim == 3 && c->nqp == 8))))
     return set_error(c, FEM_ERR_ARG, "compliance adjoint supports Q1 quad^2 (2x2 Gauss) and Q1 hex^3 (2x2x2 Gauss)");
   const double lam = nu / ((1 + nu) * (1 - 2 * nu)), mu = 1.0 / (2 * (1 + nu));
   const int64_t nc = c->ncells, nn = c->nnodes;

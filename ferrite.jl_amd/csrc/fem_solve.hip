@@ -524,7 +524,7 @@ static int ensure_work(fem_ctx* c, bool mixed) {
     destroy_graph(w->g64);
     destroy_graph(w->g32);
     w->sh64 = spmv_shape(c, c->tune.spmv_cap64 ? c->tune.spmv_cap64 : 2048, c->tune.spmv_lanes64 ? c->tune.spmv_lanes64 : 2);
-    w->sh32 = spmv_shape(c, c->tune.spmv_cap32 ? c->tune.spmv_cap32 : 2048, c->tune.spmv_lanes32 ? c->tune.spmv_lanes32 : 2);
+    w->sh32 = spmv_shape(c, c->tune.spmv_cap32 ? c->tune.spmv_cap32 : 2048, c->tune.spmv_lanes32 ? c->tune.spmv_lanes32 : 4);
     dev_free(&w->roff32);
     dev_free(&w->bbase32);
     w->comp_checked = false;
