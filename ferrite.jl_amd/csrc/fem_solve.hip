@@ -32,7 +32,13 @@ constexpr int kBatch = 16;           // CG iterations per graph replay
 
 struct Scal {
   double rz, pq, rr, alpha, beta, bnorm2;
-  int done, iters;
+  double hold2;   // stop the recurrence (done = 1) once rr < hold2: hand over to the host
+  double rzp;     // r·z of the step before the hand-over (β of the rebuilt direction)
+  double gam;     // (r·p)/(r·z) at a reseed: p is made orthogonal to the new residual
+  int done;       // 1: recurrence stopped (converged by its own residual, or handed over)
+  int iters;
+  int restart;    // p·Kp <= 0 happened: the next reseed restarts p = z
+  int pad;
 };
 
 struct SpmvShape {
@@ -273,7 +279,13 @@ __global__ __launch_bounds__(kWide) void k_alpha(int m, const double* __restrict
   const double pq = wide_sum(part, m, 1, sh);
   if (threadIdx.x == 0) {
     s->pq = pq;
-    s->alpha = s->rz / pq;
+    // p·Kp > 0 for SPD K unless p vanished: stop and let the host restart from the true residual
+    const bool ok = pq > 0.0;
+    s->alpha = ok ? s->rz / pq : 0.0;
+    if (!ok) {
+      s->done = 1;
+      s->restart = 1;
+    }
   }
 }
 
@@ -326,11 +338,15 @@ __global__ __launch_bounds__(kWide) void k_beta(int m, const double* __restrict_
   const double rzn = wide_sum(part, m, 2, sh);
   const double rrn = wide_sum(part + 1, m, 2, sh);
   if (threadIdx.x == 0) {
-    s->beta = rzn / s->rz;
+    s->beta = s->rz != 0.0 ? rzn / s->rz : 0.0;
+    s->rzp = s->rz;
     s->rz = rzn;
     s->rr = rrn;
     s->iters += 1;
-    if (rrn <= tol2) s->done = 1;
+    // converged by the recursive residual, or (mixed) dropped far enough for a reliable
+    // update: the FP32 recurrence must not run on towards underflow.  p is not updated in
+    // this iteration (it stays the last search direction, orthogonal to the recursive r).
+    if (rrn <= tol2 || rrn < s->hold2) s->done = 1;
   }
 }
 
@@ -376,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void k_init(int64_t n, const X* __restrict_
 }
 
 __global__ __launch_bounds__(kBlock) void k_init_scal(int m, const double* __restrict__ part, Scal* __restrict__ s,
-                                                      double tol2) {
+                                                      double tol2, double hold_factor2) {
   __shared__ double sh[kBlock / 64];
   const double rz = sum_parts(part, m, 2, sh);
   const double rr = sum_parts(part + 1, m, 2, sh);
@@ -384,6 +400,8 @@ __global__ __launch_bounds__(kBlock) void k_init_scal(int m, const double* __res
     s->rz = rz;
     s->rr = rr;
     s->iters = 0;
+    s->restart = 0;
+    s->hold2 = hold_factor2 * rr;
     s->done = (rr <= tol2) ? 1 : 0;
   }
 }
@@ -431,31 +449,58 @@ __global__ void k_fold(int64_t n, float* __restrict__ x32, double* __restrict__ 
 }
 
 // Reliable update, step 2: r32 = res (the FP64 true residual), partials of r32·D r32.
+// Reliable update, step 2: r = res (the FP64 true residual, rounded to X), partials of
+// r·D⁻¹r and r·p.
+template <typename X>
 __global__ __launch_bounds__(kBlock) void k_reseed(int64_t n, const double* __restrict__ res,
-                                                   const float* __restrict__ invd, float* __restrict__ r32,
-                                                   double* __restrict__ part) {
+                                                   const X* __restrict__ invd, X* __restrict__ r,
+                                                   const X* __restrict__ p, double* __restrict__ part) {
   __shared__ double sh[kBlock / 64];
-  double rz = 0.0;
+  double rz = 0.0, rp = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const float ri = (float)res[i];
-    r32[i] = ri;
+    const X ri = (X)res[i];
+    r[i] = ri;
     rz += (double)ri * (double)(ri * invd[i]);
+    rp += (double)ri * (double)p[i];
   }
-  const double b = block_sum(rz, sh);
+  double b = block_sum(rz, sh);
   if (threadIdx.x == 0) part[2 * blockIdx.x] = b;
-  if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = 0.0;
+  b = block_sum(rp, sh);
+  if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = b;
+}
+
+// Reliable update, step 4: p = z + β (p − γ z): the old direction made orthogonal to the
+// new residual, then extended as a CG step would (β = r·z / r·z of the step before).
+template <typename X>
+__global__ __launch_bounds__(kBlock) void k_pfix(int64_t n, const X* __restrict__ r, const X* __restrict__ invd,
+                                                 X* __restrict__ p, const Scal* __restrict__ s) {
+  const X beta = (X)s->beta, gam = (X)s->gam;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const X zi = r[i] * invd[i];
+    p[i] = zi + beta * (p[i] - gam * zi);
+  }
 }
 
 // Reliable update, step 3: rz from the reseeded residual; the search direction p is kept.
+// Reliable update, step 3 (scalars).  s->rr holds the true residual ||f - K u||² when this
+// runs; the next hand-over is at hold_factor2 times that.  After a breakdown (or with no
+// previous step) the direction restarts: β = 0, p = z.
 __global__ __launch_bounds__(kBlock) void k_reseed_scal(int m, const double* __restrict__ part,
-                                                        Scal* __restrict__ s) {
+                                                        Scal* __restrict__ s, double hold_factor2) {
   __shared__ double sh[kBlock / 64];
   const double rz = sum_parts(part, m, 2, sh);
+  const double rp = sum_parts(part + 1, m, 2, sh);
   if (threadIdx.x == 0) {
+    const bool fresh = s->restart || !(s->rzp > 0.0) || !(rz > 0.0);
+    s->gam = rz > 0.0 ? rp / rz : 0.0;
+    s->beta = fresh ? 0.0 : rz / s->rzp;
     s->rz = rz;
+    s->hold2 = hold_factor2 * s->rr;
     s->done = 0;
+    s->restart = 0;
   }
 }
+
 
 __global__ void k_f32_from(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -620,32 +665,51 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const V* val, X* x, X* r, X* p
   return FEM_OK;
 }
 
-// Preconditioned CG on K x = r from x = 0 (r is overwritten with the residual).
-template <typename V, typename X>
-static int pcg(fem_ctx* c, SolverWork* w, const V* val, X* x, X* r, X* p, X* q, const X* invd, double tol2,
-               int maxit, GraphCache& gc, int* iters, double* ms) {
-  k_init<X><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, r, invd, p, w->part_v);
-  k_init_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2);
+static void residual(fem_ctx* c, SolverWork* w);
+
+// FP64 preconditioned CG on K u = f from u = 0 (r holds f on entry).  When the recursive
+// residual claims convergence the true residual f − K u is computed; if it misses the
+// tolerance it replaces r (residual replacement; p is re-orthogonalised against it) and the
+// iteration goes on, at most kMaxReplace times.
+constexpr int kMaxReplace = 8;
+
+static int pcg64(fem_ctx* c, SolverWork* w, double tol2, int maxit, GraphCache& gc, int* iters, int* replaced,
+                 double* ms) {
+  double *x = c->d_u, *r = w->r, *p = w->p, *q = w->q;
+  const double* invd = w->invd;
+  k_init<double><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, r, invd, p, w->part_v);
+  k_init_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2, 0.0);
   FEM_HIP(c, hipGetLastError());
-  int rc = batch_graph<V, X>(c, w, val, x, r, p, q, invd, tol2, gc);
+  int rc = batch_graph<double, double>(c, w, c->d_nzval, x, r, p, q, invd, tol2, gc);
   if (rc) return rc;
   Scal h{};
-  int launched = 0;
+  int launched = 0, nrep = 0;
+  bool converged = false;
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   while (launched < maxit) {
     FEM_HIP(c, hipGraphLaunch(gc.exec, c->stream));
     launched += kBatch;
-    rc = read_scal(c, w, &h);
-    if (rc) return rc;
-    if (h.done) break;
+    if ((rc = read_scal(c, w, &h))) return rc;
+    if (!h.done) continue;
+    residual(c, w);
+    Scal ht{};
+    if ((rc = read_scal(c, w, &ht))) return rc;
+    if (ht.rr <= tol2) { converged = true; break; }
+    if (nrep++ >= kMaxReplace) break;
+    k_reseed<double><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, w->res, invd, r, p, w->part_v);
+    k_reseed_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, 0.0);
+    k_pfix<double><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, r, invd, p, w->s);
   }
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
   *ms += event_ms(c);
+  if ((rc = read_scal(c, w, &h))) return rc;
   *iters = h.iters;
-  return h.done ? FEM_OK : FEM_ERR_NOCONV;
+  *replaced = nrep;
+  return converged ? FEM_OK : FEM_ERR_NOCONV;
 }
 
 static void residual(fem_ctx* c, SolverWork* w) {
+  // res = f − K u and s->rr = ||res||² (FP64, fixed reduction order)
   launch_spmv<double, double, 1>(c, w->sh64, c->d_nzval, c->d_u, c->d_f, w->res, w->part_s, w->s);
   k_sum_to<<<1, kWide, 0, c->stream>>>(w->sh64.blocks, w->part_s, w->s, 0);
 }
@@ -685,26 +749,26 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
     k_invdiag<double><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd);
     k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_u);
     k_copy<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->r);
-    int iters = 0;
-    rc = pcg<double, double>(c, w, c->d_nzval, c->d_u, w->r, w->p, w->q, w->invd, tol2, maxit, w->g64, &iters,
-                             &cg_ms);
+    int iters = 0, replaced = 0;
+    rc = pcg64(c, w, tol2, maxit, w->g64, &iters, &replaced, &cg_ms);
     out.iterations = iters;
-    out.outer_iterations = 1;
+    out.outer_iterations = 1 + replaced;
     if (rc && rc != FEM_ERR_NOCONV) return rc;
   } else {
     if ((rc = ensure_operator32(c, w))) return rc;
     k_invdiag<float><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd32);
     // Mixed CG with reliable updates: one FP32 Krylov sequence (FP32 SpMV and vectors,
     // FP64 dot products); whenever its recursive residual has dropped by δ since the last
-    // refinement (or claims convergence), the FP32 iterate is folded into the FP64 u, the
-    // true residual f − K u is recomputed with the FP64 K and fed back as the FP32
-    // residual, keeping the search direction.
+    // refinement (or claims convergence) the device stops the recurrence, the FP32 iterate
+    // is folded into the FP64 u, the true residual f − K u is recomputed with the FP64 K and
+    // fed back as the FP32 residual, and the last search direction is made orthogonal to it
+    // and extended (p = z + β (p − γ z)), as in reliable-update CG.
     const double delta = cfg->inner_rtol > 0 ? cfg->inner_rtol : 0.1;
     k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_u);
     k_zero32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, w->x32);
     k_f32_from<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->r32);
     k_init<float><<<w->vblocks, kBlock, 0, c->stream>>>(n, w->r32, w->invd32, w->p32, w->part_v);
-    k_init_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2);
+    k_init_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2, delta * delta);
     if ((rc = batch_graph<float, float>(c, w, c->d_nzval32, w->x32, w->r32, w->p32, w->q32, w->invd32, tol2, w->g32)))
       return rc;
     double maxrr = bnorm2;
@@ -724,8 +788,9 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
         if ((rc = read_scal(c, w, &ht))) return rc;
         if (ht.rr <= tol2) { converged = true; break; }
         maxrr = ht.rr;
-        k_reseed<<<w->vblocks, kBlock, 0, c->stream>>>(n, w->res, w->invd32, w->r32, w->part_v);
-        k_reseed_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s);
+        k_reseed<float><<<w->vblocks, kBlock, 0, c->stream>>>(n, w->res, w->invd32, w->r32, w->p32, w->part_v);
+        k_reseed_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, delta * delta);
+        k_pfix<float><<<w->vblocks, kBlock, 0, c->stream>>>(n, w->r32, w->invd32, w->p32, w->s);
       }
     }
     FEM_HIP(c, hipEventRecord(c->ev1, c->stream));

@@ -6,6 +6,7 @@ Names follow Ferrite.jl 1.6.0; mutating Julia functions drop the `!`
 """
 from ._lib import (FemArgumentError, FemConvergenceError, FemError, FemPatternError, LIB_PATH,
                    exported_symbols)
+from .batch import BatchedPoisson, batch_sizes, make_batch
 from .constraints import ConstraintHandler, Dirichlet
 from .device import (MI355XCSC, Context, ElasticityElement, PoissonElement, add_point_load, allocate_matrix,
                      allocate_vector, apply, apply_traction, apply_zero, compliance_gradient, simp_modulus, assemble_global, get_tuning, set_cell_parameters, set_tuning, solve,
@@ -24,5 +25,5 @@ __all__ = [
     "solve", "spmv_bench", "set_tuning", "get_tuning", "start_assemble", "timings", "DofHandler", "add", "close", "create_coloring",
     "create_sparsity_pattern", "Grid", "Hexahedron", "Quadrilateral", "generate_grid", "FacetQuadratureRule",
     "Lagrange", "QuadratureRule", "RefHexahedron", "RefLine", "RefQuadrilateral", "VectorizedInterpolation",
-    "CellValues", "FacetValues",
+    "CellValues", "FacetValues", "BatchedPoisson", "batch_sizes", "make_batch",
 ]

@@ -65,6 +65,16 @@ class Timings(C.Structure):
                 ("spmv_ms", C.c_double), ("spmv_launches", C.c_int32), ("assemble_launches", C.c_int32)]
 
 
+class BatchCfg(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("cells_per_wg", C.c_int32), ("chunk", C.c_int32),
+                ("maxit", C.c_int32), ("rtol", C.c_double), ("ru_delta", C.c_double)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("assemble_ms", C.c_double), ("apply_ms", C.c_double), ("solve_ms", C.c_double),
+                ("total_ms", C.c_double), ("converged", C.c_int32), ("total_iterations", C.c_int32)]
+
+
 P = C.c_void_p
 I32, I64, D = C.c_int32, C.c_int64, C.c_double
 PI64, PI32, PD = C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_double)
@@ -99,6 +109,17 @@ SIGNATURES = {
     "fem_setup_distribute_dofs": (I32, [I32, I64, I32, PI64, I64, I32, PI32, PI32, PI64, PI32, PI64]),
     "fem_setup_pattern": (I32, [I64, I32, PI64, I64, PI64, PI64, I64, PI64]),
     "fem_setup_coloring": (I32, [I64, I32, PI64, I64, I32, PI32, PI32]),
+    "fem_batch_create": (I32, [I32, C.POINTER(P)]),
+    "fem_batch_destroy": (None, [P]),
+    "fem_batch_last_error": (C.c_char_p, [P]),
+    "fem_batch_add_problem": (I32, [P, I32, I64, PD, I64, PI64, I64, PI64, PI64, PI64, I32, PI64, PI64, PD,
+                                    I64, PI64, D]),
+    "fem_batch_finalize": (I32, [P, PD, PD, PD, PD, PD, PD]),
+    "fem_batch_autotune": (I32, [P, I32, PI32, PI32]),
+    "fem_batch_run": (I32, [P, C.POINTER(BatchCfg), C.POINTER(BatchInfo)]),
+    "fem_batch_problem_stats": (I32, [P, PD, PI32]),
+    "fem_batch_download": (I32, [P, I32, PD, PD, PD]),
+    "fem_batch_assemble_only": (I32, [P, I32, I32, I32, PD]),
 }
 
 _lib = None
@@ -127,12 +148,12 @@ def exported_symbols():
     return {name for name in SIGNATURES if getattr(h, name, None) is not None}
 
 
-def raise_for(code, ctx=None, what=""):
+def raise_for(code, ctx=None, what="", batch=False):
     if code == FEM_OK:
         return
     msg = what
     if ctx is not None:
-        m = lib().fem_last_error(ctx)
+        m = (lib().fem_batch_last_error if batch else lib().fem_last_error)(ctx)
         msg = (m.decode() if m else "") or what
     if code == FEM_ERR_PATTERN:
         raise FemPatternError(code, msg)

@@ -10,7 +10,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     names = set()
-    for h in ("ferrite_mi355x.h", "ferrite_mi355x_setup.h"):
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
         with open(os.path.join(ROOT, "include", h)) as fh:
             txt = fh.read()
         names |= set(re.findall(r"^\s*(?:int32_t|void|const char\*)\s+(fem_\w+)\s*\(", txt, re.M))

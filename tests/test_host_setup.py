@@ -117,3 +117,32 @@ def test_c2_scale_setup_properties():
     for col in colors[:3]:
         nodes = g.cells[col - 1].ravel()
         assert len(np.unique(nodes)) == nodes.size
+
+
+def test_batch_problem_sizes():
+    """The 64-problem batch: 2D/3D mix, cell counts log-uniform in [2e3, 2e5], near-square/cube grids."""
+    from ferrite_amd.batch import batch_sizes
+    sz = batch_sizes(64, 0)
+    cells = np.array([np.prod(n) for _, n in sz])
+    assert len(sz) == 64
+    assert 0.8 * 2e3 <= cells.min() and cells.max() <= 1.2 * 2e5
+    assert {ct for ct, _ in sz} == {"Quadrilateral", "Hexahedron"}
+    for ct, n in sz:
+        assert len(n) == (2 if ct == "Quadrilateral" else 3)
+        assert max(n) <= 1.2 * min(n) + 1
+    assert batch_sizes(64, 0) == sz and batch_sizes(64, 1) != sz
+
+
+def test_batch_problem_setup_matches_oracle():
+    """A batch problem's dofs, pattern and prescribed dofs equal the oracle's setup_poisson."""
+    from ferrite_amd.batch import poisson_problem
+    from ferrite_amd.dofs import create_sparsity_pattern
+    from oracle.problems import setup_poisson
+    for ct, nel in (("Quadrilateral", (7, 5)), ("Hexahedron", (3, 4, 2))):
+        g, dh, colors, ch, _ = poisson_problem(ct, nel, np.ones(int(np.prod(nel))))
+        og, ocv, odh, oK, of, och = setup_poisson(ct, nel)
+        assert np.array_equal(dh.cell_dofs.ravel(), np.asarray(odh.cell_dofs).ravel())
+        cp, rv = create_sparsity_pattern(dh)
+        assert np.array_equal(cp, oK.colptr) and np.array_equal(rv, oK.rowval)
+        assert np.array_equal(ch.prescribed_dofs, np.asarray(och.prescribed_dofs))
+        assert sorted(np.concatenate(colors).tolist()) == list(range(1, g.ncells + 1))
