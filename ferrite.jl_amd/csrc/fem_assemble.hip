@@ -17,48 +17,9 @@
 // 8 columns instead of 64 scattered lines.  Cells of one colour share no dof,
 // so the read-modify-write of nzval/f needs no atomics and the sum order per
 // entry is the colour order: results are bitwise reproducible run to run.
-#include "fem_internal.h"
+#include "fem_element.h"
 
 namespace fem {
-
-// Packed reference tables: N[NQ][NB], dN[NQ][NB][DIM], dM[NQ][NG][DIM], w[NQ].
-template <int DIM, int NG, int NB, int NQ>
-struct TabLayout {
-  static constexpr int oN = 0;
-  static constexpr int odN = oN + NQ * NB;
-  static constexpr int odM = odN + NQ * NB * DIM;
-  static constexpr int ow = odM + NQ * NG * DIM;
-  static constexpr int size = ow + NQ;
-};
-
-template <typename T, int DIM>
-__device__ __forceinline__ T det_inv(const T (&J)[DIM][DIM], T (&Ji)[DIM][DIM]) {
-  if constexpr (DIM == 2) {
-    const T d = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const T id = T(1) / d;
-    Ji[0][0] = J[1][1] * id;  Ji[0][1] = -J[0][1] * id;
-    Ji[1][0] = -J[1][0] * id; Ji[1][1] = J[0][0] * id;
-    return d;
-  } else {
-    const T a00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    const T a01 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
-    const T a02 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
-    const T d = J[0][0] * a00 + J[1][0] * a01 + J[2][0] * a02;
-    const T id = T(1) / d;
-    Ji[0][0] = a00 * id; Ji[0][1] = a01 * id; Ji[0][2] = a02 * id;
-    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) * id;
-    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
-    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-    return d;
-  }
-}
-
-constexpr __host__ __device__ int tri_idx(int lo, int hi, int nb) {
-  return lo * (2 * nb - lo + 1) / 2 + (hi - lo);
-}
 
 __global__ void k_fill_zero(double* __restrict__ a, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -79,7 +40,6 @@ __global__ __launch_bounds__(BLOCK) void k_poisson(
     const int32_t* __restrict__ cnz, const int32_t* __restrict__ ccell, const double* __restrict__ cparam,
     const T* __restrict__ tab, T src, double* __restrict__ nzval, double* __restrict__ f,
     DevError* __restrict__ err) {
-  using L = TabLayout<DIM, NG, NB, NQ>;
   constexpr int TRI = NB * (NB + 1) / 2;
   __shared__ T s_ke[BLOCK * TRI];
   __shared__ T s_fe[BLOCK * NB];
@@ -98,57 +58,9 @@ __global__ __launch_bounds__(BLOCK) void k_poisson(
       x[a][1] = (T)X1[nd];
       if constexpr (DIM == 3) x[a][2] = (T)X2[nd];
     }
-    const T k = (T)cparam[s];
     T ke[TRI];
     T fe[NB];
-#pragma unroll
-    for (int i = 0; i < TRI; ++i) ke[i] = T(0);
-#pragma unroll
-    for (int i = 0; i < NB; ++i) fe[i] = T(0);
-    bool bad = false;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      // J = Σ_j x_j ⊗ dMdξ_j   (GeometryMapping.jl:124-130)
-      T J[DIM][DIM];
-#pragma unroll
-      for (int i = 0; i < DIM; ++i)
-#pragma unroll
-        for (int j = 0; j < DIM; ++j) J[i][j] = T(0);
-#pragma unroll
-      for (int a = 0; a < NG; ++a)
-#pragma unroll
-        for (int i = 0; i < DIM; ++i)
-#pragma unroll
-          for (int j = 0; j < DIM; ++j) J[i][j] += x[a][i] * tab[L::odM + (q * NG + a) * DIM + j];
-      T Ji[DIM][DIM];
-      const T detJ = det_inv<T, DIM>(J, Ji);
-      bad |= !(detJ > T(0));
-      const T dV = detJ * tab[L::ow + q];
-      // dNdx = dNdξ ⋅ J⁻¹  (FunctionValues.jl:186-192)
-      T g[NB][DIM];
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-#pragma unroll
-        for (int j = 0; j < DIM; ++j) {
-          T acc = T(0);
-#pragma unroll
-          for (int i = 0; i < DIM; ++i) acc += tab[L::odN + (q * NB + b) * DIM + i] * Ji[i][j];
-          g[b][j] = acc;
-        }
-      const T kdV = k * dV;
-      const T sdV = src * dV;
-#pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        fe[a] += tab[L::oN + q * NB + a] * sdV;
-#pragma unroll
-        for (int b = a; b < NB; ++b) {
-          T dot = g[a][0] * g[b][0];
-#pragma unroll
-          for (int j = 1; j < DIM; ++j) dot += g[a][j] * g[b][j];
-          ke[tri_idx(a, b, NB)] += dot * kdV;
-        }
-      }
-    }
+    const bool bad = !poisson_cell<T, DIM, NG, NB, NQ>(x, (T)cparam[s], src, tab, ke, fe);
     if (bad) atomicMin(&err->detj_cell, ccell[s]);
 #pragma unroll
     for (int i = 0; i < TRI; ++i) s_ke[lc * TRI + i] = ke[i];
@@ -311,12 +223,30 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
   if (!c || !cfg) return FEM_ERR_ARG;
   if (!c->have_cv) return set_error(c, FEM_ERR_STATE, "upload CellValues tables first");
   if (!c->have_param) return set_error(c, FEM_ERR_STATE, "upload the per-cell parameter first");
-  int rc = build_tables(c);
-  if (rc) return rc;
   FEM_HIP(c, hipSetDevice(c->device));
   c->err_cell = c->err_row = c->err_col = -1;
   c->timings.assemble_launches = 0;
   k_reset_err<<<1, 1, 0, c->stream>>>(c->d_err);
+  const bool mixed = cfg->precision == FEM_MIXED;
+  // Scalar Q1 Poisson: owner-computes gather (each nzval/f entry written once, no fill
+  // needed) unless the colored scatter is asked for.
+  if (cfg->kernel == FEM_KERNEL_POISSON && c->tune.asm_mode != 1) {
+    if (c->vdim != 1 || c->ndpc != c->nbf) return set_error(c, FEM_ERR_ARG, "Poisson kernel needs a scalar field");
+    if (gather_supported(c)) {
+      int rg = c->rdim == 3 ? upload_tables<3, 8, 8, 8>(c) : upload_tables<2, 4, 4, 4>(c);
+      if (rg) return rg;
+      c->timings.fill_ms = 0.0;
+      rg = gather_poisson(c, mixed, cfg->source, cfg->fillzero != 0, c->d_tab, c->d_tabf);
+      if (rg) return rg;
+      c->nzval32_dirty = true;
+      ++c->nzval_version;
+      return check_assembly_errors(c);
+    }
+    if (c->tune.asm_mode == 2)
+      return set_error(c, FEM_ERR_ARG, "gather assembly needs a scalar Q1 quad/hex with 2x2(x2) Gauss");
+  }
+  int rc = build_tables(c);
+  if (rc) return rc;
   if (cfg->fillzero) {
     FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
     k_fill_zero<<<2048, 256, 0, c->stream>>>(c->d_nzval, c->nnz);
@@ -325,7 +255,6 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
     FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
     c->timings.fill_ms = event_ms(c);
   }
-  const bool mixed = cfg->precision == FEM_MIXED;
   if (cfg->kernel == FEM_KERNEL_POISSON) {
     if (c->vdim != 1 || c->ndpc != c->nbf) return set_error(c, FEM_ERR_ARG, "Poisson kernel needs a scalar field");
     const int key = c->rdim * 1000 + c->ngeo * 100 + c->nbf * 10 + (c->nqp == 8 ? 8 : c->nqp);
@@ -355,5 +284,6 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
   }
   if (rc) return rc;
   c->nzval32_dirty = true;
+  ++c->nzval_version;
   return check_assembly_errors(c);
 }

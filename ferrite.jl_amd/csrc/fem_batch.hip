@@ -83,6 +83,11 @@ struct fem_batch {
   fem::BProbDev* d_probs = nullptr;
   int32_t* d_rowprob = nullptr;     // [ndofs] problem of each global dof
   int64_t* d_pdof = nullptr;        // [nprob+1] dof offsets
+  int32_t* d_sprob = nullptr;       // [nseg] problem of each reduction segment
+  int64_t* d_srow = nullptr;        // [nseg+1] first row of each segment
+  int32_t* d_pseg = nullptr;        // [nprob+1] first segment of each problem
+  double* d_spart = nullptr;        // [2 nseg] segment partials
+  int nseg = 0;
   double* d_tab = nullptr;          // reference tables: quad block then hex block (float copy below)
   float* d_tabf = nullptr;
   int64_t nslots = 0;
@@ -364,23 +369,35 @@ struct BScal {
   int pad;
 };
 
-enum BDot { kDotPQ = 0, kDotRZRR = 1, kDotTrue = 3, kDotF = 4 };
+enum BDot { kDotPQ = 0, kDotRZRR = 1, kDotTrue = 3, kDotF = 4, kDotReseed = 5 };
 
-// one block per problem, fixed summation order (deterministic)
+// Which problems a reduction runs for.
+__device__ __forceinline__ bool b_skip(const BScal& sp, int which) {
+  if (which == kDotF) return false;
+  if (sp.done) return true;
+  if (which == kDotTrue || which == kDotReseed) return !sp.active;
+  return sp.hold != 0;
+}
+
+// Per-problem reductions in two levels: every segment (≤ kSeg consecutive rows of one
+// problem) gets one block that writes t = Σ a·b and t2 = Σ a·c; k_b_dot_fin then sums a
+// problem's segments in order.  Fixed order throughout (deterministic).
+constexpr int kSeg = 2048;
+
 template <typename X>
-__global__ __launch_bounds__(256) void k_b_dot(const int64_t* __restrict__ pdof, const X* __restrict__ a,
-                                               const X* __restrict__ b, BScal* __restrict__ s, int which) {
+__global__ __launch_bounds__(256) void k_b_dot_seg(const int32_t* __restrict__ sprob, const int64_t* __restrict__ srow,
+                                                   const X* __restrict__ a, const X* __restrict__ b,
+                                                   const X* __restrict__ c, const BScal* __restrict__ s, int which,
+                                                   double* __restrict__ part) {
   __shared__ double sh[2][4];
-  const int p = blockIdx.x;
-  if (which != kDotF && s[p].done) return;
-  if (which == kDotTrue && !s[p].active) return;
-  if ((which == kDotPQ || which == kDotRZRR) && s[p].hold) return;
-  const int64_t j0 = pdof[p], j1 = pdof[p + 1];
+  const int seg = blockIdx.x;
+  if (b_skip(s[sprob[seg]], which)) return;
+  const int64_t j0 = srow[seg], j1 = srow[seg + 1];
   double acc = 0.0, acc2 = 0.0;
   for (int64_t j = j0 + threadIdx.x; j < j1; j += 256) {
     const double aj = (double)a[j];
     acc += aj * (double)b[j];
-    acc2 += aj * aj;
+    acc2 += aj * (double)c[j];
   }
   for (int o = 32; o > 0; o >>= 1) {
     acc += __shfl_down(acc, o, 64);
@@ -392,34 +409,57 @@ __global__ __launch_bounds__(256) void k_b_dot(const int64_t* __restrict__ pdof,
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const double t = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
-    const double t2 = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
-    switch (which) {
-      case kDotPQ: {
-        // p·Kp > 0 for SPD K unless p vanished: hand over to the host
-        const bool ok = t > 0.0;
-        s[p].pq = t;
-        s[p].alpha = ok ? s[p].rz / t : 0.0;
-        if (!ok) {
-          s[p].hold = 1;
-          s[p].restart = 1;
-        }
-        break;
+    part[2 * seg] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+    part[2 * seg + 1] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
+  }
+}
+
+// One thread per problem: sum its segments, then the scalar step of `which`.
+__global__ void k_b_dot_fin(int nprob, const int32_t* __restrict__ pseg, const double* __restrict__ part,
+                            BScal* __restrict__ s, int which) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob || b_skip(s[p], which)) return;
+  double t = 0.0, t2 = 0.0;
+  for (int k = pseg[p]; k < pseg[p + 1]; ++k) {
+    t += part[2 * k];
+    t2 += part[2 * k + 1];
+  }
+  BScal& sp = s[p];
+  switch (which) {
+    case kDotPQ: {
+      // p·Kp > 0 for SPD K unless p vanished: hand over to the host
+      const bool ok = t > 0.0;
+      sp.pq = t;
+      sp.alpha = ok ? sp.rz / t : 0.0;
+      if (!ok) {
+        sp.hold = 1;
+        sp.restart = 1;
       }
-      case kDotRZRR:  // r·z and r·r after the update: β, iteration count
-        s[p].beta = s[p].rz != 0.0 ? t / s[p].rz : 0.0;
-        s[p].rzp = s[p].rz;
-        s[p].rz = t;
-        s[p].rr = t2;
-        s[p].iters += 1;
-        // converged by the recurrence, or dropped by δ since the last true residual: stop
-        // this problem's FP32 recurrence (it would otherwise run on towards underflow); p is
-        // not updated (it stays the last direction, orthogonal to the recursive residual)
-        if (t2 <= s[p].tol2 || t2 < s[p].d2 * s[p].maxrr) s[p].hold = 1;
-        break;
-      case kDotTrue: s[p].rrt = t; break;   // FP64 true residual (a = b = res)
-      default: s[p].bnorm2 = t; break;      // ||f||²
+      break;
     }
+    case kDotRZRR:  // t = r·z, t2 = r·r after the update: β, iteration count
+      sp.beta = sp.rz != 0.0 ? t / sp.rz : 0.0;
+      sp.rzp = sp.rz;
+      sp.rz = t;
+      sp.rr = t2;
+      sp.iters += 1;
+      // converged by the recurrence, or dropped by δ since the last true residual: stop
+      // this problem's FP32 recurrence (it would otherwise run on towards underflow); p is
+      // not updated (it stays the last direction, orthogonal to the recursive residual)
+      if (t2 <= sp.tol2 || t2 < sp.d2 * sp.maxrr) sp.hold = 1;
+      break;
+    case kDotTrue: sp.rrt = t; break;   // FP64 true residual (a = b = res)
+    case kDotReseed: {                  // t = r·z, t2 = r·p of the reseeded residual
+      // γ = (r·p)/(r·z), β = r·z / r·z of the step before (β = 0: restart p = z after a
+      // breakdown or with no previous step)
+      const bool fresh = sp.restart || !(sp.rzp > 0.0) || !(t > 0.0);
+      sp.gam = t > 0.0 ? t2 / t : 0.0;
+      sp.beta = fresh ? 0.0 : t / sp.rzp;
+      sp.rz = t;
+      sp.restart = 0;
+      break;
+    }
+    default: sp.bnorm2 = t; break;      // ||f||²
   }
 }
 
@@ -449,17 +489,24 @@ __global__ void k_b_pupd(int64_t n, const int32_t* __restrict__ rowprob, const B
 }
 
 // y = K x with 4 lanes per CSC column (K symmetric: column j is row j), fixed combine order
+// Rows of finished or held problems are skipped (their q is not used before it is rewritten).
 __global__ void k_b_spmv32(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
-                           const float* __restrict__ val, const float* __restrict__ x, float* __restrict__ y) {
+                           const float* __restrict__ val, const float* __restrict__ x, float* __restrict__ y,
+                           const int32_t* __restrict__ rowprob, const BScal* __restrict__ s) {
   const int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t j = gt >> 2;
   const int sub = (int)(gt & 3);
+  bool run = j < n;
+  if (run) {
+    const BScal& sp = s[rowprob[j]];
+    run = !(sp.done || sp.hold);
+  }
   float acc = 0.f;
-  if (j < n)
+  if (run)
     for (int32_t p = colptr[j] + sub; p < colptr[j + 1]; p += 4) acc += val[p] * x[rowval[p]];
   acc += __shfl_down(acc, 2, 4);
   acc += __shfl_down(acc, 1, 4);
-  if (j < n && sub == 0) y[j] = acc;
+  if (run && sub == 0) y[j] = acc;
 }
 
 __global__ void k_b_resid64(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
@@ -497,41 +544,6 @@ __global__ void k_b_reseed(int64_t n, const int32_t* __restrict__ rowprob, const
     const float ri = (float)res[i];
     r32[i] = ri;
     z32[i] = ri * invd[i];
-  }
-}
-
-// per active, unfinished problem: rz = r·z, γ = (r·p)/(r·z), β = rz / rz of the step before
-// (β = 0: restart p = z after a breakdown or with no previous step)
-__global__ __launch_bounds__(256) void k_b_reseed_dot(const int64_t* __restrict__ pdof, const float* __restrict__ r,
-                                                      const float* __restrict__ z, const float* __restrict__ p,
-                                                      BScal* __restrict__ s) {
-  __shared__ double sh[2][4];
-  const int q = blockIdx.x;
-  if (s[q].done || !s[q].active) return;
-  const int64_t j0 = pdof[q], j1 = pdof[q + 1];
-  double rz = 0.0, rp = 0.0;
-  for (int64_t j = j0 + threadIdx.x; j < j1; j += 256) {
-    const double rj = (double)r[j];
-    rz += rj * (double)z[j];
-    rp += rj * (double)p[j];
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    rz += __shfl_down(rz, o, 64);
-    rp += __shfl_down(rp, o, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    sh[0][threadIdx.x >> 6] = rz;
-    sh[1][threadIdx.x >> 6] = rp;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double t = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
-    const double tp = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
-    const bool fresh = s[q].restart || !(s[q].rzp > 0.0) || !(t > 0.0);
-    s[q].gam = t > 0.0 ? tp / t : 0.0;
-    s[q].beta = fresh ? 0.0 : t / s[q].rzp;
-    s[q].rz = t;
-    s[q].restart = 0;
   }
 }
 
@@ -726,24 +738,33 @@ int timed_assemble(fem_batch* B, int cpw, int chunk, bool mixed, float* ms) {
   return assemble_errors(B);
 }
 
+// Per-problem reduction t = Σ a·b, t2 = Σ a·c (two levels) followed by the scalar step `which`.
+template <typename X>
+void bdot(fem_batch* B, hipStream_t st, const X* a, const X* b, const X* c, int which) {
+  BScal* s = (BScal*)B->d_scal;
+  const int np_ = (int)B->probs.size();
+  k_b_dot_seg<X><<<B->nseg, 256, 0, st>>>(B->d_sprob, B->d_srow, a, b, c, s, which, B->d_spart);
+  k_b_dot_fin<<<(np_ + 63) / 64, 64, 0, st>>>(np_, B->d_pseg, B->d_spart, s, which);
+}
+
 // One CG iteration batch (16 iterations) captured once per batch object.
 constexpr int kCgBatch = 16;
 
 int ensure_cg_graph(fem_batch* B) {
   if (B->cg_graph) return FEM_OK;
   fem_ctx* c = B->c;
-  const int np_ = (int)B->probs.size();
   const int64_t n = c->ndofs;
   BScal* s = (BScal*)B->d_scal;
   hipStream_t st = c->stream;
   hipGraph_t g = nullptr;
   FEM_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int it = 0; it < kCgBatch; ++it) {
-    k_b_spmv32<<<grid_rows4(n), 256, 0, st>>>(n, c->d_colptr, c->d_rowval, B->d_val32, B->d_p32, B->d_q32);
-    k_b_dot<float><<<np_, 256, 0, st>>>(B->d_pdof, B->d_p32, B->d_q32, s, kDotPQ);
+    k_b_spmv32<<<grid_rows4(n), 256, 0, st>>>(n, c->d_colptr, c->d_rowval, B->d_val32, B->d_p32, B->d_q32,
+                                              B->d_rowprob, s);
+    bdot<float>(B, st, B->d_p32, B->d_q32, B->d_p32, kDotPQ);
     k_b_update<float><<<grid_n(n), 256, 0, st>>>(n, B->d_rowprob, s, B->d_x32, B->d_r32, B->d_p32, B->d_q32,
                                                  B->d_invd);
-    k_b_dot<float><<<np_, 256, 0, st>>>(B->d_pdof, B->d_r32, B->d_q32, s, kDotRZRR);
+    bdot<float>(B, st, B->d_r32, B->d_q32, B->d_r32, kDotRZRR);
     k_b_pupd<float><<<grid_n(n), 256, 0, st>>>(n, B->d_rowprob, s, B->d_p32, B->d_q32);
   }
   FEM_HIP(c, hipStreamEndCapture(st, &g));
@@ -787,6 +808,7 @@ void fem_batch_destroy(fem_batch* B) {
   dev_free(&B->d_cnodes8); dev_free(&B->d_cdofs8); dev_free(&B->d_cnz); dev_free(&B->d_kslot);
   dev_free(&B->d_slotprob); dev_free(&B->d_items); dev_free(&B->d_done); dev_free(&B->d_q);
   dev_free(&B->d_probs); dev_free(&B->d_rowprob); dev_free(&B->d_pdof); dev_free(&B->d_tab); dev_free(&B->d_tabf);
+  dev_free(&B->d_sprob); dev_free(&B->d_srow); dev_free(&B->d_pseg); dev_free(&B->d_spart);
   dev_free(&B->d_m); dev_free(&B->d_val32); dev_free(&B->d_x32); dev_free(&B->d_r32); dev_free(&B->d_p32);
   dev_free(&B->d_q32); dev_free(&B->d_invd); dev_free(&B->d_res); dev_free(&B->d_scal);
   if (c) fem_ctx_destroy(c);
@@ -991,6 +1013,28 @@ int32_t fem_batch_finalize(fem_batch* B, const double* quad_N, const double* qua
     for (size_t i = 0; i < pres.size(); ++i) pres1[i] = (int64_t)pres[i] + 1;   // ascending by construction
     if ((rc = fem_constraints_upload(c, (int64_t)pres.size(), pres1.data(), g.data()))) return rc;
   }
+  // reduction segments: ≤ kSeg consecutive rows, never crossing a problem boundary
+  {
+    std::vector<int32_t> sprob, pseg(np_ + 1);
+    std::vector<int64_t> srow;
+    for (int p = 0; p < np_; ++p) {
+      pseg[p] = (int32_t)sprob.size();
+      for (int64_t r = pdof[p]; r < pdof[p + 1]; r += kSeg) {
+        sprob.push_back(p);
+        srow.push_back(r);
+      }
+    }
+    pseg[np_] = (int32_t)sprob.size();
+    srow.push_back(nd);
+    B->nseg = (int)sprob.size();
+    if ((rc = dev_alloc(c, &B->d_sprob, B->nseg)) || (rc = dev_alloc(c, &B->d_srow, B->nseg + 1)) ||
+        (rc = dev_alloc(c, &B->d_pseg, np_ + 1)) || (rc = dev_alloc(c, &B->d_spart, 2 * B->nseg)))
+      return rc;
+    FEM_HIP(c, hipMemcpyAsync(B->d_sprob, sprob.data(), 4 * sprob.size(), hipMemcpyHostToDevice, st));
+    FEM_HIP(c, hipMemcpyAsync(B->d_srow, srow.data(), 8 * srow.size(), hipMemcpyHostToDevice, st));
+    FEM_HIP(c, hipMemcpyAsync(B->d_pseg, pseg.data(), 4 * pseg.size(), hipMemcpyHostToDevice, st));
+    FEM_HIP(c, hipStreamSynchronize(st));
+  }
   B->nslots = ns;
   B->finalized = true;
   B->chunk = 0;
@@ -1055,7 +1099,7 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
   k_b_f32<<<grid_n(nnz), 256, 0, st>>>(nnz, c->d_nzval, B->d_val32);
   k_b_invd<<<grid_n(n), 256, 0, st>>>(n, c->d_diag, c->d_nzval, B->d_invd);
   k_b_init_scal<<<(np_ + 63) / 64, 64, 0, st>>>(np_, d_s);
-  k_b_dot<double><<<np_, 256, 0, st>>>(B->d_pdof, c->d_f, c->d_f, d_s, kDotF);
+  bdot<double>(B, st, c->d_f, c->d_f, c->d_f, kDotF);
   k_b_init<<<grid_n(n), 256, 0, st>>>(n, c->d_f, B->d_invd, B->d_r32, B->d_p32, B->d_q32, B->d_x32, c->d_u);
   std::vector<BScal> hs(np_);
   FEM_HIP(c, hipMemcpyAsync(hs.data(), d_s, sizeof(BScal) * np_, hipMemcpyDeviceToHost, st));
@@ -1063,7 +1107,7 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
   if ((rc = assemble_errors(B))) return rc;
   for (int p = 0; p < np_; ++p) hs[p].active = 1;   // r·z of every problem (p = z already)
   FEM_HIP(c, hipMemcpyAsync(d_s, hs.data(), sizeof(BScal) * np_, hipMemcpyHostToDevice, st));
-  k_b_reseed_dot<<<np_, 256, 0, st>>>(B->d_pdof, B->d_r32, B->d_q32, B->d_p32, d_s);
+  bdot<float>(B, st, B->d_r32, B->d_q32, B->d_p32, kDotReseed);
   FEM_HIP(c, hipMemcpyAsync(hs.data(), d_s, sizeof(BScal) * np_, hipMemcpyDeviceToHost, st));
   FEM_HIP(c, hipStreamSynchronize(st));
   std::vector<double> done_ms(np_, -1.0);
@@ -1111,10 +1155,10 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
     k_b_fold<<<grid_n(n), 256, 0, st>>>(n, B->d_rowprob, d_s, B->d_x32, c->d_u);
     k_b_resid64<<<grid_rows4(n), 256, 0, st>>>(n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_u, c->d_f,
                                                B->d_rowprob, d_s, B->d_res);
-    k_b_dot<double><<<np_, 256, 0, st>>>(B->d_pdof, B->d_res, B->d_res, d_s, kDotTrue);
+    bdot<double>(B, st, B->d_res, B->d_res, B->d_res, kDotTrue);
     k_b_check<<<(np_ + 63) / 64, 64, 0, st>>>(np_, d_s);
     k_b_reseed<<<grid_n(n), 256, 0, st>>>(n, B->d_rowprob, d_s, B->d_res, B->d_invd, B->d_r32, B->d_q32);
-    k_b_reseed_dot<<<np_, 256, 0, st>>>(B->d_pdof, B->d_r32, B->d_q32, B->d_p32, d_s);
+    bdot<float>(B, st, B->d_r32, B->d_q32, B->d_p32, kDotReseed);
     k_b_pfix<<<grid_n(n), 256, 0, st>>>(n, B->d_rowprob, d_s, B->d_q32, B->d_p32);
     FEM_HIP(c, hipEventRecord(B->e_mark, st));
     FEM_HIP(c, hipMemcpyAsync(hs.data(), d_s, sizeof(BScal) * np_, hipMemcpyDeviceToHost, st));
@@ -1137,6 +1181,7 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
   B->last_times_ms = done_ms;
   B->last_iters = iters;
   c->nzval32_dirty = true;
+  ++c->nzval_version;
   if (info) {
     info->assemble_ms = t_asm;
     info->apply_ms = t_apply;
@@ -1199,6 +1244,7 @@ int32_t fem_batch_assemble_only(fem_batch* B, int32_t precision, int32_t cells_p
   int rc = timed_assemble(B, cpw, ch, mixed, &t);
   if (ms) *ms = t;
   c->nzval32_dirty = true;
+  ++c->nzval_version;
   return rc;
 }
 

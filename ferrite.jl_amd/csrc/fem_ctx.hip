@@ -150,7 +150,9 @@ void fem_ctx_destroy(fem_ctx* c) {
   dev_free(&c->d_colptr); dev_free(&c->d_rowval); dev_free(&c->d_diag);
   dev_free(&c->d_nzval); dev_free(&c->d_nzval32);
   dev_free(&c->d_ccell); dev_free(&c->d_cnodes); dev_free(&c->d_cdofs); dev_free(&c->d_cnz);
-  dev_free(&c->d_cparam); dev_free(&c->d_f); dev_free(&c->d_u);
+  dev_free(&c->d_cparam); dev_free(&c->d_param_cell); dev_free(&c->d_f); dev_free(&c->d_u);
+  dev_free(&c->d_gcnodes); dev_free(&c->d_d2c_ptr); dev_free(&c->d_d2c); dev_free(&c->d_gslot);
+  dev_free(&c->d_keb64); dev_free(&c->d_feb64); dev_free(&c->d_keb32); dev_free(&c->d_feb32);
   dev_free(&c->d_pres); dev_free(&c->d_g); dev_free(&c->d_isc);
   dev_free(&c->d_scalar); dev_free(&c->d_partials); dev_free(&c->d_err);
   dev_free(&c->d_tab); dev_free(&c->d_tabf);
@@ -193,6 +195,7 @@ int32_t fem_mesh_upload(fem_ctx* c, int32_t sdim, int64_t nnodes, const double* 
   }
   c->sdim = sdim; c->nnodes = nnodes; c->ncells = ncells; c->npc = npc;
   c->tables_ready = false;
+  c->gather_ready = false;
   return FEM_OK;
 }
 
@@ -215,6 +218,7 @@ int32_t fem_dofs_upload(fem_ctx* c, int64_t ndofs, int32_t ndpc, const int64_t* 
   FEM_HIP(c, hipMemsetAsync(c->d_u, 0, sizeof(double) * ndofs, c->stream));
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   c->tables_ready = false;
+  c->gather_ready = false;
   return FEM_OK;
 }
 
@@ -255,6 +259,7 @@ int32_t fem_pattern_upload(fem_ctx* c, int64_t n, const int64_t* colptr, const i
   for (int64_t j = 0; j < n; ++j) c->max_row_nnz = std::max<int>(c->max_row_nnz, cp[j + 1] - cp[j]);
   free_solver(c);
   c->tables_ready = false;
+  c->gather_ready = false;
   return FEM_OK;
 }
 
@@ -309,14 +314,12 @@ int32_t fem_cellparam_upload(fem_ctx* c, int64_t ncells, const double* param) {
   if (!c || !param || ncells != c->ncells) return set_error(c, FEM_ERR_ARG, "fem_cellparam_upload: length must equal ncells");
   if (c->ncolors == 0) return set_error(c, FEM_ERR_STATE, "upload the coloring before cell parameters");
   int rc;
-  double* tmp = nullptr;
-  if ((rc = dev_alloc(c, &tmp, ncells))) return rc;
-  if ((rc = dev_alloc(c, &c->d_cparam, ncells))) { dev_free(&tmp); return rc; }
-  FEM_HIP(c, hipMemcpyAsync(tmp, param, sizeof(double) * ncells, hipMemcpyHostToDevice, c->stream));
-  k_gather_param<<<grid_for(ncells), 256, 0, c->stream>>>(ncells, c->d_ccell, tmp, c->d_cparam);
+  if ((rc = dev_alloc(c, &c->d_param_cell, ncells))) return rc;
+  if ((rc = dev_alloc(c, &c->d_cparam, ncells))) return rc;
+  FEM_HIP(c, hipMemcpyAsync(c->d_param_cell, param, sizeof(double) * ncells, hipMemcpyHostToDevice, c->stream));
+  k_gather_param<<<grid_for(ncells), 256, 0, c->stream>>>(ncells, c->d_ccell, c->d_param_cell, c->d_cparam);
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipStreamSynchronize(c->stream));
-  dev_free(&tmp);
   c->have_param = true;
   return FEM_OK;
 }

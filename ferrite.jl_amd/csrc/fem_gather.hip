@@ -1,0 +1,285 @@
+// Two-phase, owner-computes assembly of the scalar Q1 Poisson element (hex and quad).
+//
+// Replaces the same cell loop as the colored scatter (threaded_assembly.jl:232-265:
+// reinit! + element routine + assemble!), but writes every K.nzval and f entry exactly
+// once instead of read-modify-writing it once per incident cell:
+//
+//   phase 1  k_elem_poisson   one lane per cell (cell-id order): gather the cell's node
+//                             coordinates and k, reinit! + quadrature loop, store the upper
+//                             triangle of Ke and fe as planes Ke[t][cell], fe[a][cell]
+//                             (unit-stride stores across a wave).
+//   phase 2  k_gather_cols    one lane per column j of K (= dof j): for every cell that
+//                             holds dof j (ascending cell id) and every local row a, add
+//                             Ke[a, b_j] into the slot of row dofs[a] of column j, kept in
+//                             LDS; then the block writes its contiguous range of nzval (and
+//                             f) with unit-stride stores.
+//
+// Each nzval entry is summed in ascending cell order — the order of Ferrite's serial
+// loop — so the result is deterministic and independent of launch shape and colouring.
+// A row missing from column j's pattern is recorded (first cell) like assemble!'s error.
+//
+// Tables (built once per mesh/dofs/pattern): cell-order node ids, the dof → (cell, local
+// index) lists d2c (packed cell*NB + b), and for each list entry and local row a the slot
+// of dofs[a] inside column j (one byte; 255 = absent).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "fem_element.h"
+
+namespace fem {
+
+constexpr int kGatherBlock = 128;
+constexpr uint8_t kNoSlot = 255;
+
+template <typename T, int DIM, int NG, int NB, int NQ>
+__global__ __launch_bounds__(256) void k_elem_poisson(int64_t ncells, const double* __restrict__ X0,
+                                                      const double* __restrict__ X1, const double* __restrict__ X2,
+                                                      const int32_t* __restrict__ cnodes,
+                                                      const double* __restrict__ param, const T* __restrict__ tab,
+                                                      T src, T* __restrict__ keb, T* __restrict__ feb,
+                                                      DevError* __restrict__ err) {
+  constexpr int TRI = NB * (NB + 1) / 2;
+  for (int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cell < ncells;
+       cell += (int64_t)gridDim.x * blockDim.x) {
+    int32_t nd[NG];
+    if constexpr (NG == 8) {
+      const int4* p = reinterpret_cast<const int4*>(cnodes + cell * 8);
+      const int4 u0 = p[0], u1 = p[1];
+      nd[0] = u0.x; nd[1] = u0.y; nd[2] = u0.z; nd[3] = u0.w;
+      nd[4] = u1.x; nd[5] = u1.y; nd[6] = u1.z; nd[7] = u1.w;
+    } else if constexpr (NG == 4) {
+      const int4 u0 = *reinterpret_cast<const int4*>(cnodes + cell * 4);
+      nd[0] = u0.x; nd[1] = u0.y; nd[2] = u0.z; nd[3] = u0.w;
+    } else {
+#pragma unroll
+      for (int a = 0; a < NG; ++a) nd[a] = cnodes[cell * NG + a];
+    }
+    T x[NG][DIM];
+#pragma unroll
+    for (int a = 0; a < NG; ++a) {
+      x[a][0] = (T)X0[nd[a]];
+      x[a][1] = (T)X1[nd[a]];
+      if constexpr (DIM == 3) x[a][2] = (T)X2[nd[a]];
+    }
+    T ke[TRI], fe[NB];
+    const bool ok = poisson_cell<T, DIM, NG, NB, NQ>(x, (T)param[cell], src, tab, ke, fe);
+    if (!ok) atomicMin(&err->detj_cell, (int)cell);
+#pragma unroll
+    for (int t = 0; t < TRI; ++t) keb[t * ncells + cell] = ke[t];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) feb[a * ncells + cell] = fe[a];
+  }
+}
+
+// One lane per column; LDS slots acc[k][lane] for the column's entries k < MAXL.
+template <typename T, int NB, int MAXL>
+__global__ __launch_bounds__(kGatherBlock) void k_gather_cols(
+    int64_t n, int64_t ncells, const int32_t* __restrict__ colptr, const int32_t* __restrict__ d2c_ptr,
+    const int32_t* __restrict__ d2c, const uint8_t* __restrict__ slot, const T* __restrict__ keb,
+    const T* __restrict__ feb, double* __restrict__ nzval, double* __restrict__ f, int fillzero,
+    DevError* __restrict__ err) {
+  __shared__ double acc[MAXL * kGatherBlock];
+  __shared__ int32_t scp[kGatherBlock + 1];
+  const int tid = threadIdx.x;
+  const int64_t j0 = (int64_t)blockIdx.x * kGatherBlock;
+  const int ncol = (int)min<int64_t>(kGatherBlock, n - j0);
+  for (int t = tid; t <= ncol; t += kGatherBlock) scp[t] = colptr[j0 + t];
+  __syncthreads();
+  const int32_t base0 = scp[0], end0 = scp[ncol];
+  // start values: 0, or the current K when accumulating (start_assemble(...; fillzero=false))
+  if (fillzero) {
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) acc[k * kGatherBlock + tid] = 0.0;
+  } else {
+    for (int32_t q = base0 + tid; q < end0; q += kGatherBlock) {
+      int lo = 0, hi = ncol;   // column of entry q: largest c with scp[c] <= q
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (scp[mid] <= q) lo = mid; else hi = mid;
+      }
+      acc[(q - scp[lo]) * kGatherBlock + lo] = nzval[q];
+    }
+  }
+  __syncthreads();
+  double fj = 0.0;
+  if (tid < ncol) {
+    const int64_t j = j0 + tid;
+    if (!fillzero) fj = f[j];
+    const int32_t e0 = d2c_ptr[j], e1 = d2c_ptr[j + 1];
+    for (int32_t e = e0; e < e1; ++e) {
+      const int32_t pk = d2c[e];
+      const int64_t cell = pk / NB;
+      const int b = pk - (int)cell * NB;
+      uint8_t sl[NB];
+      if constexpr (NB == 8) {
+        const uint2 u = *reinterpret_cast<const uint2*>(slot + (int64_t)e * 8);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          sl[a] = (uint8_t)(u.x >> (8 * a));
+          sl[a + 4] = (uint8_t)(u.y >> (8 * a));
+        }
+      } else if constexpr (NB == 4) {
+        const uint32_t u = *reinterpret_cast<const uint32_t*>(slot + (int64_t)e * 4);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) sl[a] = (uint8_t)(u >> (8 * a));
+      } else {
+#pragma unroll
+        for (int a = 0; a < NB; ++a) sl[a] = slot[(int64_t)e * NB + a];
+      }
+      T v[NB];
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        v[a] = keb[(int64_t)tri_idx(lo, hi, NB) * ncells + cell];
+      }
+      const T fv = feb[(int64_t)b * ncells + cell];
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        if (sl[a] != kNoSlot) {
+          acc[sl[a] * kGatherBlock + tid] += (double)v[a];
+        } else if (v[a] != T(0)) {
+          atomicMin(&err->pattern_cell, (int)cell);
+        }
+      }
+      fj += (double)fv;
+    }
+    f[j] = fj;
+  }
+  __syncthreads();
+  // unit-stride write of the block's nzval range
+  for (int32_t q = base0 + tid; q < end0; q += kGatherBlock) {
+    int lo = 0, hi = ncol;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (scp[mid] <= q) lo = mid; else hi = mid;
+    }
+    nzval[q] = acc[(q - scp[lo]) * kGatherBlock + lo];
+  }
+}
+
+// slot[e][a] = position of dofs(cell_e)[a] in column j (0-based within the column), or 255.
+template <int NB>
+__global__ void k_build_slots(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
+                              const int32_t* __restrict__ d2c_ptr, const int32_t* __restrict__ d2c,
+                              const int32_t* __restrict__ cdofs, uint8_t* __restrict__ slot) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c0 = colptr[j], c1 = colptr[j + 1];
+    for (int32_t e = d2c_ptr[j]; e < d2c_ptr[j + 1]; ++e) {
+      const int64_t cell = d2c[e] / NB;
+      for (int a = 0; a < NB; ++a) {
+        const int32_t row = cdofs[cell * NB + a];
+        int32_t lo = c0, hi = c1;
+        while (lo < hi) {
+          const int32_t mid = (lo + hi) >> 1;
+          if (rowval[mid] < row) lo = mid + 1; else hi = mid;
+        }
+        const bool found = lo < c1 && rowval[lo] == row;
+        slot[(int64_t)e * NB + a] = found ? (uint8_t)(lo - c0) : kNoSlot;
+      }
+    }
+  }
+}
+
+static int gridn(int64_t n, int b = 256) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + b - 1) / b, 16384)); }
+
+// Cell-order node table, dof -> cell lists and the slot table (once per mesh/dofs/pattern).
+int build_gather_tables(fem_ctx* c) {
+  if (c->gather_ready) return FEM_OK;
+  const int64_t nc = c->ncells, nd = c->ndofs;
+  const int nb = c->ndpc;
+  if (nc == 0 || nd == 0 || c->n != nd) return set_error(c, FEM_ERR_STATE, "mesh, dofs and pattern must be uploaded");
+  if (nc * nb >= INT32_MAX || nc >= INT32_MAX / nb) return set_error(c, FEM_ERR_ARG, "mesh too large for the gather tables");
+  int rc;
+  if ((rc = dev_alloc(c, &c->d_gcnodes, nc * c->npc))) return rc;
+  FEM_HIP(c, hipMemcpyAsync(c->d_gcnodes, c->h_cell_nodes.data(), 4 * nc * c->npc, hipMemcpyHostToDevice, c->stream));
+  // dof -> (cell, local index), ascending cell id per dof (counting sort over the cell loop)
+  std::vector<int32_t> ptr(nd + 1, 0), ent(nc * nb);
+  for (int64_t i = 0; i < nc * nb; ++i) ++ptr[c->h_cell_dofs[i] + 1];
+  for (int64_t j = 0; j < nd; ++j) ptr[j + 1] += ptr[j];
+  std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t cell = 0; cell < nc; ++cell)
+    for (int b = 0; b < nb; ++b) {
+      const int32_t d = c->h_cell_dofs[cell * nb + b];
+      ent[fill[d]++] = (int32_t)(cell * nb + b);
+    }
+  int32_t* dcd = nullptr;
+  if ((rc = dev_alloc(c, &c->d_d2c_ptr, nd + 1)) || (rc = dev_alloc(c, &c->d_d2c, nc * nb)) ||
+      (rc = dev_alloc(c, &c->d_gslot, nc * (int64_t)nb * nb)) || (rc = dev_alloc(c, &dcd, nc * nb)))
+    return rc;
+  FEM_HIP(c, hipMemcpyAsync(c->d_d2c_ptr, ptr.data(), 4 * (nd + 1), hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipMemcpyAsync(c->d_d2c, ent.data(), 4 * nc * nb, hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipMemcpyAsync(dcd, c->h_cell_dofs.data(), 4 * nc * nb, hipMemcpyHostToDevice, c->stream));
+  if (nb == 8)
+    k_build_slots<8><<<gridn(nd), 256, 0, c->stream>>>(nd, c->d_colptr, c->d_rowval, c->d_d2c_ptr, c->d_d2c, dcd,
+                                                        c->d_gslot);
+  else if (nb == 4)
+    k_build_slots<4><<<gridn(nd), 256, 0, c->stream>>>(nd, c->d_colptr, c->d_rowval, c->d_d2c_ptr, c->d_d2c, dcd,
+                                                        c->d_gslot);
+  else {
+    dev_free(&dcd);
+    return set_error(c, FEM_ERR_ARG, "gather assembly supports 4 or 8 dofs per cell");
+  }
+  FEM_HIP(c, hipGetLastError());
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  dev_free(&dcd);
+  c->gather_ready = true;
+  return FEM_OK;
+}
+
+// True when the gather path handles this assembly (scalar Q1 Poisson with short columns).
+bool gather_supported(const fem_ctx* c) {
+  const bool hex = c->rdim == 3 && c->ngeo == 8 && c->nbf == 8 && c->nqp == 8 && c->npc == 8 && c->sdim == 3;
+  const bool quad = c->rdim == 2 && c->ngeo == 4 && c->nbf == 4 && c->nqp == 4 && c->npc == 4 && c->sdim == 2;
+  return (hex || quad) && c->vdim == 1 && c->ndpc == c->nbf && c->max_row_nnz <= 32;
+}
+
+template <typename T, int DIM, int NG, int NB, int NQ>
+static int run_gather(fem_ctx* c, const T* tab, double src, bool fillzero) {
+  constexpr int TRI = NB * (NB + 1) / 2;
+  const int64_t nc = c->ncells;
+  int rc;
+  T*& keb = sizeof(T) == 8 ? reinterpret_cast<T*&>(c->d_keb64) : reinterpret_cast<T*&>(c->d_keb32);
+  T*& feb = sizeof(T) == 8 ? reinterpret_cast<T*&>(c->d_feb64) : reinterpret_cast<T*&>(c->d_feb32);
+  int64_t& cap = sizeof(T) == 8 ? c->keb64_cells : c->keb32_cells;
+  if (!keb || cap != nc) {
+    if ((rc = dev_alloc(c, &keb, nc * TRI)) || (rc = dev_alloc(c, &feb, nc * NB))) return rc;
+    cap = nc;
+  }
+  FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+  k_elem_poisson<T, DIM, NG, NB, NQ><<<gridn(nc), 256, 0, c->stream>>>(
+      nc, c->d_x[0], c->d_x[1], c->d_x[2], c->d_gcnodes, c->d_param_cell, tab, (T)src, keb, feb, c->d_err);
+  const int nblk = (int)((c->n + kGatherBlock - 1) / kGatherBlock);
+  if (c->max_row_nnz <= 9)
+    k_gather_cols<T, NB, 9><<<nblk, kGatherBlock, 0, c->stream>>>(c->n, nc, c->d_colptr, c->d_d2c_ptr, c->d_d2c,
+                                                                  c->d_gslot, keb, feb, c->d_nzval, c->d_f,
+                                                                  fillzero ? 1 : 0, c->d_err);
+  else if (c->max_row_nnz <= 27)
+    k_gather_cols<T, NB, 27><<<nblk, kGatherBlock, 0, c->stream>>>(c->n, nc, c->d_colptr, c->d_d2c_ptr, c->d_d2c,
+                                                                   c->d_gslot, keb, feb, c->d_nzval, c->d_f,
+                                                                   fillzero ? 1 : 0, c->d_err);
+  else
+    k_gather_cols<T, NB, 32><<<nblk, kGatherBlock, 0, c->stream>>>(c->n, nc, c->d_colptr, c->d_d2c_ptr, c->d_d2c,
+                                                                   c->d_gslot, keb, feb, c->d_nzval, c->d_f,
+                                                                   fillzero ? 1 : 0, c->d_err);
+  FEM_HIP(c, hipGetLastError());
+  FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
+  c->timings.assemble_kernel_ms = event_ms(c);
+  c->timings.assemble_launches = 2;
+  return FEM_OK;
+}
+
+// Entry from fem_assemble (tables uploaded by the caller, error word reset).
+int gather_poisson(fem_ctx* c, bool mixed, double src, bool fillzero, const double* tab64, const float* tab32) {
+  int rc = build_gather_tables(c);
+  if (rc) return rc;
+  const bool hex = c->rdim == 3;
+  if (hex) {
+    return mixed ? run_gather<float, 3, 8, 8, 8>(c, tab32, src, fillzero)
+                 : run_gather<double, 3, 8, 8, 8>(c, tab64, src, fillzero);
+  }
+  return mixed ? run_gather<float, 2, 4, 4, 4>(c, tab32, src, fillzero)
+               : run_gather<double, 2, 4, 4, 4>(c, tab64, src, fillzero);
+}
+
+}  // namespace fem

@@ -71,6 +71,7 @@ struct fem_ctx {
   double* d_nzval = nullptr;
   float* d_nzval32 = nullptr;          // FP32 copy used by the mixed solver
   bool nzval32_dirty = true;           // nzval changed since the FP32 copy was made
+  uint64_t nzval_version = 1;          // bumped whenever nzval changes (solver operator copies track it)
 
   // colors: cells permuted into color-major order on the device
   int ncolors = 0;
@@ -82,9 +83,23 @@ struct fem_ctx {
   int32_t* d_cnz = nullptr;            // [ncells][ndpc*ndpc] nzval position, -1 = missing
   bool tables_ready = false;
 
-  // per-cell parameter, color order
+  // per-cell parameter, color order (d_cparam) and cell-id order (d_param_cell)
   double* d_cparam = nullptr;
+  double* d_param_cell = nullptr;
   bool have_param = false;
+
+  // owner-computes (gather) assembly: cell-order nodes, dof -> (cell, local) lists, slot
+  // table, element-matrix planes (FP64 and FP32 element math)
+  int32_t* d_gcnodes = nullptr;
+  int32_t* d_d2c_ptr = nullptr;
+  int32_t* d_d2c = nullptr;
+  uint8_t* d_gslot = nullptr;
+  double* d_keb64 = nullptr;
+  double* d_feb64 = nullptr;
+  float* d_keb32 = nullptr;
+  float* d_feb32 = nullptr;
+  int64_t keb64_cells = 0, keb32_cells = 0;
+  bool gather_ready = false;
 
   // cellvalues
   int rdim = 0, nqp = 0, nbf = 0, ngeo = 0, vdim = 1;
@@ -153,6 +168,9 @@ float event_ms(fem_ctx* c);
 
 int build_tables(fem_ctx* c);
 void free_solver(fem_ctx* c);
+int build_gather_tables(fem_ctx* c);
+bool gather_supported(const fem_ctx* c);
+int gather_poisson(fem_ctx* c, bool mixed, double src, bool fillzero, const double* tab64, const float* tab32);
 
 constexpr int kReduceBlocks = 1024;   // fixed partial count -> deterministic reductions
 

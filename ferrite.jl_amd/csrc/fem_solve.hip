@@ -53,8 +53,28 @@ struct GraphCache {
   const void* key[8] = {};
 };
 
+// Sliced ELL copy of K for the solver (slices of 64 rows = one wave): entry k of row
+// 64 s + l is stored at off[s] + 64 k + l, so a wave's loads of values and column
+// offsets are unit-stride.  Rows shorter than the slice width are padded with value 0
+// and the slice's smallest column.  The values are re-gathered from nzval (src map)
+// whenever K changes; the structure is built once per pattern.
+struct SellOp {
+  int64_t nslices = 0, len = 0;
+  int32_t* off = nullptr;      // [nslices + 1] first entry of each slice
+  int32_t* base = nullptr;     // [nslices] smallest column index of the slice
+  uint16_t* roff = nullptr;    // [len] column − base (when every slice spans < 65536 columns)
+  int32_t* col = nullptr;      // [len] column (otherwise)
+  int32_t* src = nullptr;      // [len] position in nzval, −1 for padding
+  float* val32 = nullptr;
+  double* val64 = nullptr;
+  uint64_t ver32 = 0, ver64 = 0;
+  bool built = false;
+};
+
 struct SolverWork {
   int64_t n = 0;
+  SellOp sell;
+  int sell_grid = 0;           // blocks of the SELL SpMV (fixed: its partial count)
   double *r = nullptr, *p = nullptr, *q = nullptr, *invd = nullptr, *res = nullptr;
   float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr, *invd32 = nullptr;
   double* part_s = nullptr;   // SpMV partials, one per row block
@@ -82,6 +102,9 @@ void free_solver(fem_ctx* c) {
   dev_free(&w->r32); dev_free(&w->p32); dev_free(&w->q32); dev_free(&w->x32); dev_free(&w->invd32);
   dev_free(&w->part_s); dev_free(&w->part_v); dev_free(&w->s);
   dev_free(&w->roff32); dev_free(&w->bbase32);
+  SellOp& o = w->sell;
+  dev_free(&o.off); dev_free(&o.base); dev_free(&o.roff); dev_free(&o.col); dev_free(&o.src);
+  dev_free(&o.val32); dev_free(&o.val64);
   delete w;
   c->solver = nullptr;
 }
@@ -240,6 +263,132 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, int R, const int32_t
   }
   const double b = block_sum(d, sh);
   if (tid == 0) part[blockIdx.x] = b;
+}
+
+// --------------------------------------------------------------------------
+// Sliced ELL (SELL-64) SpMV: one wave per slice of 64 rows, lane l owns row 64 s + l and
+// sums its entries in pattern order (deterministic).  Value and offset loads are
+// unit-stride across the wave; x is gathered.  Fixed grid (grid-stride over slices), so
+// the partial count is fixed.
+//   MODE 0: y = K x, partial Σ x·y ;  MODE 1: y = f − K x, partial Σ y²
+// --------------------------------------------------------------------------
+template <typename V, typename X, int MODE, bool COMP>
+__global__ __launch_bounds__(kBlock) void k_sell(int64_t n, int64_t nslices, const int32_t* __restrict__ off,
+                                                 const int32_t* __restrict__ base,
+                                                 const uint16_t* __restrict__ roff,
+                                                 const int32_t* __restrict__ col, const V* __restrict__ val,
+                                                 const X* __restrict__ x, const double* __restrict__ f,
+                                                 X* __restrict__ y, double* __restrict__ part,
+                                                 const Scal* __restrict__ s) {
+  using A = typename std::conditional<sizeof(X) == 8, double, float>::type;
+  if (MODE == 0 && s->done) return;
+  __shared__ double sh[kBlock / 64];
+  const int lane = threadIdx.x & 63;
+  double d = 0.0;
+  for (int64_t sl = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); sl < nslices;
+       sl += (int64_t)gridDim.x * (kBlock / 64)) {
+    const int32_t o0 = off[sl];
+    const int W = (off[sl + 1] - o0) >> 6;
+    const int32_t b = COMP ? base[sl] : 0;
+    const int64_t row = sl * 64 + lane;
+    const int32_t e0 = o0 + lane;
+    A acc = A(0);
+    int k = 0;
+    for (; k + 4 <= W; k += 4) {
+      V v[4];
+      int32_t ci[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int32_t e = e0 + 64 * (k + u);
+        v[u] = val[e];
+        ci[u] = COMP ? b + (int32_t)roff[e] : col[e];
+      }
+      X xv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xv[u] = x[ci[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc += (A)v[u] * (A)xv[u];
+    }
+    for (; k < W; ++k) {
+      const int32_t e = e0 + 64 * k;
+      const int32_t ci = COMP ? b + (int32_t)roff[e] : col[e];
+      acc += (A)val[e] * (A)x[ci];
+    }
+    if (row < n) {
+      if (MODE == 0) {
+        y[row] = (X)acc;
+        d += (double)x[row] * (double)(X)acc;
+      } else {
+        const double rj = f[row] - (double)acc;
+        y[row] = (X)rj;
+        d += rj * rj;
+      }
+    }
+  }
+  const double bsum = block_sum(d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = bsum;
+}
+
+// Slice widths, bases and column spans (one wave per slice).
+__global__ __launch_bounds__(kBlock) void k_sell_shape(int64_t n, int64_t nslices, const int32_t* __restrict__ colptr,
+                                                       const int32_t* __restrict__ rowval, int32_t* __restrict__ width,
+                                                       int32_t* __restrict__ base, int32_t* __restrict__ span) {
+  const int64_t sl = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (sl >= nslices) return;
+  const int64_t row = sl * 64 + (threadIdx.x & 63);
+  int len = 0, mn = INT_MAX, mx = INT_MIN;
+  if (row < n) {
+    const int32_t a = colptr[row], e = colptr[row + 1];
+    len = e - a;
+    if (len > 0) {
+      mn = rowval[a];          // rows are ascending within a column of the pattern
+      mx = rowval[e - 1];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    len = max(len, __shfl_down(len, o, 64));
+    mn = min(mn, __shfl_down(mn, o, 64));
+    mx = max(mx, __shfl_down(mx, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    width[sl] = len;
+    base[sl] = mn == INT_MAX ? 0 : mn;
+    span[sl] = mn == INT_MAX ? 0 : mx - mn;
+  }
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(kBlock) void k_sell_fill(int64_t n, int64_t nslices, const int32_t* __restrict__ colptr,
+                                                      const int32_t* __restrict__ rowval,
+                                                      const int32_t* __restrict__ off,
+                                                      const int32_t* __restrict__ base, uint16_t* __restrict__ roff,
+                                                      int32_t* __restrict__ col, int32_t* __restrict__ src) {
+  const int64_t sl = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (sl >= nslices) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t row = sl * 64 + lane;
+  const int32_t o0 = off[sl];
+  const int W = (off[sl + 1] - o0) >> 6;
+  const int32_t b = base[sl];
+  const int32_t a = row < n ? colptr[row] : 0;
+  const int len = row < n ? colptr[row + 1] - a : 0;
+  for (int k = 0; k < W; ++k) {
+    const int32_t e = o0 + 64 * k + lane;
+    const bool real = k < len;
+    const int32_t cidx = real ? rowval[a + k] : b;
+    if (COMP) roff[e] = (uint16_t)(cidx - b);
+    else col[e] = cidx;
+    src[e] = real ? a + k : -1;
+  }
+}
+
+template <typename V>
+__global__ void k_sell_vals(int64_t len, const int32_t* __restrict__ src, const double* __restrict__ nzval,
+                            V* __restrict__ val) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < len; e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t p = src[e];
+    val[e] = p >= 0 ? (V)nzval[p] : V(0);
+  }
 }
 
 // Per row block of the FP32 operator: base = smallest column index, offsets = index − base.
@@ -624,6 +773,120 @@ static int ensure_operator32(fem_ctx* c, SolverWork* w) {
   return FEM_OK;
 }
 
+static bool use_sell(const fem_ctx* c) { return c->tune.spmv_format != 1; }
+
+// SELL structure (once per pattern): widths/bases on the device, offsets on the host.
+static int ensure_sell(fem_ctx* c, SolverWork* w) {
+  SellOp& o = w->sell;
+  if (o.built) return FEM_OK;
+  const int64_t n = c->n;
+  const int64_t ns = (n + 63) / 64;
+  int32_t *dw = nullptr, *ds = nullptr;
+  int rc;
+  if ((rc = dev_alloc(c, &dw, ns)) || (rc = dev_alloc(c, &ds, ns)) || (rc = dev_alloc(c, &o.base, ns))) return rc;
+  const int sb = (int)((ns + kBlock / 64 - 1) / (kBlock / 64));
+  k_sell_shape<<<sb, kBlock, 0, c->stream>>>(n, ns, c->d_colptr, c->d_rowval, dw, o.base, ds);
+  std::vector<int32_t> hw(ns), hs(ns);
+  FEM_HIP(c, hipMemcpyAsync(hw.data(), dw, 4 * ns, hipMemcpyDeviceToHost, c->stream));
+  FEM_HIP(c, hipMemcpyAsync(hs.data(), ds, 4 * ns, hipMemcpyDeviceToHost, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  dev_free(&dw);
+  dev_free(&ds);
+  std::vector<int32_t> off(ns + 1);
+  int64_t acc = 0;
+  int32_t maxspan = 0;
+  for (int64_t i = 0; i < ns; ++i) {
+    off[i] = (int32_t)acc;
+    acc += 64 * (int64_t)hw[i];
+    maxspan = std::max(maxspan, hs[i]);
+    if (acc >= INT32_MAX) return set_error(c, FEM_ERR_ARG, "matrix too large for the sliced solver operator");
+  }
+  off[ns] = (int32_t)acc;
+  const bool comp = maxspan < 65536 && c->tune.spmv_compress32 >= 0;
+  if ((rc = dev_alloc(c, &o.off, ns + 1)) || (rc = dev_alloc(c, &o.src, acc))) return rc;
+  if (comp) rc = dev_alloc(c, &o.roff, acc);
+  else rc = dev_alloc(c, &o.col, acc);
+  if (rc) return rc;
+  FEM_HIP(c, hipMemcpyAsync(o.off, off.data(), 4 * (ns + 1), hipMemcpyHostToDevice, c->stream));
+  if (comp)
+    k_sell_fill<true><<<sb, kBlock, 0, c->stream>>>(n, ns, c->d_colptr, c->d_rowval, o.off, o.base, o.roff, nullptr,
+                                                     o.src);
+  else
+    k_sell_fill<false><<<sb, kBlock, 0, c->stream>>>(n, ns, c->d_colptr, c->d_rowval, o.off, o.base, nullptr, o.col,
+                                                      o.src);
+  FEM_HIP(c, hipGetLastError());
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  o.nslices = ns;
+  o.len = acc;
+  o.ver32 = o.ver64 = 0;
+  w->sell_grid = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxVecBlocks, sb));
+  o.built = true;
+  return FEM_OK;
+}
+
+// SELL values of the current K (re-gathered only when nzval changed).
+template <typename V>
+static int ensure_sell_vals(fem_ctx* c, SolverWork* w) {
+  SellOp& o = w->sell;
+  V*& val = sizeof(V) == 4 ? reinterpret_cast<V*&>(o.val32) : reinterpret_cast<V*&>(o.val64);
+  uint64_t& ver = sizeof(V) == 4 ? o.ver32 : o.ver64;
+  if (!val) {
+    int rc = dev_alloc(c, &val, o.len);
+    if (rc) return rc;
+    ver = 0;
+  }
+  if (ver != c->nzval_version) {
+    k_sell_vals<V><<<8192, kBlock, 0, c->stream>>>(o.len, o.src, c->d_nzval, val);
+    FEM_HIP(c, hipGetLastError());
+    ver = c->nzval_version;
+  }
+  return FEM_OK;
+}
+
+// The solver's SpMV in the active format; returns the number of partials it wrote.
+template <typename X, int MODE>
+static int spmv_any(fem_ctx* c, SolverWork* w, const X* x, const double* f, X* y) {
+  if (use_sell(c)) {
+    const SellOp& o = w->sell;
+    const int g = w->sell_grid;
+    if constexpr (sizeof(X) == 4) {
+      if (o.roff)
+        k_sell<float, float, MODE, true><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, o.roff, nullptr,
+                                                                      o.val32, x, f, y, w->part_s, w->s);
+      else
+        k_sell<float, float, MODE, false><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, nullptr, o.col,
+                                                                       o.val32, x, f, y, w->part_s, w->s);
+    } else {
+      if (o.roff)
+        k_sell<double, double, MODE, true><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, o.roff,
+                                                                        nullptr, o.val64, x, f, y, w->part_s, w->s);
+      else
+        k_sell<double, double, MODE, false><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, nullptr,
+                                                                         o.col, o.val64, x, f, y, w->part_s, w->s);
+    }
+    return g;
+  }
+  if constexpr (sizeof(X) == 4) {
+    launch_spmv<float, float, MODE>(c, w->sh32, c->d_nzval32, x, f, y, w->part_s, w->s, w->roff32, w->bbase32);
+    return w->sh32.blocks;
+  } else {
+    launch_spmv<double, double, MODE>(c, w->sh64, c->d_nzval, x, f, y, w->part_s, w->s);
+    return w->sh64.blocks;
+  }
+}
+
+// The FP32 or FP64 operator of the solver, ready for spmv_any.
+template <typename X>
+static int prepare_operator(fem_ctx* c, SolverWork* w) {
+  int rc;
+  if (use_sell(c)) {
+    if ((rc = ensure_sell(c, w))) return rc;
+    return ensure_sell_vals<X>(c, w);
+  }
+  if constexpr (sizeof(X) == 4) return ensure_operator32(c, w);
+  return FEM_OK;
+}
+
 static int read_scal(fem_ctx* c, SolverWork* w, Scal* h) {
   FEM_HIP(c, hipMemcpyAsync(h, w->s, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
   FEM_HIP(c, hipStreamSynchronize(c->stream));
@@ -631,23 +894,22 @@ static int read_scal(fem_ctx* c, SolverWork* w, Scal* h) {
 }
 
 // One CG iteration's launches (q = K p, α, update, β, p update).
-template <typename V, typename X>
-static void enqueue_iteration(fem_ctx* c, SolverWork* w, const SpmvShape& sh, const V* val, X* x, X* r, X* p, X* q,
-                              const X* invd, double tol2) {
-  if constexpr (sizeof(X) == 4) launch_spmv<V, X, 0>(c, sh, val, p, nullptr, q, w->part_s, w->s, w->roff32, w->bbase32);
-  else launch_spmv<V, X, 0>(c, sh, val, p, nullptr, q, w->part_s, w->s);
-  k_alpha<<<1, kWide, 0, c->stream>>>(sh.blocks, w->part_s, w->s);
+template <typename X>
+static void enqueue_iteration(fem_ctx* c, SolverWork* w, X* x, X* r, X* p, X* q, const X* invd, double tol2) {
+  const int parts = spmv_any<X, 0>(c, w, p, nullptr, q);
+  k_alpha<<<1, kWide, 0, c->stream>>>(parts, w->part_s, w->s);
   k_update<X><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, x, r, p, q, invd, w->part_v, w->s);
   k_beta<<<1, kWide, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2);
   k_pupd<X><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, p, q, w->s);
 }
 
 // The captured batch of kBatch iterations (rebuilt only when the buffers or the tolerance change).
-template <typename V, typename X>
-static int batch_graph(fem_ctx* c, SolverWork* w, const V* val, X* x, X* r, X* p, X* q, const X* invd, double tol2,
+template <typename X>
+static int batch_graph(fem_ctx* c, SolverWork* w, X* x, X* r, X* p, X* q, const X* invd, double tol2,
                        GraphCache& gc) {
-  const SpmvShape& sh = sizeof(X) == 8 ? w->sh64 : w->sh32;
   double tkey = tol2;
+  const void* val = use_sell(c) ? (sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64)
+                                : (sizeof(X) == 4 ? (const void*)c->d_nzval32 : (const void*)c->d_nzval);
   const void* key[8] = {val, x, r, p, q, invd, w->s, nullptr};
   std::memcpy(&key[7], &tkey, sizeof(double));
   bool same = gc.exec != nullptr;
@@ -656,7 +918,7 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const V* val, X* x, X* r, X* p
   destroy_graph(gc);
   hipGraph_t graph = nullptr;
   FEM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < kBatch; ++k) enqueue_iteration<V, X>(c, w, sh, val, x, r, p, q, invd, tol2);
+  for (int k = 0; k < kBatch; ++k) enqueue_iteration<X>(c, w, x, r, p, q, invd, tol2);
   FEM_HIP(c, hipStreamEndCapture(c->stream, &graph));
   hipError_t e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -665,7 +927,7 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const V* val, X* x, X* r, X* p
   return FEM_OK;
 }
 
-static void residual(fem_ctx* c, SolverWork* w);
+static void residual(fem_ctx* c, SolverWork* w, bool sell64);
 
 // FP64 preconditioned CG on K u = f from u = 0 (r holds f on entry).  When the recursive
 // residual claims convergence the true residual f − K u is computed; if it misses the
@@ -680,22 +942,25 @@ static int pcg64(fem_ctx* c, SolverWork* w, double tol2, int maxit, GraphCache& 
   k_init<double><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, r, invd, p, w->part_v);
   k_init_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2, 0.0);
   FEM_HIP(c, hipGetLastError());
-  int rc = batch_graph<double, double>(c, w, c->d_nzval, x, r, p, q, invd, tol2, gc);
+  int rc = batch_graph<double>(c, w, x, r, p, q, invd, tol2, gc);
   if (rc) return rc;
   Scal h{};
   int launched = 0, nrep = 0;
   bool converged = false;
+  double prev_true = -1.0;
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   while (launched < maxit) {
     FEM_HIP(c, hipGraphLaunch(gc.exec, c->stream));
     launched += kBatch;
     if ((rc = read_scal(c, w, &h))) return rc;
     if (!h.done) continue;
-    residual(c, w);
+    residual(c, w, use_sell(c));
     Scal ht{};
     if ((rc = read_scal(c, w, &ht))) return rc;
     if (ht.rr <= tol2) { converged = true; break; }
-    if (nrep++ >= kMaxReplace) break;
+    // the true residual no longer improves: the tolerance is below what FP64 attains here
+    if (nrep++ >= kMaxReplace || (prev_true >= 0.0 && ht.rr > 0.25 * prev_true)) break;
+    prev_true = ht.rr;
     k_reseed<double><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, w->res, invd, r, p, w->part_v);
     k_reseed_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, 0.0);
     k_pfix<double><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, r, invd, p, w->s);
@@ -708,10 +973,17 @@ static int pcg64(fem_ctx* c, SolverWork* w, double tol2, int maxit, GraphCache& 
   return converged ? FEM_OK : FEM_ERR_NOCONV;
 }
 
-static void residual(fem_ctx* c, SolverWork* w) {
-  // res = f − K u and s->rr = ||res||² (FP64, fixed reduction order)
-  launch_spmv<double, double, 1>(c, w->sh64, c->d_nzval, c->d_u, c->d_f, w->res, w->part_s, w->s);
-  k_sum_to<<<1, kWide, 0, c->stream>>>(w->sh64.blocks, w->part_s, w->s, 0);
+// res = f − K u and s->rr = ||res||² (FP64, fixed reduction order).  With sell64 the FP64
+// sliced copy is used (the FP64 solver has it); otherwise the CSC pattern itself.
+static void residual(fem_ctx* c, SolverWork* w, bool sell64) {
+  int parts;
+  if (sell64) {
+    parts = spmv_any<double, 1>(c, w, c->d_u, c->d_f, w->res);
+  } else {
+    launch_spmv<double, double, 1>(c, w->sh64, c->d_nzval, c->d_u, c->d_f, w->res, w->part_s, w->s);
+    parts = w->sh64.blocks;
+  }
+  k_sum_to<<<1, kWide, 0, c->stream>>>(parts, w->part_s, w->s, 0);
 }
 
 }  // namespace fem
@@ -746,6 +1018,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   const double tol2 = cfg->rtol * cfg->rtol * bnorm2;
 
   if (!mixed) {
+    if ((rc = prepare_operator<double>(c, w))) return rc;
     k_invdiag<double><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd);
     k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_u);
     k_copy<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->r);
@@ -755,7 +1028,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
     out.outer_iterations = 1 + replaced;
     if (rc && rc != FEM_ERR_NOCONV) return rc;
   } else {
-    if ((rc = ensure_operator32(c, w))) return rc;
+    if ((rc = prepare_operator<float>(c, w))) return rc;
     k_invdiag<float><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd32);
     // Mixed CG with reliable updates: one FP32 Krylov sequence (FP32 SpMV and vectors,
     // FP64 dot products); whenever its recursive residual has dropped by δ since the last
@@ -769,8 +1042,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
     k_f32_from<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->r32);
     k_init<float><<<w->vblocks, kBlock, 0, c->stream>>>(n, w->r32, w->invd32, w->p32, w->part_v);
     k_init_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2, delta * delta);
-    if ((rc = batch_graph<float, float>(c, w, c->d_nzval32, w->x32, w->r32, w->p32, w->q32, w->invd32, tol2, w->g32)))
-      return rc;
+    if ((rc = batch_graph<float>(c, w, w->x32, w->r32, w->p32, w->q32, w->invd32, tol2, w->g32))) return rc;
     double maxrr = bnorm2;
     int launched = 0, refinements = 0;
     bool converged = false;
@@ -783,7 +1055,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
       if (hs.done || hs.rr < delta * delta * maxrr) {
         ++refinements;
         k_fold<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, w->x32, c->d_u);
-        residual(c, w);
+        residual(c, w, false);
         Scal ht{};
         if ((rc = read_scal(c, w, &ht))) return rc;
         if (ht.rr <= tol2) { converged = true; break; }
@@ -800,7 +1072,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
     out.iterations = hf.iters;
     out.outer_iterations = refinements;
   }
-  residual(c, w);
+  residual(c, w, !mixed && use_sell(c));
   FEM_HIP(c, hipGetLastError());
   if ((rc = read_scal(c, w, &h))) return rc;
   out.relres = std::sqrt(h.rr / bnorm2);
@@ -826,17 +1098,17 @@ extern "C" int32_t fem_spmv_bench(fem_ctx* c, int32_t mixed, int32_t reps, doubl
   Scal zero{};
   FEM_HIP(c, hipMemcpyAsync(w->s, &zero, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
   if (mixed) {
-    if ((rc = ensure_operator32(c, w))) return rc;
+    if ((rc = prepare_operator<float>(c, w))) return rc;
     k_to_f32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p32);
   } else {
+    if ((rc = prepare_operator<double>(c, w))) return rc;
     k_copy<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p);
   }
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < reps; ++i) {
-    if (mixed) launch_spmv<float, float, 0>(c, w->sh32, c->d_nzval32, w->p32, nullptr, w->q32, w->part_s, w->s,
-                                            w->roff32, w->bbase32);
-    else launch_spmv<double, double, 0>(c, w->sh64, c->d_nzval, w->p, nullptr, w->q, w->part_s, w->s);
+    if (mixed) spmv_any<float, 0>(c, w, w->p32, nullptr, w->q32);
+    else spmv_any<double, 0>(c, w, w->p, nullptr, w->q);
   }
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -849,7 +1121,8 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
   auto ok_cap = [](int v) { return v == 0 || v == 2048 || v == 4096; };
   auto ok_l = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4; };
   if (!ok_cap(t->spmv_cap64) || !ok_cap(t->spmv_cap32) || !ok_l(t->spmv_lanes64) || !ok_l(t->spmv_lanes32) ||
-      !(t->asm_block == 0 || t->asm_block == 64 || t->asm_block == 128 || t->asm_block == 256))
+      !(t->asm_block == 0 || t->asm_block == 64 || t->asm_block == 128 || t->asm_block == 256) ||
+      !(t->spmv_format == 0 || t->spmv_format == 1) || !(t->asm_mode >= 0 && t->asm_mode <= 2))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

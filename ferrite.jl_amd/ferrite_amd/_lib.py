@@ -57,7 +57,7 @@ class SolveInfo(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("spmv_cap64", C.c_int32), ("spmv_lanes64", C.c_int32), ("spmv_cap32", C.c_int32),
                 ("spmv_lanes32", C.c_int32), ("spmv_compress32", C.c_int32), ("asm_block", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("spmv_format", C.c_int32), ("asm_mode", C.c_int32)]
 
 
 class Timings(C.Structure):

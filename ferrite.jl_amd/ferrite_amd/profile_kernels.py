@@ -64,6 +64,11 @@ def main():
             r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
             print(f"phase solve {mode}: {r.iterations} iterations, relres {r.relres:.2e}", flush=True)
     if args.sweep:
+        fa.set_tuning(K, spmv_format=0)
+        a64 = fa.spmv_bench(K, mixed=False, reps=20)
+        a32 = fa.spmv_bench(K, mixed=True, reps=20)
+        print(f"sweep spmv sell: fp64 {a64:.4f} ms, fp32+u16 {a32:.4f} ms", flush=True)
+        fa.set_tuning(K, spmv_format=1)
         for cap in (2048, 4096):
             for lanes in (1, 2, 4):
                 fa.set_tuning(K, spmv_cap64=cap, spmv_lanes64=lanes, spmv_cap32=cap, spmv_lanes32=lanes,
@@ -74,6 +79,17 @@ def main():
                 a32u = fa.spmv_bench(K, mixed=True, reps=20)
                 print(f"sweep spmv cap {cap} lanes {lanes}: fp64 {a64:.4f} ms, fp32+u16 {a32:.4f} ms, "
                       f"fp32+i32 {a32u:.4f} ms", flush=True)
+        fa.set_tuning(K, spmv_format=0, spmv_compress32=0)
+        for mode, name in ((2, "gather"), (1, "colored scatter")):
+            fa.set_tuning(K, asm_mode=mode)
+            ts = []
+            for prec in ("fp64", "mixed"):
+                fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=prec)
+                fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=prec)
+                ts.append(fa.timings(K)["assemble_kernel_ms"] + fa.timings(K)["fill_ms"])
+            print(f"sweep assembly {name}: fp64 {ts[0]:.3f} ms, fp32-element {ts[1]:.3f} ms (incl. fill)",
+                  flush=True)
+        fa.set_tuning(K, asm_mode=1)
         for bs in (64, 128, 256):
             fa.set_tuning(K, asm_block=bs)
             ts = []
@@ -83,7 +99,7 @@ def main():
             print(f"sweep assembly block {bs}: fp64 {ts[0]:.3f} ms, fp32-element {ts[1]:.3f} ms", flush=True)
         for alg in ("greedy",):
             cols = fa.create_coloring(g, alg)
-            fa.set_tuning(K, asm_block=0)
+            fa.set_tuning(K, asm_block=0, asm_mode=1)
             fa.assemble_global(fa.start_assemble(K, f), cv, cols, el, precision="fp64")
             print(f"sweep coloring {alg} ({len(cols)} colors): fp64 {fa.timings(K)['assemble_kernel_ms']:.3f} ms",
                   flush=True)

@@ -115,20 +115,22 @@ int32_t fem_apply_dirichlet(fem_ctx* ctx, int32_t applyzero, double* m_out);
 /* ---- solve (replaces u = K \ f, heat_equation.jl:218; the reference has no CG) ---- */
 enum fem_solve_mode {
   FEM_SOLVE_CG_FP64 = 0,  /* CG, FP64 SpMV and vectors */
-  FEM_SOLVE_IR_MIXED = 1  /* FP64 iterative refinement around an FP32-SpMV inner CG */
+  FEM_SOLVE_IR_MIXED = 1  /* mixed CG: FP32 SpMV and vectors, FP64 dot products, FP64 reliable
+                             updates (u += x32, r = f − K u with the FP64 K) */
 };
 typedef struct {
   int32_t mode;        /* fem_solve_mode */
   int32_t maxit;       /* total CG iterations allowed */
   double rtol;         /* ‖f − K u‖₂ / ‖f‖₂ target (FP64 residual) */
-  double inner_rtol;   /* IR inner solve reduction (0 -> default 1e-5) */
+  double inner_rtol;   /* mixed: reliable update each time the FP32 residual fell by this factor
+                          (0 -> 0.1) */
   int32_t precond;     /* 0 = none, 1 = Jacobi */
   int32_t reserved;
 } fem_solve_cfg;
 
 typedef struct {
   int32_t iterations;        /* total CG iterations */
-  int32_t outer_iterations;  /* IR refinement steps (1 for plain CG) */
+  int32_t outer_iterations;  /* mixed: reliable updates; FP64: 1 + residual replacements */
   double relres;             /* final true relative residual ‖f − K u‖/‖f‖ in FP64 */
 } fem_solve_info;
 
@@ -150,7 +152,10 @@ typedef struct {
                                         0 = default (on when every row block spans < 65536 columns),
                                         -1 = off */
   int32_t asm_block;                 /* cells per workgroup of the element kernels (0 = default) */
-  int32_t reserved[2];
+  int32_t spmv_format;               /* solver operator: 0 = sliced ELL (64-row slices, default),
+                                        1 = CSR-stream over the CSC pattern (the cap/lanes above) */
+  int32_t asm_mode;                  /* scalar Q1 Poisson assembly: 0 = owner-computes gather when
+                                        supported (default), 1 = colored scatter, 2 = gather only */
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
 int32_t fem_get_tuning(const fem_ctx* ctx, fem_tuning* t);
@@ -169,7 +174,8 @@ int32_t fem_get_timings(const fem_ctx* ctx, fem_timings* t);
  * FP64 (mixed = 0) or FP32 values and vectors (mixed = 1).  The solver's dominant kernel. */
 int32_t fem_spmv_bench(fem_ctx* ctx, int32_t mixed, int32_t reps, double* avg_ms);
 
-/* ---- compliance adjoint (no counterpart in the reference; BASELINE differentiation config) ----
+/* ---- compliance adjoint (no counterpart in the reference; the compliance of
+ *      topology_optimization.jl:320-383 with the SIMP modulus E = E_min + θ^p (1 − E_min)) ----
  * J = fᵀu, dJ/dθ_e = −p θ_e^{p−1}(1−E_min) u_eᵀ K⁰_e u_e, dJ/dX by recomputing Ke per cell.
  * Requires a solved system (fem_solve) assembled with FEM_KERNEL_ELASTICITY. */
 int32_t fem_adjoint_compliance(fem_ctx* ctx, const double* theta, double emin, double penal,

@@ -20,12 +20,16 @@ def _relerr(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def _device_poisson(ct, nel, left, right, k, bc_sets, alg="workstream", precision="fp64", q=2):
+ASM = {"gather": 2, "scatter": 1}
+
+
+def _device_poisson(ct, nel, left, right, k, bc_sets, alg="workstream", precision="fp64", q=2, asm="gather"):
     g = fa.generate_grid(ct, nel, left, right)
     ip = fa.Lagrange(g.refshape, 1)
     cv = fa.CellValues(fa.QuadratureRule(g.refshape, q), ip)
     dh = fa.DofHandler(g).add("u", ip).close()
     K = fa.allocate_matrix(fa.MI355XCSC, dh)
+    fa.set_tuning(K, asm_mode=ASM[asm])
     f = fa.allocate_vector(K)
     colors = fa.create_coloring(g, alg)
     A = fa.start_assemble(K, f)
@@ -92,13 +96,15 @@ def test_c1_quad_100_parity():
     assert res.relres <= 1e-12
 
 
+@pytest.mark.parametrize("asm", ["gather", "scatter"])
 @pytest.mark.parametrize("alg", ["workstream", "greedy"])
-def test_hex_poisson_random_k_parity(alg):
+def test_hex_poisson_random_k_parity(alg, asm):
     """3D Poisson with k_e = exp(N(0, 0.25)), left/right Dirichlet, 6x5x4 hexes."""
-    print("case hex 6x5x4", alg, flush=True)
+    print("case hex 6x5x4", alg, asm, flush=True)
     nel = (6, 5, 4)
     k = poisson_conductivity(nel, seed=7)
-    g, dh, K, f, ch = _device_poisson("Hexahedron", nel, (0, 0, 0), (1, 1, 1), k, ["left", "right"], alg=alg)
+    g, dh, K, f, ch = _device_poisson("Hexahedron", nel, (0, 0, 0), (1, 1, 1), k, ["left", "right"], alg=alg,
+                                      asm=asm)
     grid, cv, odh, OK, of, och = setup_poisson("Hexahedron", nel, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0),
                                               bc_sets=["left", "right"])
     assemble_poisson(grid, cv, odh, OK, of, k_cell=k)
@@ -112,21 +118,23 @@ def test_hex_poisson_random_k_parity(alg):
         assert _relerr(res.u, u_ref) < 1e-8, mode
 
 
-def test_mixed_element_path_hex():
-    print("case mixed hex", flush=True)
+@pytest.mark.parametrize("asm", ["gather", "scatter"])
+def test_mixed_element_path_hex(asm):
+    print("case mixed hex", asm, flush=True)
     nel = (5, 4, 3)
     k = poisson_conductivity(nel, seed=3)
     g, dh, K, f, ch = _device_poisson("Hexahedron", nel, (0, 0, 0), (1, 1, 1), k, ["left", "right"],
-                                      precision="mixed")
+                                      precision="mixed", asm=asm)
     grid, cv, odh, OK, of, och = setup_poisson("Hexahedron", nel, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0),
                                               bc_sets=["left", "right"])
     assemble_poisson(grid, cv, odh, OK, of, k_cell=k)
     _check_K(K, f, OK, of, 2e-6)
 
 
-def test_assembly_deterministic_and_fillzero():
+@pytest.mark.parametrize("asm", ["gather", "scatter"])
+def test_assembly_deterministic_and_fillzero(asm):
     """Run-twice bitwise equality (threaded_assembly.jl:398-403) and start_assemble(fillzero)."""
-    print("case determinism", flush=True)
+    print("case determinism", asm, flush=True)
     nel = (9, 7, 5)
     k = poisson_conductivity(nel, seed=11)
     g = fa.generate_grid("Hexahedron", nel, (0, 0, 0), (1, 1, 1))
@@ -134,6 +142,7 @@ def test_assembly_deterministic_and_fillzero():
     cv = fa.CellValues(fa.QuadratureRule("RefHexahedron", 2), ip)
     dh = fa.DofHandler(g).add("u", ip).close()
     K = fa.allocate_matrix(fa.MI355XCSC, dh)
+    fa.set_tuning(K, asm_mode=ASM[asm])
     f = fa.allocate_vector(K)
     colors = fa.create_coloring(g)
     el = fa.PoissonElement(k=k)
@@ -145,11 +154,17 @@ def test_assembly_deterministic_and_fillzero():
     fa.assemble_global(fa.start_assemble(K, f, fillzero=False), cv, colors, el)
     a3, f3 = K.nzval(), f.to_numpy()
     assert np.allclose(a3, 2 * a1, rtol=1e-15, atol=0) and np.allclose(f3, 2 * f1, rtol=1e-15, atol=0)
+    # the other assembly strategy sums in another order: equal to rounding
+    fa.set_tuning(K, asm_mode=ASM["scatter" if asm == "gather" else "gather"])
+    fa.assemble_global(fa.start_assemble(K, f), cv, colors, el)
+    a4 = K.nzval()
+    assert np.abs(a4 - a1).max() <= 1e-14 * np.abs(a1).max()
 
 
-def test_inverted_cell_raises_detj():
+@pytest.mark.parametrize("asm", ["gather", "scatter"])
+def test_inverted_cell_raises_detj(asm):
     """det(J) <= 0 -> ArgumentError (common_values.jl:5), reported with the cell id."""
-    print("case detJ error", flush=True)
+    print("case detJ error", asm, flush=True)
     g = fa.generate_grid("Quadrilateral", (4, 4))
     c = 6
     g.cells[c - 1] = g.cells[c - 1][[1, 0, 3, 2]]  # mirror cell 6
@@ -157,13 +172,15 @@ def test_inverted_cell_raises_detj():
     cv = fa.CellValues(fa.QuadratureRule("RefQuadrilateral", 2), ip)
     dh = fa.DofHandler(g).add("u", ip).close()
     K = fa.allocate_matrix(fa.MI355XCSC, dh)
-    with pytest.raises(fa.FemArgumentError, match="det"):
+    fa.set_tuning(K, asm_mode=ASM[asm])
+    with pytest.raises(fa.FemArgumentError, match="det\\(J\\) is not positive in cell 6"):
         fa.assemble_global(fa.start_assemble(K), cv, fa.create_coloring(g), fa.PoissonElement())
 
 
-def test_missing_pattern_entry_raises():
+@pytest.mark.parametrize("asm", ["gather", "scatter"])
+def test_missing_pattern_entry_raises(asm):
     """A coupling absent from the pattern with a nonzero value -> the reference's ErrorException text."""
-    print("case missing pattern entry", flush=True)
+    print("case missing pattern entry", asm, flush=True)
     import ctypes as C
     from ferrite_amd import _lib as L
     g = fa.generate_grid("Quadrilateral", (3, 3))
@@ -186,6 +203,7 @@ def test_missing_pattern_entry_raises():
     ctx.call("fem_dofs_upload", dh.ndofs, 4, L.ptr(cd, C.c_int64))
     ctx.call("fem_pattern_upload", dh.ndofs, L.ptr(L.as_i64(cp2), C.c_int64), L.ptr(L.as_i64(rv2), C.c_int64))
     K = fa.device.DeviceMatrix(dh, ctx, cp2, rv2)
+    fa.set_tuning(K, asm_mode=ASM[asm])
     with pytest.raises(fa.FemPatternError, match="missing in the sparsity pattern"):
         fa.assemble_global(fa.start_assemble(K), cv, fa.create_coloring(g), fa.PoissonElement())
 
