@@ -26,6 +26,9 @@ for p in (ROOT, os.path.join(ROOT, "ferrite.jl_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the roofline kernel at this workload, from rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE runs calibrated against a 1 GiB device copy (see the profile summary it names)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_q1hex_128.json")
 ASSEMBLY_BYTES_PER_CELL = 1508.6   # Q1 hex FP64 colored scatter, algorithmic
 
 
@@ -41,6 +44,19 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
+
+
+def measured_traffic(kernel, n):
+    """Calibrated HBM bytes per launch of `kernel` at n^3, if a committed PMC summary holds them."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    e = t.get(kernel.split(" ")[0])
+    if not e or e.get("n") != n:
+        return None
+    return e.get("bytes_per_launch")
 
 
 def log(*a):
@@ -81,7 +97,7 @@ def main():
         t_asm = fa.timings(K)
         fa.apply(K, f, ch)
         res = fa.solve(K, f, mode=mode, rtol=args.rtol, download=False)
-        return res, t_asm
+        return res, t_asm, fa.timings(K)
 
     results = {}
     for name, mode in (("fp64", "cg"), ("mixed", "mixed")):
@@ -92,15 +108,24 @@ def main():
         iters = []
         asm_ms = []
         for _ in range(args.steps):
-            res, ta = step(mode)
+            res, ta, ts = step(mode)
             iters.append((res.iterations, res.outer_iterations, res.relres))
             asm_ms.append(ta["assemble_kernel_ms"])
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t1) / args.steps
         u = fa.device.DeviceVector(K, 2).to_numpy()
+        # one more step with the CG kernels launched directly and every SpMV bracketed by HIP
+        # events on the context stream: the SpMV's launch duration inside the solve (same
+        # kernels and operands as the timed graphs), used for the roofline below
+        fa.set_tuning(K, time_kernels=1)
+        _, _, ts = step(mode)
+        fa.set_tuning(K, time_kernels=0)
+        spmv_ms, spmv_n = ts["spmv_ms"], ts["spmv_launches"]
         results[name] = dict(ms=dt * 1e3, cells_per_s=g.ncells / dt, iters=iters, u=u,
-                             asm_kernel_ms=float(np.median(asm_ms)))
-        log(f"{name}: {dt * 1e3:.1f} ms/step, iters {iters[-1]}, assembly kernels {np.median(asm_ms):.3f} ms")
+                             asm_kernel_ms=float(np.median(asm_ms)), spmv_avg_ms=spmv_ms / max(spmv_n, 1),
+                             spmv_launches=spmv_n)
+        log(f"{name}: {dt * 1e3:.1f} ms/step, iters {iters[-1]}, assembly kernels {np.median(asm_ms):.3f} ms, "
+            f"SpMV {spmv_ms / max(spmv_n, 1) * 1e3:.1f} us avg over {spmv_n} launches")
 
     uerr = float(np.linalg.norm(results["mixed"]["u"] - results["fp64"]["u"]) / np.linalg.norm(results["fp64"]["u"]))
 
@@ -120,14 +145,16 @@ def main():
 
     # roofline of the dominant kernel: the FP32 SpMV of the mixed solver (algorithmic bytes of
     # y = K x with int32 indices, whatever the stored format: values + indices + colptr + x + y)
-    spmv_ms = fa.spmv_bench(K, mixed=True, reps=50)
+    spmv_iso_ms = fa.spmv_bench(K, mixed=True, reps=50)   # back to back, matrix partly cache-resident
+    spmv_ms = results["mixed"]["spmv_avg_ms"]               # inside the CG loop of the timed steps
     SPMV32_KERNEL = ("k_sell<float,float,0,true> (mixed-solver SpMV, sliced ELL, 16-bit column offsets)"
                      if fa.get_tuning(K)["spmv_format"] == 0 else
                      "k_spmv<float,float,0,2048,4,true> (mixed-solver SpMV, CSR-stream)")
     nnz, nd = K.nnz, dh.ndofs
     spmv_bytes = nnz * (4 + 4) + 4 * (nd + 1) + 4 * nd + 4 * nd   # values, int32 indices, colptr, x, y
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-    spmv64_ms = fa.spmv_bench(K, mixed=False, reps=50)
+    spmv64_iso_ms = fa.spmv_bench(K, mixed=False, reps=50)
+    spmv64_ms = results["fp64"]["spmv_avg_ms"]
     spmv64_bytes = nnz * (8 + 4) + 4 * (nd + 1) + 8 * nd + 8 * nd
     asm_ms = results["fp64"]["asm_kernel_ms"]
     asm_gbs = ASSEMBLY_BYTES_PER_CELL * g.ncells / (asm_ms * 1e-3) / 1e9
@@ -165,9 +192,11 @@ def main():
         "dofs_per_s": dh.ndofs / (results["mixed"]["ms"] * 1e-3),
         "roofline": {"bound": "hbm", "kernel": SPMV32_KERNEL,
                      "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms},
-        "spmv_fp64": {"avg_launch_ms": spmv64_ms, "bytes_per_launch": spmv64_bytes,
+                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(SPMV32_KERNEL, n),
+                     "bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms,
+                     "launches": results["mixed"]["spmv_launches"], "isolated_avg_launch_ms": spmv_iso_ms},
+        "spmv_fp64": {"avg_launch_ms": spmv64_ms, "isolated_avg_launch_ms": spmv64_iso_ms,
+                      "bytes_per_launch": spmv64_bytes,
                       "achieved_GBs": spmv64_bytes / (spmv64_ms * 1e-3) / 1e9},
         "assembly_fp64": {"kernel_ms": asm_ms, "bytes_per_cell": ASSEMBLY_BYTES_PER_CELL,
                           "achieved_GBs": asm_gbs, "frac": asm_gbs / HBM_PEAK_GBS,

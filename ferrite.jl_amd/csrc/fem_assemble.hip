@@ -43,6 +43,8 @@ __global__ __launch_bounds__(BLOCK) void k_poisson(
   constexpr int TRI = NB * (NB + 1) / 2;
   __shared__ T s_ke[BLOCK * TRI];
   __shared__ T s_fe[BLOCK * NB];
+  __shared__ T stab[TabLayout<DIM, NG, NB, NQ>::size];
+  stage_table<T, TabLayout<DIM, NG, NB, NQ>::size>(stab, tab);
 
   const int64_t sb = s0 + (int64_t)blockIdx.x * BLOCK;
   const int ncell = (int)min<int64_t>(BLOCK, s1 - sb);
@@ -60,7 +62,7 @@ __global__ __launch_bounds__(BLOCK) void k_poisson(
     }
     T ke[TRI];
     T fe[NB];
-    const bool bad = !poisson_cell<T, DIM, NG, NB, NQ>(x, (T)cparam[s], src, tab, ke, fe);
+    const bool bad = !poisson_cell<T, DIM, NG, NB, NQ>(x, (T)cparam[s], src, stab, ke, fe);
     if (bad) atomicMin(&err->detj_cell, ccell[s]);
 #pragma unroll
     for (int i = 0; i < TRI; ++i) s_ke[lc * TRI + i] = ke[i];

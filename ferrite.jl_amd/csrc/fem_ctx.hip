@@ -255,6 +255,7 @@ int32_t fem_pattern_upload(fem_ctx* c, int64_t n, const int64_t* colptr, const i
     if (!found) return set_error(c, FEM_ERR_ARG, "pattern lacks a diagonal entry");
   }
   c->n = n; c->nnz = nnz;
+  c->pattern_sym = -1;
   c->max_row_nnz = 0;
   for (int64_t j = 0; j < n; ++j) c->max_row_nnz = std::max<int>(c->max_row_nnz, cp[j + 1] - cp[j]);
   free_solver(c);

@@ -40,12 +40,48 @@ __global__ __launch_bounds__(256) void k_absdiag_partial(int64_t n, const int32_
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ void k_finish_mean(int nparts, const double* __restrict__ partial, int64_t n,
-                              double* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < nparts; ++i) s += partial[i];
-    out[0] = s / (double)n;
+// One block: fixed-order sum of the partials, then the mean.
+__global__ __launch_bounds__(256) void k_finish_mean(int nparts, const double* __restrict__ partial, int64_t n,
+                                                     double* __restrict__ out) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) acc += partial[i];
+  const double s = block_sum<256>(acc, sh);
+  if (threadIdx.x == 0) out[0] = s / (double)n;
+}
+
+// Structurally symmetric pattern: zero column d and row d of every prescribed d directly
+// (row d's entries are found in the columns r listed in column d, by binary search).
+__global__ void k_zero_prescribed(int64_t np, const int32_t* __restrict__ pres, const int32_t* __restrict__ colptr,
+                                  const int32_t* __restrict__ rowval, double* __restrict__ nzval) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t d = pres[i];
+    for (int32_t p = colptr[d]; p < colptr[d + 1]; ++p) {
+      nzval[p] = 0.0;
+      const int32_t r = rowval[p];
+      int32_t lo = colptr[r], hi = colptr[r + 1];
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (rowval[mid] < d) lo = mid + 1; else hi = mid;
+      }
+      if (lo < colptr[r + 1] && rowval[lo] == d) nzval[lo] = 0.0;
+    }
+  }
+}
+
+// flag = 0 if some entry (r, j) has no transposed entry (j, r) in the pattern.
+__global__ void k_check_symmetric(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
+                                  int* __restrict__ flag) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    for (int32_t p = colptr[j]; p < colptr[j + 1]; ++p) {
+      const int32_t r = rowval[p];
+      int32_t lo = colptr[r], hi = colptr[r + 1];
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (rowval[mid] < (int32_t)j) lo = mid + 1; else hi = mid;
+      }
+      if (!(lo < colptr[r + 1] && rowval[lo] == (int32_t)j)) atomicExch(flag, 0);
+    }
   }
 }
 
@@ -153,10 +189,25 @@ int32_t fem_apply_dirichlet(fem_ctx* c, int32_t applyzero, double* m_out) {
   if (!c->d_isc) return set_error(c, FEM_ERR_STATE, "upload constraints first");
   FEM_HIP(c, hipSetDevice(c->device));
   const int64_t n = c->n;
+  if (c->pattern_sym < 0) {   // once per pattern
+    int* dflag = nullptr;
+    int rc = dev_alloc(c, &dflag, 1);
+    if (rc) return rc;
+    const int one = 1;
+    int h = 0;
+    FEM_HIP(c, hipMemcpyAsync(dflag, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    k_check_symmetric<<<grid_of(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, dflag);
+    FEM_HIP(c, hipMemcpyAsync(&h, dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    dev_free(&dflag);
+    c->pattern_sym = h;
+  }
+  if (!applyzero && c->any_inhom && c->pattern_sym != 1)
+    return set_error(c, FEM_ERR_ARG, "apply! with inhomogeneous Dirichlet values needs a structurally symmetric pattern");
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   const int nparts = 1024;
   k_absdiag_partial<<<nparts, 256, 0, c->stream>>>(n, c->d_diag, c->d_nzval, c->d_partials);
-  k_finish_mean<<<1, 64, 0, c->stream>>>(nparts, c->d_partials, n, c->d_scalar);
+  k_finish_mean<<<1, 256, 0, c->stream>>>(nparts, c->d_partials, n, c->d_scalar);
   if (!applyzero && c->any_inhom) {
     double* gfull = nullptr;
     int rc = dev_alloc(c, &gfull, n);
@@ -167,7 +218,13 @@ int32_t fem_apply_dirichlet(fem_ctx* c, int32_t applyzero, double* m_out) {
     FEM_HIP(c, hipStreamSynchronize(c->stream));
     dev_free(&gfull);
   }
-  k_zero_rows_cols<<<grid_of(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, c->d_isc, c->d_nzval);
+  if (c->pattern_sym == 1) {
+    if (c->np > 0)
+      k_zero_prescribed<<<grid_of(c->np), 256, 0, c->stream>>>(c->np, c->d_pres, c->d_colptr, c->d_rowval,
+                                                                 c->d_nzval);
+  } else {
+    k_zero_rows_cols<<<grid_of(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, c->d_isc, c->d_nzval);
+  }
   if (c->np > 0)
     k_set_diag<<<grid_of(c->np), 256, 0, c->stream>>>(c->np, c->d_pres, c->d_diag, c->d_g, c->d_scalar, applyzero,
                                                         c->d_nzval, c->d_f);

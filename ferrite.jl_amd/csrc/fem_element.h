@@ -51,6 +51,8 @@ constexpr __host__ __device__ int tri_idx(int lo, int hi, int nb) {
 //   per quadrature point: J = Σ_j x_j ⊗ dMdξ_j (GeometryMapping.jl:124-130), det J > 0
 //   (CellValues.jl:110-116), dNdx = dNdξ ⋅ J⁻¹ (FunctionValues.jl:186-192), then
 //   Ke[a,b] += k (∇N_a·∇N_b) dΩ (upper triangle) and fe[a] += src N_a dΩ.
+// ∇N_a·∇N_b dΩ is formed as dNdξ_aᵀ A dNdξ_b with A = dΩ J⁻¹J⁻ᵀ (symmetric), which needs
+// no per-point array of pushed-forward gradients.  `tab` is best an LDS copy of the table.
 // Returns false if some det J <= 0.
 template <typename T, int DIM, int NG, int NB, int NQ>
 __device__ __forceinline__ bool poisson_cell(const T (&x)[NG][DIM], T k, T src, const T* __restrict__ tab,
@@ -62,7 +64,8 @@ __device__ __forceinline__ bool poisson_cell(const T (&x)[NG][DIM], T k, T src, 
 #pragma unroll
   for (int i = 0; i < NB; ++i) fe[i] = T(0);
   bool ok = true;
-#pragma unroll
+  // one quadrature point at a time (not unrolled): only that point's table entries are live
+#pragma unroll 1
   for (int q = 0; q < NQ; ++q) {
     T J[DIM][DIM];
 #pragma unroll
@@ -79,31 +82,47 @@ __device__ __forceinline__ bool poisson_cell(const T (&x)[NG][DIM], T k, T src, 
     const T detJ = det_inv<T, DIM>(J, Ji);
     ok &= detJ > T(0);
     const T dV = detJ * tab[L::ow + q];
-    T g[NB][DIM];
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int j = 0; j < DIM; ++j) {
-        T acc = T(0);
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) acc += tab[L::odN + (q * NB + b) * DIM + i] * Ji[i][j];
-        g[b][j] = acc;
-      }
     const T kdV = k * dV;
     const T sdV = src * dV;
+    T A[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+      for (int l = i; l < DIM; ++l) {
+        T acc = Ji[i][0] * Ji[l][0];
+#pragma unroll
+        for (int m = 1; m < DIM; ++m) acc += Ji[i][m] * Ji[l][m];
+        A[i][l] = kdV * acc;
+        A[l][i] = A[i][l];
+      }
 #pragma unroll
     for (int a = 0; a < NB; ++a) {
       fe[a] += tab[L::oN + q * NB + a] * sdV;
+      T wa[DIM];
+#pragma unroll
+      for (int i = 0; i < DIM; ++i) {
+        T acc = A[i][0] * tab[L::odN + (q * NB + a) * DIM];
+#pragma unroll
+        for (int j = 1; j < DIM; ++j) acc += A[i][j] * tab[L::odN + (q * NB + a) * DIM + j];
+        wa[i] = acc;
+      }
 #pragma unroll
       for (int b = a; b < NB; ++b) {
-        T dot = g[a][0] * g[b][0];
+        T dot = tab[L::odN + (q * NB + b) * DIM] * wa[0];
 #pragma unroll
-        for (int j = 1; j < DIM; ++j) dot += g[a][j] * g[b][j];
-        ke[tri_idx(a, b, NB)] += dot * kdV;
+        for (int i = 1; i < DIM; ++i) dot += tab[L::odN + (q * NB + b) * DIM + i] * wa[i];
+        ke[tri_idx(a, b, NB)] += dot;
       }
     }
   }
   return ok;
+}
+
+// Copies the packed reference table into LDS (all threads of the block take part).
+template <typename T, int N>
+__device__ __forceinline__ void stage_table(T* __restrict__ dst, const T* __restrict__ src) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
 }
 
 }  // namespace fem

@@ -40,6 +40,9 @@ __global__ __launch_bounds__(256) void k_elem_poisson(int64_t ncells, const doub
                                                       T src, T* __restrict__ keb, T* __restrict__ feb,
                                                       DevError* __restrict__ err) {
   constexpr int TRI = NB * (NB + 1) / 2;
+  using L = TabLayout<DIM, NG, NB, NQ>;
+  __shared__ T stab[L::size];
+  stage_table<T, L::size>(stab, tab);
   for (int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cell < ncells;
        cell += (int64_t)gridDim.x * blockDim.x) {
     int32_t nd[NG];
@@ -63,7 +66,7 @@ __global__ __launch_bounds__(256) void k_elem_poisson(int64_t ncells, const doub
       if constexpr (DIM == 3) x[a][2] = (T)X2[nd[a]];
     }
     T ke[TRI], fe[NB];
-    const bool ok = poisson_cell<T, DIM, NG, NB, NQ>(x, (T)param[cell], src, tab, ke, fe);
+    const bool ok = poisson_cell<T, DIM, NG, NB, NQ>(x, (T)param[cell], src, stab, ke, fe);
     if (!ok) atomicMin(&err->detj_cell, (int)cell);
 #pragma unroll
     for (int t = 0; t < TRI; ++t) keb[t * ncells + cell] = ke[t];
@@ -82,7 +85,13 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_cols(
   __shared__ double acc[MAXL * kGatherBlock];
   __shared__ int32_t scp[kGatherBlock + 1];
   const int tid = threadIdx.x;
-  const int64_t j0 = (int64_t)blockIdx.x * kGatherBlock;
+  // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2): give every XCD
+  // one contiguous range of column blocks, so neighbouring columns, which read the same
+  // cells' element planes, share an L2.  gridDim.x is a multiple of 8.
+  const int64_t per = gridDim.x / 8;
+  const int64_t lb = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  const int64_t j0 = lb * kGatherBlock;
+  if (j0 >= n) return;
   const int ncol = (int)min<int64_t>(kGatherBlock, n - j0);
   for (int t = tid; t <= ncol; t += kGatherBlock) scp[t] = colptr[j0 + t];
   __syncthreads();
@@ -249,7 +258,7 @@ static int run_gather(fem_ctx* c, const T* tab, double src, bool fillzero) {
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   k_elem_poisson<T, DIM, NG, NB, NQ><<<gridn(nc), 256, 0, c->stream>>>(
       nc, c->d_x[0], c->d_x[1], c->d_x[2], c->d_gcnodes, c->d_param_cell, tab, (T)src, keb, feb, c->d_err);
-  const int nblk = (int)((c->n + kGatherBlock - 1) / kGatherBlock);
+  const int nblk = (int)(((c->n + kGatherBlock - 1) / kGatherBlock + 7) / 8 * 8);   // multiple of 8
   if (c->max_row_nnz <= 9)
     k_gather_cols<T, NB, 9><<<nblk, kGatherBlock, 0, c->stream>>>(c->n, nc, c->d_colptr, c->d_d2c_ptr, c->d_d2c,
                                                                   c->d_gslot, keb, feb, c->d_nzval, c->d_f,

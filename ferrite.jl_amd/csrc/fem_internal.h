@@ -65,6 +65,7 @@ struct fem_ctx {
   // pattern (CSC, 0-based, int32 indices; nnz < 2^31)
   int64_t n = 0, nnz = 0;
   int max_row_nnz = 0;                 // longest column of the pattern
+  int pattern_sym = -1;                // structurally symmetric pattern: -1 unknown, 0 no, 1 yes
   int32_t* d_colptr = nullptr;
   int32_t* d_rowval = nullptr;
   int32_t* d_diag = nullptr;           // position of K[j,j] in column j (-1 if absent)

@@ -31,14 +31,19 @@ constexpr int kMaxVecBlocks = 2048;
 constexpr int kBatch = 16;           // CG iterations per graph replay
 
 struct Scal {
-  double rz, pq, rr, alpha, beta, bnorm2;
+  double rz;      // γ = r·z (z = D⁻¹ r) of the current residual
+  double pq;      // p·A p at a reseed
+  double rr;      // r·r (recursive residual; after residual(): the true residual)
+  double alpha;   // step length of the last step
+  double beta;
+  double bnorm2;
   double hold2;   // stop the recurrence (done = 1) once rr < hold2: hand over to the host
-  double rzp;     // r·z of the step before the hand-over (β of the rebuilt direction)
+  double rzp;     // γ of the previous step
   double gam;     // (r·p)/(r·z) at a reseed: p is made orthogonal to the new residual
   int done;       // 1: recurrence stopped (converged by its own residual, or handed over)
   int iters;
-  int restart;    // p·Kp <= 0 happened: the next reseed restarts p = z
-  int pad;
+  int restart;    // breakdown happened: the next reseed restarts p = z
+  int fresh;      // p, s = A p and α were just set by a reseed: the next step only updates x, r
 };
 
 struct SpmvShape {
@@ -75,11 +80,14 @@ struct SolverWork {
   int64_t n = 0;
   SellOp sell;
   int sell_grid = 0;           // blocks of the SELL SpMV (fixed: its partial count)
-  double *r = nullptr, *p = nullptr, *q = nullptr, *invd = nullptr, *res = nullptr;
-  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr, *invd32 = nullptr;
+  // CG vectors (FP64 solver / FP32 recurrence of the mixed solver): q holds w = A u
+  double *r = nullptr, *p = nullptr, *q = nullptr, *u = nullptr, *sv = nullptr, *invd = nullptr, *res = nullptr;
+  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *u32 = nullptr, *sv32 = nullptr, *x32 = nullptr,
+        *invd32 = nullptr;
   double* part_s = nullptr;   // SpMV partials, one per row block
-  double* part_v = nullptr;   // vector-kernel partials, 2 per block
-  Scal* s = nullptr;
+  double* part_v = nullptr;   // vector-kernel partials, two buffers of 2 per block
+  Scal* s = nullptr;          // two scalar records (read one, write the other)
+  hipEvent_t tev[2 * kBatch] = {};   // SpMV start/end events inside the graph (time_kernels)
   SpmvShape sh64, sh32;
   int vblocks = 0;
   GraphCache g64, g32;
@@ -98,8 +106,12 @@ void free_solver(fem_ctx* c) {
   SolverWork* w = c->solver;
   destroy_graph(w->g64);
   destroy_graph(w->g32);
-  dev_free(&w->r); dev_free(&w->p); dev_free(&w->q); dev_free(&w->invd); dev_free(&w->res);
-  dev_free(&w->r32); dev_free(&w->p32); dev_free(&w->q32); dev_free(&w->x32); dev_free(&w->invd32);
+  for (hipEvent_t& e : w->tev)
+    if (e) (void)hipEventDestroy(e);
+  dev_free(&w->r); dev_free(&w->p); dev_free(&w->q); dev_free(&w->u); dev_free(&w->sv); dev_free(&w->invd);
+  dev_free(&w->res);
+  dev_free(&w->r32); dev_free(&w->p32); dev_free(&w->q32); dev_free(&w->u32); dev_free(&w->sv32); dev_free(&w->x32);
+  dev_free(&w->invd32);
   dev_free(&w->part_s); dev_free(&w->part_v); dev_free(&w->s);
   dev_free(&w->roff32); dev_free(&w->bbase32);
   SellOp& o = w->sell;
@@ -421,140 +433,6 @@ __global__ __launch_bounds__(kBlock) void k_compress_rows(int64_t n, int R, cons
   for (int32_t e = e0 + threadIdx.x; e < e1; e += kBlock) roff[e] = (uint16_t)min(rowval[e] - mn, 65535);
 }
 
-// α = rz / (p·q)
-__global__ __launch_bounds__(kWide) void k_alpha(int m, const double* __restrict__ part, Scal* __restrict__ s) {
-  if (s->done) return;
-  __shared__ double sh[kWide / 64];
-  const double pq = wide_sum(part, m, 1, sh);
-  if (threadIdx.x == 0) {
-    s->pq = pq;
-    // p·Kp > 0 for SPD K unless p vanished: stop and let the host restart from the true residual
-    const bool ok = pq > 0.0;
-    s->alpha = ok ? s->rz / pq : 0.0;
-    if (!ok) {
-      s->done = 1;
-      s->restart = 1;
-    }
-  }
-}
-
-// x += α p ; r -= α q ; z = D⁻¹ r (into q) ; partials r·z, r·r
-template <typename X>
-__global__ __launch_bounds__(kBlock) void k_update(int64_t n, X* __restrict__ x, X* __restrict__ r,
-                                                   const X* __restrict__ p, X* q, const X* __restrict__ invd,
-                                                   double* __restrict__ part, const Scal* __restrict__ s) {
-  if (s->done) return;
-  __shared__ double sh[kBlock / 64];
-  const X alpha = (X)s->alpha;
-  double rz = 0.0, rr = 0.0;
-  for (int64_t base = (int64_t)blockIdx.x * kTile; base < n; base += (int64_t)gridDim.x * kTile) {
-    X pv[4], qv[4], xv[4], rv[4], dv[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int64_t i = base + k * kBlock + threadIdx.x;
-      const bool ok = i < n;
-      pv[k] = ok ? p[i] : X(0);
-      qv[k] = ok ? q[i] : X(0);
-      xv[k] = ok ? x[i] : X(0);
-      rv[k] = ok ? r[i] : X(0);
-      dv[k] = ok ? invd[i] : X(0);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int64_t i = base + k * kBlock + threadIdx.x;
-      if (i < n) {
-        const X ri = rv[k] - alpha * qv[k];
-        const X zi = ri * dv[k];
-        x[i] = xv[k] + alpha * pv[k];
-        r[i] = ri;
-        q[i] = zi;
-        rz += (double)ri * (double)zi;
-        rr += (double)ri * (double)ri;
-      }
-    }
-  }
-  double b = block_sum(rz, sh);
-  if (threadIdx.x == 0) part[2 * blockIdx.x] = b;
-  b = block_sum(rr, sh);
-  if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = b;
-}
-
-// β = rz_new / rz ; convergence flag
-__global__ __launch_bounds__(kWide) void k_beta(int m, const double* __restrict__ part, Scal* __restrict__ s,
-                                                double tol2) {
-  if (s->done) return;
-  __shared__ double sh[kWide / 64];
-  const double rzn = wide_sum(part, m, 2, sh);
-  const double rrn = wide_sum(part + 1, m, 2, sh);
-  if (threadIdx.x == 0) {
-    s->beta = s->rz != 0.0 ? rzn / s->rz : 0.0;
-    s->rzp = s->rz;
-    s->rz = rzn;
-    s->rr = rrn;
-    s->iters += 1;
-    // converged by the recursive residual, or (mixed) dropped far enough for a reliable
-    // update: the FP32 recurrence must not run on towards underflow.  p is not updated in
-    // this iteration (it stays the last search direction, orthogonal to the recursive r).
-    if (rrn <= tol2 || rrn < s->hold2) s->done = 1;
-  }
-}
-
-// p = z + β p
-template <typename X>
-__global__ __launch_bounds__(kBlock) void k_pupd(int64_t n, X* __restrict__ p, const X* __restrict__ z,
-                                                 const Scal* __restrict__ s) {
-  if (s->done) return;
-  const X beta = (X)s->beta;
-  for (int64_t base = (int64_t)blockIdx.x * kTile; base < n; base += (int64_t)gridDim.x * kTile) {
-    X zv[4], pv[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int64_t i = base + k * kBlock + threadIdx.x;
-      zv[k] = i < n ? z[i] : X(0);
-      pv[k] = i < n ? p[i] : X(0);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int64_t i = base + k * kBlock + threadIdx.x;
-      if (i < n) p[i] = zv[k] + beta * pv[k];
-    }
-  }
-}
-
-// z = D⁻¹ r, p = z ; partials r·z, r·r
-template <typename X>
-__global__ __launch_bounds__(kBlock) void k_init(int64_t n, const X* __restrict__ r, const X* __restrict__ invd,
-                                                 X* __restrict__ p, double* __restrict__ part) {
-  __shared__ double sh[kBlock / 64];
-  double rz = 0.0, rr = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const X ri = r[i];
-    const X zi = ri * invd[i];
-    p[i] = zi;
-    rz += (double)ri * (double)zi;
-    rr += (double)ri * (double)ri;
-  }
-  double b = block_sum(rz, sh);
-  if (threadIdx.x == 0) part[2 * blockIdx.x] = b;
-  b = block_sum(rr, sh);
-  if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = b;
-}
-
-__global__ __launch_bounds__(kBlock) void k_init_scal(int m, const double* __restrict__ part, Scal* __restrict__ s,
-                                                      double tol2, double hold_factor2) {
-  __shared__ double sh[kBlock / 64];
-  const double rz = sum_parts(part, m, 2, sh);
-  const double rr = sum_parts(part + 1, m, 2, sh);
-  if (threadIdx.x == 0) {
-    s->rz = rz;
-    s->rr = rr;
-    s->iters = 0;
-    s->restart = 0;
-    s->hold2 = hold_factor2 * rr;
-    s->done = (rr <= tol2) ? 1 : 0;
-  }
-}
-
 // s->rr (which = 0) or s->bnorm2 (which = 1) = Σ part, fixed order
 __global__ __launch_bounds__(kWide) void k_sum_to(int m, const double* __restrict__ part, Scal* __restrict__ s,
                                                   int which) {
@@ -597,59 +475,280 @@ __global__ void k_fold(int64_t n, float* __restrict__ x32, double* __restrict__ 
   }
 }
 
-// Reliable update, step 2: r32 = res (the FP64 true residual), partials of r32·D r32.
-// Reliable update, step 2: r = res (the FP64 true residual, rounded to X), partials of
-// r·D⁻¹r and r·p.
+// --------------------------------------------------------------------------
+// Chronopoulos–Gear preconditioned CG (one reduction phase per iteration).
+//   state: x, r, u = D⁻¹ r, w = A u, p, s = A p
+//   step i (after the SpMV w = A u, partial δ = w·u):
+//     γ = r·u, rr = r·r (partials of the previous step), β = γ/γ_prev,
+//     α = γ / (δ − β γ / α_prev), p = u + β p, s = w + β s, x += α p, r −= α s,
+//     u = D⁻¹ r, partials of the next γ and rr.
+// Every block reduces the partials itself (same fixed order in every block, hence the
+// same scalars bitwise), so an iteration is two kernels: SpMV, k_cg_step.  The scalar
+// records and the vector partials are double-buffered (read one, write the other).
+// --------------------------------------------------------------------------
+constexpr int kCgBlocks = 2048;  // most blocks of k_cg_step (its partial count)
+
+// Fixed-order sum of part[0, m) (stride apart) by the whole block; the result in every thread.
+// Each thread issues its first 16 loads before adding them, so for m <= 16 kBlock the whole
+// sum costs about one memory latency.
+__device__ __forceinline__ double block_sum_all(const double* __restrict__ part, int m, int stride, double* sh) {
+  constexpr int kPer = 16;
+  double t[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int i = k * kBlock + threadIdx.x;
+    t[k] = i < m ? part[(int64_t)i * stride] : 0.0;
+  }
+  double v = 0.0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) v += t[k];
+  for (int i = kPer * kBlock + threadIdx.x; i < m; i += kBlock) v += part[(int64_t)i * stride];   // m > 16 kBlock
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < kBlock / 64; ++i) r += sh[i];
+  __syncthreads();
+  return r;
+}
+
+// γ = Σ pin[2i], rr = Σ pin[2i+1] (i < nin) and δ = Σ ps[i] (i < na) in one round: all
+// loads are issued before any add; fixed order, the same in every block and thread.
+__device__ __forceinline__ void reduce3(const double* __restrict__ pin, int nin, const double* __restrict__ ps,
+                                        int na, double (*sh)[kBlock / 64], double& gam, double& rr, double& del) {
+  constexpr int kPin = 8, kPs = 8;   // up to 2048 pairs and 2048 SpMV partials in flight
+  double a[kPin], b[kPin], d[kPs];
+#pragma unroll
+  for (int k = 0; k < kPin; ++k) {
+    const int i = k * kBlock + threadIdx.x;
+    a[k] = i < nin ? pin[2 * i] : 0.0;
+    b[k] = i < nin ? pin[2 * i + 1] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < kPs; ++k) {
+    const int i = k * kBlock + threadIdx.x;
+    d[k] = i < na ? ps[i] : 0.0;
+  }
+  double va = 0.0, vb = 0.0, vd = 0.0;
+#pragma unroll
+  for (int k = 0; k < kPin; ++k) { va += a[k]; vb += b[k]; }
+#pragma unroll
+  for (int k = 0; k < kPs; ++k) vd += d[k];
+  for (int i = kPin * kBlock + threadIdx.x; i < nin; i += kBlock) { va += pin[2 * i]; vb += pin[2 * i + 1]; }
+  for (int i = kPs * kBlock + threadIdx.x; i < na; i += kBlock) vd += ps[i];
+  va = wave_sum(va);
+  vb = wave_sum(vb);
+  vd = wave_sum(vd);
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = va;
+    sh[1][threadIdx.x >> 6] = vb;
+    sh[2][threadIdx.x >> 6] = vd;
+  }
+  __syncthreads();
+  gam = rr = del = 0.0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    gam += sh[0][w];
+    rr += sh[1][w];
+    del += sh[2][w];
+  }
+}
+
+constexpr int kCgElems = 8;   // vector elements per thread per pass of k_cg_step
+
+// Blocks of k_cg_step for n unknowns: one pass over the vectors when possible.
+static int cg_blocks(int64_t n) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kCgBlocks, (n + kBlock * kCgElems - 1) / (kBlock * kCgElems)));
+}
+
 template <typename X>
-__global__ __launch_bounds__(kBlock) void k_reseed(int64_t n, const double* __restrict__ res,
-                                                   const X* __restrict__ invd, X* __restrict__ r,
-                                                   const X* __restrict__ p, double* __restrict__ part) {
+__global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, X* __restrict__ x, X* __restrict__ r,
+                                                    X* __restrict__ u, const X* __restrict__ w, X* __restrict__ p,
+                                                    X* __restrict__ sv, const X* __restrict__ invd, int na,
+                                                    const double* __restrict__ part_s, const double* __restrict__ part_in,
+                                                    double* __restrict__ part_out, const Scal* __restrict__ sin,
+                                                    Scal* __restrict__ sout, double tol2) {
+  __shared__ double sh[3][kBlock / 64];
+  const Scal S = *sin;
+  if (S.done) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *sout = S;
+    return;
+  }
+  const bool fresh = S.fresh != 0;
+  constexpr int E = kCgElems;
+  const int64_t stride = (int64_t)gridDim.x * kBlock * E;
+  int64_t base = (int64_t)blockIdx.x * kBlock * E;
+  // this thread's first elements: their loads do not depend on the scalars, so they are in
+  // flight while the partials are reduced
+  X uv[E], wv[E], pv[E], sw[E], xv[E], rv[E], dv[E];
+  auto load = [&](int64_t b0) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int64_t i = b0 + k * kBlock + threadIdx.x;
+      const bool in = i < n;
+      uv[k] = in ? u[i] : X(0);
+      wv[k] = in ? w[i] : X(0);
+      pv[k] = in ? p[i] : X(0);
+      sw[k] = in ? sv[i] : X(0);
+      xv[k] = in ? x[i] : X(0);
+      rv[k] = in ? r[i] : X(0);
+      dv[k] = in ? invd[i] : X(0);
+    }
+  };
+  load(base);
+  double gam = S.rz, rr = S.rr, alpha = S.alpha, beta = 0.0;
+  if (!fresh) {
+    double del;
+    reduce3(part_in, gridDim.x, part_s, na, sh, gam, rr, del);
+    beta = gam / S.rzp;
+    const double den = del - beta * gam / S.alpha;
+    // converged (by the recurrence), or dropped below the hand-over level: the iterations
+    // after that point drift from the true residual and are lost at the reliable update
+    const bool stop = rr <= tol2 || rr < S.hold2;
+    const bool brk = !(S.rzp > 0.0) || !(S.alpha != 0.0) || !(den > 0.0);
+    if (stop || brk) {   // hand over to the host: x, r, u of this step, p and s of the last one
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Scal o = S;
+        o.done = 1;
+        o.rz = gam;
+        o.rr = rr;
+        if (!stop) o.restart = 1;
+        *sout = o;
+      }
+      return;
+    }
+    alpha = gam / den;
+  }
+  const X al = (X)alpha, be = (X)beta;
+  double ga = 0.0, ra = 0.0;
+  while (true) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int64_t i = base + k * kBlock + threadIdx.x;
+      if (i < n) {
+        X pi = pv[k], si = sw[k];
+        if (!fresh) {
+          pi = uv[k] + be * pi;
+          si = wv[k] + be * si;
+          p[i] = pi;
+          sv[i] = si;
+        }
+        x[i] = xv[k] + al * pi;
+        const X ri = rv[k] - al * si;
+        r[i] = ri;
+        const X ui = ri * dv[k];
+        u[i] = ui;
+        ga += (double)ri * (double)ui;
+        ra += (double)ri * (double)ri;
+      }
+    }
+    base += stride;
+    if (base >= n) break;
+    load(base);
+  }
+  double b = block_sum(ga, sh[0]);
+  if (threadIdx.x == 0) part_out[2 * blockIdx.x] = b;
+  b = block_sum(ra, sh[0]);
+  if (threadIdx.x == 0) part_out[2 * blockIdx.x + 1] = b;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Scal o = S;
+    o.fresh = 0;
+    o.rzp = gam;
+    o.alpha = alpha;
+    o.beta = beta;
+    o.rz = gam;
+    o.rr = rr;
+    o.iters = S.iters + 1;
+    *sout = o;
+  }
+}
+
+// Reseed, step 1: r = res (rounded to X), u = D⁻¹ r; partials r·u and r·p.
+template <typename X>
+__global__ __launch_bounds__(kBlock) void k_cg_reseed(int64_t n, const double* __restrict__ res,
+                                                      const X* __restrict__ invd, X* __restrict__ r, X* __restrict__ u,
+                                                      const X* __restrict__ p, double* __restrict__ part) {
   __shared__ double sh[kBlock / 64];
-  double rz = 0.0, rp = 0.0;
+  double ru = 0.0, rp = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const X ri = (X)res[i];
+    const X ui = ri * invd[i];
     r[i] = ri;
-    rz += (double)ri * (double)(ri * invd[i]);
+    u[i] = ui;
+    ru += (double)ri * (double)ui;
     rp += (double)ri * (double)p[i];
   }
-  double b = block_sum(rz, sh);
+  double b = block_sum(ru, sh);
   if (threadIdx.x == 0) part[2 * blockIdx.x] = b;
   b = block_sum(rp, sh);
   if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = b;
 }
 
-// Reliable update, step 4: p = z + β (p − γ z): the old direction made orthogonal to the
-// new residual, then extended as a CG step would (β = r·z / r·z of the step before).
-template <typename X>
-__global__ __launch_bounds__(kBlock) void k_pfix(int64_t n, const X* __restrict__ r, const X* __restrict__ invd,
-                                                 X* __restrict__ p, const Scal* __restrict__ s) {
-  const X beta = (X)s->beta, gam = (X)s->gam;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const X zi = r[i] * invd[i];
-    p[i] = zi + beta * (p[i] - gam * zi);
-  }
-}
-
-// Reliable update, step 3: rz from the reseeded residual; the search direction p is kept.
-// Reliable update, step 3 (scalars).  s->rr holds the true residual ||f - K u||² when this
-// runs; the next hand-over is at hold_factor2 times that.  After a breakdown (or with no
-// previous step) the direction restarts: β = 0, p = z.
-__global__ __launch_bounds__(kBlock) void k_reseed_scal(int m, const double* __restrict__ part,
-                                                        Scal* __restrict__ s, double hold_factor2) {
+// Reseed, step 2 (scalars): γ = r·u, γ' = (r·p)/γ, β = γ/γ_prev (0: restart p = u).
+// s->rr holds the true residual when this runs; the next hand-over is at hold_factor2 × it.
+__global__ __launch_bounds__(kBlock) void k_cg_reseed_scal(int m, const double* __restrict__ part,
+                                                           Scal* __restrict__ s, double hold_factor2, int restart) {
   __shared__ double sh[kBlock / 64];
-  const double rz = sum_parts(part, m, 2, sh);
+  const double ru = sum_parts(part, m, 2, sh);
   const double rp = sum_parts(part + 1, m, 2, sh);
   if (threadIdx.x == 0) {
-    const bool fresh = s->restart || !(s->rzp > 0.0) || !(rz > 0.0);
-    s->gam = rz > 0.0 ? rp / rz : 0.0;
-    s->beta = fresh ? 0.0 : rz / s->rzp;
-    s->rz = rz;
+    const bool fresh = restart || s->restart || !(s->rzp > 0.0) || !(ru > 0.0);
+    s->gam = ru > 0.0 ? rp / ru : 0.0;
+    s->beta = fresh ? 0.0 : ru / s->rzp;
+    s->rz = ru;
     s->hold2 = hold_factor2 * s->rr;
     s->done = 0;
     s->restart = 0;
   }
 }
 
+// Reseed, step 3 (after w = A u): p = u + β (p − γ' u), s = w + β (s − γ' w): the last
+// direction made orthogonal to the new residual, then extended; partial p·s.
+template <typename X>
+__global__ __launch_bounds__(kBlock) void k_cg_pfix(int64_t n, const X* __restrict__ u, const X* __restrict__ w,
+                                                    X* __restrict__ p, X* __restrict__ sv, const Scal* __restrict__ s,
+                                                    double* __restrict__ part) {
+  __shared__ double sh[kBlock / 64];
+  const X be = (X)s->beta, ga = (X)s->gam;
+  double ps = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const X ui = u[i], wi = w[i];
+    const X pi = ui + be * (p[i] - ga * ui);
+    const X si = wi + be * (sv[i] - ga * wi);
+    p[i] = pi;
+    sv[i] = si;
+    ps += (double)pi * (double)si;
+  }
+  const double b = block_sum(ps, sh);
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = b;
+  if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = 0.0;
+}
+
+// Reseed, step 4 (scalars): α = γ / p·s; the next k_cg_step only moves x and r.
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(int m, const double* __restrict__ part, Scal* __restrict__ s) {
+  __shared__ double sh[kBlock / 64];
+  const double ps = sum_parts(part, m, 2, sh);
+  if (threadIdx.x == 0) {
+    s->pq = ps;
+    const bool ok = ps > 0.0;
+    s->alpha = ok ? s->rz / ps : 0.0;
+    s->fresh = ok ? 1 : 0;
+    if (!ok) {   // p vanished (r already zero): nothing to step
+      s->done = 1;
+      s->restart = 1;
+    }
+  }
+}
+
+// Both scalar records cleared; record 0 starts from ||f||² as its (true) residual.
+__global__ void k_cg_init_scal(Scal* __restrict__ s, double rr0) {
+  Scal z{};
+  s[1] = z;
+  z.rr = rr0;
+  s[0] = z;
+}
 
 __global__ void k_f32_from(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -725,15 +824,17 @@ static int ensure_work(fem_ctx* c, bool mixed) {
     w->vblocks = (int)std::min<int64_t>(kMaxVecBlocks, (n + kTile - 1) / kTile);
     const int ps = std::max(w->sh64.blocks, w->sh32.blocks);
     if ((rc = dev_alloc(c, &w->r, n)) || (rc = dev_alloc(c, &w->p, n)) || (rc = dev_alloc(c, &w->q, n)) ||
+        (rc = dev_alloc(c, &w->u, n)) || (rc = dev_alloc(c, &w->sv, n)) ||
         (rc = dev_alloc(c, &w->invd, n)) || (rc = dev_alloc(c, &w->res, n)) ||
         (rc = dev_alloc(c, &w->part_s, std::max(ps, kMaxVecBlocks))) ||
-        (rc = dev_alloc(c, &w->part_v, 2 * kMaxVecBlocks)) || (rc = dev_alloc(c, &w->s, 1)))
+        (rc = dev_alloc(c, &w->part_v, 4 * std::max(kMaxVecBlocks, kCgBlocks))) || (rc = dev_alloc(c, &w->s, 2)))
       return rc;
     dev_free(&w->r32);
     w->n = n;
   }
   if (mixed && !w->r32) {
     if ((rc = dev_alloc(c, &w->r32, n)) || (rc = dev_alloc(c, &w->p32, n)) || (rc = dev_alloc(c, &w->q32, n)) ||
+        (rc = dev_alloc(c, &w->u32, n)) || (rc = dev_alloc(c, &w->sv32, n)) ||
         (rc = dev_alloc(c, &w->x32, n)) || (rc = dev_alloc(c, &w->invd32, n)))
       return rc;
   }
@@ -845,32 +946,32 @@ static int ensure_sell_vals(fem_ctx* c, SolverWork* w) {
 
 // The solver's SpMV in the active format; returns the number of partials it wrote.
 template <typename X, int MODE>
-static int spmv_any(fem_ctx* c, SolverWork* w, const X* x, const double* f, X* y) {
+static int spmv_any(fem_ctx* c, SolverWork* w, const X* x, const double* f, X* y, const Scal* sc) {
   if (use_sell(c)) {
     const SellOp& o = w->sell;
     const int g = w->sell_grid;
     if constexpr (sizeof(X) == 4) {
       if (o.roff)
         k_sell<float, float, MODE, true><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, o.roff, nullptr,
-                                                                      o.val32, x, f, y, w->part_s, w->s);
+                                                                      o.val32, x, f, y, w->part_s, sc);
       else
         k_sell<float, float, MODE, false><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, nullptr, o.col,
-                                                                       o.val32, x, f, y, w->part_s, w->s);
+                                                                       o.val32, x, f, y, w->part_s, sc);
     } else {
       if (o.roff)
         k_sell<double, double, MODE, true><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, o.roff,
-                                                                        nullptr, o.val64, x, f, y, w->part_s, w->s);
+                                                                        nullptr, o.val64, x, f, y, w->part_s, sc);
       else
         k_sell<double, double, MODE, false><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, nullptr,
-                                                                         o.col, o.val64, x, f, y, w->part_s, w->s);
+                                                                         o.col, o.val64, x, f, y, w->part_s, sc);
     }
     return g;
   }
   if constexpr (sizeof(X) == 4) {
-    launch_spmv<float, float, MODE>(c, w->sh32, c->d_nzval32, x, f, y, w->part_s, w->s, w->roff32, w->bbase32);
+    launch_spmv<float, float, MODE>(c, w->sh32, c->d_nzval32, x, f, y, w->part_s, sc, w->roff32, w->bbase32);
     return w->sh32.blocks;
   } else {
-    launch_spmv<double, double, MODE>(c, w->sh64, c->d_nzval, x, f, y, w->part_s, w->s);
+    launch_spmv<double, double, MODE>(c, w->sh64, c->d_nzval, x, f, y, w->part_s, sc);
     return w->sh64.blocks;
   }
 }
@@ -887,38 +988,68 @@ static int prepare_operator(fem_ctx* c, SolverWork* w) {
   return FEM_OK;
 }
 
-static int read_scal(fem_ctx* c, SolverWork* w, Scal* h) {
+// With time_kernels: add the SpMV durations of the batch just run (the stream is synchronized).
+static void add_spmv_times(fem_ctx* c, SolverWork* w) {
+  if (!c->tune.time_kernels) return;
+  for (int k = 0; k < kBatch; ++k) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, w->tev[2 * k], w->tev[2 * k + 1]) == hipSuccess) {
+      c->timings.spmv_ms += ms;
+      c->timings.spmv_launches += 1;
+    }
+  }
+}
+
+static int read_scal(fem_ctx* c, SolverWork* w, Scal* h) {   // scalar record 0 (the current one)
   FEM_HIP(c, hipMemcpyAsync(h, w->s, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   return FEM_OK;
 }
 
-// One CG iteration's launches (q = K p, α, update, β, p update).
+// The vectors of one CG recurrence (FP64 solver or FP32 recurrence of the mixed solver).
 template <typename X>
-static void enqueue_iteration(fem_ctx* c, SolverWork* w, X* x, X* r, X* p, X* q, const X* invd, double tol2) {
-  const int parts = spmv_any<X, 0>(c, w, p, nullptr, q);
-  k_alpha<<<1, kWide, 0, c->stream>>>(parts, w->part_s, w->s);
-  k_update<X><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, x, r, p, q, invd, w->part_v, w->s);
-  k_beta<<<1, kWide, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2);
-  k_pupd<X><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, p, q, w->s);
+struct CgVecs {
+  X *x, *r, *u, *w, *p, *sv;
+  const X* invd;
+};
+
+// One Chronopoulos–Gear iteration: w = A u (partial w·u), then the fused vector step.
+// `k` is the position in the captured batch: it picks the scalar/partial buffers.
+template <typename X>
+static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int k, double tol2, bool timed) {
+  Scal* sin = w->s + (k & 1);
+  Scal* sout = w->s + ((k + 1) & 1);
+  double* pin = w->part_v + (k & 1) * 2 * kCgBlocks;
+  double* pout = w->part_v + ((k + 1) & 1) * 2 * kCgBlocks;
+  if (timed) (void)hipEventRecord(w->tev[2 * k], c->stream);
+  const int na = spmv_any<X, 0>(c, w, v.u, nullptr, v.w, sin);
+  if (timed) (void)hipEventRecord(w->tev[2 * k + 1], c->stream);
+  k_cg_step<X><<<cg_blocks(c->n), kBlock, 0, c->stream>>>(c->n, v.x, v.r, v.u, v.w, v.p, v.sv, v.invd, na, w->part_s,
+                                                           pin, pout, sin, sout, tol2);
 }
 
 // The captured batch of kBatch iterations (rebuilt only when the buffers or the tolerance change).
+// kBatch is even, so the batch starts and ends on scalar record 0 and partial buffer 0.
 template <typename X>
-static int batch_graph(fem_ctx* c, SolverWork* w, X* x, X* r, X* p, X* q, const X* invd, double tol2,
-                       GraphCache& gc) {
+static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double tol2, GraphCache& gc) {
+  static_assert(kBatch % 2 == 0, "batch must end on scalar record 0");
   double tkey = tol2;
   const void* val = use_sell(c) ? (sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64)
                                 : (sizeof(X) == 4 ? (const void*)c->d_nzval32 : (const void*)c->d_nzval);
-  const void* key[8] = {val, x, r, p, q, invd, w->s, nullptr};
+  const void* key[8] = {val, v.x, v.r, v.p, v.w, v.invd, w->s, nullptr};
   std::memcpy(&key[7], &tkey, sizeof(double));
+  if (c->tune.time_kernels) {   // kernels launched directly (launch_batch), SpMVs bracketed by events
+    for (hipEvent_t& e : w->tev)
+      if (!e) FEM_HIP(c, hipEventCreate(&e));
+    return FEM_OK;
+  }
   bool same = gc.exec != nullptr;
   for (int i = 0; i < 8 && same; ++i) same = gc.key[i] == key[i];
   if (same) return FEM_OK;
   destroy_graph(gc);
   hipGraph_t graph = nullptr;
   FEM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < kBatch; ++k) enqueue_iteration<X>(c, w, x, r, p, q, invd, tol2);
+  for (int k = 0; k < kBatch; ++k) enqueue_iteration<X>(c, w, v, k, tol2, false);
   FEM_HIP(c, hipStreamEndCapture(c->stream, &graph));
   hipError_t e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -927,22 +1058,48 @@ static int batch_graph(fem_ctx* c, SolverWork* w, X* x, X* r, X* p, X* q, const 
   return FEM_OK;
 }
 
+// One batch of kBatch iterations: the captured graph, or (time_kernels) direct launches with
+// every SpMV bracketed by HIP events.
+template <typename X>
+static int launch_batch(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double tol2, GraphCache& gc) {
+  if (c->tune.time_kernels) {
+    for (int k = 0; k < kBatch; ++k) enqueue_iteration<X>(c, w, v, k, tol2, true);
+    FEM_HIP(c, hipGetLastError());
+    return FEM_OK;
+  }
+  FEM_HIP(c, hipGraphLaunch(gc.exec, c->stream));
+  return FEM_OK;
+}
+
+// (Re)start the recurrence from the residual `res` (FP64 array): r = res, u = D⁻¹ r,
+// w = A u, p and s = A p rebuilt (orthogonalised old direction, or p = u on restart),
+// α = γ / p·s.  Works on scalar record 0 / partial buffer 0.
+template <typename X>
+static void reseed(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, const double* res, double hold_factor2,
+                   int restart) {
+  const int nb = cg_blocks(c->n);
+  k_cg_reseed<X><<<nb, kBlock, 0, c->stream>>>(c->n, res, v.invd, v.r, v.u, v.p, w->part_v);
+  k_cg_reseed_scal<<<1, kBlock, 0, c->stream>>>(nb, w->part_v, w->s, hold_factor2, restart);
+  spmv_any<X, 0>(c, w, v.u, nullptr, v.w, w->s);
+  k_cg_pfix<X><<<nb, kBlock, 0, c->stream>>>(c->n, v.u, v.w, v.p, v.sv, w->s, w->part_v);
+  k_cg_alpha<<<1, kBlock, 0, c->stream>>>(nb, w->part_v, w->s);
+}
+
 static void residual(fem_ctx* c, SolverWork* w, bool sell64);
 
-// FP64 preconditioned CG on K u = f from u = 0 (r holds f on entry).  When the recursive
-// residual claims convergence the true residual f − K u is computed; if it misses the
-// tolerance it replaces r (residual replacement; p is re-orthogonalised against it) and the
-// iteration goes on, at most kMaxReplace times.
+// FP64 preconditioned CG on K u = f from u = 0.  When the recursive residual claims
+// convergence the true residual f − K u is computed; if it misses the tolerance it
+// replaces r (residual replacement; p is re-orthogonalised against it) and the iteration
+// goes on, at most kMaxReplace times and only while the true residual still improves.
 constexpr int kMaxReplace = 8;
 
 static int pcg64(fem_ctx* c, SolverWork* w, double tol2, int maxit, GraphCache& gc, int* iters, int* replaced,
                  double* ms) {
-  double *x = c->d_u, *r = w->r, *p = w->p, *q = w->q;
-  const double* invd = w->invd;
-  k_init<double><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, r, invd, p, w->part_v);
-  k_init_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2, 0.0);
+  const CgVecs<double> v{c->d_u, w->r, w->u, w->q, w->p, w->sv, w->invd};
+  k_cg_init_scal<<<1, 1, 0, c->stream>>>(w->s, 0.0);
+  reseed<double>(c, w, v, c->d_f, 0.0, 1);
   FEM_HIP(c, hipGetLastError());
-  int rc = batch_graph<double>(c, w, x, r, p, q, invd, tol2, gc);
+  int rc = batch_graph<double>(c, w, v, tol2, gc);
   if (rc) return rc;
   Scal h{};
   int launched = 0, nrep = 0;
@@ -950,9 +1107,10 @@ static int pcg64(fem_ctx* c, SolverWork* w, double tol2, int maxit, GraphCache& 
   double prev_true = -1.0;
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   while (launched < maxit) {
-    FEM_HIP(c, hipGraphLaunch(gc.exec, c->stream));
+    if ((rc = launch_batch<double>(c, w, v, tol2, gc))) return rc;
     launched += kBatch;
     if ((rc = read_scal(c, w, &h))) return rc;
+    add_spmv_times(c, w);
     if (!h.done) continue;
     residual(c, w, use_sell(c));
     Scal ht{};
@@ -961,9 +1119,7 @@ static int pcg64(fem_ctx* c, SolverWork* w, double tol2, int maxit, GraphCache& 
     // the true residual no longer improves: the tolerance is below what FP64 attains here
     if (nrep++ >= kMaxReplace || (prev_true >= 0.0 && ht.rr > 0.25 * prev_true)) break;
     prev_true = ht.rr;
-    k_reseed<double><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, w->res, invd, r, p, w->part_v);
-    k_reseed_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, 0.0);
-    k_pfix<double><<<w->vblocks, kBlock, 0, c->stream>>>(c->n, r, invd, p, w->s);
+    reseed<double>(c, w, v, w->res, 0.0, 0);
   }
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
   *ms += event_ms(c);
@@ -978,7 +1134,7 @@ static int pcg64(fem_ctx* c, SolverWork* w, double tol2, int maxit, GraphCache& 
 static void residual(fem_ctx* c, SolverWork* w, bool sell64) {
   int parts;
   if (sell64) {
-    parts = spmv_any<double, 1>(c, w, c->d_u, c->d_f, w->res);
+    parts = spmv_any<double, 1>(c, w, c->d_u, c->d_f, w->res, w->s);
   } else {
     launch_spmv<double, double, 1>(c, w->sh64, c->d_nzval, c->d_u, c->d_f, w->res, w->part_s, w->s);
     parts = w->sh64.blocks;
@@ -1002,6 +1158,8 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   const int maxit = cfg->maxit > 0 ? cfg->maxit : 100000;
   fem_solve_info out{};
   double cg_ms = 0.0;
+  c->timings.spmv_ms = 0.0;
+  c->timings.spmv_launches = 0;
 
   k_sumsq<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->part_s);
   k_sum_to<<<1, kWide, 0, c->stream>>>(kMaxVecBlocks, w->part_s, w->s, 1);
@@ -1021,7 +1179,6 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
     if ((rc = prepare_operator<double>(c, w))) return rc;
     k_invdiag<double><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd);
     k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_u);
-    k_copy<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->r);
     int iters = 0, replaced = 0;
     rc = pcg64(c, w, tol2, maxit, w->g64, &iters, &replaced, &cg_ms);
     out.iterations = iters;
@@ -1037,33 +1194,29 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
     // fed back as the FP32 residual, and the last search direction is made orthogonal to it
     // and extended (p = z + β (p − γ z)), as in reliable-update CG.
     const double delta = cfg->inner_rtol > 0 ? cfg->inner_rtol : 0.1;
+    const CgVecs<float> v{w->x32, w->r32, w->u32, w->q32, w->p32, w->sv32, w->invd32};
     k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_u);
     k_zero32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, w->x32);
-    k_f32_from<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_f, w->r32);
-    k_init<float><<<w->vblocks, kBlock, 0, c->stream>>>(n, w->r32, w->invd32, w->p32, w->part_v);
-    k_init_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, tol2, delta * delta);
-    if ((rc = batch_graph<float>(c, w, w->x32, w->r32, w->p32, w->q32, w->invd32, tol2, w->g32))) return rc;
-    double maxrr = bnorm2;
+    k_cg_init_scal<<<1, 1, 0, c->stream>>>(w->s, bnorm2);
+    reseed<float>(c, w, v, c->d_f, delta * delta, 1);
+    if ((rc = batch_graph<float>(c, w, v, tol2, w->g32))) return rc;
     int launched = 0, refinements = 0;
     bool converged = false;
     FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
     while (launched < maxit && !converged) {
-      FEM_HIP(c, hipGraphLaunch(w->g32.exec, c->stream));
+      if ((rc = launch_batch<float>(c, w, v, tol2, w->g32))) return rc;
       launched += kBatch;
       Scal hs{};
       if ((rc = read_scal(c, w, &hs))) return rc;
-      if (hs.done || hs.rr < delta * delta * maxrr) {
-        ++refinements;
-        k_fold<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, w->x32, c->d_u);
-        residual(c, w, false);
-        Scal ht{};
-        if ((rc = read_scal(c, w, &ht))) return rc;
-        if (ht.rr <= tol2) { converged = true; break; }
-        maxrr = ht.rr;
-        k_reseed<float><<<w->vblocks, kBlock, 0, c->stream>>>(n, w->res, w->invd32, w->r32, w->p32, w->part_v);
-        k_reseed_scal<<<1, kBlock, 0, c->stream>>>(w->vblocks, w->part_v, w->s, delta * delta);
-        k_pfix<float><<<w->vblocks, kBlock, 0, c->stream>>>(n, w->r32, w->invd32, w->p32, w->s);
-      }
+      add_spmv_times(c, w);
+      if (!hs.done) continue;
+      ++refinements;
+      k_fold<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, w->x32, c->d_u);
+      residual(c, w, false);
+      Scal ht{};
+      if ((rc = read_scal(c, w, &ht))) return rc;
+      if (ht.rr <= tol2) { converged = true; break; }
+      reseed<float>(c, w, v, w->res, delta * delta, 0);
     }
     FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
     cg_ms += event_ms(c);
@@ -1076,8 +1229,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   FEM_HIP(c, hipGetLastError());
   if ((rc = read_scal(c, w, &h))) return rc;
   out.relres = std::sqrt(h.rr / bnorm2);
-  c->timings.spmv_ms = cg_ms;
-  c->timings.spmv_launches = out.iterations;
+  c->timings.solve_ms = cg_ms;
   if (info) *info = out;
   if (out.relres > cfg->rtol * 1.0000001) {
     char buf[160];
@@ -1107,8 +1259,8 @@ extern "C" int32_t fem_spmv_bench(fem_ctx* c, int32_t mixed, int32_t reps, doubl
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < reps; ++i) {
-    if (mixed) spmv_any<float, 0>(c, w, w->p32, nullptr, w->q32);
-    else spmv_any<double, 0>(c, w, w->p, nullptr, w->q);
+    if (mixed) spmv_any<float, 0>(c, w, w->p32, nullptr, w->q32, w->s);
+    else spmv_any<double, 0>(c, w, w->p, nullptr, w->q, w->s);
   }
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -1122,7 +1274,8 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
   auto ok_l = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4; };
   if (!ok_cap(t->spmv_cap64) || !ok_cap(t->spmv_cap32) || !ok_l(t->spmv_lanes64) || !ok_l(t->spmv_lanes32) ||
       !(t->asm_block == 0 || t->asm_block == 64 || t->asm_block == 128 || t->asm_block == 256) ||
-      !(t->spmv_format == 0 || t->spmv_format == 1) || !(t->asm_mode >= 0 && t->asm_mode <= 2))
+      !(t->spmv_format == 0 || t->spmv_format == 1) || !(t->asm_mode >= 0 && t->asm_mode <= 2) ||
+      !(t->time_kernels == 0 || t->time_kernels == 1))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

@@ -57,12 +57,14 @@ class SolveInfo(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("spmv_cap64", C.c_int32), ("spmv_lanes64", C.c_int32), ("spmv_cap32", C.c_int32),
                 ("spmv_lanes32", C.c_int32), ("spmv_compress32", C.c_int32), ("asm_block", C.c_int32),
-                ("spmv_format", C.c_int32), ("asm_mode", C.c_int32)]
+                ("spmv_format", C.c_int32), ("asm_mode", C.c_int32), ("time_kernels", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Timings(C.Structure):
     _fields_ = [("assemble_kernel_ms", C.c_double), ("fill_ms", C.c_double), ("apply_ms", C.c_double),
-                ("spmv_ms", C.c_double), ("spmv_launches", C.c_int32), ("assemble_launches", C.c_int32)]
+                ("spmv_ms", C.c_double), ("spmv_launches", C.c_int32), ("assemble_launches", C.c_int32),
+                ("solve_ms", C.c_double)]
 
 
 class BatchCfg(C.Structure):

@@ -156,6 +156,9 @@ typedef struct {
                                         1 = CSR-stream over the CSC pattern (the cap/lanes above) */
   int32_t asm_mode;                  /* scalar Q1 Poisson assembly: 0 = owner-computes gather when
                                         supported (default), 1 = colored scatter, 2 = gather only */
+  int32_t time_kernels;              /* 1: time every solver SpMV with HIP events recorded inside the
+                                        captured CG graph (fem_timings.spmv_ms / spmv_launches) */
+  int32_t reserved;
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
 int32_t fem_get_tuning(const fem_ctx* ctx, fem_tuning* t);
@@ -165,9 +168,10 @@ typedef struct {
   double assemble_kernel_ms;   /* element + scatter kernels, summed over colors */
   double fill_ms;              /* start_assemble zero fill */
   double apply_ms;             /* all apply! kernels */
-  double spmv_ms;              /* SpMV kernels of the last solve, summed */
-  int32_t spmv_launches;
+  double spmv_ms;              /* solver SpMV launches of the last solve, summed (with time_kernels) */
+  int32_t spmv_launches;       /* number of those launches (0 without time_kernels) */
   int32_t assemble_launches;
+  double solve_ms;             /* the CG loop of the last solve (HIP events) */
 } fem_timings;
 int32_t fem_get_timings(const fem_ctx* ctx, fem_timings* t);
 /* Average duration (HIP events) of `reps` back-to-back SpMV launches y = K x of the solver,

@@ -111,11 +111,14 @@ def test_compliance_adjoint_vs_fd_and_torch():
     assert res.relres < 1e-12
     rng = np.random.default_rng(10)
     h = 1e-6
-    # J is evaluated as 2fᵀu − uᵀKu (error quadratic in the CG residual), so each evaluation
-    # carries about 1e-14 |J| of rounding and a central difference at h = 1e-6 has a noise
-    # floor near 2e-8 |J|: entries are checked to 1e-6 relative above that floor, and the
-    # sampled gradient as a whole to 1e-6 in norm.
-    floor = 2e-8 * abs(J)
+    # A central difference at h = 1e-6 cannot resolve gradient entries much below its
+    # noise: each FP64 evaluation of J (here ~150) carries ~1e-14 |J| of rounding even with
+    # an exact dense solve, which moves (J+ − J−)/2h by ~4e-8 |J| (measured with the torch
+    # restatement: entry 31, |g| = 0.256, FD off by 2.5e-5 relative at h = 1e-6, by 1e-7 at
+    # h = 1e-4).  Entries are therefore checked to 1e-6 relative above a floor of 1e-7 |J|,
+    # and the sampled gradient as a whole to 1e-6 in norm; the entry-wise 1e-6 check against
+    # exact derivatives is the torch autograd comparison below.
+    floor = 1e-7 * abs(J)
     fds, ads = [], []
     for e in rng.choice(g.ncells, 12, replace=False):
         tp, tm = theta.copy(), theta.copy()
@@ -151,8 +154,12 @@ def test_compliance_adjoint_vs_fd_and_torch():
     Jt = compliance_torch(X_t, g.cells, dh.cell_dofs, dh.ndofs, th_t, fvec, free)
     Jt.backward()
     assert abs(Jt.item() - J) <= 1e-10 * abs(J)
-    assert _rel(dth, th_t.grad.numpy()) <= 1e-8
-    assert _rel(dx, X_t.grad.numpy()) <= 1e-8
+    gt, gx = th_t.grad.numpy(), X_t.grad.numpy()
+    assert _rel(dth, gt) <= 1e-8
+    assert _rel(dx, gx) <= 1e-8
+    # entry by entry, to 1e-6 relative (absolute floor 1e-8 max|g| for entries that vanish)
+    assert np.all(np.abs(dth - gt) <= 1e-6 * np.abs(gt) + 1e-8 * np.abs(gt).max())
+    assert np.all(np.abs(dx - gx) <= 1e-6 * np.abs(gx) + 1e-8 * np.abs(gx).max())
 
 
 def test_torch_autograd_function():
