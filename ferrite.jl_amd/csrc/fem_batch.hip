@@ -1076,8 +1076,10 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
   FEM_HIP(c, hipSetDevice(c->device));
   const int np_ = (int)B->probs.size();
   const bool mixed = cfg->precision == FEM_MIXED;
-  const int cpw = cfg->cells_per_wg ? cfg->cells_per_wg : (B->cells_per_wg ? B->cells_per_wg : 128);
+  int cpw = cfg->cells_per_wg ? cfg->cells_per_wg : (B->cells_per_wg ? B->cells_per_wg : 128);
   const int chunk = cfg->chunk ? cfg->chunk : (B->chunk ? B->chunk : 512);
+  // a shape tuned for FP32 element math may exceed the FP64 staging limit: fall back
+  if (!cfg->cells_per_wg && !shape_ok(cpw, chunk, mixed)) cpw = 128;
   if (!shape_ok(cpw, chunk, mixed)) return set_error(c, FEM_ERR_ARG, "fem_batch_run: unsupported launch shape");
   if (!(cfg->rtol > 0.0)) return set_error(c, FEM_ERR_ARG, "fem_batch_run: rtol must be positive");
   const int64_t n = c->ndofs, nnz = c->nnz;
@@ -1237,8 +1239,9 @@ int32_t fem_batch_assemble_only(fem_batch* B, int32_t precision, int32_t cells_p
   if (!B->finalized) return set_error(c, FEM_ERR_STATE, "fem_batch_assemble_only: finalize first");
   FEM_HIP(c, hipSetDevice(c->device));
   const bool mixed = precision == FEM_MIXED;
-  const int cpw = cells_per_wg ? cells_per_wg : (B->cells_per_wg ? B->cells_per_wg : 128);
+  int cpw = cells_per_wg ? cells_per_wg : (B->cells_per_wg ? B->cells_per_wg : 128);
   const int ch = chunk ? chunk : (B->chunk ? B->chunk : 512);
+  if (!cells_per_wg && !shape_ok(cpw, ch, mixed)) cpw = 128;
   if (!shape_ok(cpw, ch, mixed)) return set_error(c, FEM_ERR_ARG, "unsupported launch shape");
   float t = 0.f;
   int rc = timed_assemble(B, cpw, ch, mixed, &t);

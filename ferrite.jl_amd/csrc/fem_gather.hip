@@ -75,15 +75,21 @@ __global__ __launch_bounds__(256) void k_elem_poisson(int64_t ncells, const doub
   }
 }
 
-// One lane per column; LDS slots acc[k][lane] for the column's entries k < MAXL.
-template <typename T, int NB, int MAXL>
+// One lane per column; LDS slots acc[k][lane] for the column's entries k < MAXL.  The
+// lane walks its column's cells G at a time: the list entries, slot bytes and element
+// values of G cells are loaded before any of them is added (loads in flight), then added
+// in ascending cell order.  ecol[q] names the column of the block's q-th nzval entry, so
+// the coalesced read-in / write-out of the block's nzval range needs no search.
+template <typename T, int NB, int MAXL, int G>
 __global__ __launch_bounds__(kGatherBlock) void k_gather_cols(
     int64_t n, int64_t ncells, const int32_t* __restrict__ colptr, const int32_t* __restrict__ d2c_ptr,
     const int32_t* __restrict__ d2c, const uint8_t* __restrict__ slot, const T* __restrict__ keb,
     const T* __restrict__ feb, double* __restrict__ nzval, double* __restrict__ f, int fillzero,
     DevError* __restrict__ err) {
+  static_assert(kGatherBlock <= 256, "column index of a block entry is one byte");
   __shared__ double acc[MAXL * kGatherBlock];
   __shared__ int32_t scp[kGatherBlock + 1];
+  __shared__ uint8_t ecol[MAXL * kGatherBlock];
   const int tid = threadIdx.x;
   // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2): give every XCD
   // one contiguous range of column blocks, so neighbouring columns, which read the same
@@ -96,74 +102,85 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_cols(
   for (int t = tid; t <= ncol; t += kGatherBlock) scp[t] = colptr[j0 + t];
   __syncthreads();
   const int32_t base0 = scp[0], end0 = scp[ncol];
-  // start values: 0, or the current K when accumulating (start_assemble(...; fillzero=false))
+  if (tid < ncol) {   // every column holds at most MAXL entries (the pattern's longest column)
+    const int32_t c0 = scp[tid] - base0, len = scp[tid + 1] - scp[tid];
+    for (int k = 0; k < len; ++k) ecol[c0 + k] = (uint8_t)tid;
+  }
   if (fillzero) {
 #pragma unroll
     for (int k = 0; k < MAXL; ++k) acc[k * kGatherBlock + tid] = 0.0;
-  } else {
-    for (int32_t q = base0 + tid; q < end0; q += kGatherBlock) {
-      int lo = 0, hi = ncol;   // column of entry q: largest c with scp[c] <= q
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (scp[mid] <= q) lo = mid; else hi = mid;
-      }
-      acc[(q - scp[lo]) * kGatherBlock + lo] = nzval[q];
-    }
   }
   __syncthreads();
-  double fj = 0.0;
+  // start values: the current K when accumulating (start_assemble(...; fillzero=false))
+  if (!fillzero) {
+    for (int32_t q = base0 + tid; q < end0; q += kGatherBlock) {
+      const int cl = ecol[q - base0];
+      acc[(q - scp[cl]) * kGatherBlock + cl] = nzval[q];
+    }
+    __syncthreads();
+  }
   if (tid < ncol) {
     const int64_t j = j0 + tid;
-    if (!fillzero) fj = f[j];
+    double fj = fillzero ? 0.0 : f[j];
     const int32_t e0 = d2c_ptr[j], e1 = d2c_ptr[j + 1];
-    for (int32_t e = e0; e < e1; ++e) {
-      const int32_t pk = d2c[e];
-      const int64_t cell = pk / NB;
-      const int b = pk - (int)cell * NB;
-      uint8_t sl[NB];
-      if constexpr (NB == 8) {
-        const uint2 u = *reinterpret_cast<const uint2*>(slot + (int64_t)e * 8);
+    for (int32_t eb = e0; eb < e1; eb += G) {
+      int32_t pk[G];
+      uint8_t sl[G][NB];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-          sl[a] = (uint8_t)(u.x >> (8 * a));
-          sl[a + 4] = (uint8_t)(u.y >> (8 * a));
-        }
-      } else if constexpr (NB == 4) {
-        const uint32_t u = *reinterpret_cast<const uint32_t*>(slot + (int64_t)e * 4);
+      for (int g = 0; g < G; ++g) {
+        const int32_t e = eb + g;
+        const bool in = e < e1;
+        pk[g] = in ? d2c[e] : -1;
+        if constexpr (NB == 8) {
+          const uint2 u = in ? *reinterpret_cast<const uint2*>(slot + (int64_t)e * 8) : make_uint2(~0u, ~0u);
 #pragma unroll
-        for (int a = 0; a < 4; ++a) sl[a] = (uint8_t)(u >> (8 * a));
-      } else {
+          for (int a = 0; a < 4; ++a) {
+            sl[g][a] = (uint8_t)(u.x >> (8 * a));
+            sl[g][a + 4] = (uint8_t)(u.y >> (8 * a));
+          }
+        } else if constexpr (NB == 4) {
+          const uint32_t u = in ? *reinterpret_cast<const uint32_t*>(slot + (int64_t)e * 4) : ~0u;
 #pragma unroll
-        for (int a = 0; a < NB; ++a) sl[a] = slot[(int64_t)e * NB + a];
-      }
-      T v[NB];
+          for (int a = 0; a < 4; ++a) sl[g][a] = (uint8_t)(u >> (8 * a));
+        } else {
 #pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        const int lo = a < b ? a : b, hi = a < b ? b : a;
-        v[a] = keb[(int64_t)tri_idx(lo, hi, NB) * ncells + cell];
-      }
-      const T fv = feb[(int64_t)b * ncells + cell];
-#pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        if (sl[a] != kNoSlot) {
-          acc[sl[a] * kGatherBlock + tid] += (double)v[a];
-        } else if (v[a] != T(0)) {
-          atomicMin(&err->pattern_cell, (int)cell);
+          for (int a = 0; a < NB; ++a) sl[g][a] = in ? slot[(int64_t)e * NB + a] : kNoSlot;
         }
       }
-      fj += (double)fv;
+      T v[G][NB], fv[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const bool in = pk[g] >= 0;
+        const int64_t cell = in ? pk[g] / NB : 0;
+        const int b = in ? pk[g] - (int)cell * NB : 0;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          const int lo = a < b ? a : b, hi = a < b ? b : a;
+          v[g][a] = in ? keb[(int64_t)tri_idx(lo, hi, NB) * ncells + cell] : T(0);
+        }
+        fv[g] = in ? feb[(int64_t)b * ncells + cell] : T(0);
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if (pk[g] < 0) break;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          if (sl[g][a] != kNoSlot) {
+            acc[sl[g][a] * kGatherBlock + tid] += (double)v[g][a];
+          } else if (v[g][a] != T(0)) {
+            atomicMin(&err->pattern_cell, (int)(pk[g] / NB));
+          }
+        }
+        fj += (double)fv[g];
+      }
     }
     f[j] = fj;
   }
   __syncthreads();
   // unit-stride write of the block's nzval range
   for (int32_t q = base0 + tid; q < end0; q += kGatherBlock) {
-    int lo = 0, hi = ncol;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (scp[mid] <= q) lo = mid; else hi = mid;
-    }
-    nzval[q] = acc[(q - scp[lo]) * kGatherBlock + lo];
+    const int cl = ecol[q - base0];
+    nzval[q] = acc[(q - scp[cl]) * kGatherBlock + cl];
   }
 }
 
@@ -259,18 +276,24 @@ static int run_gather(fem_ctx* c, const T* tab, double src, bool fillzero) {
   k_elem_poisson<T, DIM, NG, NB, NQ><<<gridn(nc), 256, 0, c->stream>>>(
       nc, c->d_x[0], c->d_x[1], c->d_x[2], c->d_gcnodes, c->d_param_cell, tab, (T)src, keb, feb, c->d_err);
   const int nblk = (int)(((c->n + kGatherBlock - 1) / kGatherBlock + 7) / 8 * 8);   // multiple of 8
-  if (c->max_row_nnz <= 9)
-    k_gather_cols<T, NB, 9><<<nblk, kGatherBlock, 0, c->stream>>>(c->n, nc, c->d_colptr, c->d_d2c_ptr, c->d_d2c,
-                                                                  c->d_gslot, keb, feb, c->d_nzval, c->d_f,
-                                                                  fillzero ? 1 : 0, c->d_err);
-  else if (c->max_row_nnz <= 27)
-    k_gather_cols<T, NB, 27><<<nblk, kGatherBlock, 0, c->stream>>>(c->n, nc, c->d_colptr, c->d_d2c_ptr, c->d_d2c,
-                                                                   c->d_gslot, keb, feb, c->d_nzval, c->d_f,
-                                                                   fillzero ? 1 : 0, c->d_err);
-  else
-    k_gather_cols<T, NB, 32><<<nblk, kGatherBlock, 0, c->stream>>>(c->n, nc, c->d_colptr, c->d_d2c_ptr, c->d_d2c,
-                                                                   c->d_gslot, keb, feb, c->d_nzval, c->d_f,
-                                                                   fillzero ? 1 : 0, c->d_err);
+  const int grp = c->tune.gather_group ? c->tune.gather_group : 4;
+#define FEM_GATHER(L, GG)                                                                               \
+  k_gather_cols<T, NB, L, GG><<<nblk, kGatherBlock, 0, c->stream>>>(c->n, nc, c->d_colptr, c->d_d2c_ptr, \
+                                                                    c->d_d2c, c->d_gslot, keb, feb,     \
+                                                                    c->d_nzval, c->d_f, fillzero ? 1 : 0, \
+                                                                    c->d_err)
+#define FEM_GATHER_G(L)         \
+  do {                          \
+    if (grp == 1) FEM_GATHER(L, 1); \
+    else if (grp == 2) FEM_GATHER(L, 2); \
+    else if (grp == 8) FEM_GATHER(L, 8); \
+    else FEM_GATHER(L, 4);      \
+  } while (0)
+  if (c->max_row_nnz <= 9) FEM_GATHER_G(9);
+  else if (c->max_row_nnz <= 27) FEM_GATHER_G(27);
+  else FEM_GATHER_G(32);
+#undef FEM_GATHER_G
+#undef FEM_GATHER
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
   c->timings.assemble_kernel_ms = event_ms(c);

@@ -280,11 +280,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, int R, const int32_t
 // --------------------------------------------------------------------------
 // Sliced ELL (SELL-64) SpMV: one wave per slice of 64 rows, lane l owns row 64 s + l and
 // sums its entries in pattern order (deterministic).  Value and offset loads are
-// unit-stride across the wave; x is gathered.  Fixed grid (grid-stride over slices), so
-// the partial count is fixed.
+// unit-stride across the wave; x is gathered.  U entries per lane are loaded before their
+// x gathers are issued (memory-level parallelism).  Fixed grid, so the partial count is
+// fixed.  CONTIG: every block owns one contiguous range of slices and the blocks that the
+// hardware deals to one XCD (blockIdx % 8) own neighbouring ranges, so an XCD's rows and
+// the x window they gather stay in that XCD's L2; otherwise blocks stride over slices.
 //   MODE 0: y = K x, partial Σ x·y ;  MODE 1: y = f − K x, partial Σ y²
 // --------------------------------------------------------------------------
-template <typename V, typename X, int MODE, bool COMP>
+template <typename V, typename X, int MODE, bool COMP, int U, bool CONTIG>
 __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, int64_t nslices, const int32_t* __restrict__ off,
                                                  const int32_t* __restrict__ base,
                                                  const uint16_t* __restrict__ roff,
@@ -295,10 +298,23 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, int64_t nslices, con
   using A = typename std::conditional<sizeof(X) == 8, double, float>::type;
   if (MODE == 0 && s->done) return;
   __shared__ double sh[kBlock / 64];
+  constexpr int WPB = kBlock / 64;
   const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  int64_t s_begin, s_end, s_step;
+  if (CONTIG) {   // gridDim.x is a multiple of 8
+    const int64_t per = gridDim.x / 8;
+    const int64_t lb = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    s_begin = lb * nslices / gridDim.x + wave;
+    s_end = (lb + 1) * nslices / gridDim.x;
+    s_step = WPB;
+  } else {
+    s_begin = (int64_t)blockIdx.x * WPB + wave;
+    s_end = nslices;
+    s_step = (int64_t)gridDim.x * WPB;
+  }
   double d = 0.0;
-  for (int64_t sl = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); sl < nslices;
-       sl += (int64_t)gridDim.x * (kBlock / 64)) {
+  for (int64_t sl = s_begin; sl < s_end; sl += s_step) {
     const int32_t o0 = off[sl];
     const int W = (off[sl + 1] - o0) >> 6;
     const int32_t b = COMP ? base[sl] : 0;
@@ -306,20 +322,20 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, int64_t nslices, con
     const int32_t e0 = o0 + lane;
     A acc = A(0);
     int k = 0;
-    for (; k + 4 <= W; k += 4) {
-      V v[4];
-      int32_t ci[4];
+    for (; k + U <= W; k += U) {
+      V v[U];
+      int32_t ci[U];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < U; ++u) {
         const int32_t e = e0 + 64 * (k + u);
         v[u] = val[e];
         ci[u] = COMP ? b + (int32_t)roff[e] : col[e];
       }
-      X xv[4];
+      X xv[U];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) xv[u] = x[ci[u]];
+      for (int u = 0; u < U; ++u) xv[u] = x[ci[u]];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc += (A)v[u] * (A)xv[u];
+      for (int u = 0; u < U; ++u) acc += (A)v[u] * (A)xv[u];
     }
     for (; k < W; ++k) {
       const int32_t e = e0 + 64 * k;
@@ -920,7 +936,10 @@ static int ensure_sell(fem_ctx* c, SolverWork* w) {
   o.nslices = ns;
   o.len = acc;
   o.ver32 = o.ver64 = 0;
-  w->sell_grid = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxVecBlocks, sb));
+  // a multiple of 8 (one share per XCD) when there are enough slices
+  int64_t sg = std::min<int64_t>(kMaxVecBlocks, sb);
+  if (sg >= 8) sg -= sg % 8;
+  w->sell_grid = (int)std::max<int64_t>(1, sg);
   o.built = true;
   return FEM_OK;
 }
@@ -944,6 +963,31 @@ static int ensure_sell_vals(fem_ctx* c, SolverWork* w) {
   return FEM_OK;
 }
 
+// One SELL launch with the unroll depth and slice mapping of the tuning.
+template <typename V, typename X, int MODE, bool COMP>
+static void launch_sell(fem_ctx* c, const SellOp& o, int g, const V* val, const X* x, const double* f, X* y,
+                        double* part, const Scal* sc) {
+  // measured at 128³ Q1 hex: 8 entries per lane in flight for FP32 values, 4 for FP64;
+  // grid-stride slices beat the XCD-contiguous ranges in both precisions
+  const int u = c->tune.sell_unroll ? c->tune.sell_unroll : (sizeof(V) == 4 ? 8 : 4);
+  const bool contig = c->tune.sell_map == 2 && g % 8 == 0;
+  const int32_t* col = COMP ? nullptr : o.col;
+  const uint16_t* roff = COMP ? o.roff : nullptr;
+#define FEM_SELL(UU, CC)                                                                                 \
+  k_sell<V, X, MODE, COMP, UU, CC><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, roff, col, \
+                                                                 val, x, f, y, part, sc)
+  if (contig) {
+    if (u == 4) FEM_SELL(4, true);
+    else if (u == 16) FEM_SELL(16, true);
+    else FEM_SELL(8, true);
+  } else {
+    if (u == 4) FEM_SELL(4, false);
+    else if (u == 16) FEM_SELL(16, false);
+    else FEM_SELL(8, false);
+  }
+#undef FEM_SELL
+}
+
 // The solver's SpMV in the active format; returns the number of partials it wrote.
 template <typename X, int MODE>
 static int spmv_any(fem_ctx* c, SolverWork* w, const X* x, const double* f, X* y, const Scal* sc) {
@@ -951,19 +995,11 @@ static int spmv_any(fem_ctx* c, SolverWork* w, const X* x, const double* f, X* y
     const SellOp& o = w->sell;
     const int g = w->sell_grid;
     if constexpr (sizeof(X) == 4) {
-      if (o.roff)
-        k_sell<float, float, MODE, true><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, o.roff, nullptr,
-                                                                      o.val32, x, f, y, w->part_s, sc);
-      else
-        k_sell<float, float, MODE, false><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, nullptr, o.col,
-                                                                       o.val32, x, f, y, w->part_s, sc);
+      if (o.roff) launch_sell<float, float, MODE, true>(c, o, g, o.val32, x, f, y, w->part_s, sc);
+      else launch_sell<float, float, MODE, false>(c, o, g, o.val32, x, f, y, w->part_s, sc);
     } else {
-      if (o.roff)
-        k_sell<double, double, MODE, true><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, o.roff,
-                                                                        nullptr, o.val64, x, f, y, w->part_s, sc);
-      else
-        k_sell<double, double, MODE, false><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, nullptr,
-                                                                         o.col, o.val64, x, f, y, w->part_s, sc);
+      if (o.roff) launch_sell<double, double, MODE, true>(c, o, g, o.val64, x, f, y, w->part_s, sc);
+      else launch_sell<double, double, MODE, false>(c, o, g, o.val64, x, f, y, w->part_s, sc);
     }
     return g;
   }
@@ -1116,8 +1152,9 @@ static int pcg64(fem_ctx* c, SolverWork* w, double tol2, int maxit, GraphCache& 
     Scal ht{};
     if ((rc = read_scal(c, w, &ht))) return rc;
     if (ht.rr <= tol2) { converged = true; break; }
-    // the true residual no longer improves: the tolerance is below what FP64 attains here
-    if (nrep++ >= kMaxReplace || (prev_true >= 0.0 && ht.rr > 0.25 * prev_true)) break;
+    // the true residual no longer improves (by 10% in norm per replacement): the tolerance
+    // is below what FP64 attains here
+    if (nrep++ >= kMaxReplace || (prev_true >= 0.0 && ht.rr > 0.81 * prev_true)) break;
     prev_true = ht.rr;
     reseed<double>(c, w, v, w->res, 0.0, 0);
   }
@@ -1275,7 +1312,11 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
   if (!ok_cap(t->spmv_cap64) || !ok_cap(t->spmv_cap32) || !ok_l(t->spmv_lanes64) || !ok_l(t->spmv_lanes32) ||
       !(t->asm_block == 0 || t->asm_block == 64 || t->asm_block == 128 || t->asm_block == 256) ||
       !(t->spmv_format == 0 || t->spmv_format == 1) || !(t->asm_mode >= 0 && t->asm_mode <= 2) ||
-      !(t->time_kernels == 0 || t->time_kernels == 1))
+      !(t->time_kernels == 0 || t->time_kernels == 1) ||
+      !(t->sell_unroll == 0 || t->sell_unroll == 4 || t->sell_unroll == 8 || t->sell_unroll == 16) ||
+      !(t->sell_map >= 0 && t->sell_map <= 2) ||
+      !(t->gather_group == 0 || t->gather_group == 1 || t->gather_group == 2 || t->gather_group == 4 ||
+        t->gather_group == 8))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

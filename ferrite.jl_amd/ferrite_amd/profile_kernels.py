@@ -20,6 +20,10 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--sweep", action="store_true", help="time SpMV / assembly launch shapes")
     ap.add_argument("--solve", action="store_true", help="also run one FP64 and one mixed solve")
+    ap.add_argument("--gather-sweep", action="store_true",
+                    help="owner-computes assembly: cells loaded per column lane before adding")
+    ap.add_argument("--sell-sweep", action="store_true",
+                    help="sliced-ELL unroll depth x slice mapping: isolated and in-solve SpMV durations")
     args = ap.parse_args()
     import numpy as np
     import torch
@@ -63,6 +67,35 @@ def main():
         for mode in ("cg", "mixed"):
             r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
             print(f"phase solve {mode}: {r.iterations} iterations, relres {r.relres:.2e}", flush=True)
+    if args.gather_sweep:
+        for grp in (1, 2, 4, 8):
+            fa.set_tuning(K, gather_group=grp)
+            ts = []
+            for prec in ("fp64", "mixed"):
+                best = 1e30
+                for _ in range(3):
+                    fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=prec)
+                    best = min(best, fa.timings(K)["assemble_kernel_ms"])
+                ts.append(best)
+            print(f"gather group {grp}: fp64 {ts[0]:.3f} ms, fp32-element {ts[1]:.3f} ms", flush=True)
+        fa.set_tuning(K, gather_group=0)
+        fa.apply(K, f, ch)
+    if args.sell_sweep:
+        for smap, mname in ((1, "grid-stride"), (2, "contiguous/XCD")):
+            for unroll in (4, 8, 16):
+                fa.set_tuning(K, sell_map=smap, sell_unroll=unroll)
+                a64 = fa.spmv_bench(K, mixed=False, reps=20)
+                a32 = fa.spmv_bench(K, mixed=True, reps=20)
+                fa.set_tuning(K, sell_map=smap, sell_unroll=unroll, time_kernels=1)
+                line = []
+                for mode in ("cg", "mixed"):
+                    r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
+                    t = fa.timings(K)
+                    line.append(f"{mode} {t['spmv_ms'] / max(t['spmv_launches'], 1) * 1e3:.1f} us x "
+                                f"{t['spmv_launches']} ({r.iterations} its, loop {t['solve_ms']:.1f} ms)")
+                print(f"sell {mname} unroll {unroll}: isolated fp64 {a64 * 1e3:.1f} us, fp32 {a32 * 1e3:.1f} us; "
+                      f"in-solve {'; '.join(line)}", flush=True)
+        fa.set_tuning(K, sell_map=0, sell_unroll=0, time_kernels=0)
     if args.sweep:
         fa.set_tuning(K, spmv_format=0)
         a64 = fa.spmv_bench(K, mixed=False, reps=20)

@@ -156,8 +156,15 @@ typedef struct {
                                         1 = CSR-stream over the CSC pattern (the cap/lanes above) */
   int32_t asm_mode;                  /* scalar Q1 Poisson assembly: 0 = owner-computes gather when
                                         supported (default), 1 = colored scatter, 2 = gather only */
-  int32_t time_kernels;              /* 1: time every solver SpMV with HIP events recorded inside the
-                                        captured CG graph (fem_timings.spmv_ms / spmv_launches) */
+  int32_t time_kernels;              /* 1: the solver launches its CG kernels directly (no graph
+                                        replay) with HIP events around every SpMV
+                                        (fem_timings.spmv_ms / spmv_launches) */
+  int32_t sell_unroll;               /* sliced-ELL SpMV: entries loaded per lane before their x
+                                        gathers, 4 / 8 / 16 (0 = default: 8 FP32, 4 FP64) */
+  int32_t sell_map;                  /* sliced-ELL slices to blocks: 0 = default (grid-stride),
+                                        1 = grid-stride, 2 = contiguous, one range per XCD share */
+  int32_t gather_group;              /* owner-computes assembly: cells whose element values a column
+                                        lane loads before adding them, 1 / 2 / 4 / 8 (0 = default 4) */
   int32_t reserved;
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);

@@ -138,4 +138,12 @@ def test_full_batch_converges_and_is_reproducible():
     for i, (nz, u) in keep.items():
         nz2, _, u2 = b.download(i)
         assert np.array_equal(nz, nz2) and np.array_equal(u, u2), i
+    # FP64 element math on the same batch: the FP32-tuned shape is replaced when it does
+    # not fit the FP64 staging (256 cells per workgroup)
+    r3 = b.run("fp64", rtol=1e-8)
+    assert r3.converged == 64
+    for i, (nz, u) in keep.items():
+        nz3, _, u3 = b.download(i)
+        assert np.linalg.norm(nz3 - nz) <= 1e-5 * np.linalg.norm(nz3), i
+        assert np.linalg.norm(u3 - u) <= 1e-4 * np.linalg.norm(u3), i   # FP32 element rounding
     b.close()
