@@ -240,7 +240,6 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
       c->timings.fill_ms = 0.0;
       rg = gather_poisson(c, mixed, cfg->source, cfg->fillzero != 0, c->d_tab, c->d_tabf);
       if (rg) return rg;
-      c->nzval32_dirty = true;
       ++c->nzval_version;
       return check_assembly_errors(c);
     }
@@ -285,7 +284,6 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
     return set_error(c, FEM_ERR_ARG, "unknown kernel id");
   }
   if (rc) return rc;
-  c->nzval32_dirty = true;
   ++c->nzval_version;
   return check_assembly_errors(c);
 }

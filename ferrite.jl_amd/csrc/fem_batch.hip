@@ -1182,7 +1182,6 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
     if (done_ms[p] < 0) iters[p] = hs[p].iters;   // unconverged: iterations so far, completion -1
   B->last_times_ms = done_ms;
   B->last_iters = iters;
-  c->nzval32_dirty = true;
   ++c->nzval_version;
   if (info) {
     info->assemble_ms = t_asm;
@@ -1246,7 +1245,6 @@ int32_t fem_batch_assemble_only(fem_batch* B, int32_t precision, int32_t cells_p
   float t = 0.f;
   int rc = timed_assemble(B, cpw, ch, mixed, &t);
   if (ms) *ms = t;
-  c->nzval32_dirty = true;
   ++c->nzval_version;
   return rc;
 }

@@ -148,7 +148,7 @@ void fem_ctx_destroy(fem_ctx* c) {
   free_solver(c);
   for (auto& p : c->d_x) dev_free(&p);
   dev_free(&c->d_colptr); dev_free(&c->d_rowval); dev_free(&c->d_diag);
-  dev_free(&c->d_nzval); dev_free(&c->d_nzval32);
+  dev_free(&c->d_nzval);
   dev_free(&c->d_ccell); dev_free(&c->d_cnodes); dev_free(&c->d_cdofs); dev_free(&c->d_cnz);
   dev_free(&c->d_cparam); dev_free(&c->d_param_cell); dev_free(&c->d_f); dev_free(&c->d_u);
   dev_free(&c->d_gcnodes); dev_free(&c->d_d2c_ptr); dev_free(&c->d_d2c); dev_free(&c->d_gslot);
@@ -240,7 +240,6 @@ int32_t fem_pattern_upload(fem_ctx* c, int64_t n, const int64_t* colptr, const i
   if ((rc = dev_alloc(c, &c->d_colptr, n + 1)) || (rc = dev_alloc(c, &c->d_rowval, nnz)) ||
       (rc = dev_alloc(c, &c->d_nzval, nnz)) || (rc = dev_alloc(c, &c->d_diag, n)))
     return rc;
-  dev_free(&c->d_nzval32);
   FEM_HIP(c, hipMemcpyAsync(c->d_colptr, cp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, c->stream));
   FEM_HIP(c, hipMemcpyAsync(c->d_rowval, rv.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->stream));
   FEM_HIP(c, hipMemsetAsync(c->d_nzval, 0, sizeof(double) * nnz, c->stream));

@@ -232,7 +232,6 @@ int32_t fem_apply_dirichlet(fem_ctx* c, int32_t applyzero, double* m_out) {
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
   c->timings.apply_ms = event_ms(c);
   if (m_out) FEM_HIP(c, hipMemcpy(m_out, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost));
-  c->nzval32_dirty = true;  // the FP32 copy must be rebuilt from the constrained K
   ++c->nzval_version;
   return FEM_OK;
 }

@@ -458,7 +458,5 @@ extern "C" int32_t fem_point_load(fem_ctx* c, int64_t n, const int64_t* dofs, co
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   dev_free(&dd);
   dev_free(&dv);
-  c->nzval32_dirty = true;
-  ++c->nzval_version;
   return FEM_OK;
 }

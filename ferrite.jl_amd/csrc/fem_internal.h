@@ -70,8 +70,6 @@ struct fem_ctx {
   int32_t* d_rowval = nullptr;
   int32_t* d_diag = nullptr;           // position of K[j,j] in column j (-1 if absent)
   double* d_nzval = nullptr;
-  float* d_nzval32 = nullptr;          // FP32 copy used by the mixed solver
-  bool nzval32_dirty = true;           // nzval changed since the FP32 copy was made
   uint64_t nzval_version = 1;          // bumped whenever nzval changes (solver operator copies track it)
 
   // colors: cells permuted into color-major order on the device

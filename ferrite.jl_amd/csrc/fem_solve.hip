@@ -2,22 +2,26 @@
 // the reference has no iterative solver on this path).
 //
 //  * FEM_SOLVE_CG_FP64   Jacobi-preconditioned CG, FP64 SpMV and vectors.
-//  * FEM_SOLVE_IR_MIXED  FP64 iterative refinement: r = f − K u in FP64, inner
-//                        Jacobi-PCG on an FP32 copy of K with FP32 vectors and
-//                        FP64-accumulated dot products, u += d in FP64.
+//  * FEM_SOLVE_IR_MIXED  the same CG with an FP32 operator and FP32 vectors (FP64 dot
+//                        products), kept on track by reliable updates: whenever its
+//                        residual has fallen by δ the FP32 iterate is folded into the FP64
+//                        u and the FP64 residual f − K u replaces the FP32 one.
 //
-// K is symmetric after apply! (rows and columns of prescribed dofs are zeroed
-// together), so the CSC columns double as CSR rows: y = K x is a row gather
-// y[j] = Σ_p nzval[p] x[rowval[p]] over column j.  The SpMV is CSR-stream: a
-// block owns R consecutive rows whose entries it streams once, fully coalesced,
-// into LDS products, then one lane per row sums its products in storage order.
+// Jacobi preconditioning is applied as a symmetric scaling: the solver works on
+// Â = S K S with S = diag(K)^-1/2, whose values are written once per assembly into the
+// solver's own copy of K.  Unpreconditioned CG on Â x̂ = S f is Jacobi-PCG on K u = f
+// (u = S x̂) in exact arithmetic, but needs neither the z = D⁻¹ r vector nor D⁻¹ in the
+// iteration, so the fused vector step moves 9 instead of 12 vectors' worth of bytes.
 //
-// Reductions are two-level and fence-free: each block writes one partial, a
-// one-block kernel sums the partials in a fixed order and updates the CG
-// scalars (α, β, convergence flag) in device memory, so results are bitwise
-// reproducible and the host never sits in the iteration.  A batch of
-// iterations is captured once into a hipGraph and replayed; kernels launched
-// past convergence return at once on the device flag.
+// K is symmetric after apply! (rows and columns of prescribed dofs are zeroed together),
+// so the CSC columns double as rows: the solver's operator is a sliced ELL copy (slices of
+// 64 rows, one wave each) built from the columns.  The FP64 residual f − K u uses the CSC
+// arrays themselves (CSR-stream kernel).
+//
+// Reductions are fence-free and fixed-order: blocks write partials that the next kernel
+// sums, every block in the same order, so results are bitwise reproducible and the host
+// never sits in the iteration.  A batch of iterations is captured once into a hipGraph and
+// replayed; kernels launched past a hand-over return at once on the device flag.
 #include <cmath>
 #include <cstring>
 
@@ -26,24 +30,24 @@
 namespace fem {
 
 constexpr int kBlock = 256;
-constexpr int kTile = 4 * kBlock;    // elements per block per pass of the vector kernels
 constexpr int kMaxVecBlocks = 2048;
 constexpr int kBatch = 16;           // CG iterations per graph replay
 
+// CG scalars of the scaled system (hat quantities): γ = r̂·r̂, δ = r̂·Â r̂.
 struct Scal {
-  double rz;      // γ = r·z (z = D⁻¹ r) of the current residual
-  double pq;      // p·A p at a reseed
-  double rr;      // r·r (recursive residual; after residual(): the true residual)
+  double rz;      // γ of the current residual
+  double pq;      // p·Â p at a reseed
+  double rr;      // true residual ||f − K u||² after residual() (unscaled)
   double alpha;   // step length of the last step
   double beta;
   double bnorm2;
-  double hold2;   // stop the recurrence (done = 1) once rr < hold2: hand over to the host
+  double gstop;   // hand the recurrence over (done = 1) once γ <= gstop
   double rzp;     // γ of the previous step
-  double gam;     // (r·p)/(r·z) at a reseed: p is made orthogonal to the new residual
-  int done;       // 1: recurrence stopped (converged by its own residual, or handed over)
+  double gam;     // (r̂·p)/(r̂·r̂) at a reseed: p is made orthogonal to the new residual
+  int done;       // 1: recurrence stopped (claims convergence or fell by δ), or broke down
   int iters;
-  int restart;    // breakdown happened: the next reseed restarts p = z
-  int fresh;      // p, s = A p and α were just set by a reseed: the next step only updates x, r
+  int restart;    // breakdown happened: the next reseed restarts p = r̂
+  int fresh;      // p, s = Â p and α were just set by a reseed: the next step only moves x̂, r̂
 };
 
 struct SpmvShape {
@@ -61,8 +65,8 @@ struct GraphCache {
 // Sliced ELL copy of K for the solver (slices of 64 rows = one wave): entry k of row
 // 64 s + l is stored at off[s] + 64 k + l, so a wave's loads of values and column
 // offsets are unit-stride.  Rows shorter than the slice width are padded with value 0
-// and the slice's smallest column.  The values are re-gathered from nzval (src map)
-// whenever K changes; the structure is built once per pattern.
+// and the slice's smallest column.  The scaled values s_i K_ij s_j are re-gathered from
+// nzval (src map) whenever K changes; the structure is built once per pattern.
 struct SellOp {
   int64_t nslices = 0, len = 0;
   int32_t* off = nullptr;      // [nslices + 1] first entry of each slice
@@ -72,7 +76,8 @@ struct SellOp {
   int32_t* src = nullptr;      // [len] position in nzval, −1 for padding
   float* val32 = nullptr;
   double* val64 = nullptr;
-  uint64_t ver32 = 0, ver64 = 0;
+  uint64_t ver32 = 0, ver64 = 0;   // nzval_version the values were scaled from (+1)
+  int jac32 = -1, jac64 = -1;      // the scaling they carry (1 Jacobi, 0 none)
   bool built = false;
 };
 
@@ -80,20 +85,18 @@ struct SolverWork {
   int64_t n = 0;
   SellOp sell;
   int sell_grid = 0;           // blocks of the SELL SpMV (fixed: its partial count)
-  // CG vectors (FP64 solver / FP32 recurrence of the mixed solver): q holds w = A u
-  double *r = nullptr, *p = nullptr, *q = nullptr, *u = nullptr, *sv = nullptr, *invd = nullptr, *res = nullptr;
-  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *u32 = nullptr, *sv32 = nullptr, *x32 = nullptr,
-        *invd32 = nullptr;
-  double* part_s = nullptr;   // SpMV partials, one per row block
-  double* part_v = nullptr;   // vector-kernel partials, two buffers of 2 per block
-  Scal* s = nullptr;          // two scalar records (read one, write the other)
-  hipEvent_t tev[2 * kBatch] = {};   // SpMV start/end events inside the graph (time_kernels)
-  SpmvShape sh64, sh32;
-  int vblocks = 0;
+  SpmvShape sh64;              // CSR-stream FP64 residual
+  // recurrence vectors of the scaled system: x̂, r̂, w = Â r̂, p, s = Â p (FP64 solver / FP32
+  // recurrence of the mixed solver)
+  double *x = nullptr, *r = nullptr, *wv = nullptr, *p = nullptr, *sv = nullptr;
+  float *x32 = nullptr, *r32 = nullptr, *wv32 = nullptr, *p32 = nullptr, *sv32 = nullptr;
+  double* sc = nullptr;        // S = diag(K)^-1/2 (1 without Jacobi)
+  double* res = nullptr;       // FP64 residual f − K u
+  double* part_s = nullptr;    // SpMV partials, one per block
+  double* part_v = nullptr;    // vector-kernel partials, two buffers of 2 per block
+  Scal* s = nullptr;           // two scalar records (read one, write the other)
+  hipEvent_t tev[2 * kBatch] = {};   // SpMV start/end events (time_kernels)
   GraphCache g64, g32;
-  uint16_t* roff32 = nullptr;  // FP32 operator: 16-bit column offsets (nullptr: use rowval)
-  int32_t* bbase32 = nullptr;
-  bool comp_checked = false;
 };
 
 static void destroy_graph(GraphCache& g) {
@@ -108,12 +111,10 @@ void free_solver(fem_ctx* c) {
   destroy_graph(w->g32);
   for (hipEvent_t& e : w->tev)
     if (e) (void)hipEventDestroy(e);
-  dev_free(&w->r); dev_free(&w->p); dev_free(&w->q); dev_free(&w->u); dev_free(&w->sv); dev_free(&w->invd);
-  dev_free(&w->res);
-  dev_free(&w->r32); dev_free(&w->p32); dev_free(&w->q32); dev_free(&w->u32); dev_free(&w->sv32); dev_free(&w->x32);
-  dev_free(&w->invd32);
+  dev_free(&w->x); dev_free(&w->r); dev_free(&w->wv); dev_free(&w->p); dev_free(&w->sv);
+  dev_free(&w->x32); dev_free(&w->r32); dev_free(&w->wv32); dev_free(&w->p32); dev_free(&w->sv32);
+  dev_free(&w->sc); dev_free(&w->res);
   dev_free(&w->part_s); dev_free(&w->part_v); dev_free(&w->s);
-  dev_free(&w->roff32); dev_free(&w->bbase32);
   SellOp& o = w->sell;
   dev_free(&o.off); dev_free(&o.base); dev_free(&o.roff); dev_free(&o.col); dev_free(&o.src);
   dev_free(&o.val32); dev_free(&o.val64);
@@ -172,15 +173,14 @@ __device__ __forceinline__ double wide_sum(const double* __restrict__ part, int 
   return r;
 }
 
+
 // --------------------------------------------------------------------------
-// CSR-stream SpMV
-//   MODE 0: y = K x, partial Σ x·y   (CG: q = K p, p·q)
-//   MODE 1: y = f − K x, partial Σ y² (FP64 residual)
-// Column indices are either the pattern's int32 rowval (COMP = false) or, for the
-// FP32 operator, 16-bit offsets from a per-block base row (COMP = true), which
-// cuts the operator stream from 8 to 6 bytes per nonzero.  In the reduction
-// phase L lanes share a row: lane k sums the row's products k, k+L, ... in order
-// and the L partial sums are combined by a fixed butterfly (deterministic).
+// CSR-stream SpMV over the CSC arrays (the FP64 residual of the unscaled K):
+//   MODE 0: y = K x, partial Σ x·y ;  MODE 1: y = f − K x, partial Σ y²
+// A block owns R consecutive rows whose entries it streams once, fully coalesced, into
+// LDS products; then L lanes per row sum them (lane k: products k, k+L, ... in order,
+// combined by a fixed butterfly: deterministic).  Column indices are the pattern's int32
+// rowval (COMP = false) or 16-bit offsets from a per-block base (COMP = true).
 // --------------------------------------------------------------------------
 // Loads N consecutive elements with 16-byte vector loads (the address is N-element aligned).
 template <typename T, int N>
@@ -277,6 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, int R, const int32_t
   if (tid == 0) part[blockIdx.x] = b;
 }
 
+
 // --------------------------------------------------------------------------
 // Sliced ELL (SELL-64) SpMV: one wave per slice of 64 rows, lane l owns row 64 s + l and
 // sums its entries in pattern order (deterministic).  Value and offset loads are
@@ -296,7 +297,9 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, int64_t nslices, con
                                                  X* __restrict__ y, double* __restrict__ part,
                                                  const Scal* __restrict__ s) {
   using A = typename std::conditional<sizeof(X) == 8, double, float>::type;
-  if (MODE == 0 && s->done) return;
+  // nothing to do once the recurrence has stopped, nor in the step right after a reseed
+  // (that step moves x̂ and r̂ along the p, s the reseed built and reads neither y nor δ)
+  if (MODE == 0 && (s->done || s->fresh)) return;
   __shared__ double sh[kBlock / 64];
   constexpr int WPB = kBlock / 64;
   const int lane = threadIdx.x & 63;
@@ -410,43 +413,44 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(int64_t n, int64_t nslices
   }
 }
 
-template <typename V>
-__global__ void k_sell_vals(int64_t len, const int32_t* __restrict__ src, const double* __restrict__ nzval,
-                            V* __restrict__ val) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < len; e += (int64_t)gridDim.x * blockDim.x) {
-    const int32_t p = src[e];
-    val[e] = p >= 0 ? (V)nzval[p] : V(0);
+// Scaled values of the sliced copy: val[e] = s_row K[row, col] s_col (0 on padding); one
+// wave per slice like the SpMV, 8 entries per lane in flight.
+template <typename V, bool COMP>
+__global__ __launch_bounds__(kBlock) void k_sell_vals(int64_t n, int64_t nslices, const int32_t* __restrict__ off,
+                                                      const int32_t* __restrict__ base,
+                                                      const uint16_t* __restrict__ roff,
+                                                      const int32_t* __restrict__ col, const int32_t* __restrict__ src,
+                                                      const double* __restrict__ nzval, const double* __restrict__ sc,
+                                                      V* __restrict__ val) {
+  constexpr int U = 8;
+  for (int64_t sl = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); sl < nslices;
+       sl += (int64_t)gridDim.x * (kBlock / 64)) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = sl * 64 + lane;
+    const int32_t o0 = off[sl];
+    const int W = (off[sl + 1] - o0) >> 6;
+    const int32_t b = COMP ? base[sl] : 0;
+    const double sr = row < n ? sc[row] : 0.0;
+    for (int k0 = 0; k0 < W; k0 += U) {
+      int32_t p[U], ci[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t e = o0 + 64 * (k0 + u) + lane;
+        const bool in = k0 + u < W;
+        p[u] = in ? src[e] : -1;
+        ci[u] = in ? (COMP ? b + (int32_t)roff[e] : col[e]) : 0;
+      }
+      double a[U], sj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        a[u] = p[u] >= 0 ? nzval[p[u]] : 0.0;
+        sj[u] = p[u] >= 0 ? sc[ci[u]] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (k0 + u < W) val[o0 + 64 * (k0 + u) + lane] = (V)(a[u] * sr * sj[u]);
+    }
   }
-}
-
-// Per row block of the FP32 operator: base = smallest column index, offsets = index − base.
-// span_out receives the largest (max − min) over all blocks (the host checks it < 65536).
-__global__ __launch_bounds__(kBlock) void k_compress_rows(int64_t n, int R, const int32_t* __restrict__ colptr,
-                                                          const int32_t* __restrict__ rowval,
-                                                          uint16_t* __restrict__ roff, int32_t* __restrict__ bbase,
-                                                          int* __restrict__ span_out) {
-  __shared__ int32_t smin[kBlock / 64], smax[kBlock / 64];
-  const int64_t r0 = (int64_t)blockIdx.x * R;
-  const int nr = (int)min<int64_t>(R, n - r0);
-  const int32_t e0 = colptr[r0], e1 = colptr[r0 + nr];
-  int32_t mn = INT32_MAX, mx = INT32_MIN;
-  for (int32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
-    mn = min(mn, rowval[e]);
-    mx = max(mx, rowval[e]);
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    mn = min(mn, __shfl_down(mn, o, 64));
-    mx = max(mx, __shfl_down(mx, o, 64));
-  }
-  if ((threadIdx.x & 63) == 0) { smin[threadIdx.x >> 6] = mn; smax[threadIdx.x >> 6] = mx; }
-  __syncthreads();
-  mn = smin[0]; mx = smax[0];
-  for (int i = 1; i < kBlock / 64; ++i) { mn = min(mn, smin[i]); mx = max(mx, smax[i]); }
-  if (threadIdx.x == 0) {
-    bbase[blockIdx.x] = mn;
-    atomicMax(span_out, mx - mn);
-  }
-  for (int32_t e = e0 + threadIdx.x; e < e1; e += kBlock) roff[e] = (uint16_t)min(rowval[e] - mn, 65535);
 }
 
 // s->rr (which = 0) or s->bnorm2 (which = 1) = Σ part, fixed order
@@ -468,13 +472,13 @@ __global__ __launch_bounds__(kBlock) void k_sumsq(int64_t n, const double* __res
   if (threadIdx.x == 0) part[blockIdx.x] = b;
 }
 
-template <typename X>
-__global__ void k_invdiag(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ nzval,
-                          int jacobi, X* __restrict__ invd) {
+// S = diag(K)^-1/2 (Jacobi; rows with a zero diagonal keep 1), or 1 without Jacobi.
+__global__ void k_scale(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ nzval, int jacobi,
+                        double* __restrict__ sc) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
     const int32_t p = diag[j];
     const double d = (p >= 0) ? nzval[p] : 0.0;
-    invd[j] = (jacobi && d != 0.0) ? (X)(1.0 / d) : (X)1.0;
+    sc[j] = (jacobi && d != 0.0) ? 1.0 / sqrt(fabs(d)) : 1.0;
   }
 }
 
@@ -483,153 +487,120 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
     b[i] = (float)a[i];
 }
 
-// Reliable update, step 1: fold the FP32 correction into the FP64 iterate: u += x32, x32 = 0.
-__global__ void k_fold(int64_t n, float* __restrict__ x32, double* __restrict__ u) {
+// Hand-over, step 1: fold the recurrence's iterate into the FP64 u: u += S x̂, x̂ = 0.
+template <typename X>
+__global__ void k_fold(int64_t n, X* __restrict__ x, const double* __restrict__ sc, double* __restrict__ u) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    u[i] += (double)x32[i];
-    x32[i] = 0.0f;
+    u[i] += sc[i] * (double)x[i];
+    x[i] = X(0);
   }
 }
 
 // --------------------------------------------------------------------------
-// Chronopoulos–Gear preconditioned CG (one reduction phase per iteration).
-//   state: x, r, u = D⁻¹ r, w = A u, p, s = A p
-//   step i (after the SpMV w = A u, partial δ = w·u):
-//     γ = r·u, rr = r·r (partials of the previous step), β = γ/γ_prev,
-//     α = γ / (δ − β γ / α_prev), p = u + β p, s = w + β s, x += α p, r −= α s,
-//     u = D⁻¹ r, partials of the next γ and rr.
+// Chronopoulos–Gear CG on Â x̂ = b̂ (one reduction phase per iteration).
+//   state: x̂, r̂, w = Â r̂, p, s = Â p
+//   step i (after the SpMV w = Â r̂, partial δ = r̂·w):
+//     γ = r̂·r̂ (partials of the previous step), β = γ/γ_prev,
+//     α = γ / (δ − β γ / α_prev), p = r̂ + β p, s = w + β s, x̂ += α p, r̂ −= α s,
+//     partials of the next γ.
 // Every block reduces the partials itself (same fixed order in every block, hence the
 // same scalars bitwise), so an iteration is two kernels: SpMV, k_cg_step.  The scalar
 // records and the vector partials are double-buffered (read one, write the other).
 // --------------------------------------------------------------------------
-constexpr int kCgBlocks = 2048;  // most blocks of k_cg_step (its partial count)
+constexpr int kCgBlocks = 4096;  // most blocks of k_cg_step (its partial count)
 
-// Fixed-order sum of part[0, m) (stride apart) by the whole block; the result in every thread.
-// Each thread issues its first 16 loads before adding them, so for m <= 16 kBlock the whole
-// sum costs about one memory latency.
-__device__ __forceinline__ double block_sum_all(const double* __restrict__ part, int m, int stride, double* sh) {
-  constexpr int kPer = 16;
-  double t[kPer];
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int i = k * kBlock + threadIdx.x;
-    t[k] = i < m ? part[(int64_t)i * stride] : 0.0;
-  }
-  double v = 0.0;
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) v += t[k];
-  for (int i = kPer * kBlock + threadIdx.x; i < m; i += kBlock) v += part[(int64_t)i * stride];   // m > 16 kBlock
-  v = wave_sum(v);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = 0.0;
-#pragma unroll
-  for (int i = 0; i < kBlock / 64; ++i) r += sh[i];
-  __syncthreads();
-  return r;
-}
-
-// γ = Σ pin[2i], rr = Σ pin[2i+1] (i < nin) and δ = Σ ps[i] (i < na) in one round: all
-// loads are issued before any add; fixed order, the same in every block and thread.
-__device__ __forceinline__ void reduce3(const double* __restrict__ pin, int nin, const double* __restrict__ ps,
-                                        int na, double (*sh)[kBlock / 64], double& gam, double& rr, double& del) {
-  constexpr int kPin = 8, kPs = 8;   // up to 2048 pairs and 2048 SpMV partials in flight
-  double a[kPin], b[kPin], d[kPs];
+// γ = Σ pin[i] (i < nin) and δ = Σ ps[i] (i < na) in one round: all loads are issued before
+// any add; fixed order, the same in every block and thread.
+__device__ __forceinline__ void reduce2(const double* __restrict__ pin, int nin, const double* __restrict__ ps,
+                                        int na, double (*sh)[kBlock / 64], double& gam, double& del) {
+  constexpr int kPin = 8, kPs = 8;   // up to 2048 partials of each kind in flight
+  double a[kPin], d[kPs];
 #pragma unroll
   for (int k = 0; k < kPin; ++k) {
     const int i = k * kBlock + threadIdx.x;
-    a[k] = i < nin ? pin[2 * i] : 0.0;
-    b[k] = i < nin ? pin[2 * i + 1] : 0.0;
+    a[k] = i < nin ? pin[i] : 0.0;
   }
 #pragma unroll
   for (int k = 0; k < kPs; ++k) {
     const int i = k * kBlock + threadIdx.x;
     d[k] = i < na ? ps[i] : 0.0;
   }
-  double va = 0.0, vb = 0.0, vd = 0.0;
+  double va = 0.0, vd = 0.0;
 #pragma unroll
-  for (int k = 0; k < kPin; ++k) { va += a[k]; vb += b[k]; }
+  for (int k = 0; k < kPin; ++k) va += a[k];
 #pragma unroll
   for (int k = 0; k < kPs; ++k) vd += d[k];
-  for (int i = kPin * kBlock + threadIdx.x; i < nin; i += kBlock) { va += pin[2 * i]; vb += pin[2 * i + 1]; }
+  for (int i = kPin * kBlock + threadIdx.x; i < nin; i += kBlock) va += pin[i];
   for (int i = kPs * kBlock + threadIdx.x; i < na; i += kBlock) vd += ps[i];
   va = wave_sum(va);
-  vb = wave_sum(vb);
   vd = wave_sum(vd);
   if ((threadIdx.x & 63) == 0) {
     sh[0][threadIdx.x >> 6] = va;
-    sh[1][threadIdx.x >> 6] = vb;
-    sh[2][threadIdx.x >> 6] = vd;
+    sh[1][threadIdx.x >> 6] = vd;
   }
   __syncthreads();
-  gam = rr = del = 0.0;
+  gam = del = 0.0;
 #pragma unroll
   for (int w = 0; w < kBlock / 64; ++w) {
     gam += sh[0][w];
-    rr += sh[1][w];
-    del += sh[2][w];
+    del += sh[1][w];
   }
 }
 
-constexpr int kCgElems = 8;   // vector elements per thread per pass of k_cg_step
+// Vector elements per thread per pass of k_cg_step (tuning cg_elems; measured default 8).
+static int cg_elems(const fem_ctx* c) { return c->tune.cg_elems ? c->tune.cg_elems : 8; }
 
-// Blocks of k_cg_step for n unknowns: one pass over the vectors when possible.
-static int cg_blocks(int64_t n) {
-  return (int)std::max<int64_t>(1, std::min<int64_t>(kCgBlocks, (n + kBlock * kCgElems - 1) / (kBlock * kCgElems)));
+// Blocks of k_cg_step (and of the reseed kernels) for n unknowns: one pass over the vectors
+// when possible.
+static int cg_blocks(const fem_ctx* c) {
+  const int64_t per = (int64_t)kBlock * cg_elems(c);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kCgBlocks, (c->n + per - 1) / per));
 }
 
-template <typename X>
+template <typename X, int E>
 __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, X* __restrict__ x, X* __restrict__ r,
-                                                    X* __restrict__ u, const X* __restrict__ w, X* __restrict__ p,
-                                                    X* __restrict__ sv, const X* __restrict__ invd, int na,
-                                                    const double* __restrict__ part_s, const double* __restrict__ part_in,
-                                                    double* __restrict__ part_out, const Scal* __restrict__ sin,
-                                                    Scal* __restrict__ sout, double tol2) {
-  __shared__ double sh[3][kBlock / 64];
+                                                    const X* __restrict__ w, X* __restrict__ p, X* __restrict__ sv,
+                                                    int na, const double* __restrict__ part_s,
+                                                    const double* __restrict__ part_in, double* __restrict__ part_out,
+                                                    const Scal* __restrict__ sin, Scal* __restrict__ sout) {
+  __shared__ double sh[2][kBlock / 64];
   const Scal S = *sin;
   if (S.done) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *sout = S;
     return;
   }
   const bool fresh = S.fresh != 0;
-  constexpr int E = kCgElems;
   const int64_t stride = (int64_t)gridDim.x * kBlock * E;
   int64_t base = (int64_t)blockIdx.x * kBlock * E;
   // this thread's first elements: their loads do not depend on the scalars, so they are in
   // flight while the partials are reduced
-  X uv[E], wv[E], pv[E], sw[E], xv[E], rv[E], dv[E];
+  X wv[E], pv[E], sw[E], xv[E], rv[E];
   auto load = [&](int64_t b0) {
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const int64_t i = b0 + k * kBlock + threadIdx.x;
       const bool in = i < n;
-      uv[k] = in ? u[i] : X(0);
-      wv[k] = in ? w[i] : X(0);
+      wv[k] = in && !fresh ? w[i] : X(0);
       pv[k] = in ? p[i] : X(0);
       sw[k] = in ? sv[i] : X(0);
       xv[k] = in ? x[i] : X(0);
       rv[k] = in ? r[i] : X(0);
-      dv[k] = in ? invd[i] : X(0);
     }
   };
   load(base);
-  double gam = S.rz, rr = S.rr, alpha = S.alpha, beta = 0.0;
+  double gam = S.rz, alpha = S.alpha, beta = 0.0;
   if (!fresh) {
     double del;
-    reduce3(part_in, gridDim.x, part_s, na, sh, gam, rr, del);
+    reduce2(part_in, gridDim.x, part_s, na, sh, gam, del);
     beta = gam / S.rzp;
     const double den = del - beta * gam / S.alpha;
-    // converged (by the recurrence), or dropped below the hand-over level: the iterations
-    // after that point drift from the true residual and are lost at the reliable update
-    const bool stop = rr <= tol2 || rr < S.hold2;
+    const bool stop = gam <= S.gstop;
     const bool brk = !(S.rzp > 0.0) || !(S.alpha != 0.0) || !(den > 0.0);
-    if (stop || brk) {   // hand over to the host: x, r, u of this step, p and s of the last one
+    if (stop || brk) {   // hand over to the host: x̂, r̂ of this step, p and s of the last one
       if (blockIdx.x == 0 && threadIdx.x == 0) {
         Scal o = S;
         o.done = 1;
         o.rz = gam;
-        o.rr = rr;
         if (!stop) o.restart = 1;
         *sout = o;
       }
@@ -638,7 +609,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, X* __restrict__ x
     alpha = gam / den;
   }
   const X al = (X)alpha, be = (X)beta;
-  double ga = 0.0, ra = 0.0;
+  double ga = 0.0;
   while (true) {
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -646,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, X* __restrict__ x
       if (i < n) {
         X pi = pv[k], si = sw[k];
         if (!fresh) {
-          pi = uv[k] + be * pi;
+          pi = rv[k] + be * pi;
           si = wv[k] + be * si;
           p[i] = pi;
           sv[i] = si;
@@ -654,20 +625,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, X* __restrict__ x
         x[i] = xv[k] + al * pi;
         const X ri = rv[k] - al * si;
         r[i] = ri;
-        const X ui = ri * dv[k];
-        u[i] = ui;
-        ga += (double)ri * (double)ui;
-        ra += (double)ri * (double)ri;
+        ga += (double)ri * (double)ri;
       }
     }
     base += stride;
     if (base >= n) break;
     load(base);
   }
-  double b = block_sum(ga, sh[0]);
-  if (threadIdx.x == 0) part_out[2 * blockIdx.x] = b;
-  b = block_sum(ra, sh[0]);
-  if (threadIdx.x == 0) part_out[2 * blockIdx.x + 1] = b;
+  const double b = block_sum(ga, sh[0]);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = b;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     Scal o = S;
     o.fresh = 0;
@@ -675,37 +641,37 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, X* __restrict__ x
     o.alpha = alpha;
     o.beta = beta;
     o.rz = gam;
-    o.rr = rr;
     o.iters = S.iters + 1;
     *sout = o;
   }
 }
 
-// Reseed, step 1: r = res (rounded to X), u = D⁻¹ r; partials r·u and r·p.
+// Reseed, step 1: r̂ = S res (rounded to X); partials r̂·r̂ and r̂·p.
 template <typename X>
 __global__ __launch_bounds__(kBlock) void k_cg_reseed(int64_t n, const double* __restrict__ res,
-                                                      const X* __restrict__ invd, X* __restrict__ r, X* __restrict__ u,
+                                                      const double* __restrict__ sc, X* __restrict__ r,
                                                       const X* __restrict__ p, double* __restrict__ part) {
   __shared__ double sh[kBlock / 64];
-  double ru = 0.0, rp = 0.0;
+  double rr = 0.0, rp = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const X ri = (X)res[i];
-    const X ui = ri * invd[i];
+    const X ri = (X)(sc[i] * res[i]);
     r[i] = ri;
-    u[i] = ui;
-    ru += (double)ri * (double)ui;
+    rr += (double)ri * (double)ri;
     rp += (double)ri * (double)p[i];
   }
-  double b = block_sum(ru, sh);
+  double b = block_sum(rr, sh);
   if (threadIdx.x == 0) part[2 * blockIdx.x] = b;
   b = block_sum(rp, sh);
   if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = b;
 }
 
-// Reseed, step 2 (scalars): γ = r·u, γ' = (r·p)/γ, β = γ/γ_prev (0: restart p = u).
-// s->rr holds the true residual when this runs; the next hand-over is at hold_factor2 × it.
+// Reseed, step 2 (scalars): γ = r̂·r̂, γ' = (r̂·p)/γ, β = γ/γ_prev (0: restart p = r̂).
+// s->rr holds the true residual ||f − K u||² when this runs.  The next hand-over is when γ
+// has fallen by hold_factor2, or to where the true residual would meet tol2 if ||r||²/γ
+// stayed as it is now (the host then checks the true residual itself).
 __global__ __launch_bounds__(kBlock) void k_cg_reseed_scal(int m, const double* __restrict__ part,
-                                                           Scal* __restrict__ s, double hold_factor2, int restart) {
+                                                           Scal* __restrict__ s, double hold_factor2, double tol2,
+                                                           int restart) {
   __shared__ double sh[kBlock / 64];
   const double ru = sum_parts(part, m, 2, sh);
   const double rp = sum_parts(part + 1, m, 2, sh);
@@ -714,25 +680,29 @@ __global__ __launch_bounds__(kBlock) void k_cg_reseed_scal(int m, const double* 
     s->gam = ru > 0.0 ? rp / ru : 0.0;
     s->beta = fresh ? 0.0 : ru / s->rzp;
     s->rz = ru;
-    s->hold2 = hold_factor2 * s->rr;
+    const double conv = s->rr > 0.0 ? tol2 / s->rr : 0.0;
+    s->gstop = ru * (hold_factor2 > conv ? hold_factor2 : conv);
     s->done = 0;
     s->restart = 0;
+    s->fresh = 0;
   }
 }
 
-// Reseed, step 3 (after w = A u): p = u + β (p − γ' u), s = w + β (s − γ' w): the last
-// direction made orthogonal to the new residual, then extended; partial p·s.
+// Reseed, step 3 (after w = Â r̂): p = r̂ + β (p − γ' r̂), s = w + β (s − γ' w): the last
+// direction made orthogonal to the new residual, then extended (p = r̂, s = w on a
+// restart); partial p·s.
 template <typename X>
-__global__ __launch_bounds__(kBlock) void k_cg_pfix(int64_t n, const X* __restrict__ u, const X* __restrict__ w,
+__global__ __launch_bounds__(kBlock) void k_cg_pfix(int64_t n, const X* __restrict__ r, const X* __restrict__ w,
                                                     X* __restrict__ p, X* __restrict__ sv, const Scal* __restrict__ s,
                                                     double* __restrict__ part) {
   __shared__ double sh[kBlock / 64];
   const X be = (X)s->beta, ga = (X)s->gam;
+  const bool keep = s->beta != 0.0;
   double ps = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    const X ui = u[i], wi = w[i];
-    const X pi = ui + be * (p[i] - ga * ui);
-    const X si = wi + be * (sv[i] - ga * wi);
+    const X ri = r[i], wi = w[i];
+    const X pi = keep ? ri + be * (p[i] - ga * ri) : ri;
+    const X si = keep ? wi + be * (sv[i] - ga * wi) : wi;
     p[i] = pi;
     sv[i] = si;
     ps += (double)pi * (double)si;
@@ -742,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_pfix(int64_t n, const X* __restri
   if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = 0.0;
 }
 
-// Reseed, step 4 (scalars): α = γ / p·s; the next k_cg_step only moves x and r.
+// Reseed, step 4 (scalars): α = γ / p·s; the next k_cg_step only moves x̂ and r̂.
 __global__ __launch_bounds__(kBlock) void k_cg_alpha(int m, const double* __restrict__ part, Scal* __restrict__ s) {
   __shared__ double sh[kBlock / 64];
   const double ps = sum_parts(part, m, 2, sh);
@@ -751,7 +721,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(int m, const double* __rest
     const bool ok = ps > 0.0;
     s->alpha = ok ? s->rz / ps : 0.0;
     s->fresh = ok ? 1 : 0;
-    if (!ok) {   // p vanished (r already zero): nothing to step
+    if (!ok) {   // p vanished (r̂ already zero): nothing to step
       s->done = 1;
       s->restart = 1;
     }
@@ -764,11 +734,6 @@ __global__ void k_cg_init_scal(Scal* __restrict__ s, double rr0) {
   s[1] = z;
   z.rr = rr0;
   s[0] = z;
-}
-
-__global__ void k_f32_from(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = (float)a[i];
 }
 
 __global__ void k_zero32(int64_t n, float* __restrict__ a) {
@@ -784,6 +749,12 @@ __global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restri
 __global__ void k_zero(int64_t n, double* __restrict__ a) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     a[i] = 0.0;
+}
+
+template <typename X>
+static void zero_vec(fem_ctx* c, X* a) {
+  if constexpr (sizeof(X) == 4) k_zero32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, a);
+  else k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, a);
 }
 
 // --------------------------------------------------------------------------
@@ -829,68 +800,28 @@ static int ensure_work(fem_ctx* c, bool mixed) {
   SolverWork* w = c->solver;
   const int64_t n = c->n;
   int rc;
-  if (w->n != n || !w->r) {
+  if (w->n != n || !w->res) {
     destroy_graph(w->g64);
     destroy_graph(w->g32);
     w->sh64 = spmv_shape(c, c->tune.spmv_cap64 ? c->tune.spmv_cap64 : 2048, c->tune.spmv_lanes64 ? c->tune.spmv_lanes64 : 2);
-    w->sh32 = spmv_shape(c, c->tune.spmv_cap32 ? c->tune.spmv_cap32 : 2048, c->tune.spmv_lanes32 ? c->tune.spmv_lanes32 : 4);
-    dev_free(&w->roff32);
-    dev_free(&w->bbase32);
-    w->comp_checked = false;
-    w->vblocks = (int)std::min<int64_t>(kMaxVecBlocks, (n + kTile - 1) / kTile);
-    const int ps = std::max(w->sh64.blocks, w->sh32.blocks);
-    if ((rc = dev_alloc(c, &w->r, n)) || (rc = dev_alloc(c, &w->p, n)) || (rc = dev_alloc(c, &w->q, n)) ||
-        (rc = dev_alloc(c, &w->u, n)) || (rc = dev_alloc(c, &w->sv, n)) ||
-        (rc = dev_alloc(c, &w->invd, n)) || (rc = dev_alloc(c, &w->res, n)) ||
-        (rc = dev_alloc(c, &w->part_s, std::max(ps, kMaxVecBlocks))) ||
+    if ((rc = dev_alloc(c, &w->sc, n)) || (rc = dev_alloc(c, &w->res, n)) ||
+        (rc = dev_alloc(c, &w->part_s, std::max(w->sh64.blocks, kMaxVecBlocks))) ||
         (rc = dev_alloc(c, &w->part_v, 4 * std::max(kMaxVecBlocks, kCgBlocks))) || (rc = dev_alloc(c, &w->s, 2)))
       return rc;
-    dev_free(&w->r32);
     w->n = n;
   }
-  if (mixed && !w->r32) {
-    if ((rc = dev_alloc(c, &w->r32, n)) || (rc = dev_alloc(c, &w->p32, n)) || (rc = dev_alloc(c, &w->q32, n)) ||
-        (rc = dev_alloc(c, &w->u32, n)) || (rc = dev_alloc(c, &w->sv32, n)) ||
-        (rc = dev_alloc(c, &w->x32, n)) || (rc = dev_alloc(c, &w->invd32, n)))
+  if (!mixed && !w->x) {
+    if ((rc = dev_alloc(c, &w->x, n)) || (rc = dev_alloc(c, &w->r, n)) || (rc = dev_alloc(c, &w->wv, n)) ||
+        (rc = dev_alloc(c, &w->p, n)) || (rc = dev_alloc(c, &w->sv, n)))
+      return rc;
+  }
+  if (mixed && !w->x32) {
+    if ((rc = dev_alloc(c, &w->x32, n)) || (rc = dev_alloc(c, &w->r32, n)) || (rc = dev_alloc(c, &w->wv32, n)) ||
+        (rc = dev_alloc(c, &w->p32, n)) || (rc = dev_alloc(c, &w->sv32, n)))
       return rc;
   }
   return FEM_OK;
 }
-
-// FP32 operator of the mixed solver: values (nzval32) and, when every row block spans fewer
-// than 65536 columns, 16-bit block-relative column offsets.
-static int ensure_operator32(fem_ctx* c, SolverWork* w) {
-  int rc;
-  if (!c->d_nzval32) {
-    if ((rc = dev_alloc(c, &c->d_nzval32, c->nnz))) return rc;
-    c->nzval32_dirty = true;
-  }
-  if (c->nzval32_dirty) {
-    k_to_f32<<<8192, kBlock, 0, c->stream>>>(c->nnz, c->d_nzval, c->d_nzval32);
-    c->nzval32_dirty = false;
-  }
-  if (w->comp_checked) return FEM_OK;
-  w->comp_checked = true;
-  if (c->tune.spmv_compress32 < 0) return FEM_OK;
-  int* dspan = nullptr;
-  if ((rc = dev_alloc(c, &w->roff32, c->nnz)) || (rc = dev_alloc(c, &w->bbase32, w->sh32.blocks)) ||
-      (rc = dev_alloc(c, &dspan, 1)))
-    return rc;
-  FEM_HIP(c, hipMemsetAsync(dspan, 0, sizeof(int), c->stream));
-  k_compress_rows<<<w->sh32.blocks, kBlock, 0, c->stream>>>(c->n, w->sh32.R, c->d_colptr, c->d_rowval, w->roff32,
-                                                            w->bbase32, dspan);
-  int span = 0;
-  FEM_HIP(c, hipMemcpyAsync(&span, dspan, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  FEM_HIP(c, hipStreamSynchronize(c->stream));
-  dev_free(&dspan);
-  if (span >= 65536) {
-    dev_free(&w->roff32);
-    dev_free(&w->bbase32);
-  }
-  return FEM_OK;
-}
-
-static bool use_sell(const fem_ctx* c) { return c->tune.spmv_format != 1; }
 
 // SELL structure (once per pattern): widths/bases on the device, offsets on the host.
 static int ensure_sell(fem_ctx* c, SolverWork* w) {
@@ -919,7 +850,7 @@ static int ensure_sell(fem_ctx* c, SolverWork* w) {
     if (acc >= INT32_MAX) return set_error(c, FEM_ERR_ARG, "matrix too large for the sliced solver operator");
   }
   off[ns] = (int32_t)acc;
-  const bool comp = maxspan < 65536 && c->tune.spmv_compress32 >= 0;
+  const bool comp = maxspan < 65536 && c->tune.sell_compress >= 0;
   if ((rc = dev_alloc(c, &o.off, ns + 1)) || (rc = dev_alloc(c, &o.src, acc))) return rc;
   if (comp) rc = dev_alloc(c, &o.roff, acc);
   else rc = dev_alloc(c, &o.col, acc);
@@ -936,6 +867,7 @@ static int ensure_sell(fem_ctx* c, SolverWork* w) {
   o.nslices = ns;
   o.len = acc;
   o.ver32 = o.ver64 = 0;
+  o.jac32 = o.jac64 = -1;
   // a multiple of 8 (one share per XCD) when there are enough slices
   int64_t sg = std::min<int64_t>(kMaxVecBlocks, sb);
   if (sg >= 8) sg -= sg % 8;
@@ -944,21 +876,29 @@ static int ensure_sell(fem_ctx* c, SolverWork* w) {
   return FEM_OK;
 }
 
-// SELL values of the current K (re-gathered only when nzval changed).
+// Scaled SELL values of the current K (re-gathered when nzval or the scaling changed).
 template <typename V>
-static int ensure_sell_vals(fem_ctx* c, SolverWork* w) {
+static int ensure_sell_vals(fem_ctx* c, SolverWork* w, int jacobi) {
   SellOp& o = w->sell;
   V*& val = sizeof(V) == 4 ? reinterpret_cast<V*&>(o.val32) : reinterpret_cast<V*&>(o.val64);
   uint64_t& ver = sizeof(V) == 4 ? o.ver32 : o.ver64;
+  int& jac = sizeof(V) == 4 ? o.jac32 : o.jac64;
   if (!val) {
     int rc = dev_alloc(c, &val, o.len);
     if (rc) return rc;
     ver = 0;
   }
-  if (ver != c->nzval_version) {
-    k_sell_vals<V><<<8192, kBlock, 0, c->stream>>>(o.len, o.src, c->d_nzval, val);
+  if (ver != c->nzval_version + 1 || jac != jacobi) {
+    const int sb = (int)std::min<int64_t>(8192, (o.nslices + kBlock / 64 - 1) / (kBlock / 64));
+    if (o.roff)
+      k_sell_vals<V, true><<<sb, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, o.roff, nullptr, o.src,
+                                                          c->d_nzval, w->sc, val);
+    else
+      k_sell_vals<V, false><<<sb, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, nullptr, o.col, o.src,
+                                                           c->d_nzval, w->sc, val);
     FEM_HIP(c, hipGetLastError());
-    ver = c->nzval_version;
+    ver = c->nzval_version + 1;
+    jac = jacobi;
   }
   return FEM_OK;
 }
@@ -988,46 +928,36 @@ static void launch_sell(fem_ctx* c, const SellOp& o, int g, const V* val, const 
 #undef FEM_SELL
 }
 
-// The solver's SpMV in the active format; returns the number of partials it wrote.
-template <typename X, int MODE>
-static int spmv_any(fem_ctx* c, SolverWork* w, const X* x, const double* f, X* y, const Scal* sc) {
-  if (use_sell(c)) {
-    const SellOp& o = w->sell;
-    const int g = w->sell_grid;
-    if constexpr (sizeof(X) == 4) {
-      if (o.roff) launch_sell<float, float, MODE, true>(c, o, g, o.val32, x, f, y, w->part_s, sc);
-      else launch_sell<float, float, MODE, false>(c, o, g, o.val32, x, f, y, w->part_s, sc);
-    } else {
-      if (o.roff) launch_sell<double, double, MODE, true>(c, o, g, o.val64, x, f, y, w->part_s, sc);
-      else launch_sell<double, double, MODE, false>(c, o, g, o.val64, x, f, y, w->part_s, sc);
-    }
-    return g;
-  }
-  if constexpr (sizeof(X) == 4) {
-    launch_spmv<float, float, MODE>(c, w->sh32, c->d_nzval32, x, f, y, w->part_s, sc, w->roff32, w->bbase32);
-    return w->sh32.blocks;
-  } else {
-    launch_spmv<double, double, MODE>(c, w->sh64, c->d_nzval, x, f, y, w->part_s, sc);
-    return w->sh64.blocks;
-  }
-}
-
-// The FP32 or FP64 operator of the solver, ready for spmv_any.
+// y = Â x with the solver's sliced operator (MODE 0, partial x·y per block); returns the
+// number of partials written.
 template <typename X>
-static int prepare_operator(fem_ctx* c, SolverWork* w) {
-  int rc;
-  if (use_sell(c)) {
-    if ((rc = ensure_sell(c, w))) return rc;
-    return ensure_sell_vals<X>(c, w);
+static int spmv_hat(fem_ctx* c, SolverWork* w, const X* x, X* y, const Scal* sc) {
+  const SellOp& o = w->sell;
+  const int g = w->sell_grid;
+  if constexpr (sizeof(X) == 4) {
+    if (o.roff) launch_sell<float, float, 0, true>(c, o, g, o.val32, x, nullptr, y, w->part_s, sc);
+    else launch_sell<float, float, 0, false>(c, o, g, o.val32, x, nullptr, y, w->part_s, sc);
+  } else {
+    if (o.roff) launch_sell<double, double, 0, true>(c, o, g, o.val64, x, nullptr, y, w->part_s, sc);
+    else launch_sell<double, double, 0, false>(c, o, g, o.val64, x, nullptr, y, w->part_s, sc);
   }
-  if constexpr (sizeof(X) == 4) return ensure_operator32(c, w);
-  return FEM_OK;
+  return g;
 }
 
-// With time_kernels: add the SpMV durations of the batch just run (the stream is synchronized).
-static void add_spmv_times(fem_ctx* c, SolverWork* w) {
+// The scaling S and the scaled operator in X precision, ready for spmv_hat.
+template <typename X>
+static int prepare_operator(fem_ctx* c, SolverWork* w, int jacobi) {
+  int rc;
+  if ((rc = ensure_sell(c, w))) return rc;
+  k_scale<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_diag, c->d_nzval, jacobi, w->sc);
+  return ensure_sell_vals<X>(c, w, jacobi);
+}
+
+// With time_kernels: add the durations of the SpMVs [first, first + count) of the batch just
+// run (the stream is synchronized) — the ones that did work, not those that returned at once.
+static void add_spmv_times(fem_ctx* c, SolverWork* w, int first, int count) {
   if (!c->tune.time_kernels) return;
-  for (int k = 0; k < kBatch; ++k) {
+  for (int k = first; k < first + count && k < kBatch; ++k) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, w->tev[2 * k], w->tev[2 * k + 1]) == hipSuccess) {
       c->timings.spmv_ms += ms;
@@ -1045,35 +975,40 @@ static int read_scal(fem_ctx* c, SolverWork* w, Scal* h) {   // scalar record 0 
 // The vectors of one CG recurrence (FP64 solver or FP32 recurrence of the mixed solver).
 template <typename X>
 struct CgVecs {
-  X *x, *r, *u, *w, *p, *sv;
-  const X* invd;
+  X *x, *r, *w, *p, *sv;
 };
 
-// One Chronopoulos–Gear iteration: w = A u (partial w·u), then the fused vector step.
+// One Chronopoulos–Gear iteration: w = Â r̂ (partial r̂·w), then the fused vector step.
 // `k` is the position in the captured batch: it picks the scalar/partial buffers.
 template <typename X>
-static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int k, double tol2, bool timed) {
+static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int k, bool timed) {
   Scal* sin = w->s + (k & 1);
   Scal* sout = w->s + ((k + 1) & 1);
   double* pin = w->part_v + (k & 1) * 2 * kCgBlocks;
   double* pout = w->part_v + ((k + 1) & 1) * 2 * kCgBlocks;
   if (timed) (void)hipEventRecord(w->tev[2 * k], c->stream);
-  const int na = spmv_any<X, 0>(c, w, v.u, nullptr, v.w, sin);
+  const int na = spmv_hat<X>(c, w, v.r, v.w, sin);
   if (timed) (void)hipEventRecord(w->tev[2 * k + 1], c->stream);
-  k_cg_step<X><<<cg_blocks(c->n), kBlock, 0, c->stream>>>(c->n, v.x, v.r, v.u, v.w, v.p, v.sv, v.invd, na, w->part_s,
-                                                           pin, pout, sin, sout, tol2);
+  const int nb = cg_blocks(c);
+#define FEM_STEP(EE)                                                                                          \
+  k_cg_step<X, EE><<<nb, kBlock, 0, c->stream>>>(c->n, v.x, v.r, v.w, v.p, v.sv, na, w->part_s, pin, pout, sin, \
+                                                 sout)
+  switch (cg_elems(c)) {
+    case 2: FEM_STEP(2); break;
+    case 4: FEM_STEP(4); break;
+    case 16: FEM_STEP(16); break;
+    default: FEM_STEP(8); break;
+  }
+#undef FEM_STEP
 }
 
-// The captured batch of kBatch iterations (rebuilt only when the buffers or the tolerance change).
+// The captured batch of kBatch iterations (rebuilt only when the buffers change).
 // kBatch is even, so the batch starts and ends on scalar record 0 and partial buffer 0.
 template <typename X>
-static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double tol2, GraphCache& gc) {
+static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache& gc) {
   static_assert(kBatch % 2 == 0, "batch must end on scalar record 0");
-  double tkey = tol2;
-  const void* val = use_sell(c) ? (sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64)
-                                : (sizeof(X) == 4 ? (const void*)c->d_nzval32 : (const void*)c->d_nzval);
-  const void* key[8] = {val, v.x, v.r, v.p, v.w, v.invd, w->s, nullptr};
-  std::memcpy(&key[7], &tkey, sizeof(double));
+  const void* val = sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64;
+  const void* key[8] = {val, v.x, v.r, v.p, v.w, v.sv, w->s, w->sell.off};
   if (c->tune.time_kernels) {   // kernels launched directly (launch_batch), SpMVs bracketed by events
     for (hipEvent_t& e : w->tev)
       if (!e) FEM_HIP(c, hipEventCreate(&e));
@@ -1085,7 +1020,7 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double tol
   destroy_graph(gc);
   hipGraph_t graph = nullptr;
   FEM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < kBatch; ++k) enqueue_iteration<X>(c, w, v, k, tol2, false);
+  for (int k = 0; k < kBatch; ++k) enqueue_iteration<X>(c, w, v, k, false);
   FEM_HIP(c, hipStreamEndCapture(c->stream, &graph));
   hipError_t e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -1097,9 +1032,9 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double tol
 // One batch of kBatch iterations: the captured graph, or (time_kernels) direct launches with
 // every SpMV bracketed by HIP events.
 template <typename X>
-static int launch_batch(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double tol2, GraphCache& gc) {
+static int launch_batch(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache& gc) {
   if (c->tune.time_kernels) {
-    for (int k = 0; k < kBatch; ++k) enqueue_iteration<X>(c, w, v, k, tol2, true);
+    for (int k = 0; k < kBatch; ++k) enqueue_iteration<X>(c, w, v, k, true);
     FEM_HIP(c, hipGetLastError());
     return FEM_OK;
   }
@@ -1107,76 +1042,87 @@ static int launch_batch(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double to
   return FEM_OK;
 }
 
-// (Re)start the recurrence from the residual `res` (FP64 array): r = res, u = D⁻¹ r,
-// w = A u, p and s = A p rebuilt (orthogonalised old direction, or p = u on restart),
-// α = γ / p·s.  Works on scalar record 0 / partial buffer 0.
+// (Re)start the recurrence from the FP64 residual `res` (= f − K u, s->rr = its norm²):
+// r̂ = S res, w = Â r̂, p and s = Â p rebuilt (orthogonalised old direction, or p = r̂ on
+// restart), α = γ / p·s.  Works on scalar record 0 / partial buffer 0.
 template <typename X>
 static void reseed(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, const double* res, double hold_factor2,
-                   int restart) {
-  const int nb = cg_blocks(c->n);
-  k_cg_reseed<X><<<nb, kBlock, 0, c->stream>>>(c->n, res, v.invd, v.r, v.u, v.p, w->part_v);
-  k_cg_reseed_scal<<<1, kBlock, 0, c->stream>>>(nb, w->part_v, w->s, hold_factor2, restart);
-  spmv_any<X, 0>(c, w, v.u, nullptr, v.w, w->s);
-  k_cg_pfix<X><<<nb, kBlock, 0, c->stream>>>(c->n, v.u, v.w, v.p, v.sv, w->s, w->part_v);
+                   double tol2, int restart) {
+  const int nb = cg_blocks(c);
+  k_cg_reseed<X><<<nb, kBlock, 0, c->stream>>>(c->n, res, w->sc, v.r, v.p, w->part_v);
+  k_cg_reseed_scal<<<1, kBlock, 0, c->stream>>>(nb, w->part_v, w->s, hold_factor2, tol2, restart);
+  spmv_hat<X>(c, w, v.r, v.w, w->s);
+  k_cg_pfix<X><<<nb, kBlock, 0, c->stream>>>(c->n, v.r, v.w, v.p, v.sv, w->s, w->part_v);
   k_cg_alpha<<<1, kBlock, 0, c->stream>>>(nb, w->part_v, w->s);
 }
 
-static void residual(fem_ctx* c, SolverWork* w, bool sell64);
+// res = f − K u and s->rr = ||res||² with the unscaled FP64 K (CSC arrays, fixed order).
+static void residual(fem_ctx* c, SolverWork* w) {
+  launch_spmv<double, double, 1>(c, w->sh64, c->d_nzval, c->d_u, c->d_f, w->res, w->part_s, w->s);
+  k_sum_to<<<1, kWide, 0, c->stream>>>(w->sh64.blocks, w->part_s, w->s, 0);
+}
 
-// FP64 preconditioned CG on K u = f from u = 0.  When the recursive residual claims
-// convergence the true residual f − K u is computed; if it misses the tolerance it
-// replaces r (residual replacement; p is re-orthogonalised against it) and the iteration
-// goes on, at most kMaxReplace times and only while the true residual still improves.
-constexpr int kMaxReplace = 8;
+// CG on Â x̂ = S f in X precision from u = 0, with hand-overs: the device stops the
+// recurrence when γ has fallen by hold_factor2 (0: never) or claims convergence; the host
+// folds x̂ into u, computes the true residual with the FP64 K, stops if it meets tol2 and
+// otherwise reseeds from it (residual replacement / reliable update).  It gives up when
+// the true residual stalls (improves by less than 10% in norm) at kMaxStall hand-overs in
+// a row, or after maxit iterations.
+constexpr int kMaxStall = 2;
 
-static int pcg64(fem_ctx* c, SolverWork* w, double tol2, int maxit, GraphCache& gc, int* iters, int* replaced,
-                 double* ms) {
-  const CgVecs<double> v{c->d_u, w->r, w->u, w->q, w->p, w->sv, w->invd};
-  k_cg_init_scal<<<1, 1, 0, c->stream>>>(w->s, 0.0);
-  reseed<double>(c, w, v, c->d_f, 0.0, 1);
+template <typename X>
+static int cg_run(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double bnorm2, double tol2, double hold_factor2,
+                  int maxit, GraphCache& gc, int* iters, int* handovers, double* ms) {
+  zero_vec<double>(c, c->d_u);
+  zero_vec<X>(c, v.x);
+  zero_vec<X>(c, v.p);
+  zero_vec<X>(c, v.sv);
+  k_cg_init_scal<<<1, 1, 0, c->stream>>>(w->s, bnorm2);
+  reseed<X>(c, w, v, c->d_f, hold_factor2, tol2, 1);
   FEM_HIP(c, hipGetLastError());
-  int rc = batch_graph<double>(c, w, v, tol2, gc);
+  int rc = batch_graph<X>(c, w, v, gc);
   if (rc) return rc;
-  Scal h{};
-  int launched = 0, nrep = 0;
-  bool converged = false;
-  double prev_true = -1.0;
+  int launched = 0, nho = 0, stall = 0, iters_before = 0;
+  bool converged = false, fresh_start = true;
+  double prev_true = bnorm2;
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   while (launched < maxit) {
-    if ((rc = launch_batch<double>(c, w, v, tol2, gc))) return rc;
+    if ((rc = launch_batch<X>(c, w, v, gc))) return rc;
     launched += kBatch;
+    Scal h{};
     if ((rc = read_scal(c, w, &h))) return rc;
-    add_spmv_times(c, w);
+    // SpMVs that did work: every executed step's except a fresh one's (it skips the SpMV),
+    // plus the SpMV of the step that found the hand-over
+    const int first = fresh_start ? 1 : 0;
+    const int ran = h.iters - iters_before + (h.done ? 1 : 0);
+    add_spmv_times(c, w, first, std::max(0, ran - first));
+    iters_before = h.iters;
+    fresh_start = false;
     if (!h.done) continue;
-    residual(c, w, use_sell(c));
+    fresh_start = true;
+    ++nho;
+    k_fold<X><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, v.x, w->sc, c->d_u);
+    residual(c, w);
     Scal ht{};
     if ((rc = read_scal(c, w, &ht))) return rc;
     if (ht.rr <= tol2) { converged = true; break; }
-    // the true residual no longer improves (by 10% in norm per replacement): the tolerance
-    // is below what FP64 attains here
-    if (nrep++ >= kMaxReplace || (prev_true >= 0.0 && ht.rr > 0.81 * prev_true)) break;
-    prev_true = ht.rr;
-    reseed<double>(c, w, v, w->res, 0.0, 0);
+    stall = ht.rr > 0.81 * prev_true ? stall + 1 : 0;
+    if (stall >= kMaxStall) break;
+    prev_true = std::min(prev_true, ht.rr);
+    reseed<X>(c, w, v, w->res, hold_factor2, tol2, 0);
+  }
+  if (!converged) {   // fold what the recurrence holds and leave the true residual in s->rr
+    k_fold<X><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, v.x, w->sc, c->d_u);
+    residual(c, w);
   }
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
+  FEM_HIP(c, hipGetLastError());
   *ms += event_ms(c);
+  Scal h{};
   if ((rc = read_scal(c, w, &h))) return rc;
   *iters = h.iters;
-  *replaced = nrep;
+  *handovers = nho;
   return converged ? FEM_OK : FEM_ERR_NOCONV;
-}
-
-// res = f − K u and s->rr = ||res||² (FP64, fixed reduction order).  With sell64 the FP64
-// sliced copy is used (the FP64 solver has it); otherwise the CSC pattern itself.
-static void residual(fem_ctx* c, SolverWork* w, bool sell64) {
-  int parts;
-  if (sell64) {
-    parts = spmv_any<double, 1>(c, w, c->d_u, c->d_f, w->res, w->s);
-  } else {
-    launch_spmv<double, double, 1>(c, w->sh64, c->d_nzval, c->d_u, c->d_f, w->res, w->part_s, w->s);
-    parts = w->sh64.blocks;
-  }
-  k_sum_to<<<1, kWide, 0, c->stream>>>(parts, w->part_s, w->s, 0);
 }
 
 }  // namespace fem
@@ -1186,6 +1132,7 @@ using namespace fem;
 extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_info* info) {
   if (!c || !cfg) return FEM_ERR_ARG;
   if (c->nnz == 0 || !c->d_f) return set_error(c, FEM_ERR_STATE, "assemble a system first");
+  if (!(cfg->precond == 0 || cfg->precond == 1)) return set_error(c, FEM_ERR_ARG, "precond must be 0 or 1");
   FEM_HIP(c, hipSetDevice(c->device));
   const bool mixed = cfg->mode == FEM_SOLVE_IR_MIXED;
   int rc = ensure_work(c, mixed);
@@ -1211,60 +1158,22 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
     return FEM_OK;
   }
   const double tol2 = cfg->rtol * cfg->rtol * bnorm2;
-
+  int iters = 0, handovers = 0;
   if (!mixed) {
-    if ((rc = prepare_operator<double>(c, w))) return rc;
-    k_invdiag<double><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd);
-    k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_u);
-    int iters = 0, replaced = 0;
-    rc = pcg64(c, w, tol2, maxit, w->g64, &iters, &replaced, &cg_ms);
-    out.iterations = iters;
-    out.outer_iterations = 1 + replaced;
-    if (rc && rc != FEM_ERR_NOCONV) return rc;
+    if ((rc = prepare_operator<double>(c, w, cfg->precond))) return rc;
+    const CgVecs<double> v{w->x, w->r, w->wv, w->p, w->sv};
+    rc = cg_run<double>(c, w, v, bnorm2, tol2, 0.0, maxit, w->g64, &iters, &handovers, &cg_ms);
   } else {
-    if ((rc = prepare_operator<float>(c, w))) return rc;
-    k_invdiag<float><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_diag, c->d_nzval, cfg->precond, w->invd32);
-    // Mixed CG with reliable updates: one FP32 Krylov sequence (FP32 SpMV and vectors,
-    // FP64 dot products); whenever its recursive residual has dropped by δ since the last
-    // refinement (or claims convergence) the device stops the recurrence, the FP32 iterate
-    // is folded into the FP64 u, the true residual f − K u is recomputed with the FP64 K and
-    // fed back as the FP32 residual, and the last search direction is made orthogonal to it
-    // and extended (p = z + β (p − γ z)), as in reliable-update CG.
+    // the FP32 recurrence hands over whenever its residual has fallen by δ (inner_rtol)
+    if ((rc = prepare_operator<float>(c, w, cfg->precond))) return rc;
     const double delta = cfg->inner_rtol > 0 ? cfg->inner_rtol : 0.1;
-    const CgVecs<float> v{w->x32, w->r32, w->u32, w->q32, w->p32, w->sv32, w->invd32};
-    k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, c->d_u);
-    k_zero32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, w->x32);
-    k_cg_init_scal<<<1, 1, 0, c->stream>>>(w->s, bnorm2);
-    reseed<float>(c, w, v, c->d_f, delta * delta, 1);
-    if ((rc = batch_graph<float>(c, w, v, tol2, w->g32))) return rc;
-    int launched = 0, refinements = 0;
-    bool converged = false;
-    FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
-    while (launched < maxit && !converged) {
-      if ((rc = launch_batch<float>(c, w, v, tol2, w->g32))) return rc;
-      launched += kBatch;
-      Scal hs{};
-      if ((rc = read_scal(c, w, &hs))) return rc;
-      add_spmv_times(c, w);
-      if (!hs.done) continue;
-      ++refinements;
-      k_fold<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, w->x32, c->d_u);
-      residual(c, w, false);
-      Scal ht{};
-      if ((rc = read_scal(c, w, &ht))) return rc;
-      if (ht.rr <= tol2) { converged = true; break; }
-      reseed<float>(c, w, v, w->res, delta * delta, 0);
-    }
-    FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
-    cg_ms += event_ms(c);
-    Scal hf{};
-    if ((rc = read_scal(c, w, &hf))) return rc;
-    out.iterations = hf.iters;
-    out.outer_iterations = refinements;
+    const CgVecs<float> v{w->x32, w->r32, w->wv32, w->p32, w->sv32};
+    rc = cg_run<float>(c, w, v, bnorm2, tol2, delta * delta, maxit, w->g32, &iters, &handovers, &cg_ms);
   }
-  residual(c, w, !mixed && use_sell(c));
-  FEM_HIP(c, hipGetLastError());
-  if ((rc = read_scal(c, w, &h))) return rc;
+  if (rc && rc != FEM_ERR_NOCONV) return rc;
+  out.iterations = iters;
+  out.outer_iterations = handovers;
+  if ((rc = read_scal(c, w, &h))) return rc;   // s->rr: the true residual of the returned u
   out.relres = std::sqrt(h.rr / bnorm2);
   c->timings.solve_ms = cg_ms;
   if (info) *info = out;
@@ -1276,7 +1185,8 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   return FEM_OK;
 }
 
-// Average duration (HIP events) of `reps` back-to-back SpMV launches q = K p of the solver.
+// Average duration (HIP events) of `reps` back-to-back launches of the solver's SpMV
+// w = Â x (sliced operator, FP64 or FP32 values and vectors).
 extern "C" int32_t fem_spmv_bench(fem_ctx* c, int32_t mixed, int32_t reps, double* avg_ms) {
   if (!c || reps <= 0 || !avg_ms) return FEM_ERR_ARG;
   if (c->nnz == 0) return set_error(c, FEM_ERR_STATE, "no matrix");
@@ -1287,17 +1197,17 @@ extern "C" int32_t fem_spmv_bench(fem_ctx* c, int32_t mixed, int32_t reps, doubl
   Scal zero{};
   FEM_HIP(c, hipMemcpyAsync(w->s, &zero, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
   if (mixed) {
-    if ((rc = prepare_operator<float>(c, w))) return rc;
+    if ((rc = prepare_operator<float>(c, w, 1))) return rc;
     k_to_f32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p32);
   } else {
-    if ((rc = prepare_operator<double>(c, w))) return rc;
+    if ((rc = prepare_operator<double>(c, w, 1))) return rc;
     k_copy<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p);
   }
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < reps; ++i) {
-    if (mixed) spmv_any<float, 0>(c, w, w->p32, nullptr, w->q32, w->s);
-    else spmv_any<double, 0>(c, w, w->p, nullptr, w->q, w->s);
+    if (mixed) spmv_hat<float>(c, w, w->p32, w->wv32, w->s);
+    else spmv_hat<double>(c, w, w->p, w->wv, w->s);
   }
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -1309,14 +1219,14 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
   if (!c || !t) return FEM_ERR_ARG;
   auto ok_cap = [](int v) { return v == 0 || v == 2048 || v == 4096; };
   auto ok_l = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4; };
-  if (!ok_cap(t->spmv_cap64) || !ok_cap(t->spmv_cap32) || !ok_l(t->spmv_lanes64) || !ok_l(t->spmv_lanes32) ||
+  if (!ok_cap(t->spmv_cap64) || !ok_l(t->spmv_lanes64) || !(t->sell_compress == 0 || t->sell_compress == -1) ||
       !(t->asm_block == 0 || t->asm_block == 64 || t->asm_block == 128 || t->asm_block == 256) ||
-      !(t->spmv_format == 0 || t->spmv_format == 1) || !(t->asm_mode >= 0 && t->asm_mode <= 2) ||
-      !(t->time_kernels == 0 || t->time_kernels == 1) ||
+      !(t->asm_mode >= 0 && t->asm_mode <= 2) || !(t->time_kernels == 0 || t->time_kernels == 1) ||
       !(t->sell_unroll == 0 || t->sell_unroll == 4 || t->sell_unroll == 8 || t->sell_unroll == 16) ||
       !(t->sell_map >= 0 && t->sell_map <= 2) ||
       !(t->gather_group == 0 || t->gather_group == 1 || t->gather_group == 2 || t->gather_group == 4 ||
-        t->gather_group == 8))
+        t->gather_group == 8) ||
+      !(t->cg_elems == 0 || t->cg_elems == 2 || t->cg_elems == 4 || t->cg_elems == 8 || t->cg_elems == 16))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

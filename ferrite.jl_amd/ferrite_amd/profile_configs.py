@@ -13,7 +13,9 @@ elasticity  Q2 hex^3 small-strain elasticity on generate_grid(Hexahedron, (n, n,
 cantilever  Plane-strain Q1^2 cantilever on (nx, ny) cells over [0,2]x[0,1], E = E_min +
             theta^3 (1 - E_min), theta ~ U[0.2, 1.0] (seed 0), clamped at x = 0, point load
             (0, -1) at (2, 0.5).  Times J and the adjoint gradients; checks `fd` random entries
-            of dJ/dtheta and of dJ/dX against central differences (h = 1e-6).
+            of dJ/dtheta and of dJ/dX against central differences (h = 1e-6 and 1e-4; FP64 CG
+            attains a relative residual of about 1e-9 on this system, which bounds how well a
+            difference of two J values resolves the small entries).
 batch       `problems` seeded scalar Poisson problems (2D/3D, log-uniform cell counts in
             [2e3, 2e5]) assembled and solved (mixed CG to 1e-8) in one batched context after
             the launch-shape autotuner; total cells/s and p50 / p99 completion times.
@@ -117,22 +119,27 @@ def cantilever(args):
     fa.compliance_gradient(K, theta)
     tg = time.perf_counter() - t1
     log(f"J = {J:.10e}: forward {tj:.2f}s ({res.iterations} CG its), adjoint kernel {tg * 1e3:.2f} ms")
-    h = 1e-6
     pick = np.random.default_rng(args.seed + 1)
-    rows_t, rows_x = [], []
-    for e in pick.choice(g.ncells, args.fd, replace=False):
-        tp, tm = theta.copy(), theta.copy()
-        tp[e] += h
-        tm[e] -= h
-        fd = (prob.compliance(tp)[0] - prob.compliance(tm)[0]) / (2 * h)
-        rows_t.append((fd, dth[e]))
-    for k in pick.choice(g.nnodes * 2, args.fd, replace=False):
-        nd, d = divmod(int(k), 2)
-        Xp, Xm = X0.copy(), X0.copy()
-        Xp[nd, d] += h
-        Xm[nd, d] -= h
-        fd = (prob.compliance(theta, Xp)[0] - prob.compliance(theta, Xm)[0]) / (2 * h)
-        rows_x.append((fd, dx[nd, d]))
+    cells = pick.choice(g.ncells, args.fd, replace=False)
+    coords = pick.choice(g.nnodes * 2, args.fd, replace=False)
+    fd_rows = {}
+    for h in (1e-6, 1e-4):
+        rows_t, rows_x = [], []
+        for e in cells:
+            tp, tm = theta.copy(), theta.copy()
+            tp[e] += h
+            tm[e] -= h
+            fd = (prob.compliance(tp)[0] - prob.compliance(tm)[0]) / (2 * h)
+            rows_t.append((fd, dth[e]))
+        for k in coords:
+            nd, d = divmod(int(k), 2)
+            Xp, Xm = X0.copy(), X0.copy()
+            Xp[nd, d] += h
+            Xm[nd, d] -= h
+            fd = (prob.compliance(theta, Xp)[0] - prob.compliance(theta, Xm)[0]) / (2 * h)
+            rows_x.append((fd, dx[nd, d]))
+        fd_rows[h] = (rows_t, rows_x)
+        log(f"h = {h:g}: {len(rows_t)} + {len(rows_x)} central differences done")
     prob.compliance(theta, X0)
 
     def stats(rows):
@@ -143,9 +150,12 @@ def cantilever(args):
                 "max_entry_rel_err": float(rel.max()), "median_entry_rel_err": float(np.median(rel)),
                 "entries_above_1e-6": int(np.sum(rel > 1e-6)),
                 "entries_above_1e-6_and_floor": int(np.sum(np.abs(fd - ad) > 1e-6 * np.abs(ad) + 1e-7 * abs(J)))}
-    return {"workload": f"plane-strain Q1^2 cantilever {nx}x{ny}, SIMP p=3, E_min 1e-3, point load at node {tip}",
-            "cells": g.ncells, "dofs": dh.ndofs, "J": J, "forward_s": tj, "cg_iterations": res.iterations,
-            "adjoint_kernel_ms": tg * 1e3, "fd_h": h, "dJdtheta_vs_fd": stats(rows_t), "dJdX_vs_fd": stats(rows_x)}
+    out = {"workload": f"plane-strain Q1^2 cantilever {nx}x{ny}, SIMP p=3, E_min 1e-3, point load at node {tip}",
+           "cells": g.ncells, "dofs": dh.ndofs, "J": J, "forward_s": tj, "cg_iterations": res.iterations,
+           "cg_rtol": args.rtol, "relres": res.relres, "adjoint_kernel_ms": tg * 1e3}
+    for h, (rows_t, rows_x) in fd_rows.items():
+        out[f"fd_h{h:g}"] = {"dJdtheta": stats(rows_t), "dJdX": stats(rows_x)}
+    return out
 
 
 def batch(args):
@@ -182,11 +192,11 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--rtol", type=float, default=0.0,
-                    help="CG relative residual (default 1e-8 elasticity, 1e-10 cantilever)")
+                    help="CG relative residual (default 1e-8 elasticity, 2e-9 cantilever)")
     args = ap.parse_args()
     fn = {"elasticity": elasticity, "cantilever": cantilever, "batch": batch}[args.config]
     if args.rtol == 0.0:
-        args.rtol = {"elasticity": 1e-8, "cantilever": 1e-10, "batch": 1e-8}[args.config]
+        args.rtol = {"elasticity": 1e-8, "cantilever": 2e-9, "batch": 1e-8}[args.config]
     print(json.dumps({"config": args.config, **fn(args)}), flush=True)
 
 

@@ -20,6 +20,8 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--sweep", action="store_true", help="time SpMV / assembly launch shapes")
     ap.add_argument("--solve", action="store_true", help="also run one FP64 and one mixed solve")
+    ap.add_argument("--cg-sweep", action="store_true",
+                    help="CG vector step: elements per thread (solve loop times, both solvers)")
     ap.add_argument("--gather-sweep", action="store_true",
                     help="owner-computes assembly: cells loaded per column lane before adding")
     ap.add_argument("--sell-sweep", action="store_true",
@@ -67,6 +69,18 @@ def main():
         for mode in ("cg", "mixed"):
             r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
             print(f"phase solve {mode}: {r.iterations} iterations, relres {r.relres:.2e}", flush=True)
+    if args.cg_sweep:
+        for ce in (2, 4, 8, 16):
+            fa.set_tuning(K, cg_elems=ce)
+            line = []
+            for mode in ("cg", "mixed"):
+                best = 1e30
+                for _ in range(2):
+                    r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
+                    best = min(best, fa.timings(K)["solve_ms"])
+                line.append(f"{mode} loop {best:.2f} ms ({r.iterations} its, {best / r.iterations * 1e3:.1f} us/it)")
+            print(f"cg elems {ce}: {'; '.join(line)}", flush=True)
+        fa.set_tuning(K, cg_elems=0)
     if args.gather_sweep:
         for grp in (1, 2, 4, 8):
             fa.set_tuning(K, gather_group=grp)
@@ -97,22 +111,14 @@ def main():
                       f"in-solve {'; '.join(line)}", flush=True)
         fa.set_tuning(K, sell_map=0, sell_unroll=0, time_kernels=0)
     if args.sweep:
-        fa.set_tuning(K, spmv_format=0)
         a64 = fa.spmv_bench(K, mixed=False, reps=20)
         a32 = fa.spmv_bench(K, mixed=True, reps=20)
         print(f"sweep spmv sell: fp64 {a64:.4f} ms, fp32+u16 {a32:.4f} ms", flush=True)
-        fa.set_tuning(K, spmv_format=1)
-        for cap in (2048, 4096):
-            for lanes in (1, 2, 4):
-                fa.set_tuning(K, spmv_cap64=cap, spmv_lanes64=lanes, spmv_cap32=cap, spmv_lanes32=lanes,
-                              spmv_compress32=0)
-                a64 = fa.spmv_bench(K, mixed=False, reps=20)
-                a32 = fa.spmv_bench(K, mixed=True, reps=20)
-                fa.set_tuning(K, spmv_compress32=-1)
-                a32u = fa.spmv_bench(K, mixed=True, reps=20)
-                print(f"sweep spmv cap {cap} lanes {lanes}: fp64 {a64:.4f} ms, fp32+u16 {a32:.4f} ms, "
-                      f"fp32+i32 {a32u:.4f} ms", flush=True)
-        fa.set_tuning(K, spmv_format=0, spmv_compress32=0)
+        fa.set_tuning(K, sell_compress=-1)
+        a64 = fa.spmv_bench(K, mixed=False, reps=20)
+        a32 = fa.spmv_bench(K, mixed=True, reps=20)
+        print(f"sweep spmv sell int32 columns: fp64 {a64:.4f} ms, fp32 {a32:.4f} ms", flush=True)
+        fa.set_tuning(K, sell_compress=0)
         for mode, name in ((2, "gather"), (1, "colored scatter")):
             fa.set_tuning(K, asm_mode=mode)
             ts = []

@@ -145,15 +145,13 @@ int32_t fem_device_pointer(fem_ctx* ctx, int32_t which, void** ptr, int64_t* nel
 
 /* ---- launch-shape tuning (autotuner interface; no counterpart in the reference) ---- */
 typedef struct {
-  int32_t spmv_cap64, spmv_lanes64;  /* FP64 solver SpMV: LDS products per chunk (2048/4096) and
-                                        lanes per row in its reduction (1/2/4); 0 = default */
-  int32_t spmv_cap32, spmv_lanes32;  /* FP32 operator of the mixed solver, same meaning */
-  int32_t spmv_compress32;           /* 16-bit block-relative column offsets for the FP32 operator:
-                                        0 = default (on when every row block spans < 65536 columns),
-                                        -1 = off */
+  int32_t spmv_cap64, spmv_lanes64;  /* FP64 residual f − K u (CSR-stream over the CSC arrays): LDS
+                                        products per chunk (2048/4096) and lanes per row (1/2/4);
+                                        0 = default */
+  int32_t sell_compress;             /* solver operator (sliced ELL): 16-bit slice-relative column
+                                        offsets when every slice spans < 65536 columns (0, default)
+                                        or int32 columns (-1) */
   int32_t asm_block;                 /* cells per workgroup of the element kernels (0 = default) */
-  int32_t spmv_format;               /* solver operator: 0 = sliced ELL (64-row slices, default),
-                                        1 = CSR-stream over the CSC pattern (the cap/lanes above) */
   int32_t asm_mode;                  /* scalar Q1 Poisson assembly: 0 = owner-computes gather when
                                         supported (default), 1 = colored scatter, 2 = gather only */
   int32_t time_kernels;              /* 1: the solver launches its CG kernels directly (no graph
@@ -165,6 +163,8 @@ typedef struct {
                                         1 = grid-stride, 2 = contiguous, one range per XCD share */
   int32_t gather_group;              /* owner-computes assembly: cells whose element values a column
                                         lane loads before adding them, 1 / 2 / 4 / 8 (0 = default 4) */
+  int32_t cg_elems;                  /* CG vector step: elements per thread per pass, 2 / 4 / 8 / 16
+                                        (0 = default 8) */
   int32_t reserved;
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
