@@ -86,8 +86,14 @@ struct fem_batch {
   int32_t* d_sprob = nullptr;       // [nseg] problem of each reduction segment
   int64_t* d_srow = nullptr;        // [nseg+1] first row of each segment
   int32_t* d_pseg = nullptr;        // [nprob+1] first segment of each problem
-  double* d_spart = nullptr;        // [2 nseg] segment partials
+  double* d_spart = nullptr;        // [6 nseg] segment partials: γ and ||r||² (two buffers of 2
+                                    // per segment), then δ / the hand-over partials (2 per segment)
   int nseg = 0;
+  int32_t* d_aseg = nullptr;        // [nseg] segments of problems in the recurrence
+  int32_t* d_hseg = nullptr;        // [nseg] segments of problems being handed over
+  int32_t* d_nlist = nullptr;       // [2] lengths of the two lists
+  int32_t* d_handing = nullptr;     // [nprob] 1 for the problems of the current hand-over
+  std::vector<int32_t> h_pseg;      // [nprob+1] first segment of each problem (host copy)
   double* d_tab = nullptr;          // reference tables: quad block then hex block (float copy below)
   float* d_tabf = nullptr;
   int64_t nslots = 0;
@@ -97,10 +103,15 @@ struct fem_batch {
   // persistent solver work (allocated at finalize)
   double* d_m = nullptr;            // [nprob] meandiag per problem
   float* d_val32 = nullptr;         // [nnz] FP32 operator
-  float *d_x32 = nullptr, *d_r32 = nullptr, *d_p32 = nullptr, *d_q32 = nullptr, *d_invd = nullptr;
+  // FP32 recurrence of the scaled system Â = S K S: x̂, r̂, w = Â r̂, p, s = Â p
+  float *d_x32 = nullptr, *d_r32 = nullptr, *d_w32 = nullptr, *d_p32 = nullptr, *d_s32 = nullptr;
+  double* d_sc = nullptr;           // [ndofs] S = diag(K)^-1/2
+  float* d_dsq = nullptr;           // [ndofs] S⁻² = |diag(K)|
   double* d_res = nullptr;          // [ndofs] FP64 true residual
-  void* d_scal = nullptr;           // [nprob] per-problem CG scalars
-  hipGraphExec_t cg_graph = nullptr;
+  void* d_scal = nullptr;           // [2 nprob] per-problem CG scalars (two records each)
+  // captured CG batches, one per launch width (a power of two ≥ the recurring segments):
+  // blocks past the list's length return at once, so a narrow graph serves short lists
+  std::vector<std::pair<int, hipGraphExec_t>> cg_graphs;
   hipEvent_t e_start = nullptr, e_asm = nullptr, e_apply = nullptr, e_mark = nullptr;
   std::vector<double> last_times_ms;
   std::vector<int> last_iters;
@@ -356,243 +367,413 @@ __global__ void k_b_setdiag(int64_t np, const int32_t* __restrict__ pres, const 
 // batched mixed PCG: FP32 Krylov recurrence on the block-diagonal operator with
 // per-problem scalars; dot products and the reliable-update residual in FP64
 // ---------------------------------------------------------------------------
+// Per-problem scalars of the batched Chronopoulos–Gear CG on the scaled system (two
+// records per problem, read one / write the other, like the single-problem solver).
 struct BScal {
-  double rz, pq, rr, alpha, beta, tol2, maxrr, bnorm2, d2;
-  double rrt;   // FP64 true residual ||f - K u||² of the last reliable update
-  double rzp;   // r·z of the step before the hand-over
-  double gam;   // (r·p)/(r·z) at a reseed
-  int done;     // converged (FP64 true residual <= rtol ||f||)
-  int hold;     // recurrence stopped, waiting for a reliable update on the host
-  int active;   // selected for the current reliable update
+  double rz;      // γ = r̂·r̂ of the current residual
+  double pq;      // p·Â p at a reseed
+  double rr;      // FP64 true residual ||f − K u||² of the last hand-over
+  double rrec;    // recursive residual ||S⁻¹ r̂||² of the last step
+  double alpha, beta;
+  double tol2;    // rtol² ||f||²
+  double d2;      // δ²: hand over once the recursive residual fell by this factor
+  double bnorm2;  // ||f||²
+  double rzp;     // γ of the previous step
+  double gam;     // (r̂·p)/(r̂·r̂) at a reseed
+  int done;       // converged (FP64 true residual <= rtol ||f||)
+  int hold;       // recurrence stopped, waiting for a hand-over on the host
   int iters;
-  int restart;  // p·Kp <= 0 happened: the reliable update restarts p = z
+  int restart;    // breakdown: the next reseed restarts p = r̂
+  int fresh;      // p, s, α just set by a reseed: the next step only moves x̂, r̂
   int pad;
 };
 
-enum BDot { kDotPQ = 0, kDotRZRR = 1, kDotTrue = 3, kDotF = 4, kDotReseed = 5 };
-
-// Which problems a reduction runs for.
-__device__ __forceinline__ bool b_skip(const BScal& sp, int which) {
-  if (which == kDotF) return false;
-  if (sp.done) return true;
-  if (which == kDotTrue || which == kDotReseed) return !sp.active;
-  return sp.hold != 0;
-}
-
 // Per-problem reductions in two levels: every segment (≤ kSeg consecutive rows of one
-// problem) gets one block that writes t = Σ a·b and t2 = Σ a·c; k_b_dot_fin then sums a
-// problem's segments in order.  Fixed order throughout (deterministic).
-constexpr int kSeg = 2048;
+// problem) is one block and writes its partials; the problem's blocks (or a per-problem
+// kernel) sum them in segment order.  Fixed order throughout (deterministic).
+constexpr int kSeg = 256;
+constexpr int kSegThreads = 256;     // vector kernels: one thread per row of a segment
+constexpr int kSpmvThreads = 1024;   // SpMV kernels: 4 lanes per row of a segment
+static_assert(kSpmvThreads == 4 * kSeg, "one SpMV pass covers a segment");
 
-template <typename X>
-__global__ __launch_bounds__(256) void k_b_dot_seg(const int32_t* __restrict__ sprob, const int64_t* __restrict__ srow,
-                                                   const X* __restrict__ a, const X* __restrict__ b,
-                                                   const X* __restrict__ c, const BScal* __restrict__ s, int which,
-                                                   double* __restrict__ part) {
-  __shared__ double sh[2][4];
-  const int seg = blockIdx.x;
-  if (b_skip(s[sprob[seg]], which)) return;
-  const int64_t j0 = srow[seg], j1 = srow[seg + 1];
-  double acc = 0.0, acc2 = 0.0;
-  for (int64_t j = j0 + threadIdx.x; j < j1; j += 256) {
-    const double aj = (double)a[j];
-    acc += aj * (double)b[j];
-    acc2 += aj * (double)c[j];
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    acc += __shfl_down(acc, o, 64);
-    acc2 += __shfl_down(acc2, o, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    sh[0][threadIdx.x >> 6] = acc;
-    sh[1][threadIdx.x >> 6] = acc2;
-  }
+// Sum over the kSpmvThreads threads of a block, valid in every thread; `sh` holds 16.
+__device__ __forceinline__ double spmv_block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    part[2 * seg] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
-    part[2 * seg + 1] = (sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3]);
-  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < kSpmvThreads / 64; ++i) t += sh[i];
+  return t;
 }
 
-// One thread per problem: sum its segments, then the scalar step of `which`.
-__global__ void k_b_dot_fin(int nprob, const int32_t* __restrict__ pseg, const double* __restrict__ part,
-                            BScal* __restrict__ s, int which) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nprob || b_skip(s[p], which)) return;
-  double t = 0.0, t2 = 0.0;
-  for (int k = pseg[p]; k < pseg[p + 1]; ++k) {
-    t += part[2 * k];
-    t2 += part[2 * k + 1];
-  }
-  BScal& sp = s[p];
-  switch (which) {
-    case kDotPQ: {
-      // p·Kp > 0 for SPD K unless p vanished: hand over to the host
-      const bool ok = t > 0.0;
-      sp.pq = t;
-      sp.alpha = ok ? sp.rz / t : 0.0;
-      if (!ok) {
-        sp.hold = 1;
-        sp.restart = 1;
-      }
-      break;
-    }
-    case kDotRZRR:  // t = r·z, t2 = r·r after the update: β, iteration count
-      sp.beta = sp.rz != 0.0 ? t / sp.rz : 0.0;
-      sp.rzp = sp.rz;
-      sp.rz = t;
-      sp.rr = t2;
-      sp.iters += 1;
-      // converged by the recurrence, or dropped by δ since the last true residual: stop
-      // this problem's FP32 recurrence (it would otherwise run on towards underflow); p is
-      // not updated (it stays the last direction, orthogonal to the recursive residual)
-      if (t2 <= sp.tol2 || t2 < sp.d2 * sp.maxrr) sp.hold = 1;
-      break;
-    case kDotTrue: sp.rrt = t; break;   // FP64 true residual (a = b = res)
-    case kDotReseed: {                  // t = r·z, t2 = r·p of the reseeded residual
-      // γ = (r·p)/(r·z), β = r·z / r·z of the step before (β = 0: restart p = z after a
-      // breakdown or with no previous step)
-      const bool fresh = sp.restart || !(sp.rzp > 0.0) || !(t > 0.0);
-      sp.gam = t > 0.0 ? t2 / t : 0.0;
-      sp.beta = fresh ? 0.0 : t / sp.rzp;
-      sp.rz = t;
-      sp.restart = 0;
-      break;
-    }
-    default: sp.bnorm2 = t; break;      // ||f||²
-  }
+__device__ __forceinline__ double seg_block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (sh[0] + sh[1]) + (sh[2] + sh[3]);   // valid in every thread
 }
 
-template <typename X>
-__global__ void k_b_update(int64_t n, const int32_t* __restrict__ rowprob, const BScal* __restrict__ s,
-                           X* __restrict__ x, X* __restrict__ r, const X* __restrict__ p, X* q,
-                           const X* __restrict__ invd) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const BScal& sp = s[rowprob[i]];
-    if (sp.done || sp.hold) continue;
-    const X al = (X)sp.alpha;
-    x[i] += al * p[i];
-    const X ri = r[i] - al * q[i];
-    r[i] = ri;
-    q[i] = ri * invd[i];   // z = D⁻¹ r (Jacobi)
-  }
-}
-
-template <typename X>
-__global__ void k_b_pupd(int64_t n, const int32_t* __restrict__ rowprob, const BScal* __restrict__ s,
-                         X* __restrict__ p, const X* __restrict__ z) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const BScal& sp = s[rowprob[i]];
-    if (sp.done || sp.hold) continue;
-    p[i] = z[i] + (X)sp.beta * p[i];
-  }
-}
-
-// y = K x with 4 lanes per CSC column (K symmetric: column j is row j), fixed combine order
-// Rows of finished or held problems are skipped (their q is not used before it is rewritten).
-__global__ void k_b_spmv32(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
-                           const float* __restrict__ val, const float* __restrict__ x, float* __restrict__ y,
-                           const int32_t* __restrict__ rowprob, const BScal* __restrict__ s) {
-  const int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t j = gt >> 2;
-  const int sub = (int)(gt & 3);
-  bool run = j < n;
-  if (run) {
-    const BScal& sp = s[rowprob[j]];
-    run = !(sp.done || sp.hold);
-  }
+// w = Â r̂ on one segment per block (4 lanes per row, K symmetric: column j is row j), and
+// the segment's partial δ = r̂·w.  list/nlist: the segments to run; blocks past the list's
+// length return.  Problems held or right after a reseed (fresh) are skipped.
+__global__ __launch_bounds__(kSpmvThreads) void k_bc_spmv(const int32_t* __restrict__ list,
+                                                         const int32_t* __restrict__ nlist, int which,
+                                                         const int32_t* __restrict__ sprob,
+                                                         const int64_t* __restrict__ srow,
+                                                         const int32_t* __restrict__ colptr,
+                                                         const int32_t* __restrict__ rowval,
+                                                         const float* __restrict__ val, const float* __restrict__ r,
+                                                         float* __restrict__ w, const BScal* __restrict__ sin,
+                                                         double* __restrict__ part_d) {
+  __shared__ double sh[kSpmvThreads / 64];
+  if ((int)blockIdx.x >= nlist[which]) return;
+  const int seg = list[blockIdx.x];
+  const BScal& S = sin[sprob[seg]];
+  if (S.done || S.hold || S.fresh) return;
+  const int64_t j = srow[seg] + (threadIdx.x >> 2);   // one pass: kSeg rows x 4 lanes
+  const int sub = threadIdx.x & 3;
+  double d = 0.0;
   float acc = 0.f;
-  if (run)
-    for (int32_t p = colptr[j] + sub; p < colptr[j + 1]; p += 4) acc += val[p] * x[rowval[p]];
+  const bool in = j < srow[seg + 1];
+  if (in) {
+    // entries sub, sub + 4, ... of the row, two loads in flight (fixed order per lane)
+    const int32_t q1 = colptr[j + 1];
+    int32_t q = colptr[j] + sub;
+    for (; q + 4 < q1; q += 8) {
+      const float v0 = val[q], v1 = val[q + 4];
+      const int32_t c0 = rowval[q], c1 = rowval[q + 4];
+      acc += v0 * r[c0];
+      acc += v1 * r[c1];
+    }
+    if (q < q1) acc += val[q] * r[rowval[q]];
+  }
+  // the 4 lanes of a row are adjacent and share `in`: the shuffles stay inside the row
   acc += __shfl_down(acc, 2, 4);
   acc += __shfl_down(acc, 1, 4);
-  if (run && sub == 0) y[j] = acc;
+  if (in && sub == 0) {
+    w[j] = acc;
+    d = (double)r[j] * (double)acc;
+  }
+  const double t = spmv_block_sum(d, sh);
+  if (threadIdx.x == 0) part_d[seg] = t;
 }
 
-__global__ void k_b_resid64(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
-                            const double* __restrict__ val, const double* __restrict__ u,
-                            const double* __restrict__ f, const int32_t* __restrict__ rowprob,
-                            const BScal* __restrict__ s, double* __restrict__ res) {
-  const int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t j = gt >> 2;
-  const int sub = (int)(gt & 3);
-  const bool act = j < n && s[rowprob[j]].active;
-  double acc = 0.0;
-  if (act)
-    for (int32_t p = colptr[j] + sub; p < colptr[j + 1]; p += 4) acc += val[p] * u[rowval[p]];
+// The fused vector step on one segment per block.  Every block of a problem sums that
+// problem's partials (γ of the previous step, δ of the SpMV) in segment order, so all its
+// blocks hold the same scalars; the problem's first segment writes its next record.
+__global__ __launch_bounds__(kSegThreads) void k_bc_step(const int32_t* __restrict__ list,
+                                                         const int32_t* __restrict__ nlist,
+                                                         const int32_t* __restrict__ sprob,
+                                                         const int64_t* __restrict__ srow,
+                                                         const int32_t* __restrict__ pseg, float* __restrict__ x,
+                                                         float* __restrict__ r, const float* __restrict__ w,
+                                                         float* __restrict__ p, float* __restrict__ sv,
+                                                         const float* __restrict__ dsq,
+                                                         const double* __restrict__ part_d,
+                                                         const double* __restrict__ part_in,
+                                                         double* __restrict__ part_out, const BScal* __restrict__ sin,
+                                                         BScal* __restrict__ sout) {
+  __shared__ double sh[4];
+  if ((int)blockIdx.x >= nlist[0]) return;
+  const int seg = list[blockIdx.x];
+  const int pr = sprob[seg];
+  const bool writer = seg == pseg[pr] && threadIdx.x == 0;
+  const BScal S = sin[pr];
+  if (S.done || S.hold) {
+    if (writer) sout[pr] = S;
+    return;
+  }
+  const bool fresh = S.fresh != 0;
+  double gam = S.rz, rrec = S.rrec, alpha = S.alpha, beta = 0.0;
+  if (!fresh) {
+    double g = 0.0, q = 0.0, dl = 0.0;
+    for (int k = pseg[pr] + threadIdx.x; k < pseg[pr + 1]; k += kSegThreads) {
+      g += part_in[2 * k];
+      q += part_in[2 * k + 1];
+      dl += part_d[k];
+    }
+    gam = seg_block_sum(g, sh);
+    rrec = seg_block_sum(q, sh);
+    const double del = seg_block_sum(dl, sh);
+    beta = gam / S.rzp;
+    const double den = del - beta * gam / S.alpha;
+    // converged by the recursive residual, or fallen by δ since the last true residual
+    const bool stop = rrec <= S.tol2 || rrec < S.d2 * S.rr;
+    const bool brk = !(S.rzp > 0.0) || !(S.alpha != 0.0) || !(den > 0.0);
+    if (stop || brk) {
+      if (writer) {
+        BScal o = S;
+        o.hold = 1;
+        o.rz = gam;
+        o.rrec = rrec;
+        if (!stop) o.restart = 1;
+        sout[pr] = o;
+      }
+      return;
+    }
+    alpha = gam / den;
+  }
+  const float al = (float)alpha, be = (float)beta;
+  const int64_t j0 = srow[seg], j1 = srow[seg + 1];
+  double ga = 0.0, gr = 0.0;
+  for (int64_t j = j0 + threadIdx.x; j < j1; j += kSegThreads) {
+    float pj = p[j], sj = sv[j];
+    const float rj0 = r[j];
+    const float dj = dsq[j];
+    if (!fresh) {
+      pj = rj0 + be * pj;
+      sj = w[j] + be * sj;
+      p[j] = pj;
+      sv[j] = sj;
+    }
+    x[j] += al * pj;
+    const float rj = rj0 - al * sj;
+    r[j] = rj;
+    const double rd = (double)rj * (double)rj;
+    ga += rd;
+    gr += rd * (double)dj;
+  }
+  const double t = seg_block_sum(ga, sh);
+  const double t2 = seg_block_sum(gr, sh);
+  if (threadIdx.x == 0) {
+    part_out[2 * seg] = t;
+    part_out[2 * seg + 1] = t2;
+  }
+  if (writer) {
+    BScal o = S;
+    o.fresh = 0;
+    o.rrec = rrec;
+    o.rzp = gam;
+    o.alpha = alpha;
+    o.beta = beta;
+    o.rz = gam;
+    o.iters = S.iters + 1;
+    sout[pr] = o;
+  }
+}
+
+// Hand-over, step 1 (segments of the handed-over problems): u += S x̂, x̂ = 0, then
+// res = f − K u with the FP64 CSC arrays; partial ||res||² per segment.
+__global__ __launch_bounds__(kSpmvThreads) void k_bc_resid(const int32_t* __restrict__ list,
+                                                          const int32_t* __restrict__ nlist,
+                                                          const int64_t* __restrict__ srow,
+                                                          const int32_t* __restrict__ colptr,
+                                                          const int32_t* __restrict__ rowval,
+                                                          const double* __restrict__ val, const double* __restrict__ u,
+                                                          const double* __restrict__ f, double* __restrict__ res,
+                                                          double* __restrict__ part) {
+  __shared__ double sh[kSpmvThreads / 64];
+  if ((int)blockIdx.x >= nlist[1]) return;
+  const int seg = list[blockIdx.x];
+  const int64_t j = srow[seg] + (threadIdx.x >> 2);
+  const int sub = threadIdx.x & 3;
+  const bool in = j < srow[seg + 1];
+  double acc = 0.0, d = 0.0;
+  if (in)
+    for (int32_t q = colptr[j] + sub; q < colptr[j + 1]; q += 4) acc += val[q] * u[rowval[q]];
   acc += __shfl_down(acc, 2, 4);
   acc += __shfl_down(acc, 1, 4);
-  if (act && sub == 0) res[j] = f[j] - acc;
+  if (in && sub == 0) {
+    const double rj = f[j] - acc;
+    res[j] = rj;
+    d = rj * rj;
+  }
+  const double t = spmv_block_sum(d, sh);
+  if (threadIdx.x == 0) part[seg] = t;
 }
 
-// reliable update of the active problems: u += x32, x32 = 0
-__global__ void k_b_fold(int64_t n, const int32_t* __restrict__ rowprob, const BScal* __restrict__ s,
-                         float* __restrict__ x32, double* __restrict__ u) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    if (!s[rowprob[i]].active) continue;
-    u[i] += (double)x32[i];
-    x32[i] = 0.f;
+__global__ __launch_bounds__(kSegThreads) void k_bc_fold(const int32_t* __restrict__ list,
+                                                         const int32_t* __restrict__ nlist,
+                                                         const int64_t* __restrict__ srow,
+                                                         const double* __restrict__ sc, float* __restrict__ x,
+                                                         double* __restrict__ u) {
+  if ((int)blockIdx.x >= nlist[1]) return;
+  const int seg = list[blockIdx.x];
+  for (int64_t j = srow[seg] + threadIdx.x; j < srow[seg + 1]; j += kSegThreads) {
+    u[j] += sc[j] * (double)x[j];
+    x[j] = 0.f;
   }
 }
 
-// r32 = res, z = D⁻¹ r32 for the active problems
-__global__ void k_b_reseed(int64_t n, const int32_t* __restrict__ rowprob, const BScal* __restrict__ s,
-                           const double* __restrict__ res, const float* __restrict__ invd, float* __restrict__ r32,
-                           float* __restrict__ z32) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    if (!s[rowprob[i]].active) continue;
-    const float ri = (float)res[i];
-    r32[i] = ri;
-    z32[i] = ri * invd[i];
+// Hand-over, step 2 (one thread per problem): the true residual; converged problems are
+// done, the others are set up for a reseed (hold stays 1 until then).
+// Fixed-order sum of part[stride·k] over a problem's segments by one wave (lane l takes
+// k ≡ l mod 64, then a fixed butterfly), valid in every lane.
+__device__ __forceinline__ double wave_seg_sum(const double* __restrict__ part, int k0, int k1, int stride) {
+  double t = 0.0;
+  for (int k = k0 + (int)(threadIdx.x & 63); k < k1; k += 64) t += part[(int64_t)stride * k];
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  return t;
+}
+
+__global__ __launch_bounds__(64) void k_bc_true(int nprob, const int32_t* __restrict__ pseg,
+                                                const double* __restrict__ part, const int32_t* __restrict__ handing,
+                                                BScal* __restrict__ s) {
+  const int pr = blockIdx.x;
+  if (pr >= nprob || !handing[pr]) return;
+  const double t = wave_seg_sum(part, pseg[pr], pseg[pr + 1], 1);
+  if (threadIdx.x == 0) {
+    s[pr].rr = t;
+    if (t <= s[pr].tol2) s[pr].done = 1;
   }
 }
 
-// p = z + β (p − γ z): the last direction made orthogonal to the new residual, then extended
-__global__ void k_b_pfix(int64_t n, const int32_t* __restrict__ rowprob, const BScal* __restrict__ s,
-                         const float* __restrict__ z, float* __restrict__ p) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const BScal& sp = s[rowprob[i]];
-    if (!sp.active || sp.done) continue;
-    const float zi = z[i];
-    p[i] = zi + (float)sp.beta * (p[i] - (float)sp.gam * zi);
+// Hand-over, step 3 (segments): r̂ = S res; partials r̂·r̂ and r̂·p.
+__global__ __launch_bounds__(kSegThreads) void k_bc_reseed(const int32_t* __restrict__ list,
+                                                           const int32_t* __restrict__ nlist,
+                                                           const int32_t* __restrict__ sprob,
+                                                           const int64_t* __restrict__ srow,
+                                                           const double* __restrict__ sc,
+                                                           const double* __restrict__ res, float* __restrict__ r,
+                                                           const float* __restrict__ p, const BScal* __restrict__ s,
+                                                           double* __restrict__ part) {
+  __shared__ double sh[4];
+  if ((int)blockIdx.x >= nlist[1]) return;
+  const int seg = list[blockIdx.x];
+  if (s[sprob[seg]].done) return;
+  double rr = 0.0, rp = 0.0;
+  for (int64_t j = srow[seg] + threadIdx.x; j < srow[seg + 1]; j += kSegThreads) {
+    const float rj = (float)(sc[j] * res[j]);
+    r[j] = rj;
+    rr += (double)rj * (double)rj;
+    rp += (double)rj * (double)p[j];
+  }
+  const double a = seg_block_sum(rr, sh);
+  const double b = seg_block_sum(rp, sh);
+  if (threadIdx.x == 0) {
+    part[2 * seg] = a;
+    part[2 * seg + 1] = b;
   }
 }
 
-__global__ void k_b_check(int nprob, BScal* __restrict__ s) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nprob || !s[p].active) return;
-  const double rrt = s[p].rrt;
-  if (rrt <= s[p].tol2) s[p].done = 1;
-  s[p].maxrr = rrt;
-  s[p].rr = rrt;
-  s[p].hold = 0;
+// Hand-over, step 4 (one wave per problem): γ, γ', β (0: restart p = r̂); the recurrence
+// runs again (hold = 0).
+__global__ __launch_bounds__(64) void k_bc_reseed_fin(int nprob, const int32_t* __restrict__ pseg,
+                                                      const double* __restrict__ part,
+                                                      const int32_t* __restrict__ handing, BScal* __restrict__ s) {
+  const int pr = blockIdx.x;
+  if (pr >= nprob || !handing[pr] || s[pr].done) return;
+  const double ru = wave_seg_sum(part, pseg[pr], pseg[pr + 1], 2);
+  const double rp = wave_seg_sum(part + 1, pseg[pr], pseg[pr + 1], 2);
+  if (threadIdx.x != 0) return;
+  BScal& sp = s[pr];
+  const bool fresh = sp.restart || !(sp.rzp > 0.0) || !(ru > 0.0);
+  sp.gam = ru > 0.0 ? rp / ru : 0.0;
+  sp.beta = fresh ? 0.0 : ru / sp.rzp;
+  sp.rz = ru;
+  sp.rrec = sp.rr;
+  sp.hold = 0;
+  sp.restart = 0;
+  sp.fresh = 0;
 }
 
-__global__ void k_b_init(int64_t n, const double* __restrict__ f, const float* __restrict__ invd,
-                         float* __restrict__ r32, float* __restrict__ p32, float* __restrict__ z32,
-                         float* __restrict__ x32, double* __restrict__ u) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const float ri = (float)f[i];
-    const float zi = ri * invd[i];
-    r32[i] = ri;
-    p32[i] = zi;
-    z32[i] = zi;
-    x32[i] = 0.f;
-    u[i] = 0.0;
+// Hand-over, step 5 (segments, after w = Â r̂): p = r̂ + β(p − γ' r̂), s = w + β(s − γ' w)
+// (p = r̂, s = w on a restart); partial p·s.
+__global__ __launch_bounds__(kSegThreads) void k_bc_pfix(const int32_t* __restrict__ list,
+                                                         const int32_t* __restrict__ nlist,
+                                                         const int32_t* __restrict__ sprob,
+                                                         const int64_t* __restrict__ srow,
+                                                         const float* __restrict__ r, const float* __restrict__ w,
+                                                         float* __restrict__ p, float* __restrict__ sv,
+                                                         const BScal* __restrict__ s, double* __restrict__ part) {
+  __shared__ double sh[4];
+  if ((int)blockIdx.x >= nlist[1]) return;
+  const int seg = list[blockIdx.x];
+  const BScal& sp = s[sprob[seg]];
+  if (sp.done) return;
+  const float be = (float)sp.beta, ga = (float)sp.gam;
+  const bool keep = sp.beta != 0.0;
+  double ps = 0.0;
+  for (int64_t j = srow[seg] + threadIdx.x; j < srow[seg + 1]; j += kSegThreads) {
+    const float rj = r[j], wj = w[j];
+    const float pj = keep ? rj + be * (p[j] - ga * rj) : rj;
+    const float sj = keep ? wj + be * (sv[j] - ga * wj) : wj;
+    p[j] = pj;
+    sv[j] = sj;
+    ps += (double)pj * (double)sj;
+  }
+  const double t = seg_block_sum(ps, sh);
+  if (threadIdx.x == 0) part[seg] = t;
+}
+
+// Hand-over, step 6 (one thread per problem): α = γ / p·s; the next step only moves x̂, r̂.
+__global__ __launch_bounds__(64) void k_bc_alpha(int nprob, const int32_t* __restrict__ pseg,
+                                                 const double* __restrict__ part, const int32_t* __restrict__ handing,
+                                                 BScal* __restrict__ s) {
+  const int pr = blockIdx.x;
+  if (pr >= nprob || !handing[pr] || s[pr].done) return;
+  const double ps = wave_seg_sum(part, pseg[pr], pseg[pr + 1], 1);
+  if (threadIdx.x != 0) return;
+  BScal& sp = s[pr];
+  sp.pq = ps;
+  const bool ok = ps > 0.0;
+  sp.alpha = ok ? sp.rz / ps : 0.0;
+  sp.fresh = ok ? 1 : 0;
+  if (!ok) {   // p vanished (r̂ already zero): hand over again at once
+    sp.hold = 1;
+    sp.restart = 1;
   }
 }
 
-__global__ void k_b_invd(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ nzval,
-                         float* __restrict__ invd) {
+// ||f||² per problem (segment partials, then k_bc_bnorm), and the initial records.
+__global__ __launch_bounds__(kSegThreads) void k_bc_fsq(int nseg, const int64_t* __restrict__ srow,
+                                                        const double* __restrict__ f, double* __restrict__ part) {
+  __shared__ double sh[4];
+  const int seg = blockIdx.x;
+  if (seg >= nseg) return;
+  double d = 0.0;
+  for (int64_t j = srow[seg] + threadIdx.x; j < srow[seg + 1]; j += kSegThreads) d += f[j] * f[j];
+  const double t = seg_block_sum(d, sh);
+  if (threadIdx.x == 0) part[seg] = t;
+}
+
+__global__ __launch_bounds__(64) void k_bc_init(int nprob, const int32_t* __restrict__ pseg,
+                                                const double* __restrict__ part, double rtol2, double d2,
+                                                BScal* __restrict__ s) {
+  const int pr = blockIdx.x;
+  if (pr >= nprob) return;
+  const double t = wave_seg_sum(part, pseg[pr], pseg[pr + 1], 1);
+  if (threadIdx.x != 0) return;
+  BScal z{};
+  z.bnorm2 = t;
+  z.tol2 = rtol2 * t;
+  z.d2 = d2;
+  z.rr = t;
+  z.hold = 1;          // every problem starts with a hand-over from u = 0
+  if (t == 0.0) z.done = 1;   // f = 0: u = 0 is exact
+  s[pr] = z;
+  s[nprob + pr] = z;
+}
+
+// S = diag(K)^-1/2 (1 where the diagonal is zero) and the scaled FP32 operator.
+// ... and dsq = S⁻² in FP32, which turns r̂·r̂ into the unscaled ||r||² row by row.
+__global__ void k_bc_scale(int64_t n, const int32_t* __restrict__ diag, const double* __restrict__ nzval,
+                           double* __restrict__ sc, float* __restrict__ dsq) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
     const double d = nzval[diag[j]];
-    invd[j] = d != 0.0 ? (float)(1.0 / d) : 1.0f;
+    sc[j] = d != 0.0 ? 1.0 / sqrt(fabs(d)) : 1.0;
+    dsq[j] = d != 0.0 ? (float)fabs(d) : 1.0f;
   }
 }
 
-__global__ void k_b_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+// val = s_i K_ij s_j in FP32, 4 lanes per column (grid-stride over column groups).
+__global__ void k_bc_scaled_vals(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
+                                 const double* __restrict__ nzval, const double* __restrict__ sc,
+                                 float* __restrict__ val) {
+  const int sub = threadIdx.x & 3;
+  for (int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 2; j < n;
+       j += ((int64_t)gridDim.x * blockDim.x) >> 2) {
+    const double sj = sc[j];
+    for (int32_t q = colptr[j] + sub; q < colptr[j + 1]; q += 4) val[q] = (float)(nzval[q] * sj * sc[rowval[q]]);
+  }
+}
+
+__global__ void k_bc_zero32(int64_t n, float* __restrict__ a) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = (float)a[i];
+    a[i] = 0.f;
 }
 
 __global__ void k_b_zero(int64_t n, double* __restrict__ a) {
@@ -600,12 +781,6 @@ __global__ void k_b_zero(int64_t n, double* __restrict__ a) {
     a[i] = 0.0;
 }
 
-__global__ void k_b_init_scal(int nprob, BScal* __restrict__ s) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nprob) return;
-  BScal z{};
-  s[p] = z;
-}
 
 }  // namespace fem
 
@@ -614,8 +789,6 @@ using namespace fem;
 namespace {
 
 int grid_n(int64_t n, int b = 256) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + b - 1) / b, 8192)); }
-// 4 lanes per row, one row group per 4 threads, no grid-stride loop: every row gets a group
-int grid_rows4(int64_t n) { return (int)std::max<int64_t>(1, (4 * n + 255) / 256); }
 
 int build_items(fem_batch* B, int chunk) {
   fem_ctx* c = B->c;
@@ -739,38 +912,44 @@ int timed_assemble(fem_batch* B, int cpw, int chunk, bool mixed, float* ms) {
 }
 
 // Per-problem reduction t = Σ a·b, t2 = Σ a·c (two levels) followed by the scalar step `which`.
-template <typename X>
-void bdot(fem_batch* B, hipStream_t st, const X* a, const X* b, const X* c, int which) {
-  BScal* s = (BScal*)B->d_scal;
-  const int np_ = (int)B->probs.size();
-  k_b_dot_seg<X><<<B->nseg, 256, 0, st>>>(B->d_sprob, B->d_srow, a, b, c, s, which, B->d_spart);
-  k_b_dot_fin<<<(np_ + 63) / 64, 64, 0, st>>>(np_, B->d_pseg, B->d_spart, s, which);
-}
-
-// One CG iteration batch (16 iterations) captured once per batch object.
+// One batch of kCgBatch iterations, captured once per batch object: SpMV + fused step per
+// iteration over the list of recurring segments (its length is read on the device, so the
+// same graph serves every active set).  kCgBatch is even: a replay starts and ends on
+// scalar record 0 and partial buffer 0.
 constexpr int kCgBatch = 16;
 
-int ensure_cg_graph(fem_batch* B) {
-  if (B->cg_graph) return FEM_OK;
+hipGraphExec_t* cg_graph_for(fem_batch* B, int width) {
+  for (auto& g : B->cg_graphs)
+    if (g.first == width) return &g.second;
+  return nullptr;
+}
+
+int ensure_cg_graph(fem_batch* B, int width) {
+  if (cg_graph_for(B, width)) return FEM_OK;
   fem_ctx* c = B->c;
-  const int64_t n = c->ndofs;
+  const int np_ = (int)B->probs.size();
   BScal* s = (BScal*)B->d_scal;
+  double* pg[2] = {B->d_spart, B->d_spart + 2 * B->nseg};
+  double* pd = B->d_spart + 4 * B->nseg;
   hipStream_t st = c->stream;
   hipGraph_t g = nullptr;
+  FEM_HIP(c, hipStreamSynchronize(st));   // captured on first use of this width, mid-solve
   FEM_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int it = 0; it < kCgBatch; ++it) {
-    k_b_spmv32<<<grid_rows4(n), 256, 0, st>>>(n, c->d_colptr, c->d_rowval, B->d_val32, B->d_p32, B->d_q32,
-                                              B->d_rowprob, s);
-    bdot<float>(B, st, B->d_p32, B->d_q32, B->d_p32, kDotPQ);
-    k_b_update<float><<<grid_n(n), 256, 0, st>>>(n, B->d_rowprob, s, B->d_x32, B->d_r32, B->d_p32, B->d_q32,
-                                                 B->d_invd);
-    bdot<float>(B, st, B->d_r32, B->d_q32, B->d_r32, kDotRZRR);
-    k_b_pupd<float><<<grid_n(n), 256, 0, st>>>(n, B->d_rowprob, s, B->d_p32, B->d_q32);
+    BScal* sin = s + (it & 1) * np_;
+    BScal* sout = s + ((it + 1) & 1) * np_;
+    k_bc_spmv<<<width, kSpmvThreads, 0, st>>>(B->d_aseg, B->d_nlist, 0, B->d_sprob, B->d_srow, c->d_colptr,
+                                               c->d_rowval, B->d_val32, B->d_r32, B->d_w32, sin, pd);
+    k_bc_step<<<width, kSegThreads, 0, st>>>(B->d_aseg, B->d_nlist, B->d_sprob, B->d_srow, B->d_pseg, B->d_x32,
+                                               B->d_r32, B->d_w32, B->d_p32, B->d_s32, B->d_dsq, pd, pg[it & 1],
+                                               pg[(it + 1) & 1], sin, sout);
   }
   FEM_HIP(c, hipStreamEndCapture(st, &g));
-  hipError_t e = hipGraphInstantiate(&B->cg_graph, g, nullptr, nullptr, 0);
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return check_hip(c, e, "hipGraphInstantiate (batch CG)");
+  B->cg_graphs.emplace_back(width, ex);
   return FEM_OK;
 }
 
@@ -802,7 +981,7 @@ void fem_batch_destroy(fem_batch* B) {
     (void)hipStreamSynchronize(c->stream);
   }
   if (B->graph) (void)hipGraphExecDestroy(B->graph);
-  if (B->cg_graph) (void)hipGraphExecDestroy(B->cg_graph);
+  for (auto& g : B->cg_graphs) (void)hipGraphExecDestroy(g.second);
   for (hipEvent_t* e : {&B->e_start, &B->e_asm, &B->e_apply, &B->e_mark})
     if (*e) (void)hipEventDestroy(*e);
   dev_free(&B->d_cnodes8); dev_free(&B->d_cdofs8); dev_free(&B->d_cnz); dev_free(&B->d_kslot);
@@ -810,7 +989,8 @@ void fem_batch_destroy(fem_batch* B) {
   dev_free(&B->d_probs); dev_free(&B->d_rowprob); dev_free(&B->d_pdof); dev_free(&B->d_tab); dev_free(&B->d_tabf);
   dev_free(&B->d_sprob); dev_free(&B->d_srow); dev_free(&B->d_pseg); dev_free(&B->d_spart);
   dev_free(&B->d_m); dev_free(&B->d_val32); dev_free(&B->d_x32); dev_free(&B->d_r32); dev_free(&B->d_p32);
-  dev_free(&B->d_q32); dev_free(&B->d_invd); dev_free(&B->d_res); dev_free(&B->d_scal);
+  dev_free(&B->d_w32); dev_free(&B->d_s32); dev_free(&B->d_sc); dev_free(&B->d_dsq); dev_free(&B->d_res); dev_free(&B->d_scal);
+  dev_free(&B->d_aseg); dev_free(&B->d_hseg); dev_free(&B->d_nlist); dev_free(&B->d_handing);
   if (c) fem_ctx_destroy(c);
   delete B;
 }
@@ -991,9 +1171,11 @@ int32_t fem_batch_finalize(fem_batch* B, const double* quad_N, const double* qua
       (rc = dev_alloc(c, &B->d_tab, kTabSize)) || (rc = dev_alloc(c, &B->d_tabf, kTabSize)) ||
       (rc = dev_alloc(c, &B->d_m, np_)) || (rc = dev_alloc(c, &B->d_val32, nz)) ||
       (rc = dev_alloc(c, &B->d_x32, nd)) || (rc = dev_alloc(c, &B->d_r32, nd)) ||
-      (rc = dev_alloc(c, &B->d_p32, nd)) || (rc = dev_alloc(c, &B->d_q32, nd)) ||
-      (rc = dev_alloc(c, &B->d_invd, nd)) || (rc = dev_alloc(c, &B->d_res, nd)) ||
-      (rc = dev_alloc(c, (BScal**)&B->d_scal, np_)))
+      (rc = dev_alloc(c, &B->d_p32, nd)) || (rc = dev_alloc(c, &B->d_w32, nd)) ||
+      (rc = dev_alloc(c, &B->d_s32, nd)) || (rc = dev_alloc(c, &B->d_sc, nd)) ||
+      (rc = dev_alloc(c, &B->d_dsq, nd)) ||
+      (rc = dev_alloc(c, &B->d_res, nd)) || (rc = dev_alloc(c, (BScal**)&B->d_scal, 2 * np_)) ||
+      (rc = dev_alloc(c, &B->d_nlist, 2)) || (rc = dev_alloc(c, &B->d_handing, np_)))
     return rc;
   FEM_HIP(c, hipMemcpyAsync(B->d_cnodes8, cn8.data(), 4 * cn8.size(), hipMemcpyHostToDevice, st));
   FEM_HIP(c, hipMemcpyAsync(B->d_cdofs8, cd8.data(), 4 * cd8.size(), hipMemcpyHostToDevice, st));
@@ -1028,12 +1210,14 @@ int32_t fem_batch_finalize(fem_batch* B, const double* quad_N, const double* qua
     srow.push_back(nd);
     B->nseg = (int)sprob.size();
     if ((rc = dev_alloc(c, &B->d_sprob, B->nseg)) || (rc = dev_alloc(c, &B->d_srow, B->nseg + 1)) ||
-        (rc = dev_alloc(c, &B->d_pseg, np_ + 1)) || (rc = dev_alloc(c, &B->d_spart, 2 * B->nseg)))
+        (rc = dev_alloc(c, &B->d_pseg, np_ + 1)) || (rc = dev_alloc(c, &B->d_spart, 6 * B->nseg)) ||
+        (rc = dev_alloc(c, &B->d_aseg, B->nseg)) || (rc = dev_alloc(c, &B->d_hseg, B->nseg)))
       return rc;
     FEM_HIP(c, hipMemcpyAsync(B->d_sprob, sprob.data(), 4 * sprob.size(), hipMemcpyHostToDevice, st));
     FEM_HIP(c, hipMemcpyAsync(B->d_srow, srow.data(), 8 * srow.size(), hipMemcpyHostToDevice, st));
     FEM_HIP(c, hipMemcpyAsync(B->d_pseg, pseg.data(), 4 * pseg.size(), hipMemcpyHostToDevice, st));
     FEM_HIP(c, hipStreamSynchronize(st));
+    B->h_pseg = pseg;
   }
   B->nslots = ns;
   B->finalized = true;
@@ -1082,11 +1266,10 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
   if (!cfg->cells_per_wg && !shape_ok(cpw, chunk, mixed)) cpw = 128;
   if (!shape_ok(cpw, chunk, mixed)) return set_error(c, FEM_ERR_ARG, "fem_batch_run: unsupported launch shape");
   if (!(cfg->rtol > 0.0)) return set_error(c, FEM_ERR_ARG, "fem_batch_run: rtol must be positive");
-  const int64_t n = c->ndofs, nnz = c->nnz;
+  const int64_t n = c->ndofs;
   hipStream_t st = c->stream;
   BScal* d_s = (BScal*)B->d_scal;
   int rc;
-  if ((rc = ensure_cg_graph(B))) return rc;
   // assembly: zero + persistent work-queue kernel (one graph)
   FEM_HIP(c, hipEventRecord(B->e_start, st));
   if ((rc = assemble_graph_launch(B, cpw, chunk, mixed))) return rc;
@@ -1097,71 +1280,64 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
   if (c->np > 0)
     k_b_setdiag<<<grid_n(c->np), 256, 0, st>>>(c->np, c->d_pres, c->d_diag, B->d_rowprob, B->d_m, c->d_nzval, c->d_f);
   FEM_HIP(c, hipEventRecord(B->e_apply, st));
-  // solver set-up: FP32 operator, Jacobi, per-problem scalars
-  k_b_f32<<<grid_n(nnz), 256, 0, st>>>(nnz, c->d_nzval, B->d_val32);
-  k_b_invd<<<grid_n(n), 256, 0, st>>>(n, c->d_diag, c->d_nzval, B->d_invd);
-  k_b_init_scal<<<(np_ + 63) / 64, 64, 0, st>>>(np_, d_s);
-  bdot<double>(B, st, c->d_f, c->d_f, c->d_f, kDotF);
-  k_b_init<<<grid_n(n), 256, 0, st>>>(n, c->d_f, B->d_invd, B->d_r32, B->d_p32, B->d_q32, B->d_x32, c->d_u);
+  // solver set-up: S = diag(K)^-1/2, the scaled FP32 operator, records (every problem
+  // starts with a hand-over from u = 0)
+  double* ph = B->d_spart + 4 * B->nseg;   // hand-over partials (2 per segment)
+  k_bc_scale<<<grid_n(n), 256, 0, st>>>(n, c->d_diag, c->d_nzval, B->d_sc, B->d_dsq);
+  k_bc_scaled_vals<<<grid_n(n), 256, 0, st>>>(n, c->d_colptr, c->d_rowval, c->d_nzval, B->d_sc, B->d_val32);
+  k_b_zero<<<grid_n(n), 256, 0, st>>>(n, c->d_u);
+  k_bc_zero32<<<grid_n(n), 256, 0, st>>>(n, B->d_x32);
+  k_bc_zero32<<<grid_n(n), 256, 0, st>>>(n, B->d_p32);
+  k_bc_zero32<<<grid_n(n), 256, 0, st>>>(n, B->d_s32);
+  const double delta = cfg->ru_delta > 0 ? cfg->ru_delta : 0.1;
+  k_bc_fsq<<<B->nseg, kSegThreads, 0, st>>>(B->nseg, B->d_srow, c->d_f, ph);
+  k_bc_init<<<np_, 64, 0, st>>>(np_, B->d_pseg, ph, cfg->rtol * cfg->rtol, delta * delta, d_s);
+  FEM_HIP(c, hipGetLastError());
   std::vector<BScal> hs(np_);
   FEM_HIP(c, hipMemcpyAsync(hs.data(), d_s, sizeof(BScal) * np_, hipMemcpyDeviceToHost, st));
   FEM_HIP(c, hipStreamSynchronize(st));
   if ((rc = assemble_errors(B))) return rc;
-  for (int p = 0; p < np_; ++p) hs[p].active = 1;   // r·z of every problem (p = z already)
-  FEM_HIP(c, hipMemcpyAsync(d_s, hs.data(), sizeof(BScal) * np_, hipMemcpyHostToDevice, st));
-  bdot<float>(B, st, B->d_r32, B->d_q32, B->d_p32, kDotReseed);
-  FEM_HIP(c, hipMemcpyAsync(hs.data(), d_s, sizeof(BScal) * np_, hipMemcpyDeviceToHost, st));
-  FEM_HIP(c, hipStreamSynchronize(st));
   std::vector<double> done_ms(np_, -1.0);
   std::vector<int> iters(np_, 0);
   int ndone = 0;
-  const double delta = cfg->ru_delta > 0 ? cfg->ru_delta : 0.1;
   float t_asm = 0.f, t_apply = 0.f;
   (void)hipEventElapsedTime(&t_asm, B->e_start, B->e_asm);
   (void)hipEventElapsedTime(&t_apply, B->e_asm, B->e_apply);
-  for (int p = 0; p < np_; ++p) {
-    hs[p].active = 0;
-    hs[p].hold = 0;
-    hs[p].restart = 0;
-    hs[p].d2 = delta * delta;
-    hs[p].tol2 = cfg->rtol * cfg->rtol * hs[p].bnorm2;
-    hs[p].maxrr = hs[p].bnorm2;
-    hs[p].rr = hs[p].bnorm2;
-    if (hs[p].bnorm2 == 0.0) {   // f = 0: u = 0 is exact
-      hs[p].done = 1;
+  for (int p = 0; p < np_; ++p)
+    if (hs[p].done) {   // f = 0: u = 0 is exact
       done_ms[p] = t_asm + t_apply;
       ++ndone;
     }
-  }
-  FEM_HIP(c, hipMemcpyAsync(d_s, hs.data(), sizeof(BScal) * np_, hipMemcpyHostToDevice, st));
-  const int maxit = cfg->maxit > 0 ? cfg->maxit : 20000;
-  int launched = 0;
-  while (ndone < np_ && launched < maxit) {
-    FEM_HIP(c, hipGraphLaunch(B->cg_graph, st));
-    launched += kCgBatch;
-    FEM_HIP(c, hipMemcpyAsync(hs.data(), d_s, sizeof(BScal) * np_, hipMemcpyDeviceToHost, st));
-    FEM_HIP(c, hipStreamSynchronize(st));
-    // reliable update for every unfinished problem whose recurrence stopped (fell by δ,
-    // met the tolerance or broke down)
+  std::vector<int32_t> list, handing(np_);
+  // hand-over of every problem whose recurrence stopped (or of all unfinished ones at maxit):
+  // fold x̂ into u, FP64 true residual, done or reseed (r̂ = S res, re-orthogonalised p, α)
+  auto hand_over = [&](bool all) -> int {
+    list.clear();
     bool any = false;
     for (int p = 0; p < np_; ++p) {
-      hs[p].active = 0;
-      if (hs[p].done) continue;
-      if (hs[p].hold || launched >= maxit) {
-        hs[p].active = 1;
-        any = true;
-      }
+      handing[p] = !hs[p].done && (hs[p].hold || all);
+      if (!handing[p]) continue;
+      any = true;
+      for (int k = B->h_pseg[p]; k < B->h_pseg[p + 1]; ++k) list.push_back(k);
     }
-    if (!any) continue;
-    FEM_HIP(c, hipMemcpyAsync(d_s, hs.data(), sizeof(BScal) * np_, hipMemcpyHostToDevice, st));
-    k_b_fold<<<grid_n(n), 256, 0, st>>>(n, B->d_rowprob, d_s, B->d_x32, c->d_u);
-    k_b_resid64<<<grid_rows4(n), 256, 0, st>>>(n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_u, c->d_f,
-                                               B->d_rowprob, d_s, B->d_res);
-    bdot<double>(B, st, B->d_res, B->d_res, B->d_res, kDotTrue);
-    k_b_check<<<(np_ + 63) / 64, 64, 0, st>>>(np_, d_s);
-    k_b_reseed<<<grid_n(n), 256, 0, st>>>(n, B->d_rowprob, d_s, B->d_res, B->d_invd, B->d_r32, B->d_q32);
-    bdot<float>(B, st, B->d_r32, B->d_q32, B->d_p32, kDotReseed);
-    k_b_pfix<<<grid_n(n), 256, 0, st>>>(n, B->d_rowprob, d_s, B->d_q32, B->d_p32);
+    if (!any) return FEM_OK;
+    const int32_t len = (int32_t)list.size();
+    FEM_HIP(c, hipMemcpyAsync(B->d_hseg, list.data(), 4 * list.size(), hipMemcpyHostToDevice, st));
+    FEM_HIP(c, hipMemcpyAsync(B->d_nlist + 1, &len, 4, hipMemcpyHostToDevice, st));
+    FEM_HIP(c, hipMemcpyAsync(B->d_handing, handing.data(), 4 * np_, hipMemcpyHostToDevice, st));
+    k_bc_fold<<<len, kSegThreads, 0, st>>>(B->d_hseg, B->d_nlist, B->d_srow, B->d_sc, B->d_x32, c->d_u);
+    k_bc_resid<<<len, kSpmvThreads, 0, st>>>(B->d_hseg, B->d_nlist, B->d_srow, c->d_colptr, c->d_rowval, c->d_nzval,
+                                            c->d_u, c->d_f, B->d_res, ph);
+    k_bc_true<<<np_, 64, 0, st>>>(np_, B->d_pseg, ph, B->d_handing, d_s);
+    k_bc_reseed<<<len, kSegThreads, 0, st>>>(B->d_hseg, B->d_nlist, B->d_sprob, B->d_srow, B->d_sc, B->d_res,
+                                             B->d_r32, B->d_p32, d_s, ph);
+    k_bc_reseed_fin<<<np_, 64, 0, st>>>(np_, B->d_pseg, ph, B->d_handing, d_s);
+    k_bc_spmv<<<len, kSpmvThreads, 0, st>>>(B->d_hseg, B->d_nlist, 1, B->d_sprob, B->d_srow, c->d_colptr, c->d_rowval,
+                                           B->d_val32, B->d_r32, B->d_w32, d_s, ph);
+    k_bc_pfix<<<len, kSegThreads, 0, st>>>(B->d_hseg, B->d_nlist, B->d_sprob, B->d_srow, B->d_r32, B->d_w32,
+                                           B->d_p32, B->d_s32, d_s, ph);
+    k_bc_alpha<<<np_, 64, 0, st>>>(np_, B->d_pseg, ph, B->d_handing, d_s);
+    FEM_HIP(c, hipGetLastError());
     FEM_HIP(c, hipEventRecord(B->e_mark, st));
     FEM_HIP(c, hipMemcpyAsync(hs.data(), d_s, sizeof(BScal) * np_, hipMemcpyDeviceToHost, st));
     FEM_HIP(c, hipStreamSynchronize(st));
@@ -1173,6 +1349,34 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
         iters[p] = hs[p].iters;
         ++ndone;
       }
+    return FEM_OK;
+  };
+  if ((rc = hand_over(false))) return rc;
+  const int maxit = cfg->maxit > 0 ? cfg->maxit : 20000;
+  int launched = 0, idle = 0;
+  while (ndone < np_ && launched < maxit) {
+    // the segments of every problem still in its recurrence
+    list.clear();
+    for (int p = 0; p < np_; ++p)
+      if (!hs[p].done && !hs[p].hold)
+        for (int k = B->h_pseg[p]; k < B->h_pseg[p + 1]; ++k) list.push_back(k);
+    if (!list.empty()) {
+      const int32_t len = (int32_t)list.size();
+      FEM_HIP(c, hipMemcpyAsync(B->d_aseg, list.data(), 4 * list.size(), hipMemcpyHostToDevice, st));
+      FEM_HIP(c, hipMemcpyAsync(B->d_nlist, &len, 4, hipMemcpyHostToDevice, st));
+      int width = 64;
+      while (width < len) width *= 2;
+      width = std::min(width, B->nseg);
+      if ((rc = ensure_cg_graph(B, width))) return rc;
+      FEM_HIP(c, hipGraphLaunch(*cg_graph_for(B, width), st));
+      launched += kCgBatch;
+      FEM_HIP(c, hipMemcpyAsync(hs.data(), d_s, sizeof(BScal) * np_, hipMemcpyDeviceToHost, st));
+      FEM_HIP(c, hipStreamSynchronize(st));
+      idle = 0;
+    } else if (++idle > 64) {
+      break;   // every unfinished problem breaks down at once on each reseed: give up
+    }
+    if ((rc = hand_over(launched >= maxit))) return rc;
   }
   FEM_HIP(c, hipEventRecord(B->e_mark, st));
   FEM_HIP(c, hipEventSynchronize(B->e_mark));
@@ -1197,8 +1401,8 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
     for (int p = 0; p < np_ && shown < 8; ++p) {
       if (hs[p].done) continue;
       char buf[200];
-      snprintf(buf, sizeof buf, " [%d: iters %d, true %.3e, recursive %.3e, hold %d]", p, hs[p].iters,
-               std::sqrt(hs[p].rrt / hs[p].bnorm2), std::sqrt(std::fabs(hs[p].rr) / hs[p].bnorm2), hs[p].hold);
+      snprintf(buf, sizeof buf, " [%d: iters %d, true %.3e, hold %d]", p, hs[p].iters,
+               std::sqrt(hs[p].rr / hs[p].bnorm2), hs[p].hold);
       msg += buf;
       ++shown;
     }

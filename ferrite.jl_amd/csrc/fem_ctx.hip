@@ -153,6 +153,12 @@ void fem_ctx_destroy(fem_ctx* c) {
   dev_free(&c->d_cparam); dev_free(&c->d_param_cell); dev_free(&c->d_f); dev_free(&c->d_u);
   dev_free(&c->d_gcnodes); dev_free(&c->d_d2c_ptr); dev_free(&c->d_d2c); dev_free(&c->d_gslot);
   dev_free(&c->d_keb64); dev_free(&c->d_feb64); dev_free(&c->d_keb32); dev_free(&c->d_feb32);
+  dev_free(&c->d_gblist);
+  for (hipEvent_t e : c->gchunk_ev) (void)hipEventDestroy(e);
+  if (c->stream2) {
+    (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamDestroy(c->stream2);
+  }
   dev_free(&c->d_pres); dev_free(&c->d_g); dev_free(&c->d_isc);
   dev_free(&c->d_scalar); dev_free(&c->d_partials); dev_free(&c->d_err);
   dev_free(&c->d_tab); dev_free(&c->d_tabf);
@@ -196,6 +202,7 @@ int32_t fem_mesh_upload(fem_ctx* c, int32_t sdim, int64_t nnodes, const double* 
   c->sdim = sdim; c->nnodes = nnodes; c->ncells = ncells; c->npc = npc;
   c->tables_ready = false;
   c->gather_ready = false;
+  c->gchunks_built = 0;
   return FEM_OK;
 }
 
@@ -219,6 +226,7 @@ int32_t fem_dofs_upload(fem_ctx* c, int64_t ndofs, int32_t ndpc, const int64_t* 
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   c->tables_ready = false;
   c->gather_ready = false;
+  c->gchunks_built = 0;
   return FEM_OK;
 }
 
@@ -260,6 +268,7 @@ int32_t fem_pattern_upload(fem_ctx* c, int64_t n, const int64_t* colptr, const i
   free_solver(c);
   c->tables_ready = false;
   c->gather_ready = false;
+  c->gchunks_built = 0;
   return FEM_OK;
 }
 

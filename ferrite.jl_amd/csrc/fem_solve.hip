@@ -1226,7 +1226,9 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
       !(t->sell_map >= 0 && t->sell_map <= 2) ||
       !(t->gather_group == 0 || t->gather_group == 1 || t->gather_group == 2 || t->gather_group == 4 ||
         t->gather_group == 8) ||
-      !(t->cg_elems == 0 || t->cg_elems == 2 || t->cg_elems == 4 || t->cg_elems == 8 || t->cg_elems == 16))
+      !(t->cg_elems == 0 || t->cg_elems == 2 || t->cg_elems == 4 || t->cg_elems == 8 || t->cg_elems == 16) ||
+      !(t->gather_chunks == 0 || t->gather_chunks == 1 || t->gather_chunks == 2 || t->gather_chunks == 4 ||
+        t->gather_chunks == 8 || t->gather_chunks == 16 || t->gather_chunks == 32))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

@@ -82,6 +82,17 @@ def main():
             print(f"cg elems {ce}: {'; '.join(line)}", flush=True)
         fa.set_tuning(K, cg_elems=0)
     if args.gather_sweep:
+        for chunks in (1, 2, 4, 8, 16, 32):
+            fa.set_tuning(K, gather_chunks=chunks)
+            ts = []
+            for prec in ("fp64", "mixed"):
+                best = 1e30
+                for _ in range(3):
+                    fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=prec)
+                    best = min(best, fa.timings(K)["assemble_kernel_ms"])
+                ts.append(best)
+            print(f"gather chunks {chunks}: fp64 {ts[0]:.3f} ms, fp32-element {ts[1]:.3f} ms", flush=True)
+        fa.set_tuning(K, gather_chunks=0)
         for grp in (1, 2, 4, 8):
             fa.set_tuning(K, gather_group=grp)
             ts = []

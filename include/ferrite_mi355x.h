@@ -165,6 +165,9 @@ typedef struct {
                                         lane loads before adding them, 1 / 2 / 4 / 8 (0 = default 4) */
   int32_t cg_elems;                  /* CG vector step: elements per thread per pass, 2 / 4 / 8 / 16
                                         (0 = default 8) */
+  int32_t gather_chunks;             /* owner-computes assembly: cell chunks whose element kernel
+                                        overlaps the previous chunk's gather on a second stream,
+                                        1 (no overlap) / 2 / 4 / 8 / 16 / 32 (0 = default 1) */
   int32_t reserved;
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
