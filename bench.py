@@ -147,7 +147,7 @@ def main():
     # y = K x with int32 indices, whatever the stored format: values + indices + colptr + x + y)
     spmv_iso_ms = fa.spmv_bench(K, mixed=True, reps=50)   # back to back, matrix partly cache-resident
     spmv_ms = results["mixed"]["spmv_avg_ms"]               # inside the CG loop of the timed steps
-    SPMV32_KERNEL = "k_sell<float, float, 0, true, 8, false>"   # sliced ELL, 16-bit column offsets
+    SPMV32_KERNEL = "k_sell<float, float, 0, true, 8, false, true>"   # sliced ELL, u16 offsets, NT loads
     nnz, nd = K.nnz, dh.ndofs
     spmv_bytes = nnz * (4 + 4) + 4 * (nd + 1) + 4 * nd + 4 * nd   # values, int32 indices, colptr, x, y
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9

@@ -288,7 +288,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, int R, const int32_t
 // the x window they gather stay in that XCD's L2; otherwise blocks stride over slices.
 //   MODE 0: y = K x, partial Σ x·y ;  MODE 1: y = f − K x, partial Σ y²
 // --------------------------------------------------------------------------
-template <typename V, typename X, int MODE, bool COMP, int U, bool CONTIG>
+// Streamed once per launch: loaded non-temporally (NT) when asked, so that they do not
+// evict the gathered vector from L2.
+template <bool NT, typename T>
+__device__ __forceinline__ T stream_load(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+template <typename V, typename X, int MODE, bool COMP, int U, bool CONTIG, bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, int64_t nslices, const int32_t* __restrict__ off,
                                                  const int32_t* __restrict__ base,
                                                  const uint16_t* __restrict__ roff,
@@ -331,8 +339,8 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, int64_t nslices, con
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int32_t e = e0 + 64 * (k + u);
-        v[u] = val[e];
-        ci[u] = COMP ? b + (int32_t)roff[e] : col[e];
+        v[u] = stream_load<NT>(val + e);
+        ci[u] = COMP ? b + (int32_t)stream_load<NT>(roff + e) : stream_load<NT>(col + e);
       }
       X xv[U];
 #pragma unroll
@@ -342,8 +350,8 @@ __global__ __launch_bounds__(kBlock) void k_sell(int64_t n, int64_t nslices, con
     }
     for (; k < W; ++k) {
       const int32_t e = e0 + 64 * k;
-      const int32_t ci = COMP ? b + (int32_t)roff[e] : col[e];
-      acc += (A)val[e] * (A)x[ci];
+      const int32_t ci = COMP ? b + (int32_t)stream_load<NT>(roff + e) : stream_load<NT>(col + e);
+      acc += (A)stream_load<NT>(val + e) * (A)x[ci];
     }
     if (row < n) {
       if (MODE == 0) {
@@ -911,11 +919,24 @@ static void launch_sell(fem_ctx* c, const SellOp& o, int g, const V* val, const 
   // grid-stride slices beat the XCD-contiguous ranges in both precisions
   const int u = c->tune.sell_unroll ? c->tune.sell_unroll : (sizeof(V) == 4 ? 8 : 4);
   const bool contig = c->tune.sell_map == 2 && g % 8 == 0;
+  // non-temporal operator loads by default: measured at 128³, FP64 CG loop 103.9 → 96.5 ms,
+  // mixed loop 69.0 → 66.3 ms (the gathered vectors stay in L2)
+  const bool nt = c->tune.sell_nt != -1;
   const int32_t* col = COMP ? nullptr : o.col;
   const uint16_t* roff = COMP ? o.roff : nullptr;
 #define FEM_SELL(UU, CC)                                                                                 \
   k_sell<V, X, MODE, COMP, UU, CC><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, roff, col, \
                                                                  val, x, f, y, part, sc)
+  if (nt && !contig) {
+    const int32_t* cl = col;
+    if (u == 4)
+      k_sell<V, X, MODE, COMP, 4, false, true><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, roff,
+                                                                           cl, val, x, f, y, part, sc);
+    else
+      k_sell<V, X, MODE, COMP, 8, false, true><<<g, kBlock, 0, c->stream>>>(c->n, o.nslices, o.off, o.base, roff,
+                                                                           cl, val, x, f, y, part, sc);
+    return;
+  }
   if (contig) {
     if (u == 4) FEM_SELL(4, true);
     else if (u == 16) FEM_SELL(16, true);
@@ -1228,7 +1249,8 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
         t->gather_group == 8) ||
       !(t->cg_elems == 0 || t->cg_elems == 2 || t->cg_elems == 4 || t->cg_elems == 8 || t->cg_elems == 16) ||
       !(t->gather_chunks == 0 || t->gather_chunks == 1 || t->gather_chunks == 2 || t->gather_chunks == 4 ||
-        t->gather_chunks == 8 || t->gather_chunks == 16 || t->gather_chunks == 32))
+        t->gather_chunks == 8 || t->gather_chunks == 16 || t->gather_chunks == 32) ||
+      !(t->sell_nt == 0 || t->sell_nt == 1 || t->sell_nt == -1))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

@@ -106,12 +106,17 @@ def main():
         fa.set_tuning(K, gather_group=0)
         fa.apply(K, f, ch)
     if args.sell_sweep:
-        for smap, mname in ((1, "grid-stride"), (2, "contiguous/XCD")):
-            for unroll in (4, 8, 16):
-                fa.set_tuning(K, sell_map=smap, sell_unroll=unroll)
+        for smap, mname, unrolls in ((1, "grid-stride, ordinary loads", (4, 8, 16)), (2, "contiguous/XCD", (4, 8)),
+                                     (3, "grid-stride, non-temporal loads", (4, 8))):
+            for unroll in unrolls:
+                if smap == 3:
+                    fa.set_tuning(K, sell_map=1, sell_nt=1, sell_unroll=unroll)
+                else:
+                    fa.set_tuning(K, sell_map=smap, sell_nt=-1, sell_unroll=unroll)
                 a64 = fa.spmv_bench(K, mixed=False, reps=20)
                 a32 = fa.spmv_bench(K, mixed=True, reps=20)
-                fa.set_tuning(K, sell_map=smap, sell_unroll=unroll, time_kernels=1)
+                fa.set_tuning(K, sell_map=smap if smap != 3 else 1, sell_unroll=unroll, time_kernels=1,
+                              sell_nt=1 if smap == 3 else -1)
                 line = []
                 for mode in ("cg", "mixed"):
                     r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
@@ -120,7 +125,7 @@ def main():
                                 f"{t['spmv_launches']} ({r.iterations} its, loop {t['solve_ms']:.1f} ms)")
                 print(f"sell {mname} unroll {unroll}: isolated fp64 {a64 * 1e3:.1f} us, fp32 {a32 * 1e3:.1f} us; "
                       f"in-solve {'; '.join(line)}", flush=True)
-        fa.set_tuning(K, sell_map=0, sell_unroll=0, time_kernels=0)
+        fa.set_tuning(K, sell_map=0, sell_unroll=0, time_kernels=0, sell_nt=0)
     if args.sweep:
         a64 = fa.spmv_bench(K, mixed=False, reps=20)
         a32 = fa.spmv_bench(K, mixed=True, reps=20)

@@ -168,6 +168,8 @@ typedef struct {
   int32_t gather_chunks;             /* owner-computes assembly: cell chunks whose element kernel
                                         overlaps the previous chunk's gather on a second stream,
                                         1 (no overlap) / 2 / 4 / 8 / 16 / 32 (0 = default 1) */
+  int32_t sell_nt;                   /* sliced-ELL SpMV: non-temporal loads of the operator's values
+                                        and offsets (0 = default, 1) or ordinary loads (-1) */
   int32_t reserved;
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
