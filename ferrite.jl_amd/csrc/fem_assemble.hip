@@ -138,6 +138,38 @@ static void pack_tables(const RefTables& r, std::vector<T>& out) {
   }
 }
 
+// Trilinear hex geometry: Ferrite's node order (reference coordinates below) with
+// dM_a(ξ) = s_a ⊙ [(1 + s_η η)(1 + s_ζ ζ), (1 + s_ξ ξ)(1 + s_ζ ζ), (1 + s_ξ ξ)(1 + s_η η)] / 8.
+// Recovers each point's (ξ, η, ζ) from the uploaded dM table and accepts the point only if
+// the whole table is reproduced to 1e-14.  qc[q] = ξ, η, ζ, ηζ, ξζ, ξη.
+static bool detect_trilinear(const fem_ctx* c, std::vector<double>& qc) {
+  if (c->rdim != 3 || c->ngeo != 8) return false;
+  static const int sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                               {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  qc.assign(6 * c->nqp, 0.0);
+  for (int q = 0; q < c->nqp; ++q) {
+    const auto& dM = c->ref.dM[q];
+    // ∂M/∂ξ of nodes 0 (η = ζ = −1 signs) and 3 (η = +1): η = (d3 − d0)/(d3 + d0), etc.
+    const double e0 = dM[0][0], e3 = dM[3][0], e4 = dM[4][0];
+    const double f0 = dM[0][1], f1 = dM[1][1];
+    if (e0 + e3 == 0.0 || e0 + e4 == 0.0 || f0 + f1 == 0.0) return false;
+    const double eta = (e3 - e0) / (e3 + e0);
+    const double zeta = (e4 - e0) / (e4 + e0);
+    const double xi = (f1 - f0) / (f1 + f0);
+    const double p[3] = {xi, eta, zeta};
+    for (int a = 0; a < 8; ++a)
+      for (int d = 0; d < 3; ++d) {
+        double v = sg[a][d] / 8.0;
+        for (int e = 0; e < 3; ++e)
+          if (e != d) v *= 1.0 + sg[a][e] * p[e];
+        if (std::fabs(v - dM[a][d]) > 1e-14) return false;
+      }
+    double* o = &qc[6 * q];
+    o[0] = xi; o[1] = eta; o[2] = zeta; o[3] = eta * zeta; o[4] = xi * zeta; o[5] = xi * eta;
+  }
+  return true;
+}
+
 template <int DIM, int NG, int NB, int NQ>
 static int upload_tables(fem_ctx* c) {
   if (!c->tab_dirty) return FEM_OK;
@@ -149,6 +181,41 @@ static int upload_tables(fem_ctx* c) {
   if ((rc = dev_alloc(c, &c->d_tab, td.size())) || (rc = dev_alloc(c, &c->d_tabf, tf.size()))) return rc;
   FEM_HIP(c, hipMemcpyAsync(c->d_tab, td.data(), td.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   FEM_HIP(c, hipMemcpyAsync(c->d_tabf, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  std::vector<double> qc;
+  c->trilinear = detect_trilinear(c, qc);
+  if (c->trilinear) {
+    std::vector<float> qf(qc.begin(), qc.end());
+    if ((rc = dev_alloc(c, &c->d_qc, qc.size())) || (rc = dev_alloc(c, &c->d_qcf, qf.size()))) return rc;
+    FEM_HIP(c, hipMemcpyAsync(c->d_qc, qc.data(), qc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    FEM_HIP(c, hipMemcpyAsync(c->d_qcf, qf.data(), qf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    // reference integrals of the affine fast path (same quadrature as the element loop),
+    // strict upper triangle in row-major order (the stored element planes), then Σ_q w_q N_a
+    const int nb = c->nbf, nq = c->nqp;
+    static const int mi[6] = {0, 1, 2, 0, 0, 1}, mj[6] = {0, 1, 2, 1, 2, 2};
+    std::vector<double> aff;
+    for (int a = 0; a < nb; ++a)
+      for (int b = a + 1; b < nb; ++b)
+        for (int m = 0; m < 6; ++m) {
+          double v = 0.0;
+          for (int q = 0; q < nq; ++q) {
+            const auto& g = c->ref.dN[q];
+            double t = g[a][mi[m]] * g[b][mj[m]];
+            if (mi[m] != mj[m]) t += g[a][mj[m]] * g[b][mi[m]];
+            v += c->ref.w[q] * t;
+          }
+          aff.push_back(v);
+        }
+    for (int a = 0; a < nb; ++a) {
+      double v = 0.0;
+      for (int q = 0; q < nq; ++q) v += c->ref.w[q] * c->ref.N[q][a];
+      aff.push_back(v);
+    }
+    std::vector<float> afff(aff.begin(), aff.end());
+    if ((rc = dev_alloc(c, &c->d_aff, aff.size())) || (rc = dev_alloc(c, &c->d_afff, afff.size()))) return rc;
+    FEM_HIP(c, hipMemcpyAsync(c->d_aff, aff.data(), aff.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    FEM_HIP(c, hipMemcpyAsync(c->d_afff, afff.data(), afff.size() * sizeof(float), hipMemcpyHostToDevice,
+                              c->stream));
+  }
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   c->tab_dirty = false;
   return FEM_OK;

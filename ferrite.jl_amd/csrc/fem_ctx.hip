@@ -161,7 +161,8 @@ void fem_ctx_destroy(fem_ctx* c) {
   }
   dev_free(&c->d_pres); dev_free(&c->d_g); dev_free(&c->d_isc);
   dev_free(&c->d_scalar); dev_free(&c->d_partials); dev_free(&c->d_err);
-  dev_free(&c->d_tab); dev_free(&c->d_tabf);
+  dev_free(&c->d_tab); dev_free(&c->d_tabf); dev_free(&c->d_qc); dev_free(&c->d_qcf);
+  dev_free(&c->d_aff); dev_free(&c->d_afff);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -203,6 +204,7 @@ int32_t fem_mesh_upload(fem_ctx* c, int32_t sdim, int64_t nnodes, const double* 
   c->tables_ready = false;
   c->gather_ready = false;
   c->gchunks_built = 0;
+  c->affine_mesh = -1;
   return FEM_OK;
 }
 
@@ -227,6 +229,7 @@ int32_t fem_dofs_upload(fem_ctx* c, int64_t ndofs, int32_t ndpc, const int64_t* 
   c->tables_ready = false;
   c->gather_ready = false;
   c->gchunks_built = 0;
+  c->affine_mesh = -1;
   return FEM_OK;
 }
 
@@ -269,6 +272,7 @@ int32_t fem_pattern_upload(fem_ctx* c, int64_t n, const int64_t* colptr, const i
   c->tables_ready = false;
   c->gather_ready = false;
   c->gchunks_built = 0;
+  c->affine_mesh = -1;
   return FEM_OK;
 }
 

@@ -5,9 +5,14 @@
 // once instead of read-modify-writing it once per incident cell:
 //
 //   phase 1  k_elem_poisson   one lane per cell (cell-id order): gather the cell's node
-//                             coordinates and k, reinit! + quadrature loop, store the upper
-//                             triangle of Ke and fe as planes Ke[t][cell], fe[a][cell]
-//                             (unit-stride stores across a wave).
+//                             coordinates and k, reinit! + quadrature loop, store the strict
+//                             upper triangle of Ke and fe as planes Ke[t][cell], fe[a][cell]
+//                             (unit-stride stores across a wave).  The diagonal is not
+//                             stored: every row of a Lagrange Poisson Ke sums to zero
+//                             (∇ of the partition of unity Σ_b N_b = 1 vanishes), so
+//                             Ke[b,b] = −Σ_{a≠b} Ke[a,b], which phase 2 forms from the values
+//                             it loads anyway (equal to the directly integrated diagonal to
+//                             rounding).
 //   phase 2  k_gather_cols    one lane per column j of K (= dof j): for every cell that
 //                             holds dof j (ascending cell id) and every local row a, add
 //                             Ke[a, b_j] into the slot of row dofs[a] of column j, kept in
@@ -29,11 +34,19 @@
 
 namespace fem {
 
+// Index of (lo, hi), lo < hi, in the row-major strict upper triangle of an nb x nb matrix.
+constexpr __host__ __device__ int off_idx(int lo, int hi, int nb) {
+  return lo * (2 * nb - lo - 1) / 2 + (hi - lo - 1);
+}
+
 constexpr int kGatherBlock = 128;
 constexpr uint8_t kNoSlot = 255;
 
-template <typename T, int DIM, int NG, int NB, int NQ>
+// GEO: 0 = the uploaded dM tables (any geometry), 1 = trilinear hex (coefficient form of J),
+// 2 = affine hex (every cell of the mesh a parallelepiped; constant-J form)
+template <typename T, int DIM, int NG, int NB, int NQ, int GEO = 0>
 __global__ __launch_bounds__(256) void k_elem_poisson(int64_t ncells, int64_t cbeg, int64_t cend,
+                                                      const T* __restrict__ qc, const T* __restrict__ aff,
                                                       const double* __restrict__ X0,
                                                       const double* __restrict__ X1, const double* __restrict__ X2,
                                                       const int32_t* __restrict__ cnodes,
@@ -42,7 +55,15 @@ __global__ __launch_bounds__(256) void k_elem_poisson(int64_t ncells, int64_t cb
                                                       DevError* __restrict__ err) {
   constexpr int TRI = NB * (NB + 1) / 2;
   using L = TabLayout<DIM, NG, NB, NQ>;
-  __shared__ T stab[L::size];
+  constexpr int NOFF = NB * (NB - 1) / 2;
+  constexpr int NAFF = 6 * NOFF + NB;
+  __shared__ T stab[L::size + 6 * NQ + NAFF];
+  if constexpr (GEO == 1) {
+    for (int i = threadIdx.x; i < 6 * NQ; i += blockDim.x) stab[L::size + i] = qc[i];
+  }
+  if constexpr (GEO == 2) {
+    for (int i = threadIdx.x; i < NAFF; i += blockDim.x) stab[L::size + 6 * NQ + i] = aff[i];
+  }
   stage_table<T, L::size>(stab, tab);
   for (int64_t cell = cbeg + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cell < cend;
        cell += (int64_t)gridDim.x * blockDim.x) {
@@ -66,11 +87,33 @@ __global__ __launch_bounds__(256) void k_elem_poisson(int64_t ncells, int64_t cb
       x[a][1] = (T)X1[nd[a]];
       if constexpr (DIM == 3) x[a][2] = (T)X2[nd[a]];
     }
+    if constexpr (GEO == 2) {
+      // constant-J form, each plane value stored as soon as it is formed (no Ke array live)
+      const T* P = stab + L::size + 6 * NQ;
+      T A[6];
+      const T detJ = affine_metric<T>(x, (T)param[cell], A);
+      if (!(detJ > T(0))) atomicMin(&err->detj_cell, (int)cell);
+#pragma unroll 4
+      for (int t = 0; t < NOFF; ++t) {
+        T acc = A[0] * P[6 * t];
+#pragma unroll
+        for (int m = 1; m < 6; ++m) acc += A[m] * P[6 * t + m];
+        keb[t * ncells + cell] = acc;
+      }
+      const T sd = src * detJ;
+#pragma unroll
+      for (int a = 0; a < NB; ++a) feb[a * ncells + cell] = sd * P[6 * NOFF + a];
+      continue;
+    }
     T ke[TRI], fe[NB];
-    const bool ok = poisson_cell<T, DIM, NG, NB, NQ>(x, (T)param[cell], src, stab, ke, fe);
+    bool ok;
+    if constexpr (GEO == 1) ok = poisson_cell_tri<T, NB, NQ>(x, (T)param[cell], src, stab, stab + L::size, ke, fe);
+    else ok = poisson_cell<T, DIM, NG, NB, NQ>(x, (T)param[cell], src, stab, ke, fe);
     if (!ok) atomicMin(&err->detj_cell, (int)cell);
 #pragma unroll
-    for (int t = 0; t < TRI; ++t) keb[t * ncells + cell] = ke[t];
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int b = a + 1; b < NB; ++b) keb[off_idx(a, b, NB) * ncells + cell] = ke[tri_idx(a, b, NB)];
 #pragma unroll
     for (int a = 0; a < NB; ++a) feb[a * ncells + cell] = fe[a];
   }
@@ -157,11 +200,17 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_cols(
         const bool in = pk[g] >= 0;
         const int64_t cell = in ? pk[g] / NB : 0;
         const int b = in ? pk[g] - (int)cell * NB : 0;
+        T diag = T(0);
 #pragma unroll
         for (int a = 0; a < NB; ++a) {
           const int lo = a < b ? a : b, hi = a < b ? b : a;
-          v[g][a] = in ? keb[(int64_t)tri_idx(lo, hi, NB) * ncells + cell] : T(0);
+          v[g][a] = (in && a != b) ? keb[(int64_t)off_idx(lo, hi, NB) * ncells + cell] : T(0);
         }
+#pragma unroll
+        for (int a = 0; a < NB; ++a) diag -= v[g][a];   // v[g][b] is 0 here; ascending a
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+          if (a == b) v[g][a] = diag;
         fv[g] = in ? feb[(int64_t)b * ncells + cell] : T(0);
       }
 #pragma unroll
@@ -302,6 +351,43 @@ static int build_gather_chunks(fem_ctx* c, int nchunk) {
   return FEM_OK;
 }
 
+// flag = 0 when some cell's trilinear map is not a parallelepiped (FP64, tol 1e-14).
+__global__ __launch_bounds__(256) void k_check_affine(int64_t ncells, const double* __restrict__ X0,
+                                                      const double* __restrict__ X1, const double* __restrict__ X2,
+                                                      const int32_t* __restrict__ cnodes, int* __restrict__ flag) {
+  for (int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cell < ncells;
+       cell += (int64_t)gridDim.x * blockDim.x) {
+    double x[8][3];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int32_t nd = cnodes[cell * 8 + a];
+      x[a][0] = X0[nd];
+      x[a][1] = X1[nd];
+      x[a][2] = X2[nd];
+    }
+    double Y[8][3];
+    trilinear_coeffs<double>(x, Y);
+    if (!is_affine<double>(Y, 1e-14)) *flag = 0;   // every writer stores the same 0
+  }
+}
+
+// Whether every cell is affine (cached until the mesh changes).
+static int check_affine_mesh(fem_ctx* c) {
+  if (c->affine_mesh >= 0) return FEM_OK;
+  int* d = nullptr;
+  int rc = dev_alloc(c, &d, 1);
+  if (rc) return rc;
+  const int one = 1;
+  FEM_HIP(c, hipMemcpyAsync(d, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  k_check_affine<<<gridn(c->ncells), 256, 0, c->stream>>>(c->ncells, c->d_x[0], c->d_x[1], c->d_x[2], c->d_gcnodes, d);
+  int h = 0;
+  FEM_HIP(c, hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  dev_free(&d);
+  c->affine_mesh = h;
+  return FEM_OK;
+}
+
 // True when the gather path handles this assembly (scalar Q1 Poisson with short columns).
 bool gather_supported(const fem_ctx* c) {
   const bool hex = c->rdim == 3 && c->ngeo == 8 && c->nbf == 8 && c->nqp == 8 && c->npc == 8 && c->sdim == 3;
@@ -311,14 +397,14 @@ bool gather_supported(const fem_ctx* c) {
 
 template <typename T, int DIM, int NG, int NB, int NQ>
 static int run_gather(fem_ctx* c, const T* tab, double src, bool fillzero) {
-  constexpr int TRI = NB * (NB + 1) / 2;
+  constexpr int NOFF = NB * (NB - 1) / 2;   // stored planes: the strict upper triangle
   const int64_t nc = c->ncells;
   int rc;
   T*& keb = sizeof(T) == 8 ? reinterpret_cast<T*&>(c->d_keb64) : reinterpret_cast<T*&>(c->d_keb32);
   T*& feb = sizeof(T) == 8 ? reinterpret_cast<T*&>(c->d_feb64) : reinterpret_cast<T*&>(c->d_feb32);
   int64_t& cap = sizeof(T) == 8 ? c->keb64_cells : c->keb32_cells;
   if (!keb || cap != nc) {
-    if ((rc = dev_alloc(c, &keb, nc * TRI)) || (rc = dev_alloc(c, &feb, nc * NB))) return rc;
+    if ((rc = dev_alloc(c, &keb, nc * NOFF)) || (rc = dev_alloc(c, &feb, nc * NB))) return rc;
     cap = nc;
   }
   // default: no overlap.  Measured at 128³ (FP64): 1 chunk 0.661 ms, 2: 0.703, 8: 0.843 —
@@ -345,9 +431,35 @@ static int run_gather(fem_ctx* c, const T* tab, double src, bool fillzero) {
     else if (c->max_row_nnz <= 27) FEM_GATHER_G(27, nblk, st, bl);
     else FEM_GATHER_G(32, nblk, st, bl);
   };
+  const T* qc = sizeof(T) == 8 ? (const T*)c->d_qc : (const T*)c->d_qcf;
+  const T* aff = sizeof(T) == 8 ? (const T*)c->d_aff : (const T*)c->d_afff;
+  int geo = 0;
+  if constexpr (DIM == 3 && NG == 8) {
+    if (c->trilinear) {
+      geo = 1;
+      if (c->tune.affine != -1) {
+        if ((rc = check_affine_mesh(c))) return rc;
+        if (c->affine_mesh == 1) geo = 2;
+      }
+    }
+  }
   auto elem = [&](int64_t c0, int64_t c1, hipStream_t st) {
+    if constexpr (DIM == 3 && NG == 8) {
+      if (geo == 2) {
+        k_elem_poisson<T, DIM, NG, NB, NQ, 2><<<gridn(c1 - c0), 256, 0, st>>>(
+            nc, c0, c1, qc, aff, c->d_x[0], c->d_x[1], c->d_x[2], c->d_gcnodes, c->d_param_cell, tab, (T)src, keb, feb,
+            c->d_err);
+        return;
+      }
+      if (geo == 1) {
+        k_elem_poisson<T, DIM, NG, NB, NQ, 1><<<gridn(c1 - c0), 256, 0, st>>>(
+            nc, c0, c1, qc, aff, c->d_x[0], c->d_x[1], c->d_x[2], c->d_gcnodes, c->d_param_cell, tab, (T)src, keb, feb,
+            c->d_err);
+        return;
+      }
+    }
     k_elem_poisson<T, DIM, NG, NB, NQ><<<gridn(c1 - c0), 256, 0, st>>>(
-        nc, c0, c1, c->d_x[0], c->d_x[1], c->d_x[2], c->d_gcnodes, c->d_param_cell, tab, (T)src, keb, feb,
+        nc, c0, c1, qc, nullptr, c->d_x[0], c->d_x[1], c->d_x[2], c->d_gcnodes, c->d_param_cell, tab, (T)src, keb, feb,
         c->d_err);
   };
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));

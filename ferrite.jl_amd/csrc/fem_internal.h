@@ -117,6 +117,17 @@ struct fem_ctx {
   bool tab_dirty = true;
   double* d_tab = nullptr;             // packed N, dN, dM, w for the element kernels (FP64)
   float* d_tabf = nullptr;             // the same in FP32
+  // trilinear hex geometry (Ferrite's node order, tables checked on the host): per
+  // quadrature point ξ, η, ζ, ηζ, ξζ, ξη for the coefficient form of the Jacobian
+  bool trilinear = false;
+  double* d_qc = nullptr;
+  float* d_qcf = nullptr;
+  // affine (parallelepiped) cells of that geometry: Ke = Σ_m A_m P[t][m] with the reference
+  // integrals P[t][m] = Σ_q w_q (∇N_a,i ∇N_b,j + ∇N_a,j ∇N_b,i)/(1 + δ_ij) for the 6 unique
+  // (i, j), t = (a, b), a < b, row-major, followed by Σ_q w_q N_a (8 values)
+  double* d_aff = nullptr;
+  float* d_afff = nullptr;
+  int affine_mesh = -1;                // every cell a parallelepiped: 1 / 0, −1 = not checked
 
   // vectors
   double* d_f = nullptr;

@@ -1250,7 +1250,7 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
       !(t->cg_elems == 0 || t->cg_elems == 2 || t->cg_elems == 4 || t->cg_elems == 8 || t->cg_elems == 16) ||
       !(t->gather_chunks == 0 || t->gather_chunks == 1 || t->gather_chunks == 2 || t->gather_chunks == 4 ||
         t->gather_chunks == 8 || t->gather_chunks == 16 || t->gather_chunks == 32) ||
-      !(t->sell_nt == 0 || t->sell_nt == 1 || t->sell_nt == -1))
+      !(t->sell_nt == 0 || t->sell_nt == 1 || t->sell_nt == -1) || !(t->affine == 0 || t->affine == -1))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

@@ -170,6 +170,9 @@ typedef struct {
                                         1 (no overlap) / 2 / 4 / 8 / 16 / 32 (0 = default 1) */
   int32_t sell_nt;                   /* sliced-ELL SpMV: non-temporal loads of the operator's values
                                         and offsets (0 = default, 1) or ordinary loads (-1) */
+  int32_t affine;                    /* owner-computes assembly on trilinear hexes: constant-Jacobian
+                                        form for parallelepiped cells (0 = default, on) or the
+                                        quadrature loop for every cell (-1) */
   int32_t reserved;
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);

@@ -161,6 +161,43 @@ def test_assembly_deterministic_and_fillzero(asm):
     assert np.abs(a4 - a1).max() <= 1e-14 * np.abs(a1).max()
 
 
+def test_hex_geometry_paths():
+    """The element kernel's three geometry forms: generic dM tables, trilinear coefficient
+    form (a distorted mesh), and the constant-Jacobian form of affine meshes — each against
+    the oracle's quadrature loop to 1e-12, and affine vs quadrature on one mesh to 1e-14."""
+    print("case hex geometry paths", flush=True)
+    nel = (5, 4, 3)
+    k = poisson_conductivity(nel, seed=21)
+    # distorted: interior nodes moved by up to 15% of the cell size (det J stays positive)
+    g = fa.generate_grid("Hexahedron", nel, (0, 0, 0), (1, 1, 1))
+    X = g.nodes
+    inner = np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1)
+    h = np.array([1.0 / n for n in nel])
+    X[inner] += np.random.default_rng(22).uniform(-0.15, 0.15, size=(int(inner.sum()), 3)) * h
+    ip = fa.Lagrange("RefHexahedron", 1)
+    cv = fa.CellValues(fa.QuadratureRule("RefHexahedron", 2), ip)
+    dh = fa.DofHandler(g).add("u", ip).close()
+    K = fa.allocate_matrix(fa.MI355XCSC, dh)
+    f = fa.allocate_vector(K)
+    colors = fa.create_coloring(g)
+    fa.assemble_global(fa.start_assemble(K, f), cv, colors, fa.PoissonElement(k=k))
+    grid, ocv, odh, OK, of, _ = setup_poisson("Hexahedron", nel, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    grid.nodes[...] = X
+    assemble_poisson(grid, ocv, odh, OK, of, k_cell=k)
+    _check_K(K, f, OK, of, 1e-12)
+    # affine mesh: constant-J form (default) vs the per-point form, and vs the oracle
+    g2, dh2, K2, f2, _ = _device_poisson("Hexahedron", nel, (0, 0, 0), (1, 2, 3), k, ["left"])
+    a_aff, f_aff = K2.nzval(), f2.to_numpy()
+    fa.set_tuning(K2, affine=-1)
+    fa.assemble_global(fa.start_assemble(K2, f2), cv, fa.create_coloring(g2), fa.PoissonElement(k=k))
+    a_q, f_q = K2.nzval(), f2.to_numpy()
+    assert np.abs(a_aff - a_q).max() <= 1e-14 * np.abs(a_q).max()
+    assert np.abs(f_aff - f_q).max() <= 1e-14 * np.abs(f_q).max()
+    grid2, ocv2, odh2, OK2, of2, _ = setup_poisson("Hexahedron", nel, (0.0, 0.0, 0.0), (1.0, 2.0, 3.0))
+    assemble_poisson(grid2, ocv2, odh2, OK2, of2, k_cell=k)
+    _check_K(K2, f2, OK2, of2, 1e-12)
+
+
 def test_pipelined_gather_bitwise():
     """Element kernel of chunk k+1 overlapping the gather of chunk k on a second stream: every
     entry still sums its cells in ascending order, so nzval and f are bitwise those of the
