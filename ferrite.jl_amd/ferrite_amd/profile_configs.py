@@ -12,10 +12,12 @@ elasticity  Q2 hex^3 small-strain elasticity on generate_grid(Hexahedron, (n, n,
             27x27x27 products the kernel performs).
 cantilever  Plane-strain Q1^2 cantilever on (nx, ny) cells over [0,2]x[0,1], E = E_min +
             theta^3 (1 - E_min), theta ~ U[0.2, 1.0] (seed 0), clamped at x = 0, point load
-            (0, -1) at (2, 0.5).  Times J and the adjoint gradients; checks `fd` random entries
-            of dJ/dtheta and of dJ/dX against central differences (h = 1e-6 and 1e-4; FP64 CG
-            attains a relative residual of about 1e-9 on this system, which bounds how well a
-            difference of two J values resolves the small entries).
+            (0, -1) at (2, 0.5).  Times J (device CG) and the adjoint kernel, then checks `fd`
+            entries of dJ/dtheta and dJ/dX (random cells / node coordinates whose supports share
+            no node) against central differences at h = 1e-6 and 1e-4 that carry no solve
+            error: K_ff is factored once on the host and each J(theta +- h) - J(theta) follows
+            from a Woodbury update (exact_central_differences); the adjoint kernel runs on the
+            LU solution.
 batch       `problems` seeded scalar Poisson problems (2D/3D, log-uniform cell counts in
             [2e3, 2e5]) assembled and solved (mixed CG to 1e-8) in one batched context after
             the launch-shape autotuner; total cells/s and p50 / p99 completion times.
@@ -85,9 +87,168 @@ def elasticity(args):
     return out
 
 
+def _disjoint_picks(rng, candidates, node_sets, want):
+    """Up to `want` of `candidates` (in the random order of rng) whose node sets are pairwise disjoint."""
+    used, out = set(), []
+    for k in rng.permutation(candidates):
+        s = node_sets(int(k))
+        if used.isdisjoint(s):
+            used.update(s)
+            out.append(int(k))
+            if len(out) == want:
+                break
+    return out
+
+
+def pick_fd_entries(g, rng, n_cells, n_coords):
+    """Random cells, and random (node, direction) coordinates, for exact_central_differences:
+    no two chosen cells share a node, and no two chosen nodes' cell patches share a node."""
+    import numpy as np
+    cells0 = g.cells - 1
+    node_cells = {}
+    for e, row in enumerate(cells0):
+        for nd in row:
+            node_cells.setdefault(int(nd), []).append(e)
+    cells = _disjoint_picks(rng, np.arange(g.ncells), lambda e: set(cells0[e].tolist()), n_cells)
+    nodes = _disjoint_picks(rng, np.arange(g.nnodes), lambda nd: set(cells0[node_cells[nd]].ravel().tolist()),
+                            n_coords)
+    return cells, [(nd, int(rng.integers(2))) for nd in nodes]
+
+
+def exact_central_differences(prob, theta, cells, coords, hs, chunk=64):
+    """Central differences of J(θ, X) = fᵀu, K u = f, that carry no solve error.
+
+    The free-dof block K_ff of the assembled, constrained system is factored once (sparse LU
+    on the host, two steps of iterative refinement).  Changing θ_e or one node coordinate
+    changes K_ff only on the free dofs U of the cells involved, K_ff + U M Uᵀ, and by the
+    Woodbury identity
+        J(M) − J = −vᵀ M (I + G M)⁻¹ v,   v = Uᵀ u,   G = Uᵀ K_ff⁻¹ U,
+    which is formed directly instead of as the difference of two J values.  M comes from device
+    assemblies: Ke⁰ of the chosen cells from one assembly with E = their indicator, and ΔK of a
+    node coordinate as K(X ± h) − K(X) from assemblies with every chosen coordinate moved at
+    once.  The chosen cells, and the cell patches of the chosen nodes, must share no node (so the
+    blocks do not overlap); `coords` holds (node index, direction) pairs, 0-based.
+
+    Returns u (full vector, the LU solution), the refinement residual ‖f − K u‖/‖f‖ and
+    {h: (fd_theta, fd_x)}.  Leaves the device mesh at X and K, f assembled at θ."""
+    import numpy as np
+    import scipy.sparse.linalg as sla
+    from . import device as D
+    from .autodiff import _mesh_args
+    K, f, ch = prob.K, prob.f, prob.ch
+    g, dh = K.dh.grid, K.dh
+    n = dh.ndofs
+    E = D.simp_modulus(theta, prob.emin, prob.penal)
+
+    def assemble(Evals, X=None, constrain=False):
+        if X is not None:
+            g.nodes[...] = X
+            K.ctx.call("fem_mesh_upload", *_mesh_args(K))
+            K._colors = None
+        D.assemble_global(D.start_assemble(K, f), prob.cv, prob.colors, D.ElasticityElement(E=Evals, nu=prob.nu))
+        if constrain:
+            prob.loads(K, f)
+            D.apply(K, f, ch)
+        return K.to_scipy()
+
+    X0 = g.nodes.copy()
+    free = np.ones(n, dtype=bool)
+    free[np.asarray(ch.prescribed_dofs, dtype=np.int64) - 1] = False
+    fidx = np.flatnonzero(free)
+    pos = np.full(n, -1, dtype=np.int64)
+    pos[fidx] = np.arange(len(fidx))
+
+    # blocks first (they overwrite K and f), the constrained system last
+    ind = np.zeros(g.ncells)
+    ind[cells] = 1.0
+    K0 = assemble(ind)
+    Kx = {}
+    for h in hs:
+        for sgn in (1, -1):
+            Xs = X0.copy()
+            for nd, d in coords:
+                Xs[nd, d] += sgn * h
+            Kx[(h, sgn)] = assemble(E, Xs)
+    Kraw = assemble(E, X0)
+
+    cell_dofs = dh.cell_dofs - 1
+    node_cells = {}
+    for c_, row in enumerate(g.cells - 1):
+        for nd in row:
+            node_cells.setdefault(int(nd), []).append(c_)
+    Ut = [np.unique(cell_dofs[e]) for e in cells]
+    Ux = [np.unique(cell_dofs[node_cells[nd]]) for nd, _ in coords]
+    # a block whose dofs are all free is annihilated by rigid translations (per component)
+    whole_t = [bool(free[u_].all()) for u_ in Ut]
+    whole_x = [bool(free[u_].all()) for u_ in Ux]
+    Ut = [u_[free[u_]] for u_ in Ut]
+    Ux = [u_[free[u_]] for u_ in Ux]
+    comp = np.zeros(n, dtype=np.int64)
+    for d in range(g.sdim):
+        comp[cell_dofs[:, d::g.sdim]] = d
+
+    A = assemble(E, None, constrain=True)
+    Aff = A[fidx][:, fidx].tocsc()
+    fvec = D.DeviceVector(K, 1).to_numpy()
+    ff = fvec[fidx]
+    t0 = time.perf_counter()
+    lu = sla.splu(Aff, permc_spec="MMD_AT_PLUS_A", options=dict(SymmetricMode=True))
+    log(f"sparse LU of K_ff ({len(fidx)} dofs): {time.perf_counter() - t0:.1f}s, fill {lu.L.nnz + lu.U.nnz}")
+    uf = lu.solve(ff)
+    for _ in range(2):
+        uf += lu.solve(ff - Aff @ uf)
+    refine_res = float(np.linalg.norm(ff - Aff @ uf) / np.linalg.norm(ff))
+    u = np.zeros(n)
+    u[fidx] = uf
+
+    # G = Uᵀ K_ff⁻¹ U for every block, from one set of unit right-hand sides
+    allU = np.unique(np.concatenate(Ut + Ux))
+    col = {int(d): i for i, d in enumerate(allU)}
+    Z = np.empty((len(fidx), len(allU)))
+    t0 = time.perf_counter()
+    for s in range(0, len(allU), chunk):
+        B = np.zeros((len(fidx), min(chunk, len(allU) - s)))
+        B[pos[allU[s:s + B.shape[1]]], np.arange(B.shape[1])] = 1.0
+        Z[:, s:s + B.shape[1]] = lu.solve(B)
+        log(f"K_ff^-1 columns {s + B.shape[1]}/{len(allU)}: {time.perf_counter() - t0:.1f}s")
+
+    def dJ(U, M, shift):
+        # With t a translation of the block (M t = 0, so (I + G M)⁻¹ t = t), vᵀM(I + GM)⁻¹v equals
+        # wᵀM(I + GM)⁻¹w for w = v − t.  Taking t = the per-component mean of v removes the bulk
+        # displacement, which would otherwise multiply the rounding of M (entries ~E, ~eps E
+        # error, not translation-free) by |v|².
+        p = pos[U]
+        G = Z[p][:, [col[int(d)] for d in U]]
+        v = u[U].copy()
+        if shift:
+            for d in range(g.sdim):
+                m = comp[U] == d
+                v[m] -= v[m].mean()
+        return -float(v @ M @ np.linalg.solve(np.eye(len(U)) + G @ M, v))
+
+    def dense(S, U):
+        return S[:, U][U].toarray()
+
+    out = {}
+    for h in hs:
+        fdt = []
+        for e, U, sh in zip(cells, Ut, whole_t):
+            K0e = dense(K0, U)
+            Ep = D.simp_modulus(theta[e] + h, prob.emin, prob.penal) - E[e]
+            Em = D.simp_modulus(theta[e] - h, prob.emin, prob.penal) - E[e]
+            fdt.append((dJ(U, Ep * K0e, sh) - dJ(U, Em * K0e, sh)) / (2 * h))
+        fdx = []
+        for U, sh in zip(Ux, whole_x):
+            base = dense(Kraw, U)
+            fdx.append((dJ(U, dense(Kx[(h, 1)], U) - base, sh) - dJ(U, dense(Kx[(h, -1)], U) - base, sh)) / (2 * h))
+        out[h] = (np.array(fdt), np.array(fdx))
+    return u, refine_res, out
+
+
 def cantilever(args):
     import numpy as np
     import ferrite_amd as fa
+    from ferrite_amd import _lib as L
     from ferrite_amd.autodiff import ComplianceProblem
     nx, ny = args.nx, args.ny
     t0 = time.perf_counter()
@@ -111,50 +272,51 @@ def cantilever(args):
         fa.add_point_load(K_, f_, [ydof], [-1.0])
     prob = ComplianceProblem(K, f, cv, colors, ch, loads, rtol=args.rtol)
     log(f"setup {time.perf_counter() - t0:.1f}s: {g.ncells} cells, {dh.ndofs} dofs, tip node {tip}")
-    X0 = g.nodes.copy()
     t1 = time.perf_counter()
-    J, dth, dx, res = prob.compliance(theta)
+    J, dth_cg, dx_cg, res = prob.compliance(theta)
     tj = time.perf_counter() - t1
     t1 = time.perf_counter()
     fa.compliance_gradient(K, theta)
     tg = time.perf_counter() - t1
     log(f"J = {J:.10e}: forward {tj:.2f}s ({res.iterations} CG its), adjoint kernel {tg * 1e3:.2f} ms")
-    pick = np.random.default_rng(args.seed + 1)
-    cells = pick.choice(g.ncells, args.fd, replace=False)
-    coords = pick.choice(g.nnodes * 2, args.fd, replace=False)
-    fd_rows = {}
-    for h in (1e-6, 1e-4):
-        rows_t, rows_x = [], []
-        for e in cells:
-            tp, tm = theta.copy(), theta.copy()
-            tp[e] += h
-            tm[e] -= h
-            fd = (prob.compliance(tp)[0] - prob.compliance(tm)[0]) / (2 * h)
-            rows_t.append((fd, dth[e]))
-        for k in coords:
-            nd, d = divmod(int(k), 2)
-            Xp, Xm = X0.copy(), X0.copy()
-            Xp[nd, d] += h
-            Xm[nd, d] -= h
-            fd = (prob.compliance(theta, Xp)[0] - prob.compliance(theta, Xm)[0]) / (2 * h)
-            rows_x.append((fd, dx[nd, d]))
-        fd_rows[h] = (rows_t, rows_x)
-        log(f"h = {h:g}: {len(rows_t)} + {len(rows_x)} central differences done")
-    prob.compliance(theta, X0)
 
-    def stats(rows):
-        fd = np.array([r[0] for r in rows])
-        ad = np.array([r[1] for r in rows])
-        rel = np.abs(fd - ad) / np.maximum(np.abs(ad), 1e-300)
-        return {"entries": len(rows), "normwise_rel_err": float(np.linalg.norm(fd - ad) / np.linalg.norm(ad)),
+    cells, coords = pick_fd_entries(g, np.random.default_rng(args.seed + 1), args.fd // 2, args.fd - args.fd // 2)
+    hs = (1e-6, 2e-6, 1e-4)
+    t1 = time.perf_counter()
+    u, refine_res, fd = exact_central_differences(prob, theta, cells, coords, hs)
+    tfd = time.perf_counter() - t1
+    K.ctx.call("fem_solution_upload", L.ptr(u, L.C.c_double))
+    J_lu, dth, dx = fa.compliance_gradient(K, theta)
+    ad_t = dth[cells]
+    ad_x = np.array([dx[nd, d] for nd, d in coords])
+
+    def stats(fdv, ad):
+        rel = np.abs(fdv - ad) / np.abs(ad)
+        return {"entries": len(ad), "normwise_rel_err": float(np.linalg.norm(fdv - ad) / np.linalg.norm(ad)),
                 "max_entry_rel_err": float(rel.max()), "median_entry_rel_err": float(np.median(rel)),
                 "entries_above_1e-6": int(np.sum(rel > 1e-6)),
-                "entries_above_1e-6_and_floor": int(np.sum(np.abs(fd - ad) > 1e-6 * np.abs(ad) + 1e-7 * abs(J)))}
+                "min_abs_entry": float(np.abs(ad).min()), "max_abs_entry": float(np.abs(ad).max())}
     out = {"workload": f"plane-strain Q1^2 cantilever {nx}x{ny}, SIMP p=3, E_min 1e-3, point load at node {tip}",
            "cells": g.ncells, "dofs": dh.ndofs, "J": J, "forward_s": tj, "cg_iterations": res.iterations,
-           "cg_rtol": args.rtol, "relres": res.relres, "adjoint_kernel_ms": tg * 1e3}
-    for h, (rows_t, rows_x) in fd_rows.items():
-        out[f"fd_h{h:g}"] = {"dJdtheta": stats(rows_t), "dJdX": stats(rows_x)}
+           "cg_rtol": args.rtol, "relres": res.relres, "adjoint_kernel_ms": tg * 1e3,
+           "fd_method": "exact central differences: one sparse LU of K_ff (host), Woodbury update per "
+                        "entry with M from device assemblies; adjoint kernel evaluated on the LU solution",
+           "fd_entries": {"cells": [int(e) for e in cells], "coords": [[int(nd), int(d)] for nd, d in coords]},
+           "lu_refined_relres": refine_res, "fd_wall_s": tfd, "J_lu": J_lu,
+           "J_rel_diff_cg_vs_lu": abs(J - J_lu) / abs(J_lu),
+           "grad_rel_diff_cg_vs_lu": {
+               "dJdtheta": float(np.linalg.norm(dth_cg - dth) / np.linalg.norm(dth)),
+               "dJdX": float(np.linalg.norm(dx_cg - dx) / np.linalg.norm(dx))}}
+    for h in hs:
+        fdt, fdx = fd[h]
+        out[f"fd_h{h:g}"] = {"dJdtheta": stats(fdt, ad_t), "dJdX": stats(fdx, ad_x)}
+        log(f"h = {h:g}: dJ/dtheta max rel {out[f'fd_h{h:g}']['dJdtheta']['max_entry_rel_err']:.2e}, "
+            f"dJ/dX max rel {out[f'fd_h{h:g}']['dJdX']['max_entry_rel_err']:.2e}")
+    (t1_, x1), (t2_, x2) = fd[1e-6], fd[2e-6]
+    out["richardson_h1e-6"] = {"dJdtheta": stats((4 * t1_ - t2_) / 3, ad_t), "dJdX": stats((4 * x1 - x2) / 3, ad_x)}
+    trunc = (x2 - x1) / 3
+    out["dJdX_h1e-6_error_vs_truncation_estimate"] = float(np.max(np.abs((x1 - ad_x) - trunc) / np.abs(trunc)))
+    log(f"Richardson h = 1e-6: dJ/dX max rel {out['richardson_h1e-6']['dJdX']['max_entry_rel_err']:.2e}")
     return out
 
 

@@ -162,6 +162,36 @@ def test_compliance_adjoint_vs_fd_and_torch():
     assert np.all(np.abs(dx - gx) <= 1e-6 * np.abs(gx) + 1e-8 * np.abs(gx).max())
 
 
+def test_compliance_adjoint_vs_exact_central_differences():
+    """dJ/dθ and dJ/dX against central differences at h = 1e-6 that carry no solve noise: each
+    J(· ± h) − J is a Woodbury update of one factored K_ff, and the adjoint kernel runs on the
+    same LU solution.  Every entry to 1e-6 relative; dJ/dX entries that are small through
+    cancellation between the four cells around the node (here one entry of 3e-4 beside entries
+    of 1-10) keep an absolute floor of 1e-9 max|g|, the rounding of the assembled K(X ± h)
+    (~eps E per entry) times the patch displacement squared, divided by 2h."""
+    print("case compliance adjoint exact differences 96x48", flush=True)
+    from ferrite_amd import _lib as L
+    from ferrite_amd.profile_configs import exact_central_differences, pick_fd_entries
+    g, dh, K, f, ch, prob, theta, ydof = _cantilever(96, 48, seed=4)
+    cells, coords = pick_fd_entries(g, np.random.default_rng(5), 16, 16)
+    assert len(cells) == 16 and len(coords) == 16
+    h = 1e-6
+    u, refine_res, fd = exact_central_differences(prob, theta, cells, coords, (h, 2 * h))
+    assert refine_res < 1e-10          # ‖f − K u‖/‖f‖ of the LU solution after refinement
+    K.ctx.call("fem_solution_upload", L.ptr(u, L.C.c_double))
+    J, dth, dx = fa.compliance_gradient(K, theta)
+    assert abs(J - (-u[ydof - 1])) <= 1e-12 * abs(J)
+    adt = dth[cells]
+    adx = np.array([dx[nd, d] for nd, d in coords])
+    (t1, x1), (t2, x2) = fd[h], fd[2 * h]
+    print("dJ/dtheta rel err", np.abs(t1 - adt) / np.abs(adt), flush=True)
+    print("dJ/dX rel err h", np.abs(x1 - adx) / np.abs(adx), flush=True)
+    print("dJ/dX rel err 2h", np.abs(x2 - adx) / np.abs(adx), flush=True)
+    assert np.all(np.abs(t1 - adt) <= 1e-6 * np.abs(adt)), (t1, adt)
+    assert np.all(np.abs(x1 - adx) <= 1e-6 * np.abs(adx) + 1e-9 * np.abs(adx).max()), (x1, adx)
+    assert _rel(x1, adx) <= 1e-8 and _rel(t1, adt) <= 1e-9
+
+
 def test_torch_autograd_function():
     print("case torch autograd function", flush=True)
     import torch
