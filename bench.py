@@ -121,7 +121,7 @@ def main():
         _, _, ts = step(mode)
         fa.set_tuning(K, time_kernels=0)
         spmv_ms, spmv_n = ts["spmv_ms"], ts["spmv_launches"]
-        results[name] = dict(ms=dt * 1e3, cells_per_s=g.ncells / dt, iters=iters, u=u,
+        results[name] = dict(ms=dt * 1e3, cells_per_s=g.ncells / dt, iters=iters, u=u, operator=res.operator,
                              asm_kernel_ms=float(np.median(asm_ms)), spmv_avg_ms=spmv_ms / max(spmv_n, 1),
                              spmv_launches=spmv_n)
         log(f"{name}: {dt * 1e3:.1f} ms/step, iters {iters[-1]}, assembly kernels {np.median(asm_ms):.3f} ms, "
@@ -143,13 +143,22 @@ def main():
     fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision="fp64")
     fa.apply(K, f, ch)
 
-    # roofline of the dominant kernel: the FP32 SpMV of the mixed solver (algorithmic bytes of
-    # y = K x with int32 indices, whatever the stored format: values + indices + colptr + x + y)
+    # roofline of the dominant kernel: the FP32 SpMV of the mixed solver.  Lattice operator:
+    # algorithmic bytes of its format (14 FP32 value planes over the computed rows, x over the
+    # padded lattice, y over the computed rows; the backward neighbours re-read planes another
+    # row already read).  Sliced ELL: y = K x with int32 indices (values + indices + colptr + x + y).
     spmv_iso_ms = fa.spmv_bench(K, mixed=True, reps=50)   # back to back, matrix partly cache-resident
     spmv_ms = results["mixed"]["spmv_avg_ms"]               # inside the CG loop of the timed steps
-    SPMV32_KERNEL = "k_sell<float, float, 0, true, 8, false, true>"   # sliced ELL, u16 offsets, NT loads
     nnz, nd = K.nnz, dh.ndofs
-    spmv_bytes = nnz * (4 + 4) + 4 * (nd + 1) + 4 * nd + 4 * nd   # values, int32 indices, colptr, x, y
+    if results["mixed"]["operator"] == "lattice":
+        SPMV32_KERNEL = "k_lat_spmv<float, float, 14, 1>"   # lattice planes, XCD-range rows
+        px = n + 3                                           # n + 1 nodes per axis, one padding node each side
+        npad, omax = px ** 3, 1 + px + px * px
+        nrows = npad - 2 * omax
+        spmv_bytes = nrows * (14 * 4 + 4) + 4 * npad
+    else:
+        SPMV32_KERNEL = "k_sell<float, float, 0, true, 8, false, true>"   # sliced ELL, u16 offsets, NT loads
+        spmv_bytes = nnz * (4 + 4) + 4 * (nd + 1) + 4 * nd + 4 * nd
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     spmv64_iso_ms = fa.spmv_bench(K, mixed=False, reps=50)
     spmv64_ms = results["fp64"]["spmv_avg_ms"]
@@ -179,8 +188,10 @@ def main():
                    "cells": g.ncells, "dofs": dh.ndofs, "nnz": nnz, "colors": len(colors),
                    "coloring": args.coloring, "seed": args.seed},
         "fp64": {"cells_per_s": results["fp64"]["cells_per_s"], "ms_per_step": results["fp64"]["ms"],
+                 "operator": results["fp64"]["operator"],
                  "cg_iterations": results["fp64"]["iters"][-1][0], "relres": results["fp64"]["iters"][-1][2]},
         "mixed": {"cells_per_s": results["mixed"]["cells_per_s"], "ms_per_step": results["mixed"]["ms"],
+                  "operator": results["mixed"]["operator"],
                   "cg_iterations": results["mixed"]["iters"][-1][0],
                   "refinements": results["mixed"]["iters"][-1][1], "relres": results["mixed"]["iters"][-1][2]},
         "mixed_speedup_vs_fp64": results["fp64"]["ms"] / results["mixed"]["ms"],

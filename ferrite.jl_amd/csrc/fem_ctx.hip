@@ -345,6 +345,15 @@ int32_t fem_matrix_download(fem_ctx* c, double* nzval, double* f) {
   return FEM_OK;
 }
 
+int32_t fem_matrix_upload(fem_ctx* c, const double* nzval, const double* f) {
+  if (!c || c->nnz == 0) return set_error(c, FEM_ERR_STATE, "no matrix");
+  if (nzval) FEM_HIP(c, hipMemcpyAsync(c->d_nzval, nzval, sizeof(double) * c->nnz, hipMemcpyHostToDevice, c->stream));
+  if (f) FEM_HIP(c, hipMemcpyAsync(c->d_f, f, sizeof(double) * c->ndofs, hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  if (nzval) ++c->nzval_version;
+  return FEM_OK;
+}
+
 int32_t fem_solution_download(fem_ctx* c, double* u) {
   if (!c || !u || c->ndofs == 0) return set_error(c, FEM_ERR_STATE, "no solution");
   FEM_HIP(c, hipMemcpyAsync(u, c->d_u, sizeof(double) * c->ndofs, hipMemcpyDeviceToHost, c->stream));
@@ -362,7 +371,11 @@ int32_t fem_solution_upload(fem_ctx* c, const double* u) {
 int32_t fem_device_pointer(fem_ctx* c, int32_t which, void** ptr, int64_t* nelem) {
   if (!c || !ptr) return FEM_ERR_ARG;
   switch (which) {
-    case 0: *ptr = c->d_nzval; if (nelem) *nelem = c->nnz; break;
+    case 0:   // writable: the solver's operator copies are rebuilt from nzval at the next solve
+      *ptr = c->d_nzval;
+      if (nelem) *nelem = c->nnz;
+      ++c->nzval_version;
+      break;
     case 1: *ptr = c->d_f; if (nelem) *nelem = c->ndofs; break;
     case 2: *ptr = c->d_u; if (nelem) *nelem = c->ndofs; break;
     default: return set_error(c, FEM_ERR_ARG, "unknown array");

@@ -194,4 +194,13 @@ int gather_poisson(fem_ctx* c, bool mixed, double src, bool fillzero, const doub
 
 constexpr int kReduceBlocks = 1024;   // fixed partial count -> deterministic reductions
 
+// Tensor-product numbering of the dofs of a scalar Q1 hex (quad) field, from the cell
+// topology (fem_lattice.hip).
+struct Lattice {
+  int dim = 0;                 // 3 or 2 when found
+  int64_t nx = 0, ny = 0, nz = 1;
+  std::vector<int32_t> pos;    // [n] ix + nx (iy + ny iz) of each dof
+};
+bool find_lattice(const fem_ctx* c, Lattice* L);
+
 }  // namespace fem

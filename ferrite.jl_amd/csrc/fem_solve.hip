@@ -81,9 +81,44 @@ struct SellOp {
   bool built = false;
 };
 
+// Lattice copy of Â, used in place of the sliced ELL when the dofs sit on a tensor-product
+// lattice (find_lattice) and every pattern entry couples lattice neighbours.  The lattice is
+// padded by one node on each side of every used axis: padded row P = (ix + 1) +
+// px ((iy + 1) + py (iz + 1)).  Neighbour codes are (dx + 1) + 3 (dy + 1) + 9 (dz + 1)
+// (centre 13); plane s holds Â(P, P + o_s) for the centre (s = 0) and the forward neighbours
+// (code 13 + s, s = 1 .. ns − 1: 13 in 3D, 4 in 2D).  Row P's backward neighbour P − o_s is
+// read from plane s at row P − o_s (Â is symmetric, which the value fill checks), so the
+// operator is half the values and no column indices.  Padding rows hold zero values, so
+// the solver's vectors (in padded numbering) stay zero there.
+struct LatOffsets {
+  int32_t o[14];
+};
+
+struct LatOp {
+  int state = 0;               // 0 not tried, 1 usable, −1 not a lattice
+  int ns = 0;
+  int64_t px = 0, py = 0, pxy = 0, npad = 0;
+  int64_t r_lo = 0, nrows = 0; // rows the SpMV computes
+  LatOffsets off{};
+  int32_t* map = nullptr;      // [n] padded row of each dof
+  int32_t* dofp = nullptr;     // [npad] dof of each padded row, −1 on padding
+  float* val32 = nullptr;
+  double* val64 = nullptr;
+  float* mir32 = nullptr;      // [ns − 1][npad] mirror planes of the symmetry check
+  double* mir64 = nullptr;
+  uint64_t ver32 = 0, ver64 = 0;   // nzval_version the planes were filled from (+1)
+  int jac32 = -1, jac64 = -1;
+  bool sym32 = false, sym64 = false;   // the last fill found Â symmetric
+  unsigned long long* cnt = nullptr;   // [4] fill counters
+  int grid = 0;
+};
+
 struct SolverWork {
   int64_t n = 0;
   SellOp sell;
+  LatOp lat;
+  bool use_lat = false;        // the current solve runs on the lattice operator
+  int64_t cap32 = 0, cap64 = 0;   // length of the recurrence vectors
   int sell_grid = 0;           // blocks of the SELL SpMV (fixed: its partial count)
   SpmvShape sh64;              // CSR-stream FP64 residual
   // recurrence vectors of the scaled system: x̂, r̂, w = Â r̂, p, s = Â p (FP64 solver / FP32
@@ -118,6 +153,9 @@ void free_solver(fem_ctx* c) {
   SellOp& o = w->sell;
   dev_free(&o.off); dev_free(&o.base); dev_free(&o.roff); dev_free(&o.col); dev_free(&o.src);
   dev_free(&o.val32); dev_free(&o.val64);
+  LatOp& l = w->lat;
+  dev_free(&l.map); dev_free(&l.dofp); dev_free(&l.val32); dev_free(&l.val64); dev_free(&l.cnt);
+  dev_free(&l.mir32); dev_free(&l.mir64);
   delete w;
   c->solver = nullptr;
 }
@@ -461,6 +499,151 @@ __global__ __launch_bounds__(kBlock) void k_sell_vals(int64_t n, int64_t nslices
   }
 }
 
+// --------------------------------------------------------------------------
+// Lattice operator (LatOp): plane fill, symmetry check and SpMV.
+// --------------------------------------------------------------------------
+struct LatGeom {
+  uint32_t px, pxy, ofs;       // ofs = 1 + px + pxy: shifts every neighbour offset to [0, 2 ofs]
+  int64_t npad;
+};
+
+// Neighbour code (dx + 1) + 3 (dy + 1) + 9 (dz + 1) of the padded row at offset d, or −1 when
+// it is not a lattice neighbour.  d + ofs read as the digits (dz + 1, dy + 1, dx + 1) in the
+// mixed radix (pxy, px, 1); as every real node lies at least one padding node away from the
+// lattice edge, digits ≤ 2 identify a neighbour uniquely.  In 2D (pz = 1) dz + 1 is always 1.
+__device__ __forceinline__ int lat_code(int64_t d, const LatGeom& g) {
+  const int64_t t = d + g.ofs;
+  if (t < 0 || t > 2 * (int64_t)g.ofs) return -1;
+  const uint32_t u = (uint32_t)t;
+  const uint32_t cz = u / g.pxy, rz = u - cz * g.pxy;
+  const uint32_t cy = rz / g.px, cx = rz - cy * g.px;
+  if (cz > 2 || cy > 2 || cx > 2) return -1;
+  return (int)(cx + 3 * cy + 9 * cz);
+}
+
+// Adds block totals of two counters to cnt[i0], cnt[i0 + 1].
+__device__ __forceinline__ void count2(double a, double b, unsigned long long* cnt, int i0, double* sh) {
+  a = block_sum(a, sh);
+  if (threadIdx.x == 0 && a != 0.0) atomicAdd(cnt + i0, (unsigned long long)a);
+  b = block_sum(b, sh);
+  if (threadIdx.x == 0 && b != 0.0) atomicAdd(cnt + i0 + 1, (unsigned long long)b);
+}
+
+// Plane fill, kLatLanes lanes per padded row P (dofp[P] = its dof j, −1 on padding), one
+// entry of column j of the CSC arrays (= row j of the symmetric K) per lane, v = K[i, j] s_j s_i
+// (the product order of the sliced copy):
+//   i = j or a forward neighbour (code ≥ 13): val[code − 13][P] = v;
+//   i a backward neighbour:                     mir[12 − code][P(i)] = v, i.e. where row P(i)
+//                                               stores K[j, i] in val[13 − code][P(i)].
+// Â is symmetric iff planes 1.. of val equal mir (k_lat_cmp).  cnt[1] counts entries that
+// are not lattice neighbours.
+constexpr int kLatLanes = 32;
+
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_lat_fill(int64_t r_lo, int64_t nrows, const int32_t* __restrict__ dofp,
+                                                     const int32_t* __restrict__ colptr,
+                                                     const int32_t* __restrict__ rowval,
+                                                     const double* __restrict__ nzval, const double* __restrict__ sc,
+                                                     const int32_t* __restrict__ map, LatGeom g, V* __restrict__ val,
+                                                     V* __restrict__ mir, unsigned long long* __restrict__ cnt) {
+  __shared__ double sh[kBlock / 64];
+  const int lane = threadIdx.x % kLatLanes;
+  const int64_t groups = (int64_t)gridDim.x * (kBlock / kLatLanes);
+  double bad = 0.0;
+  for (int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kLatLanes; q < nrows; q += groups) {
+    const int64_t P = r_lo + q;
+    const int32_t j = dofp[P];
+    if (j < 0) continue;
+    const double sr = sc[j];
+    const int32_t p1 = colptr[j + 1];
+    for (int32_t p = colptr[j] + lane; p < p1; p += kLatLanes) {
+      const int32_t i = rowval[p];
+      const int64_t Pi = map[i];
+      const int code = lat_code(Pi - P, g);
+      if (code < 0) {
+        bad += 1.0;
+        continue;
+      }
+      const V v = (V)(nzval[p] * sr * sc[i]);
+      if (code >= 13) val[(int64_t)(code - 13) * g.npad + P] = v;
+      else mir[(int64_t)(12 - code) * g.npad + Pi] = v;
+    }
+  }
+  count2(0.0, bad, cnt, 0, sh);
+}
+
+// Symmetry check: a[k] vs b[k] for k < m (the forward planes against their mirrors) to tol
+// relative (the operator's precision); cnt[3] counts mismatches.
+template <typename V>
+__global__ __launch_bounds__(kBlock) void k_lat_cmp(int64_t m, const V* __restrict__ a, const V* __restrict__ b,
+                                                    double tol, unsigned long long* __restrict__ cnt) {
+  __shared__ double sh[kBlock / 64];
+  double bad = 0.0;
+  for (int64_t k = blockIdx.x * (int64_t)kBlock + threadIdx.x; k < m; k += (int64_t)gridDim.x * kBlock) {
+    const double x = (double)a[k], y = (double)b[k];
+    if (!(fabs(x - y) <= tol * fmax(fabs(x), fabs(y)))) bad += 1.0;
+  }
+  count2(0.0, bad, cnt, 2, sh);
+}
+
+// Element at a 32-bit byte offset from a wave-uniform base (the global-load form with a
+// scalar base and a 32-bit vector offset: no 64-bit address per stream in VGPRs).
+template <typename T>
+__device__ __forceinline__ T at_byte(const T* __restrict__ base, uint32_t boff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+
+// w = Â x on the lattice planes, rows [r_lo, r_lo + nrows) of the padded numbering; partial
+// x·w per block.  All 2 ns − 1 values and x entries of a row are loaded before the sum (fixed
+// order).  Byte offsets are 32-bit (the planes of one operator span < 4 GiB: checked by the
+// host).  MAP 0: grid-stride over all rows; MAP 1 (gridDim.x a multiple of 8): the blocks of
+// one XCD (b ≡ x mod 8) sweep one contiguous eighth of the rows in order, so the rows a
+// backward neighbour reads were loaded by the same XCD shortly before.
+template <typename V, typename X, int NS, int MAP>
+__global__ __launch_bounds__(kBlock) void k_lat_spmv(int64_t r_lo, int64_t nrows, int64_t npad, LatOffsets lo,
+                                                     const V* __restrict__ val, const X* __restrict__ x,
+                                                     X* __restrict__ y, double* __restrict__ part,
+                                                     const Scal* __restrict__ s) {
+  using A = typename std::conditional<sizeof(X) == 8, double, float>::type;
+  if (s->done || s->fresh) return;
+  __shared__ double sh[kBlock / 64];
+  int64_t i0, iend, step;
+  if (MAP == 1) {
+    const int64_t per = gridDim.x / 8, xcd = blockIdx.x % 8, k = blockIdx.x / 8;
+    i0 = xcd * nrows / 8 + k * kBlock + threadIdx.x;
+    iend = (xcd + 1) * nrows / 8;
+    step = per * kBlock;
+  } else {
+    i0 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    iend = nrows;
+    step = (int64_t)gridDim.x * kBlock;
+  }
+  double d = 0.0;
+  for (int64_t i = i0; i < iend; i += step) {
+    const uint32_t r = (uint32_t)(r_lo + i);
+    V v[2 * NS - 1];
+    X xv[2 * NS - 1];
+    v[0] = at_byte(val, r * (uint32_t)sizeof(V));
+    xv[0] = at_byte(x, r * (uint32_t)sizeof(X));
+#pragma unroll
+    for (int k = 1; k < NS; ++k) {
+      const uint32_t o = (uint32_t)lo.o[k];
+      const V* pl = val + k * npad;
+      v[2 * k - 1] = at_byte(pl, r * (uint32_t)sizeof(V));
+      v[2 * k] = at_byte(pl, (r - o) * (uint32_t)sizeof(V));
+      xv[2 * k - 1] = at_byte(x, (r + o) * (uint32_t)sizeof(X));
+      xv[2 * k] = at_byte(x, (r - o) * (uint32_t)sizeof(X));
+    }
+    A acc = (A)v[0] * (A)xv[0];
+#pragma unroll
+    for (int k = 1; k < 2 * NS - 1; ++k) acc += (A)v[k] * (A)xv[k];
+    y[r] = (X)acc;
+    d += (double)xv[0] * (double)(X)acc;
+  }
+  const double b = block_sum(d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
 // s->rr (which = 0) or s->bnorm2 (which = 1) = Σ part, fixed order
 __global__ __launch_bounds__(kWide) void k_sum_to(int m, const double* __restrict__ part, Scal* __restrict__ s,
                                                   int which) {
@@ -496,11 +679,14 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
 }
 
 // Hand-over, step 1: fold the recurrence's iterate into the FP64 u: u += S x̂, x̂ = 0.
+// `map` (lattice operator) gives the recurrence's index of each dof; null: the same index.
 template <typename X>
-__global__ void k_fold(int64_t n, X* __restrict__ x, const double* __restrict__ sc, double* __restrict__ u) {
+__global__ void k_fold(int64_t n, X* __restrict__ x, const double* __restrict__ sc, double* __restrict__ u,
+                       const int32_t* __restrict__ map) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    u[i] += sc[i] * (double)x[i];
-    x[i] = X(0);
+    const int64_t k = map ? (int64_t)map[i] : i;
+    u[i] += sc[i] * (double)x[k];
+    x[k] = X(0);
   }
 }
 
@@ -560,9 +746,9 @@ static int cg_elems(const fem_ctx* c) { return c->tune.cg_elems ? c->tune.cg_ele
 
 // Blocks of k_cg_step (and of the reseed kernels) for n unknowns: one pass over the vectors
 // when possible.
-static int cg_blocks(const fem_ctx* c) {
+static int cg_blocks(const fem_ctx* c, int64_t n) {
   const int64_t per = (int64_t)kBlock * cg_elems(c);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(kCgBlocks, (c->n + per - 1) / per));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kCgBlocks, (n + per - 1) / per));
 }
 
 template <typename X, int E>
@@ -655,17 +841,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, X* __restrict__ x
 }
 
 // Reseed, step 1: r̂ = S res (rounded to X); partials r̂·r̂ and r̂·p.
+// `map` as in k_fold (the recurrence's entries outside the map stay zero).
 template <typename X>
 __global__ __launch_bounds__(kBlock) void k_cg_reseed(int64_t n, const double* __restrict__ res,
                                                       const double* __restrict__ sc, X* __restrict__ r,
-                                                      const X* __restrict__ p, double* __restrict__ part) {
+                                                      const X* __restrict__ p, double* __restrict__ part,
+                                                      const int32_t* __restrict__ map) {
   __shared__ double sh[kBlock / 64];
   double rr = 0.0, rp = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t k = map ? (int64_t)map[i] : i;
     const X ri = (X)(sc[i] * res[i]);
-    r[i] = ri;
+    r[k] = ri;
     rr += (double)ri * (double)ri;
-    rp += (double)ri * (double)p[i];
+    rp += (double)ri * (double)p[k];
   }
   double b = block_sum(rr, sh);
   if (threadIdx.x == 0) part[2 * blockIdx.x] = b;
@@ -760,9 +949,9 @@ __global__ void k_zero(int64_t n, double* __restrict__ a) {
 }
 
 template <typename X>
-static void zero_vec(fem_ctx* c, X* a) {
-  if constexpr (sizeof(X) == 4) k_zero32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, a);
-  else k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, a);
+static void zero_vec(fem_ctx* c, X* a, int64_t n) {
+  if constexpr (sizeof(X) == 4) k_zero32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, a);
+  else k_zero<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(n, a);
 }
 
 // --------------------------------------------------------------------------
@@ -818,17 +1007,142 @@ static int ensure_work(fem_ctx* c, bool mixed) {
       return rc;
     w->n = n;
   }
-  if (!mixed && !w->x) {
-    if ((rc = dev_alloc(c, &w->x, n)) || (rc = dev_alloc(c, &w->r, n)) || (rc = dev_alloc(c, &w->wv, n)) ||
-        (rc = dev_alloc(c, &w->p, n)) || (rc = dev_alloc(c, &w->sv, n)))
-      return rc;
-  }
-  if (mixed && !w->x32) {
-    if ((rc = dev_alloc(c, &w->x32, n)) || (rc = dev_alloc(c, &w->r32, n)) || (rc = dev_alloc(c, &w->wv32, n)) ||
-        (rc = dev_alloc(c, &w->p32, n)) || (rc = dev_alloc(c, &w->sv32, n)))
-      return rc;
-  }
+  (void)mixed;
   return FEM_OK;
+}
+
+// The five recurrence vectors in X precision, at least nv long (the lattice operator's
+// padded numbering is longer than n).
+template <typename X>
+static int ensure_vecs(fem_ctx* c, SolverWork* w, int64_t nv) {
+  int64_t& cap = sizeof(X) == 4 ? w->cap32 : w->cap64;
+  if (cap >= nv) return FEM_OK;
+  X** v[5];
+  if constexpr (sizeof(X) == 4) {
+    X** t[5] = {&w->x32, &w->r32, &w->wv32, &w->p32, &w->sv32};
+    std::memcpy(v, t, sizeof v);
+  } else {
+    X** t[5] = {&w->x, &w->r, &w->wv, &w->p, &w->sv};
+    std::memcpy(v, t, sizeof v);
+  }
+  cap = 0;
+  for (X** a : v) {
+    const int rc = dev_alloc(c, a, nv);
+    if (rc) return rc;
+  }
+  cap = nv;
+  return FEM_OK;
+}
+
+// Lattice numbering and padded geometry (once per pattern): find_lattice on the host and the
+// map to padded rows.  state −1 when the dofs form no lattice; the first plane fill also sets
+// it when a pattern entry couples dofs that are not lattice neighbours (the sliced copy is
+// used instead).
+static int ensure_lattice(fem_ctx* c, SolverWork* w) {
+  LatOp& l = w->lat;
+  if (l.state != 0) return FEM_OK;
+  l.state = -1;
+  Lattice L;
+  if (!find_lattice(c, &L)) return FEM_OK;
+  const int64_t px = L.nx + 2, py = L.ny + 2, pz = L.dim == 3 ? L.nz + 2 : 1;
+  const int64_t pxy = px * py, npad = pxy * pz;
+  if (8 * npad >= (int64_t)UINT32_MAX) return FEM_OK;   // 32-bit byte offsets within a plane / vector
+  std::vector<int32_t> map(c->n), dofp(npad, -1);
+  for (int64_t i = 0; i < c->n; ++i) {
+    const int64_t q = L.pos[i];
+    const int64_t ix = q % L.nx, iy = (q / L.nx) % L.ny, iz = q / (L.nx * L.ny);
+    map[i] = (int32_t)((ix + 1) + px * ((iy + 1) + py * (L.dim == 3 ? iz + 1 : 0)));
+    dofp[map[i]] = (int32_t)i;
+  }
+  int rc;
+  if ((rc = dev_alloc(c, &l.map, c->n)) || (rc = dev_alloc(c, &l.dofp, npad)) || (rc = dev_alloc(c, &l.cnt, 4)))
+    return rc;
+  FEM_HIP(c, hipMemcpyAsync(l.map, map.data(), 4 * c->n, hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipMemcpyAsync(l.dofp, dofp.data(), 4 * npad, hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  l.px = px; l.py = py; l.pxy = pxy; l.npad = npad;
+  l.ns = L.dim == 3 ? 14 : 5;
+  for (int k = 0; k < 14; ++k) l.off.o[k] = 0;
+  for (int k = 1; k < l.ns; ++k) {
+    const int code = 13 + k;
+    const int dx = code % 3 - 1, dy = (code / 3) % 3 - 1, dz = code / 9 - 1;
+    l.off.o[k] = (int32_t)(dx + px * dy + pxy * dz);
+  }
+  const int64_t omax = 1 + px + (L.dim == 3 ? pxy : 0);   // largest offset
+  l.r_lo = omax;
+  l.nrows = npad - 2 * omax;
+  // a multiple of 8 (one share per XCD)
+  int64_t gr = std::min<int64_t>(kMaxVecBlocks, (l.nrows + kBlock - 1) / kBlock);
+  if (gr >= 8) gr -= gr % 8;
+  l.grid = (int)std::max<int64_t>(1, gr);
+  l.state = 1;
+  return FEM_OK;
+}
+
+// Planes of the scaled K in V precision (refilled when nzval or the scaling changed), with the
+// symmetry check; *sym = false when it fails (the caller then uses the sliced copy).
+template <typename V>
+static int ensure_lat_vals(fem_ctx* c, SolverWork* w, int jacobi, bool* sym) {
+  LatOp& l = w->lat;
+  V*& val = sizeof(V) == 4 ? reinterpret_cast<V*&>(l.val32) : reinterpret_cast<V*&>(l.val64);
+  V*& mir = sizeof(V) == 4 ? reinterpret_cast<V*&>(l.mir32) : reinterpret_cast<V*&>(l.mir64);
+  uint64_t& ver = sizeof(V) == 4 ? l.ver32 : l.ver64;
+  int& jac = sizeof(V) == 4 ? l.jac32 : l.jac64;
+  bool& ok = sizeof(V) == 4 ? l.sym32 : l.sym64;
+  int rc;
+  if (!val) {   // positions never written (padding, missing neighbours) stay zero
+    if ((rc = dev_alloc(c, &val, (size_t)l.ns * l.npad)) || (rc = dev_alloc(c, &mir, (size_t)(l.ns - 1) * l.npad)))
+      return rc;
+    FEM_HIP(c, hipMemsetAsync(val, 0, sizeof(V) * l.ns * l.npad, c->stream));
+    FEM_HIP(c, hipMemsetAsync(mir, 0, sizeof(V) * (l.ns - 1) * l.npad, c->stream));
+    ver = 0;
+  }
+  if (ver != c->nzval_version + 1 || jac != jacobi) {
+    const LatGeom g{(uint32_t)l.px, (uint32_t)l.pxy, (uint32_t)(1 + l.px + l.pxy), l.npad};
+    const int64_t rows_per_block = kBlock / kLatLanes;
+    const int fb = (int)std::min<int64_t>(16384, (l.nrows + rows_per_block - 1) / rows_per_block);
+    FEM_HIP(c, hipMemsetAsync(l.cnt, 0, 4 * sizeof(unsigned long long), c->stream));
+    k_lat_fill<V><<<fb, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.dofp, c->d_colptr, c->d_rowval, c->d_nzval,
+                                                  w->sc, l.map, g, val, mir, l.cnt);
+    // FP32 planes: asymmetry below FP32 resolution does not change the operator
+    const double tol = sizeof(V) == 4 ? 1.0 / (1 << 22) : 1e-12;
+    k_lat_cmp<V><<<kMaxVecBlocks, kBlock, 0, c->stream>>>((int64_t)(l.ns - 1) * l.npad, val + l.npad, mir, tol,
+                                                            l.cnt);
+    unsigned long long h[4] = {};
+    FEM_HIP(c, hipMemcpyAsync(h, l.cnt, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    if (h[1] != 0) {   // not a lattice pattern: never try again for this pattern
+      l.state = -1;
+      dev_free(&l.val32); dev_free(&l.val64); dev_free(&l.mir32); dev_free(&l.mir64);
+      *sym = false;
+      return FEM_OK;
+    }
+    ok = h[3] == 0;    // every forward value equals its mirror (a missing mirror leaves a zero)
+    ver = c->nzval_version + 1;
+    jac = jacobi;
+  }
+  *sym = ok;
+  return FEM_OK;
+}
+
+template <typename V, typename X, int NS>
+static void launch_lat_ns(fem_ctx* c, const LatOp& l, const V* val, const X* x, X* y, double* part,
+                          const Scal* sc) {
+  if (c->tune.sell_map != 1 && l.grid % 8 == 0)
+    k_lat_spmv<V, X, NS, 1><<<l.grid, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.npad, l.off, val, x, y, part, sc);
+  else
+    k_lat_spmv<V, X, NS, 0><<<l.grid, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.npad, l.off, val, x, y, part, sc);
+}
+
+template <typename X>
+static void launch_lat(fem_ctx* c, const LatOp& l, const X* x, X* y, double* part, const Scal* sc) {
+  if constexpr (sizeof(X) == 4) {
+    if (l.ns == 14) launch_lat_ns<float, float, 14>(c, l, l.val32, x, y, part, sc);
+    else launch_lat_ns<float, float, 5>(c, l, l.val32, x, y, part, sc);
+  } else {
+    if (l.ns == 14) launch_lat_ns<double, double, 14>(c, l, l.val64, x, y, part, sc);
+    else launch_lat_ns<double, double, 5>(c, l, l.val64, x, y, part, sc);
+  }
 }
 
 // SELL structure (once per pattern): widths/bases on the device, offsets on the host.
@@ -953,6 +1267,10 @@ static void launch_sell(fem_ctx* c, const SellOp& o, int g, const V* val, const 
 // number of partials written.
 template <typename X>
 static int spmv_hat(fem_ctx* c, SolverWork* w, const X* x, X* y, const Scal* sc) {
+  if (w->use_lat) {
+    launch_lat<X>(c, w->lat, x, y, w->part_s, sc);
+    return w->lat.grid;
+  }
   const SellOp& o = w->sell;
   const int g = w->sell_grid;
   if constexpr (sizeof(X) == 4) {
@@ -965,12 +1283,28 @@ static int spmv_hat(fem_ctx* c, SolverWork* w, const X* x, X* y, const Scal* sc)
   return g;
 }
 
-// The scaling S and the scaled operator in X precision, ready for spmv_hat.
+// The scaling S and the scaled operator in X precision, ready for spmv_hat: the lattice
+// planes when the dofs form a lattice and K is symmetric (the mixed solver by default; the
+// FP64 solver with tuning lattice = 1), otherwise the sliced copy.  Returns the length of
+// the recurrence vectors in *nv.
 template <typename X>
-static int prepare_operator(fem_ctx* c, SolverWork* w, int jacobi) {
+static int prepare_operator(fem_ctx* c, SolverWork* w, int jacobi, int64_t* nv) {
   int rc;
-  if ((rc = ensure_sell(c, w))) return rc;
   k_scale<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_diag, c->d_nzval, jacobi, w->sc);
+  w->use_lat = false;
+  const bool want = sizeof(X) == 4 ? c->tune.lattice != -1 : c->tune.lattice == 1;
+  if (want) {
+    if ((rc = ensure_lattice(c, w))) return rc;
+    if (w->lat.state == 1) {
+      bool sym = false;
+      if ((rc = ensure_lat_vals<X>(c, w, jacobi, &sym))) return rc;
+      w->use_lat = sym;
+    }
+  }
+  *nv = w->use_lat ? w->lat.npad : c->n;
+  if ((rc = ensure_vecs<X>(c, w, *nv))) return rc;
+  if (w->use_lat) return FEM_OK;
+  if ((rc = ensure_sell(c, w))) return rc;
   return ensure_sell_vals<X>(c, w, jacobi);
 }
 
@@ -997,6 +1331,8 @@ static int read_scal(fem_ctx* c, SolverWork* w, Scal* h) {   // scalar record 0 
 template <typename X>
 struct CgVecs {
   X *x, *r, *w, *p, *sv;
+  int64_t n;               // their length
+  const int32_t* map;      // recurrence index of each dof (lattice operator), or null
 };
 
 // One Chronopoulos–Gear iteration: w = Â r̂ (partial r̂·w), then the fused vector step.
@@ -1010,9 +1346,9 @@ static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int
   if (timed) (void)hipEventRecord(w->tev[2 * k], c->stream);
   const int na = spmv_hat<X>(c, w, v.r, v.w, sin);
   if (timed) (void)hipEventRecord(w->tev[2 * k + 1], c->stream);
-  const int nb = cg_blocks(c);
+  const int nb = cg_blocks(c, v.n);
 #define FEM_STEP(EE)                                                                                          \
-  k_cg_step<X, EE><<<nb, kBlock, 0, c->stream>>>(c->n, v.x, v.r, v.w, v.p, v.sv, na, w->part_s, pin, pout, sin, \
+  k_cg_step<X, EE><<<nb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, na, w->part_s, pin, pout, sin, \
                                                  sout)
   switch (cg_elems(c)) {
     case 2: FEM_STEP(2); break;
@@ -1028,8 +1364,9 @@ static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int
 template <typename X>
 static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache& gc) {
   static_assert(kBatch % 2 == 0, "batch must end on scalar record 0");
-  const void* val = sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64;
-  const void* key[8] = {val, v.x, v.r, v.p, v.w, v.sv, w->s, w->sell.off};
+  const void* val = w->use_lat ? (sizeof(X) == 4 ? (const void*)w->lat.val32 : (const void*)w->lat.val64)
+                               : (sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64);
+  const void* key[8] = {val, v.x, v.r, v.p, v.w, v.sv, w->s, w->use_lat ? (const void*)w->lat.map : w->sell.off};
   if (c->tune.time_kernels) {   // kernels launched directly (launch_batch), SpMVs bracketed by events
     for (hipEvent_t& e : w->tev)
       if (!e) FEM_HIP(c, hipEventCreate(&e));
@@ -1069,11 +1406,11 @@ static int launch_batch(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCach
 template <typename X>
 static void reseed(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, const double* res, double hold_factor2,
                    double tol2, int restart) {
-  const int nb = cg_blocks(c);
-  k_cg_reseed<X><<<nb, kBlock, 0, c->stream>>>(c->n, res, w->sc, v.r, v.p, w->part_v);
+  const int nb = cg_blocks(c, v.n);
+  k_cg_reseed<X><<<nb, kBlock, 0, c->stream>>>(c->n, res, w->sc, v.r, v.p, w->part_v, v.map);
   k_cg_reseed_scal<<<1, kBlock, 0, c->stream>>>(nb, w->part_v, w->s, hold_factor2, tol2, restart);
   spmv_hat<X>(c, w, v.r, v.w, w->s);
-  k_cg_pfix<X><<<nb, kBlock, 0, c->stream>>>(c->n, v.r, v.w, v.p, v.sv, w->s, w->part_v);
+  k_cg_pfix<X><<<nb, kBlock, 0, c->stream>>>(v.n, v.r, v.w, v.p, v.sv, w->s, w->part_v);
   k_cg_alpha<<<1, kBlock, 0, c->stream>>>(nb, w->part_v, w->s);
 }
 
@@ -1094,10 +1431,9 @@ constexpr int kMaxStall = 2;
 template <typename X>
 static int cg_run(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double bnorm2, double tol2, double hold_factor2,
                   int maxit, GraphCache& gc, int* iters, int* handovers, double* ms) {
-  zero_vec<double>(c, c->d_u);
-  zero_vec<X>(c, v.x);
-  zero_vec<X>(c, v.p);
-  zero_vec<X>(c, v.sv);
+  zero_vec<double>(c, c->d_u, c->n);
+  // all five: the lattice numbering's padding entries must start (and so stay) zero
+  for (X* a : {v.x, v.r, v.w, v.p, v.sv}) zero_vec<X>(c, a, v.n);
   k_cg_init_scal<<<1, 1, 0, c->stream>>>(w->s, bnorm2);
   reseed<X>(c, w, v, c->d_f, hold_factor2, tol2, 1);
   FEM_HIP(c, hipGetLastError());
@@ -1122,7 +1458,7 @@ static int cg_run(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double bnorm2, 
     if (!h.done) continue;
     fresh_start = true;
     ++nho;
-    k_fold<X><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, v.x, w->sc, c->d_u);
+    k_fold<X><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, v.x, w->sc, c->d_u, v.map);
     residual(c, w);
     Scal ht{};
     if ((rc = read_scal(c, w, &ht))) return rc;
@@ -1133,7 +1469,7 @@ static int cg_run(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double bnorm2, 
     reseed<X>(c, w, v, w->res, hold_factor2, tol2, 0);
   }
   if (!converged) {   // fold what the recurrence holds and leave the true residual in s->rr
-    k_fold<X><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, v.x, w->sc, c->d_u);
+    k_fold<X><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, v.x, w->sc, c->d_u, v.map);
     residual(c, w);
   }
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
@@ -1180,20 +1516,22 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   }
   const double tol2 = cfg->rtol * cfg->rtol * bnorm2;
   int iters = 0, handovers = 0;
+  int64_t nv = 0;
   if (!mixed) {
-    if ((rc = prepare_operator<double>(c, w, cfg->precond))) return rc;
-    const CgVecs<double> v{w->x, w->r, w->wv, w->p, w->sv};
+    if ((rc = prepare_operator<double>(c, w, cfg->precond, &nv))) return rc;
+    const CgVecs<double> v{w->x, w->r, w->wv, w->p, w->sv, nv, w->use_lat ? w->lat.map : nullptr};
     rc = cg_run<double>(c, w, v, bnorm2, tol2, 0.0, maxit, w->g64, &iters, &handovers, &cg_ms);
   } else {
     // the FP32 recurrence hands over whenever its residual has fallen by δ (inner_rtol)
-    if ((rc = prepare_operator<float>(c, w, cfg->precond))) return rc;
+    if ((rc = prepare_operator<float>(c, w, cfg->precond, &nv))) return rc;
     const double delta = cfg->inner_rtol > 0 ? cfg->inner_rtol : 0.1;
-    const CgVecs<float> v{w->x32, w->r32, w->wv32, w->p32, w->sv32};
+    const CgVecs<float> v{w->x32, w->r32, w->wv32, w->p32, w->sv32, nv, w->use_lat ? w->lat.map : nullptr};
     rc = cg_run<float>(c, w, v, bnorm2, tol2, delta * delta, maxit, w->g32, &iters, &handovers, &cg_ms);
   }
   if (rc && rc != FEM_ERR_NOCONV) return rc;
   out.iterations = iters;
   out.outer_iterations = handovers;
+  out.operator_format = w->use_lat ? 1 : 0;
   if ((rc = read_scal(c, w, &h))) return rc;   // s->rr: the true residual of the returned u
   out.relres = std::sqrt(h.rr / bnorm2);
   c->timings.solve_ms = cg_ms;
@@ -1217,11 +1555,14 @@ extern "C" int32_t fem_spmv_bench(fem_ctx* c, int32_t mixed, int32_t reps, doubl
   SolverWork* w = c->solver;
   Scal zero{};
   FEM_HIP(c, hipMemcpyAsync(w->s, &zero, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
-  if (mixed) {
-    if ((rc = prepare_operator<float>(c, w, 1))) return rc;
+  int64_t nv = 0;
+  if (mixed) {   // x = f (in the operator's numbering when it is the lattice's: padding zero)
+    if ((rc = prepare_operator<float>(c, w, 1, &nv))) return rc;
+    zero_vec<float>(c, w->p32, nv);
     k_to_f32<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p32);
   } else {
-    if ((rc = prepare_operator<double>(c, w, 1))) return rc;
+    if ((rc = prepare_operator<double>(c, w, 1, &nv))) return rc;
+    zero_vec<double>(c, w->p, nv);
     k_copy<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_f, w->p);
   }
   FEM_HIP(c, hipStreamSynchronize(c->stream));
@@ -1250,7 +1591,8 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
       !(t->cg_elems == 0 || t->cg_elems == 2 || t->cg_elems == 4 || t->cg_elems == 8 || t->cg_elems == 16) ||
       !(t->gather_chunks == 0 || t->gather_chunks == 1 || t->gather_chunks == 2 || t->gather_chunks == 4 ||
         t->gather_chunks == 8 || t->gather_chunks == 16 || t->gather_chunks == 32) ||
-      !(t->sell_nt == 0 || t->sell_nt == 1 || t->sell_nt == -1) || !(t->affine == 0 || t->affine == -1))
+      !(t->sell_nt == 0 || t->sell_nt == 1 || t->sell_nt == -1) || !(t->affine == 0 || t->affine == -1) ||
+      !(t->lattice == 0 || t->lattice == 1 || t->lattice == -1))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

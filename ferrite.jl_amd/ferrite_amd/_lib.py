@@ -51,7 +51,8 @@ class SolveCfg(C.Structure):
 
 
 class SolveInfo(C.Structure):
-    _fields_ = [("iterations", C.c_int32), ("outer_iterations", C.c_int32), ("relres", C.c_double)]
+    _fields_ = [("iterations", C.c_int32), ("outer_iterations", C.c_int32), ("relres", C.c_double),
+                ("operator_format", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Tuning(C.Structure):
@@ -59,7 +60,7 @@ class Tuning(C.Structure):
                 ("asm_block", C.c_int32), ("asm_mode", C.c_int32), ("time_kernels", C.c_int32),
                 ("sell_unroll", C.c_int32), ("sell_map", C.c_int32), ("gather_group", C.c_int32),
                 ("cg_elems", C.c_int32), ("gather_chunks", C.c_int32), ("sell_nt", C.c_int32),
-                ("affine", C.c_int32), ("reserved", C.c_int32)]
+                ("affine", C.c_int32), ("lattice", C.c_int32)]
 
 
 class Timings(C.Structure):
@@ -101,6 +102,7 @@ SIGNATURES = {
     "fem_apply_dirichlet": (I32, [P, I32, PD]),
     "fem_solve": (I32, [P, C.POINTER(SolveCfg), C.POINTER(SolveInfo)]),
     "fem_matrix_download": (I32, [P, PD, PD]),
+    "fem_matrix_upload": (I32, [P, PD, PD]),
     "fem_solution_download": (I32, [P, PD]),
     "fem_solution_upload": (I32, [P, PD]),
     "fem_device_pointer": (I32, [P, I32, C.POINTER(P), PI64]),

@@ -76,6 +76,13 @@ class DeviceMatrix:
         self.ctx.call("fem_matrix_download", L.ptr(out, C.c_double), None)
         return out
 
+    def set_nzval(self, values):
+        """Writes K.nzval from the host (the in-place nzval write of a SparseMatrixCSC)."""
+        v = L.as_f64(values)
+        if v.shape != (self.nnz,):
+            raise ValueError("nzval length must equal nnz")
+        self.ctx.call("fem_matrix_upload", L.ptr(v, C.c_double), None)
+
     def to_scipy(self):
         import scipy.sparse as sp
         return sp.csc_matrix((self.nzval(), self.rowval - 1, self.colptr - 1), shape=self.shape)
@@ -217,6 +224,7 @@ class SolveResult:
         self.iterations = info.iterations
         self.outer_iterations = info.outer_iterations
         self.relres = info.relres
+        self.operator = "lattice" if info.operator_format == 1 else "sell"
         self.u = u
 
 

@@ -24,6 +24,8 @@ def main():
                     help="CG vector step: elements per thread (solve loop times, both solvers)")
     ap.add_argument("--gather-sweep", action="store_true",
                     help="owner-computes assembly: cells loaded per column lane before adding")
+    ap.add_argument("--lattice-sweep", action="store_true",
+                    help="lattice vs sliced-ELL solver operator x block mapping: isolated and in-solve SpMV")
     ap.add_argument("--sell-sweep", action="store_true",
                     help="sliced-ELL unroll depth x slice mapping: isolated and in-solve SpMV durations")
     args = ap.parse_args()
@@ -105,6 +107,27 @@ def main():
             print(f"gather group {grp}: fp64 {ts[0]:.3f} ms, fp32-element {ts[1]:.3f} ms", flush=True)
         fa.set_tuning(K, gather_group=0)
         fa.apply(K, f, ch)
+    if args.lattice_sweep:
+        for lat, lname in ((-1, "sliced ELL"), (1, "lattice")):
+            for smap, mname in ((1, "grid-stride"), (2, "XCD ranges")):
+                fa.set_tuning(K, lattice=lat, sell_map=smap)
+                a64 = fa.spmv_bench(K, mixed=False, reps=20)
+                a32 = fa.spmv_bench(K, mixed=True, reps=20)
+                fa.set_tuning(K, lattice=lat, sell_map=smap, time_kernels=1)
+                line = []
+                for mode in ("cg", "mixed"):
+                    r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
+                    t = fa.timings(K)
+                    line.append(f"{mode} [{r.operator}] {t['spmv_ms'] / max(t['spmv_launches'], 1) * 1e3:.1f} us x "
+                                f"{t['spmv_launches']} ({r.iterations} its, loop {t['solve_ms']:.1f} ms)")
+                fa.set_tuning(K, lattice=lat, sell_map=smap, time_kernels=0)
+                loops = []
+                for mode in ("cg", "mixed"):
+                    r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
+                    loops.append(f"{mode} {fa.timings(K)['solve_ms']:.1f} ms")
+                print(f"operator {lname} {mname}: isolated fp64 {a64 * 1e3:.1f} us, fp32 {a32 * 1e3:.1f} us; "
+                      f"in-solve {'; '.join(line)}; graph loops {', '.join(loops)}", flush=True)
+        fa.set_tuning(K, lattice=0, sell_map=0, time_kernels=0)
     if args.sell_sweep:
         for smap, mname, unrolls in ((1, "grid-stride, ordinary loads", (4, 8, 16)), (2, "contiguous/XCD", (4, 8)),
                                      (3, "grid-stride, non-temporal loads", (4, 8))):

@@ -132,15 +132,22 @@ typedef struct {
   int32_t iterations;        /* total CG iterations */
   int32_t outer_iterations;  /* mixed: reliable updates; FP64: 1 + residual replacements */
   double relres;             /* final true relative residual ‖f − K u‖/‖f‖ in FP64 */
+  int32_t operator_format;   /* the CG operator copy used: 0 sliced ELL, 1 lattice planes */
+  int32_t reserved;
 } fem_solve_info;
 
 int32_t fem_solve(fem_ctx* ctx, const fem_solve_cfg* cfg, fem_solve_info* info);
 
 /* ---- results ---- */
 int32_t fem_matrix_download(fem_ctx* ctx, double* nzval, double* f);
+/* K.nzval (and f, either may be NULL) written from the host, as a Julia caller writes the
+ * SparseMatrixCSC's nzval in place; the solver rebuilds its operator copies at the next solve. */
+int32_t fem_matrix_upload(fem_ctx* ctx, const double* nzval, const double* f);
 int32_t fem_solution_download(fem_ctx* ctx, double* u);
 int32_t fem_solution_upload(fem_ctx* ctx, const double* u);
-/* Device address of a resident array (0 = nzval, 1 = f, 2 = u) for zero-copy wrapping. */
+/* Device address of a resident array (0 = nzval, 1 = f, 2 = u) for zero-copy wrapping.  Asking
+ * for nzval marks K as changed: the solver rebuilds its operator copies at the next solve, so
+ * a caller may write K in place between solves. */
 int32_t fem_device_pointer(fem_ctx* ctx, int32_t which, void** ptr, int64_t* nelem);
 
 /* ---- launch-shape tuning (autotuner interface; no counterpart in the reference) ---- */
@@ -159,8 +166,9 @@ typedef struct {
                                         (fem_timings.spmv_ms / spmv_launches) */
   int32_t sell_unroll;               /* sliced-ELL SpMV: entries loaded per lane before their x
                                         gathers, 4 / 8 / 16 (0 = default: 8 FP32, 4 FP64) */
-  int32_t sell_map;                  /* sliced-ELL slices to blocks: 0 = default (grid-stride),
-                                        1 = grid-stride, 2 = contiguous, one range per XCD share */
+  int32_t sell_map;                  /* solver SpMV rows to blocks: 1 = grid-stride, 2 = contiguous,
+                                        one range per XCD share; 0 = default (sliced ELL: grid-stride,
+                                        lattice planes: XCD ranges) */
   int32_t gather_group;              /* owner-computes assembly: cells whose element values a column
                                         lane loads before adding them, 1 / 2 / 4 / 8 (0 = default 4) */
   int32_t cg_elems;                  /* CG vector step: elements per thread per pass, 2 / 4 / 8 / 16
@@ -173,7 +181,11 @@ typedef struct {
   int32_t affine;                    /* owner-computes assembly on trilinear hexes: constant-Jacobian
                                         form for parallelepiped cells (0 = default, on) or the
                                         quadrature loop for every cell (-1) */
-  int32_t reserved;
+  int32_t lattice;                   /* solver operator when the dofs of a scalar Q1 field form a
+                                        tensor-product lattice and K is symmetric: value planes
+                                        per neighbour, no column indices, half the values
+                                        (0 = default: mixed solver only; 1: FP64 solver too;
+                                        -1: never, the sliced ELL copy) */
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
 int32_t fem_get_tuning(const fem_ctx* ctx, fem_tuning* t);

@@ -1,0 +1,119 @@
+"""The solver's lattice operator (value planes per neighbour, no column indices) against the
+oracle's direct solve and against the sliced-ELL operator on the same system.
+
+Bar: u within 1e-8 of the oracle (CG to 1e-12), the lattice and sliced operators' solutions
+within 1e-8 of each other; the operator actually used is read back from fem_solve_info.
+Meshes with non-cube dimensions (13x9x7 hexes, 17x11 quads) so that an axis mix-up in the
+padded numbering cannot cancel out; one mesh with a cell whose local numbering is rotated
+(a valid hex the lattice walk cannot place) must fall back to the sliced operator.
+"""
+import numpy as np
+import pytest
+
+import ferrite_amd as fa
+from oracle import constraints as OC
+from oracle.problems import assemble_poisson, poisson_conductivity, setup_poisson
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def _system(ct, nel, right, k, cells_hook=None):
+    g = fa.generate_grid(ct, nel, tuple(0.0 for _ in nel), right)
+    if cells_hook is not None:
+        cells_hook(g)
+    ip = fa.Lagrange(g.refshape, 1)
+    cv = fa.CellValues(fa.QuadratureRule(g.refshape, 2), ip)
+    dh = fa.DofHandler(g).add("u", ip).close()
+    K = fa.allocate_matrix(fa.MI355XCSC, dh)
+    f = fa.allocate_vector(K)
+    fa.assemble_global(fa.start_assemble(K, f), cv, fa.create_coloring(g), fa.PoissonElement(k=k, source=1.0))
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.Dirichlet("u", g.facetsets["left"] + g.facetsets["right"], lambda x, t: 0.0)).close()
+    fa.apply(K, f, ch)
+    return g, dh, K, f
+
+
+def _oracle_u(ct, nel, right, k):
+    grid, cv, odh, OK, of, och = setup_poisson(ct, nel, tuple(0.0 for _ in nel), right, bc_sets=["left", "right"])
+    assemble_poisson(grid, cv, odh, OK, of, k_cell=k)
+    OC.apply(OK, of, och)
+    return OC.solve(OK, of)
+
+
+@pytest.mark.parametrize("ct,nel,right", [("Hexahedron", (13, 9, 7), (1.3, 0.9, 0.7)),
+                                          ("Quadrilateral", (17, 11), (1.7, 1.1))])
+def test_lattice_operator_parity(ct, nel, right):
+    print("case lattice operator", ct, nel, flush=True)
+    k = poisson_conductivity(nel, seed=11)
+    g, dh, K, f = _system(ct, nel, right, k)
+    u_ref = _oracle_u(ct, nel, right, k)
+    lat = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert lat.operator == "lattice"
+    assert lat.relres <= 1e-12 and _rel(lat.u, u_ref) < 1e-8
+    fa.set_tuning(K, lattice=1)
+    lat64 = fa.solve(K, f, mode="cg", rtol=1e-12)
+    assert lat64.operator == "lattice"
+    assert lat64.relres <= 1e-12 and _rel(lat64.u, u_ref) < 1e-8
+    fa.set_tuning(K, lattice=-1)
+    sell = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert sell.operator == "sell"
+    assert _rel(lat.u, sell.u) < 1e-8
+    sell64 = fa.solve(K, f, mode="cg", rtol=1e-12)
+    assert sell64.operator == "sell"
+    assert _rel(lat64.u, sell64.u) < 1e-9
+    fa.set_tuning(K, lattice=0)
+    again = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert again.operator == "lattice"
+    assert np.array_equal(again.u, lat.u)          # deterministic
+    assert fa.spmv_bench(K, mixed=True, reps=3) > 0.0
+
+
+def test_lattice_fallback_on_rotated_cell():
+    """A hex whose local node order is turned a quarter around z is valid for assembly, but
+    the lattice walk places its nodes inconsistently: the solver must use the sliced copy."""
+    print("case lattice fallback", flush=True)
+    nel = (6, 5, 4)
+    k = poisson_conductivity(nel, seed=12)
+
+    def rotate(g):
+        c = g.cells[7].copy()
+        g.cells[7] = c[[1, 2, 3, 0, 5, 6, 7, 4]]
+    g, dh, K, f = _system("Hexahedron", nel, (1.0, 1.0, 1.0), k, rotate)
+    res = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert res.operator == "sell"
+    ref = fa.solve(K, f, mode="cg", rtol=1e-12)
+    assert _rel(res.u, ref.u) < 1e-8
+    # same physics as the unrotated mesh: u per node agrees with the oracle's
+    u_ref = _oracle_u("Hexahedron", nel, (1.0, 1.0, 1.0), k)
+    g0, dh0, K0, f0 = _system("Hexahedron", nel, (1.0, 1.0, 1.0), k)
+    node_u, node_u0 = np.zeros(g.nnodes), np.zeros(g.nnodes)
+    node_u[g.cells.ravel() - 1] = res.u[dh.cell_dofs.ravel() - 1]
+    node_u0[g0.cells.ravel() - 1] = u_ref[dh0.cell_dofs.ravel() - 1]
+    assert _rel(node_u, node_u0) < 1e-8
+
+
+def test_lattice_symmetry_check_falls_back():
+    """K made slightly unsymmetric in place (one off-diagonal entry scaled by 1 + 1e-5, above
+    FP32 resolution): the plane fill's mirror check must reject the lattice operator; with K
+    restored the lattice operator is used again."""
+    print("case lattice symmetry check", flush=True)
+    nel = (7, 6, 5)
+    k = poisson_conductivity(nel, seed=13)
+    g, dh, K, f = _system("Hexahedron", nel, (1.0, 1.0, 1.0), k)
+    assert fa.solve(K, f, mode="mixed", rtol=1e-12).operator == "lattice"
+    nz = K.nzval()
+    rows, cols = K.rowval - 1, np.repeat(np.arange(K.n), np.diff(K.colptr))
+    off = np.nonzero((rows != cols) & (nz != 0.0))[0]
+    e = int(off[len(off) // 2])
+    bad = nz.copy()
+    bad[e] *= 1.0 + 1e-5
+    K.set_nzval(bad)
+    res = fa.solve(K, f, mode="mixed", rtol=1e-10)
+    assert res.operator == "sell"
+    K.set_nzval(nz)
+    res = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert res.operator == "lattice"
