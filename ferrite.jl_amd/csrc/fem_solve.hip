@@ -48,6 +48,8 @@ struct Scal {
   int iters;
   int restart;    // breakdown happened: the next reseed restarts p = r̂
   int fresh;      // p, s = Â p and α were just set by a reseed: the next step only moves x̂, r̂
+  int sfresh;     // (record written by k_cg_scal) the step k_cg_move runs now is such a step
+  int pad;
 };
 
 struct SpmvShape {
@@ -697,8 +699,10 @@ __global__ void k_fold(int64_t n, X* __restrict__ x, const double* __restrict__ 
 //     γ = r̂·r̂ (partials of the previous step), β = γ/γ_prev,
 //     α = γ / (δ − β γ / α_prev), p = r̂ + β p, s = w + β s, x̂ += α p, r̂ −= α s,
 //     partials of the next γ.
-// Every block reduces the partials itself (same fixed order in every block, hence the
-// same scalars bitwise), so an iteration is two kernels: SpMV, k_cg_step.  The scalar
+// By default an iteration is three kernels: SpMV, k_cg_scal (one block reduces the partials
+// in a fixed order and writes the scalars), k_cg_move (the vectors).  In the one-kernel form
+// k_cg_step, every block reduces the partials itself (same fixed order in every block, hence
+// the same scalars bitwise).  The scalar
 // records and the vector partials are double-buffered (read one, write the other).
 // --------------------------------------------------------------------------
 constexpr int kCgBlocks = 4096;  // most blocks of k_cg_step (its partial count)
@@ -838,6 +842,100 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, X* __restrict__ x
     o.iters = S.iters + 1;
     *sout = o;
   }
+}
+
+// Split form of the step (the default; tuning cg_split): k_cg_scal reduces the partials once (one block,
+// fixed order) and writes the step's scalars; k_cg_move then updates the vectors with many
+// small blocks (E elements per thread), which lets later blocks load while earlier ones
+// store.  Same recurrence as k_cg_step.
+__global__ __launch_bounds__(kWide) void k_cg_scal(int nin, const double* __restrict__ part_in, int na,
+                                                  const double* __restrict__ part_s, const Scal* __restrict__ sin,
+                                                  Scal* __restrict__ sout) {
+  __shared__ double sh[kWide / 64];
+  const Scal S = *sin;
+  if (S.done) {
+    if (threadIdx.x == 0) *sout = S;
+    return;
+  }
+  if (S.fresh) {
+    if (threadIdx.x == 0) {
+      Scal o = S;
+      o.sfresh = 1;
+      o.fresh = 0;
+      o.rzp = S.rz;
+      o.beta = 0.0;
+      o.iters = S.iters + 1;
+      *sout = o;
+    }
+    return;
+  }
+  const double gam = wide_sum(part_in, nin, 1, sh);
+  const double del = wide_sum(part_s, na, 1, sh);
+  if (threadIdx.x == 0) {
+    Scal o = S;
+    const double beta = gam / S.rzp;
+    const double den = del - beta * gam / S.alpha;
+    const bool stop = gam <= S.gstop;
+    const bool brk = !(S.rzp > 0.0) || !(S.alpha != 0.0) || !(den > 0.0);
+    if (stop || brk) {   // hand over to the host: x̂, r̂ of this step, p and s of the last one
+      o.done = 1;
+      o.rz = gam;
+      if (!stop) o.restart = 1;
+    } else {
+      o.sfresh = 0;
+      o.fresh = 0;
+      o.alpha = gam / den;
+      o.beta = beta;
+      o.rzp = gam;
+      o.rz = gam;
+      o.iters = S.iters + 1;
+    }
+    *sout = o;
+  }
+}
+
+template <typename X, int E>
+__global__ __launch_bounds__(kBlock) void k_cg_move(int64_t n, X* __restrict__ x, X* __restrict__ r,
+                                                    const X* __restrict__ w, X* __restrict__ p, X* __restrict__ sv,
+                                                    double* __restrict__ part_out, const Scal* __restrict__ sc) {
+  __shared__ double sh[kBlock / 64];
+  const Scal S = *sc;
+  if (S.done) return;
+  const bool fresh = S.sfresh != 0;
+  const X al = (X)S.alpha, be = (X)S.beta;
+  double ga = 0.0;
+  for (int64_t base = (int64_t)blockIdx.x * kBlock * E; base < n; base += (int64_t)gridDim.x * kBlock * E) {
+    X wv[E], pv[E], sw[E], xv[E], rv[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int64_t i = base + k * kBlock + threadIdx.x;
+      const bool in = i < n;
+      wv[k] = in && !fresh ? w[i] : X(0);
+      pv[k] = in ? p[i] : X(0);
+      sw[k] = in ? sv[i] : X(0);
+      xv[k] = in ? x[i] : X(0);
+      rv[k] = in ? r[i] : X(0);
+    }
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int64_t i = base + k * kBlock + threadIdx.x;
+      if (i < n) {
+        X pi = pv[k], si = sw[k];
+        if (!fresh) {
+          pi = rv[k] + be * pi;
+          si = wv[k] + be * si;
+          p[i] = pi;
+          sv[i] = si;
+        }
+        x[i] = xv[k] + al * pi;
+        const X ri = rv[k] - al * si;
+        r[i] = ri;
+        ga += (double)ri * (double)ri;
+      }
+    }
+  }
+  const double b = block_sum(ga, sh);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = b;
 }
 
 // Reseed, step 1: r̂ = S res (rounded to X); partials r̂·r̂ and r̂·p.
@@ -1071,8 +1169,10 @@ static int ensure_lattice(fem_ctx* c, SolverWork* w) {
   const int64_t omax = 1 + px + (L.dim == 3 ? pxy : 0);   // largest offset
   l.r_lo = omax;
   l.nrows = npad - 2 * omax;
-  // a multiple of 8 (one share per XCD)
-  int64_t gr = std::min<int64_t>(kMaxVecBlocks, (l.nrows + kBlock - 1) / kBlock);
+  // A multiple of 8 (one share per XCD).  Measured at 128³ (FP32, in the solve): 512 blocks
+  // 28.5 µs; 384 34.5, 448 31.1, 536 36.4, 576 35.6, 768 32.8, 2048 36.3 µs.
+  int64_t gr = c->tune.lattice_blocks ? c->tune.lattice_blocks : 512;
+  gr = std::min<int64_t>(gr, (l.nrows + kBlock - 1) / kBlock);
   if (gr >= 8) gr -= gr % 8;
   l.grid = (int)std::max<int64_t>(1, gr);
   l.state = 1;
@@ -1346,6 +1446,19 @@ static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int
   if (timed) (void)hipEventRecord(w->tev[2 * k], c->stream);
   const int na = spmv_hat<X>(c, w, v.r, v.w, sin);
   if (timed) (void)hipEventRecord(w->tev[2 * k + 1], c->stream);
+  // default: scalars once, then the vector move (measured at 128³: mixed loop 29.7 vs 34.9 ms,
+  // FP64 loop 89.7 vs 92.4 ms for the one-kernel step)
+  if (c->tune.cg_split != -1) {
+    const int mb = cg_blocks(c, v.n);
+    k_cg_scal<<<1, kWide, 0, c->stream>>>(mb, pin, na, w->part_s, sin, sout);
+    switch (cg_elems(c)) {
+      case 2: k_cg_move<X, 2><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
+      case 4: k_cg_move<X, 4><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
+      case 16: k_cg_move<X, 16><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
+      default: k_cg_move<X, 8><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
+    }
+    return;
+  }
   const int nb = cg_blocks(c, v.n);
 #define FEM_STEP(EE)                                                                                          \
   k_cg_step<X, EE><<<nb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, na, w->part_s, pin, pout, sin, \
@@ -1592,7 +1705,10 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
       !(t->gather_chunks == 0 || t->gather_chunks == 1 || t->gather_chunks == 2 || t->gather_chunks == 4 ||
         t->gather_chunks == 8 || t->gather_chunks == 16 || t->gather_chunks == 32) ||
       !(t->sell_nt == 0 || t->sell_nt == 1 || t->sell_nt == -1) || !(t->affine == 0 || t->affine == -1) ||
-      !(t->lattice == 0 || t->lattice == 1 || t->lattice == -1))
+      !(t->lattice == 0 || t->lattice == 1 || t->lattice == -1) ||
+      !(t->lattice_blocks == 0 || (t->lattice_blocks >= 8 && t->lattice_blocks <= kMaxVecBlocks &&
+                                   t->lattice_blocks % 8 == 0)) ||
+      !(t->cg_split == 0 || t->cg_split == 1 || t->cg_split == -1))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

@@ -72,8 +72,8 @@ def main():
             r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
             print(f"phase solve {mode}: {r.iterations} iterations, relres {r.relres:.2e}", flush=True)
     if args.cg_sweep:
-        for ce in (2, 4, 8, 16):
-            fa.set_tuning(K, cg_elems=ce)
+        for split, ce in ((-1, 8), (1, 2), (1, 4), (1, 8), (1, 16)):
+            fa.set_tuning(K, cg_elems=ce, cg_split=split)
             line = []
             for mode in ("cg", "mixed"):
                 best = 1e30
@@ -81,8 +81,8 @@ def main():
                     r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
                     best = min(best, fa.timings(K)["solve_ms"])
                 line.append(f"{mode} loop {best:.2f} ms ({r.iterations} its, {best / r.iterations * 1e3:.1f} us/it)")
-            print(f"cg elems {ce}: {'; '.join(line)}", flush=True)
-        fa.set_tuning(K, cg_elems=0)
+            print(f"cg split {split} elems {ce}: {'; '.join(line)}", flush=True)
+        fa.set_tuning(K, cg_elems=0, cg_split=0)
     if args.gather_sweep:
         for chunks in (1, 2, 4, 8, 16, 32):
             fa.set_tuning(K, gather_chunks=chunks)
@@ -128,6 +128,18 @@ def main():
                 print(f"operator {lname} {mname}: isolated fp64 {a64 * 1e3:.1f} us, fp32 {a32 * 1e3:.1f} us; "
                       f"in-solve {'; '.join(line)}; graph loops {', '.join(loops)}", flush=True)
         fa.set_tuning(K, lattice=0, sell_map=0, time_kernels=0)
+        for blocks in (256, 384, 512, 640, 1024, 2048):
+            fa.set_tuning(K, lattice_blocks=blocks)
+            a32 = fa.spmv_bench(K, mixed=True, reps=20)
+            fa.set_tuning(K, lattice_blocks=blocks, time_kernels=1)
+            r = fa.solve(K, f, mode="mixed", rtol=1e-10, download=False)
+            t = fa.timings(K)
+            fa.set_tuning(K, lattice_blocks=blocks, time_kernels=0)
+            r = fa.solve(K, f, mode="mixed", rtol=1e-10, download=False)
+            print(f"lattice blocks {blocks}: isolated fp32 {a32 * 1e3:.1f} us; in-solve "
+                  f"{t['spmv_ms'] / max(t['spmv_launches'], 1) * 1e3:.1f} us; graph loop "
+                  f"{fa.timings(K)['solve_ms']:.1f} ms ({r.iterations} its)", flush=True)
+        fa.set_tuning(K, lattice_blocks=0)
     if args.sell_sweep:
         for smap, mname, unrolls in ((1, "grid-stride, ordinary loads", (4, 8, 16)), (2, "contiguous/XCD", (4, 8)),
                                      (3, "grid-stride, non-temporal loads", (4, 8))):

@@ -186,6 +186,10 @@ typedef struct {
                                         per neighbour, no column indices, half the values
                                         (0 = default: mixed solver only; 1: FP64 solver too;
                                         -1: never, the sliced ELL copy) */
+  int32_t lattice_blocks;            /* blocks of the lattice SpMV, a multiple of 8 (0 = default) */
+  int32_t cg_split;                  /* CG step after the SpMV: 0 (default) or 1 = a one-block scalar
+                                        kernel, then the vector move; -1 = one kernel whose every
+                                        block reduces the partials itself */
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
 int32_t fem_get_tuning(const fem_ctx* ctx, fem_tuning* t);
