@@ -24,6 +24,8 @@ def main():
                     help="CG vector step: elements per thread (solve loop times, both solvers)")
     ap.add_argument("--gather-sweep", action="store_true",
                     help="owner-computes assembly: cells loaded per column lane before adding")
+    ap.add_argument("--ir-sweep", action="store_true",
+                    help="mixed solver: reliable-update factor δ (inner_rtol) vs iterations and loop time")
     ap.add_argument("--lattice-sweep", action="store_true",
                     help="lattice vs sliced-ELL solver operator x block mapping: isolated and in-solve SpMV")
     ap.add_argument("--sell-sweep", action="store_true",
@@ -107,6 +109,14 @@ def main():
             print(f"gather group {grp}: fp64 {ts[0]:.3f} ms, fp32-element {ts[1]:.3f} ms", flush=True)
         fa.set_tuning(K, gather_group=0)
         fa.apply(K, f, ch)
+    if args.ir_sweep:
+        for d in (0.3, 0.1, 0.03, 0.01, 0.003):
+            best, r = 1e30, None
+            for _ in range(2):
+                r = fa.solve(K, f, mode="mixed", rtol=1e-10, inner_rtol=d, download=False)
+                best = min(best, fa.timings(K)["solve_ms"])
+            print(f"ir delta {d:g}: {r.iterations} its, {r.outer_iterations} reliable updates, loop {best:.2f} ms, "
+                  f"relres {r.relres:.2e}", flush=True)
     if args.lattice_sweep:
         for lat, lname in ((-1, "sliced ELL"), (1, "lattice")):
             for smap, mname in ((1, "grid-stride"), (2, "XCD ranges")):
