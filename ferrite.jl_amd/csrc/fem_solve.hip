@@ -1482,7 +1482,9 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache
   const void* key[8] = {val, v.x, v.r, v.p, v.w, v.sv, w->s, w->use_lat ? (const void*)w->lat.map : w->sell.off};
   if (c->tune.time_kernels) {   // kernels launched directly (launch_batch), SpMVs bracketed by events
     for (hipEvent_t& e : w->tev)
-      if (!e) FEM_HIP(c, hipEventCreate(&e));
+      // timing-only events: no system-scope fence (its L2 writeback and invalidation would
+      // slow the kernel that follows each event and inflate the measured durations)
+      if (!e) FEM_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     return FEM_OK;
   }
   bool same = gc.exec != nullptr;
