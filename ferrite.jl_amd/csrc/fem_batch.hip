@@ -20,6 +20,8 @@
 // problem and FP64 reliable updates (true residual per problem) every batch of
 // iterations; a problem stops updating once its FP64 residual meets its tolerance,
 // and its completion time is taken from the batch boundary at which that happened.
+// The FP32 operator is a sliced ELL copy whose slices (≤ 64 rows, one wave) never cross
+// a reduction segment, so one block per segment runs its slices and sums its partial.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -89,6 +91,7 @@ struct fem_batch {
   double* d_spart = nullptr;        // [6 nseg] segment partials: γ and ||r||² (two buffers of 2
                                     // per segment), then δ / the hand-over partials (2 per segment)
   int nseg = 0;
+  int64_t nslices = 0, blen = 0;
   int32_t* d_aseg = nullptr;        // [nseg] segments of problems in the recurrence
   int32_t* d_hseg = nullptr;        // [nseg] segments of problems being handed over
   int32_t* d_nlist = nullptr;       // [2] lengths of the two lists
@@ -102,7 +105,16 @@ struct fem_batch {
   int graph_cpw = 0, graph_chunk = 0, graph_prec = -1;
   // persistent solver work (allocated at finalize)
   double* d_m = nullptr;            // [nprob] meandiag per problem
-  float* d_val32 = nullptr;         // [nnz] FP32 operator
+  // FP32 operator as slices of ≤ 64 rows inside each segment (a segment's rows are its first
+  // slices' rows; entry k of the slice's row l at boff[slice] + 64 k + l, padding value 0)
+  int32_t* d_bsl = nullptr;         // [nseg+1] first slice of each segment
+  int32_t* d_slrow = nullptr;       // [nslices] first row of each slice
+  int32_t* d_boff = nullptr;        // [nslices+1] first entry of each slice
+  int32_t* d_bbase = nullptr;       // [nslices] smallest column of the slice
+  uint16_t* d_broff = nullptr;      // [blen] column − base (when every slice spans < 65536 columns)
+  int32_t* d_bcol = nullptr;        // [blen] column (otherwise)
+  int32_t* d_bsrc = nullptr;        // [blen] position in nzval, −1 on padding
+  float* d_bval = nullptr;          // [blen] s_i K_ij s_j
   // FP32 recurrence of the scaled system Â = S K S: x̂, r̂, w = Â r̂, p, s = Â p
   float *d_x32 = nullptr, *d_r32 = nullptr, *d_w32 = nullptr, *d_p32 = nullptr, *d_s32 = nullptr;
   double* d_sc = nullptr;           // [ndofs] S = diag(K)^-1/2
@@ -393,8 +405,8 @@ struct BScal {
 // kernel) sum them in segment order.  Fixed order throughout (deterministic).
 constexpr int kSeg = 256;
 constexpr int kSegThreads = 256;     // vector kernels: one thread per row of a segment
-constexpr int kSpmvThreads = 1024;   // SpMV kernels: 4 lanes per row of a segment
-static_assert(kSpmvThreads == 4 * kSeg, "one SpMV pass covers a segment");
+constexpr int kSpmvThreads = 1024;   // FP64 residual kernel: 4 lanes per row of a segment
+static_assert(kSpmvThreads == 4 * kSeg, "one residual pass covers a segment");
 
 // Sum over the kSpmvThreads threads of a block, valid in every thread; `sh` holds 16.
 __device__ __forceinline__ double spmv_block_sum(double v, double* sh) {
@@ -416,49 +428,94 @@ __device__ __forceinline__ double seg_block_sum(double v, double* sh) {
   return (sh[0] + sh[1]) + (sh[2] + sh[3]);   // valid in every thread
 }
 
-// w = Â r̂ on one segment per block (4 lanes per row, K symmetric: column j is row j), and
-// the segment's partial δ = r̂·w.  list/nlist: the segments to run; blocks past the list's
-// length return.  Problems held or right after a reseed (fresh) are skipped.
-__global__ __launch_bounds__(kSpmvThreads) void k_bc_spmv(const int32_t* __restrict__ list,
+// w = Â r̂ on one segment per block from the sliced operator (one wave per slice of the
+// segment, lane = row, 8 entries per lane loaded before their r̂ gathers; the operator is
+// streamed with non-temporal loads so that r̂ stays in L2), and the segment's partial
+// δ = r̂·w.  list/nlist: the segments to run; blocks past the list's length return.
+// Problems held or right after a reseed (fresh) are skipped.
+template <bool COMP>
+__global__ __launch_bounds__(kSegThreads) void k_bc_sell(const int32_t* __restrict__ list,
                                                          const int32_t* __restrict__ nlist, int which,
                                                          const int32_t* __restrict__ sprob,
                                                          const int64_t* __restrict__ srow,
-                                                         const int32_t* __restrict__ colptr,
-                                                         const int32_t* __restrict__ rowval,
+                                                         const int32_t* __restrict__ bsl,
+                                                         const int32_t* __restrict__ slrow,
+                                                         const int32_t* __restrict__ boff,
+                                                         const int32_t* __restrict__ bbase,
+                                                         const uint16_t* __restrict__ broff,
+                                                         const int32_t* __restrict__ bcol,
                                                          const float* __restrict__ val, const float* __restrict__ r,
                                                          float* __restrict__ w, const BScal* __restrict__ sin,
                                                          double* __restrict__ part_d) {
-  __shared__ double sh[kSpmvThreads / 64];
+  __shared__ double sh[4];
   if ((int)blockIdx.x >= nlist[which]) return;
   const int seg = list[blockIdx.x];
   const BScal& S = sin[sprob[seg]];
   if (S.done || S.hold || S.fresh) return;
-  const int64_t j = srow[seg] + (threadIdx.x >> 2);   // one pass: kSeg rows x 4 lanes
-  const int sub = threadIdx.x & 3;
+  const int sl = bsl[seg] + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   double d = 0.0;
-  float acc = 0.f;
-  const bool in = j < srow[seg + 1];
-  if (in) {
-    // entries sub, sub + 4, ... of the row, two loads in flight (fixed order per lane)
-    const int32_t q1 = colptr[j + 1];
-    int32_t q = colptr[j] + sub;
-    for (; q + 4 < q1; q += 8) {
-      const float v0 = val[q], v1 = val[q + 4];
-      const int32_t c0 = rowval[q], c1 = rowval[q + 4];
-      acc += v0 * r[c0];
-      acc += v1 * r[c1];
+  if (sl < bsl[seg + 1]) {
+    const int64_t row = (int64_t)slrow[sl] + lane;
+    const int32_t o0 = boff[sl];
+    const int W = (boff[sl + 1] - o0) >> 6;
+    const int32_t b = COMP ? bbase[sl] : 0;
+    const int32_t e0 = o0 + lane;
+    constexpr int U = 8;
+    float acc = 0.f;
+    int k = 0;
+    for (; k + U <= W; k += U) {
+      float v[U];
+      int32_t ci[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t e = e0 + 64 * (k + u);
+        v[u] = __builtin_nontemporal_load(val + e);
+        ci[u] = COMP ? b + (int32_t)__builtin_nontemporal_load(broff + e) : __builtin_nontemporal_load(bcol + e);
+      }
+      float xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = r[ci[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc += v[u] * xv[u];
     }
-    if (q < q1) acc += val[q] * r[rowval[q]];
+    for (; k < W; ++k) {
+      const int32_t e = e0 + 64 * k;
+      const int32_t ci = COMP ? b + (int32_t)__builtin_nontemporal_load(broff + e) : __builtin_nontemporal_load(bcol + e);
+      acc += __builtin_nontemporal_load(val + e) * r[ci];
+    }
+    if (row < srow[seg + 1]) {
+      w[row] = acc;
+      d = (double)r[row] * (double)acc;
+    }
   }
-  // the 4 lanes of a row are adjacent and share `in`: the shuffles stay inside the row
-  acc += __shfl_down(acc, 2, 4);
-  acc += __shfl_down(acc, 1, 4);
-  if (in && sub == 0) {
-    w[j] = acc;
-    d = (double)r[j] * (double)acc;
-  }
-  const double t = spmv_block_sum(d, sh);
+  const double t = seg_block_sum(d, sh);
   if (threadIdx.x == 0) part_d[seg] = t;
+}
+
+// Scaled values of the sliced operator: val = s_row K[row, col] s_col (the product order of
+// the single-problem copy), 0 on padding; one wave per slice.
+template <bool COMP>
+__global__ void k_bc_sell_vals(int64_t nsl, int64_t n, const int32_t* __restrict__ slrow,
+                               const int32_t* __restrict__ boff, const int32_t* __restrict__ bbase,
+                               const uint16_t* __restrict__ broff, const int32_t* __restrict__ bcol,
+                               const int32_t* __restrict__ bsrc, const double* __restrict__ nzval,
+                               const double* __restrict__ sc, float* __restrict__ val) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t sl = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; sl < nsl;
+       sl += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+    const int64_t row = (int64_t)slrow[sl] + lane;
+    const double sr = row < n ? sc[row] : 0.0;
+    const int32_t o0 = boff[sl];
+    const int W = (boff[sl + 1] - o0) >> 6;
+    const int32_t b = COMP ? bbase[sl] : 0;
+    for (int k = 0; k < W; ++k) {
+      const int32_t e = o0 + 64 * k + lane;
+      const int32_t q = bsrc[e];
+      const int32_t col = COMP ? b + (int32_t)broff[e] : bcol[e];
+      val[e] = q >= 0 ? (float)(nzval[q] * sr * sc[col]) : 0.f;
+    }
+  }
 }
 
 // The fused vector step on one segment per block.  Every block of a problem sums that
@@ -759,18 +816,6 @@ __global__ void k_bc_scale(int64_t n, const int32_t* __restrict__ diag, const do
   }
 }
 
-// val = s_i K_ij s_j in FP32, 4 lanes per column (grid-stride over column groups).
-__global__ void k_bc_scaled_vals(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
-                                 const double* __restrict__ nzval, const double* __restrict__ sc,
-                                 float* __restrict__ val) {
-  const int sub = threadIdx.x & 3;
-  for (int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 2; j < n;
-       j += ((int64_t)gridDim.x * blockDim.x) >> 2) {
-    const double sj = sc[j];
-    for (int32_t q = colptr[j] + sub; q < colptr[j + 1]; q += 4) val[q] = (float)(nzval[q] * sj * sc[rowval[q]]);
-  }
-}
-
 __global__ void k_bc_zero32(int64_t n, float* __restrict__ a) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     a[i] = 0.f;
@@ -918,6 +963,20 @@ int timed_assemble(fem_batch* B, int cpw, int chunk, bool mixed, float* ms) {
 // scalar record 0 and partial buffer 0.
 constexpr int kCgBatch = 16;
 
+// The sliced SpMV over `blocks` entries of a segment list (which = 0: recurrence list,
+// 1: hand-over list).
+void launch_bc_sell(fem_batch* B, int blocks, const int32_t* list, int which, const BScal* sin, double* part) {
+  fem_ctx* c = B->c;
+  if (B->d_broff)
+    k_bc_sell<true><<<blocks, kSegThreads, 0, c->stream>>>(list, B->d_nlist, which, B->d_sprob, B->d_srow, B->d_bsl,
+                                                           B->d_slrow, B->d_boff, B->d_bbase, B->d_broff, nullptr,
+                                                           B->d_bval, B->d_r32, B->d_w32, sin, part);
+  else
+    k_bc_sell<false><<<blocks, kSegThreads, 0, c->stream>>>(list, B->d_nlist, which, B->d_sprob, B->d_srow, B->d_bsl,
+                                                            B->d_slrow, B->d_boff, B->d_bbase, nullptr, B->d_bcol,
+                                                            B->d_bval, B->d_r32, B->d_w32, sin, part);
+}
+
 hipGraphExec_t* cg_graph_for(fem_batch* B, int width) {
   for (auto& g : B->cg_graphs)
     if (g.first == width) return &g.second;
@@ -938,8 +997,7 @@ int ensure_cg_graph(fem_batch* B, int width) {
   for (int it = 0; it < kCgBatch; ++it) {
     BScal* sin = s + (it & 1) * np_;
     BScal* sout = s + ((it + 1) & 1) * np_;
-    k_bc_spmv<<<width, kSpmvThreads, 0, st>>>(B->d_aseg, B->d_nlist, 0, B->d_sprob, B->d_srow, c->d_colptr,
-                                               c->d_rowval, B->d_val32, B->d_r32, B->d_w32, sin, pd);
+    launch_bc_sell(B, width, B->d_aseg, 0, sin, pd);
     k_bc_step<<<width, kSegThreads, 0, st>>>(B->d_aseg, B->d_nlist, B->d_sprob, B->d_srow, B->d_pseg, B->d_x32,
                                                B->d_r32, B->d_w32, B->d_p32, B->d_s32, B->d_dsq, pd, pg[it & 1],
                                                pg[(it + 1) & 1], sin, sout);
@@ -988,7 +1046,8 @@ void fem_batch_destroy(fem_batch* B) {
   dev_free(&B->d_slotprob); dev_free(&B->d_items); dev_free(&B->d_done); dev_free(&B->d_q);
   dev_free(&B->d_probs); dev_free(&B->d_rowprob); dev_free(&B->d_pdof); dev_free(&B->d_tab); dev_free(&B->d_tabf);
   dev_free(&B->d_sprob); dev_free(&B->d_srow); dev_free(&B->d_pseg); dev_free(&B->d_spart);
-  dev_free(&B->d_m); dev_free(&B->d_val32); dev_free(&B->d_x32); dev_free(&B->d_r32); dev_free(&B->d_p32);
+  dev_free(&B->d_m); dev_free(&B->d_bsl); dev_free(&B->d_slrow); dev_free(&B->d_boff); dev_free(&B->d_bbase);
+  dev_free(&B->d_broff); dev_free(&B->d_bcol); dev_free(&B->d_bsrc); dev_free(&B->d_bval); dev_free(&B->d_x32); dev_free(&B->d_r32); dev_free(&B->d_p32);
   dev_free(&B->d_w32); dev_free(&B->d_s32); dev_free(&B->d_sc); dev_free(&B->d_dsq); dev_free(&B->d_res); dev_free(&B->d_scal);
   dev_free(&B->d_aseg); dev_free(&B->d_hseg); dev_free(&B->d_nlist); dev_free(&B->d_handing);
   if (c) fem_ctx_destroy(c);
@@ -1169,7 +1228,7 @@ int32_t fem_batch_finalize(fem_batch* B, const double* quad_N, const double* qua
       (rc = dev_alloc(c, &B->d_q, 1)) || (rc = dev_alloc(c, &B->d_probs, np_)) ||
       (rc = dev_alloc(c, &B->d_rowprob, nd)) || (rc = dev_alloc(c, &B->d_pdof, np_ + 1)) ||
       (rc = dev_alloc(c, &B->d_tab, kTabSize)) || (rc = dev_alloc(c, &B->d_tabf, kTabSize)) ||
-      (rc = dev_alloc(c, &B->d_m, np_)) || (rc = dev_alloc(c, &B->d_val32, nz)) ||
+      (rc = dev_alloc(c, &B->d_m, np_)) ||
       (rc = dev_alloc(c, &B->d_x32, nd)) || (rc = dev_alloc(c, &B->d_r32, nd)) ||
       (rc = dev_alloc(c, &B->d_p32, nd)) || (rc = dev_alloc(c, &B->d_w32, nd)) ||
       (rc = dev_alloc(c, &B->d_s32, nd)) || (rc = dev_alloc(c, &B->d_sc, nd)) ||
@@ -1218,6 +1277,70 @@ int32_t fem_batch_finalize(fem_batch* B, const double* quad_N, const double* qua
     FEM_HIP(c, hipMemcpyAsync(B->d_pseg, pseg.data(), 4 * pseg.size(), hipMemcpyHostToDevice, st));
     FEM_HIP(c, hipStreamSynchronize(st));
     B->h_pseg = pseg;
+    // sliced operator: slices of ≤ 64 rows inside each segment
+    std::vector<int32_t> bsl(B->nseg + 1), slrow, slend, boff(1, 0), bbase;
+    int64_t blen = 0;
+    int32_t span = 0;
+    for (int sg = 0; sg < B->nseg; ++sg) {
+      bsl[sg] = (int32_t)slrow.size();
+      for (int64_t r0 = srow[sg]; r0 < srow[sg + 1]; r0 += 64) {
+        const int64_t r1 = std::min<int64_t>(r0 + 64, srow[sg + 1]);
+        int32_t W = 0, lo = INT32_MAX, hi = 0;
+        for (int64_t r = r0; r < r1; ++r) {
+          W = std::max(W, cp[r + 1] - cp[r]);
+          for (int32_t q = cp[r]; q < cp[r + 1]; ++q) {
+            lo = std::min(lo, rv[q]);
+            hi = std::max(hi, rv[q]);
+          }
+        }
+        if (lo > hi) lo = hi = (int32_t)r0;
+        slrow.push_back((int32_t)r0);
+        slend.push_back((int32_t)r1);
+        bbase.push_back(lo);
+        span = std::max(span, hi - lo);
+        blen += 64 * (int64_t)W;
+        if (blen >= INT32_MAX) return set_error(c, FEM_ERR_ARG, "batch too large for the sliced operator");
+        boff.push_back((int32_t)blen);
+      }
+    }
+    bsl[B->nseg] = (int32_t)slrow.size();
+    const int64_t nsl = (int64_t)slrow.size();
+    const bool comp = span < 65536;
+    std::vector<int32_t> bsrc(blen, -1), bcol(comp ? 0 : blen);
+    std::vector<uint16_t> broff(comp ? blen : 0);
+    for (int64_t k = 0; k < nsl; ++k) {
+      const int64_t r0 = slrow[k], r1 = slend[k];
+      const int32_t W = (boff[k + 1] - boff[k]) / 64;
+      for (int l = 0; l < 64; ++l) {
+        const int64_t r = r0 + l;
+        for (int32_t e = 0; e < W; ++e) {
+          const int64_t at = boff[k] + 64 * (int64_t)e + l;
+          const int32_t q = r < r1 ? cp[r] + e : -1;
+          const bool real = r < r1 && q < cp[r + 1];
+          bsrc[at] = real ? q : -1;   // rows past the slice and short rows: padding
+          const int32_t col = real ? rv[q] : bbase[k];
+          if (comp) broff[at] = (uint16_t)(col - bbase[k]);
+          else bcol[at] = col;
+        }
+      }
+    }
+    if ((rc = dev_alloc(c, &B->d_bsl, B->nseg + 1)) || (rc = dev_alloc(c, &B->d_slrow, nsl)) ||
+        (rc = dev_alloc(c, &B->d_boff, nsl + 1)) || (rc = dev_alloc(c, &B->d_bbase, nsl)) ||
+        (rc = dev_alloc(c, &B->d_bsrc, blen)) || (rc = dev_alloc(c, &B->d_bval, blen)))
+      return rc;
+    if (comp) rc = dev_alloc(c, &B->d_broff, blen);
+    else rc = dev_alloc(c, &B->d_bcol, blen);
+    if (rc) return rc;
+    FEM_HIP(c, hipMemcpyAsync(B->d_bsl, bsl.data(), 4 * bsl.size(), hipMemcpyHostToDevice, st));
+    FEM_HIP(c, hipMemcpyAsync(B->d_slrow, slrow.data(), 4 * nsl, hipMemcpyHostToDevice, st));
+    FEM_HIP(c, hipMemcpyAsync(B->d_boff, boff.data(), 4 * boff.size(), hipMemcpyHostToDevice, st));
+    FEM_HIP(c, hipMemcpyAsync(B->d_bbase, bbase.data(), 4 * nsl, hipMemcpyHostToDevice, st));
+    FEM_HIP(c, hipMemcpyAsync(B->d_bsrc, bsrc.data(), 4 * blen, hipMemcpyHostToDevice, st));
+    if (comp) FEM_HIP(c, hipMemcpyAsync(B->d_broff, broff.data(), 2 * blen, hipMemcpyHostToDevice, st));
+    else FEM_HIP(c, hipMemcpyAsync(B->d_bcol, bcol.data(), 4 * blen, hipMemcpyHostToDevice, st));
+    FEM_HIP(c, hipStreamSynchronize(st));
+    B->nslices = nsl;
+    B->blen = blen;
   }
   B->nslots = ns;
   B->finalized = true;
@@ -1284,7 +1407,15 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
   // starts with a hand-over from u = 0)
   double* ph = B->d_spart + 4 * B->nseg;   // hand-over partials (2 per segment)
   k_bc_scale<<<grid_n(n), 256, 0, st>>>(n, c->d_diag, c->d_nzval, B->d_sc, B->d_dsq);
-  k_bc_scaled_vals<<<grid_n(n), 256, 0, st>>>(n, c->d_colptr, c->d_rowval, c->d_nzval, B->d_sc, B->d_val32);
+  {
+    const int vb = (int)std::min<int64_t>(8192, (B->nslices + 3) / 4);
+    if (B->d_broff)
+      k_bc_sell_vals<true><<<vb, 256, 0, st>>>(B->nslices, n, B->d_slrow, B->d_boff, B->d_bbase, B->d_broff, nullptr,
+                                               B->d_bsrc, c->d_nzval, B->d_sc, B->d_bval);
+    else
+      k_bc_sell_vals<false><<<vb, 256, 0, st>>>(B->nslices, n, B->d_slrow, B->d_boff, B->d_bbase, nullptr, B->d_bcol,
+                                                B->d_bsrc, c->d_nzval, B->d_sc, B->d_bval);
+  }
   k_b_zero<<<grid_n(n), 256, 0, st>>>(n, c->d_u);
   k_bc_zero32<<<grid_n(n), 256, 0, st>>>(n, B->d_x32);
   k_bc_zero32<<<grid_n(n), 256, 0, st>>>(n, B->d_p32);
@@ -1332,8 +1463,7 @@ int32_t fem_batch_run(fem_batch* B, const fem_batch_cfg* cfg, fem_batch_info* in
     k_bc_reseed<<<len, kSegThreads, 0, st>>>(B->d_hseg, B->d_nlist, B->d_sprob, B->d_srow, B->d_sc, B->d_res,
                                              B->d_r32, B->d_p32, d_s, ph);
     k_bc_reseed_fin<<<np_, 64, 0, st>>>(np_, B->d_pseg, ph, B->d_handing, d_s);
-    k_bc_spmv<<<len, kSpmvThreads, 0, st>>>(B->d_hseg, B->d_nlist, 1, B->d_sprob, B->d_srow, c->d_colptr, c->d_rowval,
-                                           B->d_val32, B->d_r32, B->d_w32, d_s, ph);
+    launch_bc_sell(B, len, B->d_hseg, 1, d_s, ph);
     k_bc_pfix<<<len, kSegThreads, 0, st>>>(B->d_hseg, B->d_nlist, B->d_sprob, B->d_srow, B->d_r32, B->d_w32,
                                            B->d_p32, B->d_s32, d_s, ph);
     k_bc_alpha<<<np_, 64, 0, st>>>(np_, B->d_pseg, ph, B->d_handing, d_s);
