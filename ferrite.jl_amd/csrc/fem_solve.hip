@@ -506,6 +506,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_vals(int64_t n, int64_t nslices
 // --------------------------------------------------------------------------
 struct LatGeom {
   uint32_t px, pxy, ofs;       // ofs = 1 + px + pxy: shifts every neighbour offset to [0, 2 ofs]
+  int ns;                      // planes: forward codes 13 .. 12 + ns, backward 14 − ns .. 12
   int64_t npad;
 };
 
@@ -562,7 +563,9 @@ __global__ __launch_bounds__(kBlock) void k_lat_fill(int64_t r_lo, int64_t nrows
       const int32_t i = rowval[p];
       const int64_t Pi = map[i];
       const int code = lat_code(Pi - P, g);
-      if (code < 0) {
+      // a neighbour outside the used axes (dz ≠ 0 in 2D) cannot occur for lattice rows; it
+      // is counted as a non-neighbour rather than written past the planes
+      if (code < 0 || code - 13 >= g.ns || 13 - code >= g.ns) {
         bad += 1.0;
         continue;
       }
@@ -1198,7 +1201,7 @@ static int ensure_lat_vals(fem_ctx* c, SolverWork* w, int jacobi, bool* sym) {
     ver = 0;
   }
   if (ver != c->nzval_version + 1 || jac != jacobi) {
-    const LatGeom g{(uint32_t)l.px, (uint32_t)l.pxy, (uint32_t)(1 + l.px + l.pxy), l.npad};
+    const LatGeom g{(uint32_t)l.px, (uint32_t)l.pxy, (uint32_t)(1 + l.px + l.pxy), l.ns, l.npad};
     const int64_t rows_per_block = kBlock / kLatLanes;
     const int fb = (int)std::min<int64_t>(16384, (l.nrows + rows_per_block - 1) / rows_per_block);
     FEM_HIP(c, hipMemsetAsync(l.cnt, 0, 4 * sizeof(unsigned long long), c->stream));
