@@ -14,14 +14,22 @@
 // iteration, so the fused vector step moves 9 instead of 12 vectors' worth of bytes.
 //
 // K is symmetric after apply! (rows and columns of prescribed dofs are zeroed together),
-// so the CSC columns double as rows: the solver's operator is a sliced ELL copy (slices of
-// 64 rows, one wave each) built from the columns.  The FP64 residual f − K u uses the CSC
-// arrays themselves (CSR-stream kernel).
+// so the CSC columns double as rows.  The solver's operator is one of two copies:
+//  * lattice planes, when the dofs of a scalar Q1 field sit on a tensor-product lattice
+//    (found from the cell topology, fem_lattice.hip) and every entry couples lattice
+//    neighbours: the centre and the forward neighbours' values per padded lattice row, no
+//    column indices; the backward neighbours read their mirror's plane (checked symmetric
+//    at every fill).  The mixed solver's default; the recurrence vectors then live in the
+//    padded lattice numbering (map / dofp translate at the hand-overs).
+//  * a sliced ELL copy (slices of 64 rows, one wave each) built from the columns, for
+//    everything else and for the FP64 solver by default.
+// The FP64 residual f − K u uses the CSC arrays themselves (CSR-stream kernel).
 //
 // Reductions are fence-free and fixed-order: blocks write partials that the next kernel
-// sums, every block in the same order, so results are bitwise reproducible and the host
-// never sits in the iteration.  A batch of iterations is captured once into a hipGraph and
-// replayed; kernels launched past a hand-over return at once on the device flag.
+// sums (by default one block, k_cg_scal), in the same order every time, so results are
+// bitwise reproducible and the host never sits in the iteration.  A batch of iterations is
+// captured once into a hipGraph and replayed; kernels launched past a hand-over return at
+// once on the device flag.
 #include <cmath>
 #include <cstring>
 
