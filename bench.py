@@ -124,6 +124,7 @@ def main():
         fa.set_tuning(K, time_kernels=0)
         spmv_ms, spmv_n = ts["spmv_ms"], ts["spmv_launches"]
         results[name] = dict(ms=dt * 1e3, cells_per_s=g.ncells / dt, iters=iters, u=u, operator=res.operator,
+                             unit_diagonal=res.unit_diagonal,
                              asm_kernel_ms=float(np.median(asm_ms)), spmv_avg_ms=spmv_ms / max(spmv_n, 1),
                              spmv_launches=spmv_n)
         log(f"{name}: {dt * 1e3:.1f} ms/step, iters {iters[-1]}, assembly kernels {np.median(asm_ms):.3f} ms, "
@@ -169,11 +170,14 @@ def main():
     spmv_ms = results["mixed"]["spmv_avg_ms"]               # inside the CG loop of the timed steps
     nnz, nd = K.nnz, dh.ndofs
     if results["mixed"]["operator"] == "lattice":
-        SPMV32_KERNEL = "k_lat_spmv<float, float, 14, 1>"   # lattice planes, XCD-range rows
+        # lattice planes, XCD-range rows; with the Jacobi scaling the diagonal is all ones and
+        # the centre plane is not read
+        unit = results["mixed"]["unit_diagonal"]
+        SPMV32_KERNEL = f"k_lat_spmv<float, float, 14, 1, {'true' if unit else 'false'}>"
         px = n + 3                                           # n + 1 nodes per axis, one padding node each side
         npad, omax = px ** 3, 1 + px + px * px
         nrows = npad - 2 * omax
-        spmv_bytes = nrows * (14 * 4 + 4) + 4 * npad
+        spmv_bytes = nrows * ((13 if unit else 14) * 4 + 4) + 4 * npad
     else:
         SPMV32_KERNEL = "k_sell<float, float, 0, true, 8, false, true>"   # sliced ELL, u16 offsets, NT loads
         spmv_bytes = nnz * (4 + 4) + 4 * (nd + 1) + 4 * nd + 4 * nd

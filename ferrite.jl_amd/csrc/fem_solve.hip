@@ -69,7 +69,7 @@ struct SpmvShape {
 
 struct GraphCache {
   hipGraphExec_t exec = nullptr;
-  const void* key[8] = {};
+  const void* key[9] = {};   // operands and the kernel variant the graph was captured with
 };
 
 // Sliced ELL copy of K for the solver (slices of 64 rows = one wave): entry k of row
@@ -119,6 +119,7 @@ struct LatOp {
   uint64_t ver32 = 0, ver64 = 0;   // nzval_version the planes were filled from (+1)
   int jac32 = -1, jac64 = -1;
   bool sym32 = false, sym64 = false;   // the last fill found Â symmetric
+  bool unit32 = false, unit64 = false; // ... and its diagonal all ones
   unsigned long long* cnt = nullptr;   // [4] fill counters
   int grid = 0;
 };
@@ -547,7 +548,8 @@ __device__ __forceinline__ void count2(double a, double b, unsigned long long* c
 //   i a backward neighbour:                     mir[12 − code][P(i)] = v, i.e. where row P(i)
 //                                               stores K[j, i] in val[13 − code][P(i)].
 // Â is symmetric iff planes 1.. of val equal mir (k_lat_cmp).  cnt[1] counts entries that
-// are not lattice neighbours.
+// are not lattice neighbours, cnt[0] diagonal values other than 1 (with the Jacobi scaling
+// every diagonal value is 1 after rounding, and the SpMV need not read the centre plane).
 constexpr int kLatLanes = 32;
 
 template <typename V>
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(kBlock) void k_lat_fill(int64_t r_lo, int64_t nrows
   __shared__ double sh[kBlock / 64];
   const int lane = threadIdx.x % kLatLanes;
   const int64_t groups = (int64_t)gridDim.x * (kBlock / kLatLanes);
-  double bad = 0.0;
+  double bad = 0.0, nonunit = 0.0;
   for (int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kLatLanes; q < nrows; q += groups) {
     const int64_t P = r_lo + q;
     const int32_t j = dofp[P];
@@ -580,9 +582,10 @@ __global__ __launch_bounds__(kBlock) void k_lat_fill(int64_t r_lo, int64_t nrows
       const V v = (V)(nzval[p] * sr * sc[i]);
       if (code >= 13) val[(int64_t)(code - 13) * g.npad + P] = v;
       else mir[(int64_t)(12 - code) * g.npad + Pi] = v;
+      if (code == 13 && v != V(1)) nonunit += 1.0;
     }
   }
-  count2(0.0, bad, cnt, 0, sh);
+  count2(nonunit, bad, cnt, 0, sh);
 }
 
 // Symmetry check: a[k] vs b[k] for k < m (the forward planes against their mirrors) to tol
@@ -612,7 +615,7 @@ __device__ __forceinline__ T at_byte(const T* __restrict__ base, uint32_t boff) 
 // host).  MAP 0: grid-stride over all rows; MAP 1 (gridDim.x a multiple of 8): the blocks of
 // one XCD (b ≡ x mod 8) sweep one contiguous eighth of the rows in order, so the rows a
 // backward neighbour reads were loaded by the same XCD shortly before.
-template <typename V, typename X, int NS, int MAP>
+template <typename V, typename X, int NS, int MAP, bool UNIT>
 __global__ __launch_bounds__(kBlock) void k_lat_spmv(int64_t r_lo, int64_t nrows, int64_t npad, LatOffsets lo,
                                                      const V* __restrict__ val, const X* __restrict__ x,
                                                      X* __restrict__ y, double* __restrict__ part,
@@ -636,7 +639,7 @@ __global__ __launch_bounds__(kBlock) void k_lat_spmv(int64_t r_lo, int64_t nrows
     const uint32_t r = (uint32_t)(r_lo + i);
     V v[2 * NS - 1];
     X xv[2 * NS - 1];
-    v[0] = at_byte(val, r * (uint32_t)sizeof(V));
+    v[0] = UNIT ? V(1) : at_byte(val, r * (uint32_t)sizeof(V));   // UNIT: every diagonal value is 1
     xv[0] = at_byte(x, r * (uint32_t)sizeof(X));
 #pragma unroll
     for (int k = 1; k < NS; ++k) {
@@ -1229,6 +1232,7 @@ static int ensure_lat_vals(fem_ctx* c, SolverWork* w, int jacobi, bool* sym) {
       return FEM_OK;
     }
     ok = h[3] == 0;    // every forward value equals its mirror (a missing mirror leaves a zero)
+    (sizeof(V) == 4 ? l.unit32 : l.unit64) = h[0] == 0;
     ver = c->nzval_version + 1;
     jac = jacobi;
   }
@@ -1239,10 +1243,18 @@ static int ensure_lat_vals(fem_ctx* c, SolverWork* w, int jacobi, bool* sym) {
 template <typename V, typename X, int NS>
 static void launch_lat_ns(fem_ctx* c, const LatOp& l, const V* val, const X* x, X* y, double* part,
                           const Scal* sc) {
-  if (c->tune.sell_map != 1 && l.grid % 8 == 0)
-    k_lat_spmv<V, X, NS, 1><<<l.grid, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.npad, l.off, val, x, y, part, sc);
-  else
-    k_lat_spmv<V, X, NS, 0><<<l.grid, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.npad, l.off, val, x, y, part, sc);
+  const bool xcd = c->tune.sell_map != 1 && l.grid % 8 == 0;
+  const bool unit = sizeof(V) == 4 ? l.unit32 : l.unit64;
+#define FEM_LAT(MAP, UNIT) \
+  k_lat_spmv<V, X, NS, MAP, UNIT><<<l.grid, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.npad, l.off, val, x, y, part, sc)
+  if (xcd) {
+    if (unit) FEM_LAT(1, true);
+    else FEM_LAT(1, false);
+  } else {
+    if (unit) FEM_LAT(0, true);
+    else FEM_LAT(0, false);
+  }
+#undef FEM_LAT
 }
 
 template <typename X>
@@ -1490,7 +1502,10 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache
   static_assert(kBatch % 2 == 0, "batch must end on scalar record 0");
   const void* val = w->use_lat ? (sizeof(X) == 4 ? (const void*)w->lat.val32 : (const void*)w->lat.val64)
                                : (sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64);
-  const void* key[8] = {val, v.x, v.r, v.p, v.w, v.sv, w->s, w->use_lat ? (const void*)w->lat.map : w->sell.off};
+  // the lattice SpMV variant depends on the values (a unit diagonal), not only on the operands
+  const uintptr_t variant = w->use_lat ? 1 + (sizeof(X) == 4 ? w->lat.unit32 : w->lat.unit64) : 0;
+  const void* key[9] = {val, v.x, v.r, v.p, v.w, v.sv, w->s, w->use_lat ? (const void*)w->lat.map : w->sell.off,
+                        reinterpret_cast<const void*>(variant)};
   if (c->tune.time_kernels) {   // kernels launched directly (launch_batch), SpMVs bracketed by events
     for (hipEvent_t& e : w->tev)
       // timing-only events: no system-scope fence (its L2 writeback and invalidation would
@@ -1499,7 +1514,7 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache
     return FEM_OK;
   }
   bool same = gc.exec != nullptr;
-  for (int i = 0; i < 8 && same; ++i) same = gc.key[i] == key[i];
+  for (int i = 0; i < 9 && same; ++i) same = gc.key[i] == key[i];
   if (same) return FEM_OK;
   destroy_graph(gc);
   hipGraph_t graph = nullptr;
@@ -1509,7 +1524,7 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache
   hipError_t e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return check_hip(c, e, "hipGraphInstantiate");
-  for (int i = 0; i < 8; ++i) gc.key[i] = key[i];
+  for (int i = 0; i < 9; ++i) gc.key[i] = key[i];
   return FEM_OK;
 }
 
@@ -1657,7 +1672,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   if (rc && rc != FEM_ERR_NOCONV) return rc;
   out.iterations = iters;
   out.outer_iterations = handovers;
-  out.operator_format = w->use_lat ? 1 : 0;
+  out.operator_format = !w->use_lat ? 0 : (mixed ? w->lat.unit32 : w->lat.unit64) ? 2 : 1;
   if ((rc = read_scal(c, w, &h))) return rc;   // s->rr: the true residual of the returned u
   out.relres = std::sqrt(h.rr / bnorm2);
   c->timings.solve_ms = cg_ms;

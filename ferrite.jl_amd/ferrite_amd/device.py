@@ -224,7 +224,8 @@ class SolveResult:
         self.iterations = info.iterations
         self.outer_iterations = info.outer_iterations
         self.relres = info.relres
-        self.operator = "lattice" if info.operator_format == 1 else "sell"
+        self.operator = "lattice" if info.operator_format in (1, 2) else "sell"
+        self.unit_diagonal = info.operator_format == 2
         self.u = u
 
 

@@ -132,7 +132,9 @@ typedef struct {
   int32_t iterations;        /* total CG iterations */
   int32_t outer_iterations;  /* mixed: reliable updates; FP64: 1 + residual replacements */
   double relres;             /* final true relative residual ‖f − K u‖/‖f‖ in FP64 */
-  int32_t operator_format;   /* the CG operator copy used: 0 sliced ELL, 1 lattice planes */
+  int32_t operator_format;   /* the CG operator copy used: 0 sliced ELL, 1 lattice planes,
+                                2 lattice planes whose (scaled) diagonal is all ones, so the
+                                SpMV does not read the centre plane */
   int32_t reserved;
 } fem_solve_info;
 

@@ -52,8 +52,11 @@ def test_lattice_operator_parity(ct, nel, right):
     g, dh, K, f = _system(ct, nel, right, k)
     u_ref = _oracle_u(ct, nel, right, k)
     lat = fa.solve(K, f, mode="mixed", rtol=1e-12)
-    assert lat.operator == "lattice"
+    assert lat.operator == "lattice" and lat.unit_diagonal   # Jacobi-scaled: the centre plane is not read
     assert lat.relres <= 1e-12 and _rel(lat.u, u_ref) < 1e-8
+    plain = fa.solve(K, f, mode="mixed", rtol=1e-12, precond="none")   # unscaled: the centre plane is read
+    assert plain.operator == "lattice" and not plain.unit_diagonal
+    assert plain.relres <= 1e-12 and _rel(plain.u, u_ref) < 1e-8
     fa.set_tuning(K, lattice=1)
     lat64 = fa.solve(K, f, mode="cg", rtol=1e-12)
     assert lat64.operator == "lattice"
