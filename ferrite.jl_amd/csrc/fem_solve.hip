@@ -858,14 +858,36 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, X* __restrict__ x
   }
 }
 
-// Split form of the step (the default; tuning cg_split): k_cg_scal reduces the partials once (one block,
-// fixed order) and writes the step's scalars; k_cg_move then updates the vectors with many
-// small blocks (E elements per thread), which lets later blocks load while earlier ones
-// store.  Same recurrence as k_cg_step.
-__global__ __launch_bounds__(kWide) void k_cg_scal(int nin, const double* __restrict__ part_in, int na,
-                                                  const double* __restrict__ part_s, const Scal* __restrict__ sin,
-                                                  Scal* __restrict__ sout) {
-  __shared__ double sh[kWide / 64];
+// Σ a[0..m) and Σ b[0..mb) by one wave in a fixed order: lane l takes i ≡ l (mod 64) into
+// eight accumulators (i / 64 mod 8), which are added pairwise, then a butterfly across lanes.
+__device__ __forceinline__ void wave_sum2(const double* __restrict__ a, int m, const double* __restrict__ b,
+                                          int mb, double& sa, double& sb) {
+  const int lane = threadIdx.x & 63;
+  double ua[8] = {}, ub[8] = {};
+  const int mm = m > mb ? m : mb;
+  for (int i0 = 0; i0 < mm; i0 += 8 * 64) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = i0 + 64 * k + lane;
+      ua[k] += i < m ? a[i] : 0.0;
+      ub[k] += i < mb ? b[i] : 0.0;
+    }
+  }
+  sa = ((ua[0] + ua[1]) + (ua[2] + ua[3])) + ((ua[4] + ua[5]) + (ua[6] + ua[7]));
+  sb = ((ub[0] + ub[1]) + (ub[2] + ub[3])) + ((ub[4] + ub[5]) + (ub[6] + ub[7]));
+  for (int o = 32; o > 0; o >>= 1) {
+    sa += __shfl_xor(sa, o, 64);
+    sb += __shfl_xor(sb, o, 64);
+  }
+}
+
+// Split form of the step (the default; tuning cg_split): k_cg_scal reduces the partials once
+// (one wave, fixed order) and writes the step's scalars; k_cg_move then updates the vectors
+// with many small blocks (E elements per thread), which lets later blocks load while earlier
+// ones store.  Same recurrence as k_cg_step.
+__global__ __launch_bounds__(64) void k_cg_scal(int nin, const double* __restrict__ part_in, int na,
+                                               const double* __restrict__ part_s, const Scal* __restrict__ sin,
+                                               Scal* __restrict__ sout) {
   const Scal S = *sin;
   if (S.done) {
     if (threadIdx.x == 0) *sout = S;
@@ -883,8 +905,8 @@ __global__ __launch_bounds__(kWide) void k_cg_scal(int nin, const double* __rest
     }
     return;
   }
-  const double gam = wide_sum(part_in, nin, 1, sh);
-  const double del = wide_sum(part_s, na, 1, sh);
+  double gam, del;
+  wave_sum2(part_in, nin, part_s, na, gam, del);
   if (threadIdx.x == 0) {
     Scal o = S;
     const double beta = gam / S.rzp;
@@ -1473,7 +1495,7 @@ static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int
   // FP64 loop 89.7 vs 92.4 ms for the one-kernel step)
   if (c->tune.cg_split != -1) {
     const int mb = cg_blocks(c, v.n);
-    k_cg_scal<<<1, kWide, 0, c->stream>>>(mb, pin, na, w->part_s, sin, sout);
+    k_cg_scal<<<1, 64, 0, c->stream>>>(mb, pin, na, w->part_s, sin, sout);
     switch (cg_elems(c)) {
       case 2: k_cg_move<X, 2><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
       case 4: k_cg_move<X, 4><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;

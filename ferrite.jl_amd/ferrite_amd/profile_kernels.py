@@ -74,7 +74,7 @@ def main():
             r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
             print(f"phase solve {mode}: {r.iterations} iterations, relres {r.relres:.2e}", flush=True)
     if args.cg_sweep:
-        for split, ce in ((-1, 8), (1, 2), (1, 4), (1, 8), (1, 16)):
+        for split, ce in ((-1, 8), (0, 4), (0, 8), (0, 16)):
             fa.set_tuning(K, cg_elems=ce, cg_split=split)
             line = []
             for mode in ("cg", "mixed"):

@@ -64,6 +64,12 @@ def test_heat_equation_kat(precision, tol):
     assert abs(np.linalg.norm(res.u) - 3.307743912641305) / 3.307743912641305 < (1e-10 if precision == "fp64" else 1e-5)
     assert _relerr(res.u, u_ref) < (1e-10 if precision == "fp64" else 1e-5)
     assert m > 0
+    # the mixed solver on the lattice operator (FP32 planes, FP64 reliable updates) meets the
+    # same KAT
+    mix = fa.solve(K, f, mode="mixed", rtol=1e-13)
+    assert mix.operator == "lattice"
+    assert _relerr(mix.u, res.u) < 1e-10
+    assert abs(np.linalg.norm(mix.u) - 3.307743912641305) / 3.307743912641305 < (1e-10 if precision == "fp64" else 1e-5)
 
 
 _cache = {}
