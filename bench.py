@@ -5,9 +5,11 @@ on 'left' ∪ 'right', and a CG solve to ‖f − K u‖/‖f‖ ≤ 1e-10, with
 in HBM.  Two forms are timed back to back on the same FP64-assembled K:
   * fp64  : FP64 Jacobi-PCG on the solver's general sliced-ELL copy of K
   * mixed : FP32-SpMV Jacobi-PCG (FP32 vectors, FP64 dot products) with FP64 iterative
-            refinement by reliable updates (true residual f − K u in FP64), on the lattice
-            operator form (value planes per neighbour) the structured mesh allows
-The FP64 solver on the lattice form is timed too and reported beside (`fp64_lattice`).
+            refinement by reliable updates (true residual f − K u in FP64), on the cell form of
+            the operator the structured mesh of congruent cells allows (per-cell k_e times one
+            element matrix, used after an entry-by-entry check against K)
+Timed too and reported beside: the mixed solver on the lattice value planes (`mixed_lattice`),
+the FP64 solver on the planes (`fp64_lattice`) and on the cell form (`fp64_cells`).
 `value` is the mixed form's cells/s (the headline); the FP64 form's cells/s, their
 ratio and ‖u_mixed − u_fp64‖/‖u_fp64‖ are reported beside it.  The FP32-element-math
 assembly variant is timed separately with its K and u errors (an FP32 Ke moves u by
@@ -132,21 +134,30 @@ def main():
 
     uerr = float(np.linalg.norm(results["mixed"]["u"] - results["fp64"]["u"]) / np.linalg.norm(results["fp64"]["u"]))
 
-    # context, not the headline: the FP64 solver on the same lattice operator form the mixed
-    # solver uses (the plain FP64 solver above uses the general sliced copy)
-    fa.set_tuning(K, lattice=1)
-    for _ in range(args.warmup):
-        step("cg")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        res64l, _, _ = step("cg")
-    torch.cuda.synchronize()
-    dt64l = (time.perf_counter() - t1) / args.steps
-    fa.set_tuning(K, lattice=0)
-    fp64_lattice = {"cells_per_s": g.ncells / dt64l, "ms_per_step": dt64l * 1e3, "operator": res64l.operator,
-                    "cg_iterations": res64l.iterations, "relres": res64l.relres}
-    log(f"fp64 on the lattice operator: {dt64l * 1e3:.1f} ms/step, {res64l.iterations} iterations")
+    def context(name, mode, **tuning):
+        """Context, not the headline: one solver on another operator form (same K, same steps),
+        with its SpMV's in-solve launch duration from one more step with timed SpMVs."""
+        fa.set_tuning(K, **tuning)
+        for _ in range(args.warmup):
+            step(mode)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            r, _, _ = step(mode)
+        torch.cuda.synchronize()
+        dtc = (time.perf_counter() - t1) / args.steps
+        fa.set_tuning(K, time_kernels=1)
+        _, _, ts = step(mode)
+        fa.set_tuning(K, lattice=0, cell_operator=0, time_kernels=0)
+        log(f"{name}: {dtc * 1e3:.1f} ms/step, {r.iterations} iterations, operator {r.operator}, SpMV "
+            f"{ts['spmv_ms'] / max(ts['spmv_launches'], 1) * 1e3:.1f} us")
+        return {"cells_per_s": g.ncells / dtc, "ms_per_step": dtc * 1e3, "operator": r.operator,
+                "cg_iterations": r.iterations, "relres": r.relres,
+                "spmv_avg_ms": ts["spmv_ms"] / max(ts["spmv_launches"], 1), "unit_diagonal": r.unit_diagonal}
+
+    mixed_lattice = context("mixed on the lattice planes", "mixed", cell_operator=-1)
+    fp64_lattice = context("fp64 on the lattice planes", "cg", lattice=1, cell_operator=-1)
+    fp64_cells = context("fp64 on the cell form", "cg", cell_operator=1)
 
     # FP32 element math with FP64 accumulation: K error and the u it leads to (reported beside)
     fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision="fp64")
@@ -162,26 +173,34 @@ def main():
     fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision="fp64")
     fa.apply(K, f, ch)
 
-    # roofline of the dominant kernel: the FP32 SpMV of the mixed solver.  Lattice operator:
-    # algorithmic bytes of its format (14 FP32 value planes over the computed rows, x over the
-    # padded lattice, y over the computed rows; the backward neighbours re-read planes another
-    # row already read).  Sliced ELL: y = K x with int32 indices (values + indices + colptr + x + y).
-    spmv_iso_ms = fa.spmv_bench(K, mixed=True, reps=50)   # back to back, matrix partly cache-resident
-    spmv_ms = results["mixed"]["spmv_avg_ms"]               # inside the CG loop of the timed steps
+    # roofline of the dominant kernel: the FP32 SpMV of the mixed solver, algorithmic bytes of its
+    # operator form.  Cell form (z-marching): t = S and x read over the padded lattice, k_e over
+    # it (one per cell origin row), y written over the marched planes.  Lattice planes: 14 FP32
+    # value planes over the computed rows (13 with the unit diagonal, whose centre plane is not
+    # read), x over the padded lattice, y over the computed rows; backward neighbours re-read
+    # planes another row already read.  Sliced ELL: y = K x with int32 indices.
+    spmv_iso_ms = fa.spmv_bench(K, mixed=True, reps=50)   # back to back, operator partly cache-resident
     nnz, nd = K.nnz, dh.ndofs
-    if results["mixed"]["operator"] == "lattice":
-        # lattice planes, XCD-range rows; with the Jacobi scaling the diagonal is all ones and
-        # the centre plane is not read
-        unit = results["mixed"]["unit_diagonal"]
-        SPMV32_KERNEL = f"k_lat_spmv<float, float, 14, 1, {'true' if unit else 'false'}>"
-        px = n + 3                                           # n + 1 nodes per axis, one padding node each side
-        npad, omax = px ** 3, 1 + px + px * px
-        nrows = npad - 2 * omax
-        spmv_bytes = nrows * ((13 if unit else 14) * 4 + 4) + 4 * npad
+    px = n + 3                                               # n + 1 nodes per axis, one padding node each side
+    npad, omax = px ** 3, 1 + px + px * px
+
+    def lattice_roofline(unit, ms):
+        kern = f"k_lat_spmv<float, float, 14, 1, {'true' if unit else 'false'}>"
+        nbytes = (npad - 2 * omax) * ((13 if unit else 14) * 4 + 4) + 4 * npad
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return kern, nbytes, gbs
+
+    spmv_ms = results["mixed"]["spmv_avg_ms"]               # inside the CG loop of the timed steps
+    if results["mixed"]["operator"] == "cells":
+        SPMV32_KERNEL = "k_cell_spmv_z<float, 1, 4, 3>"
+        spmv_bytes = 3 * 4 * npad + 4 * (px - 2) * px * px
+    elif results["mixed"]["operator"] == "lattice":
+        SPMV32_KERNEL, spmv_bytes, _ = lattice_roofline(results["mixed"]["unit_diagonal"], spmv_ms)
     else:
         SPMV32_KERNEL = "k_sell<float, float, 0, true, 8, false, true>"   # sliced ELL, u16 offsets, NT loads
         spmv_bytes = nnz * (4 + 4) + 4 * (nd + 1) + 4 * nd + 4 * nd
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    lat_kernel, lat_bytes, lat_gbs = lattice_roofline(mixed_lattice["unit_diagonal"], mixed_lattice["spmv_avg_ms"])
     spmv64_iso_ms = fa.spmv_bench(K, mixed=False, reps=50)
     spmv64_ms = results["fp64"]["spmv_avg_ms"]
     spmv64_bytes = nnz * (8 + 4) + 4 * (nd + 1) + 8 * nd + 8 * nd
@@ -217,8 +236,11 @@ def main():
                   "cg_iterations": results["mixed"]["iters"][-1][0],
                   "refinements": results["mixed"]["iters"][-1][1], "relres": results["mixed"]["iters"][-1][2]},
         "mixed_speedup_vs_fp64": results["fp64"]["ms"] / results["mixed"]["ms"],
+        "mixed_lattice": mixed_lattice,
         "fp64_lattice": fp64_lattice,
+        "fp64_cells": fp64_cells,
         "mixed_speedup_vs_fp64_lattice": fp64_lattice["ms_per_step"] / results["mixed"]["ms"],
+        "mixed_speedup_vs_fp64_cells": fp64_cells["ms_per_step"] / results["mixed"]["ms"],
         "u_rel_err_mixed_vs_fp64": uerr,
         "fp32_element_assembly": {"kernel_ms": asm32_ms, "K_rel_err_vs_fp64": kerr32,
                                   "u_rel_err_vs_fp64": uerr32},
@@ -228,6 +250,11 @@ def main():
                      "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(SPMV32_KERNEL, n),
                      "bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms,
                      "launches": results["mixed"]["spmv_launches"], "isolated_avg_launch_ms": spmv_iso_ms},
+        # the same workload's SpMV on the lattice value planes (the mixed_lattice run)
+        "roofline_lattice": {"bound": "hbm", "kernel": lat_kernel, "achieved": lat_gbs, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": lat_gbs / HBM_PEAK_GBS,
+                             "traffic": measured_traffic(lat_kernel, n), "bytes_per_launch": lat_bytes,
+                             "avg_launch_ms": mixed_lattice["spmv_avg_ms"]},
         "spmv_fp64": {"avg_launch_ms": spmv64_ms, "isolated_avg_launch_ms": spmv64_iso_ms,
                       "bytes_per_launch": spmv64_bytes,
                       "achieved_GBs": spmv64_bytes / (spmv64_ms * 1e-3) / 1e9},

@@ -320,6 +320,7 @@ int32_t fem_cellvalues_upload(fem_ctx* c, int32_t rdim, int32_t nqp, int32_t nbf
   c->rdim = rdim; c->nqp = nqp; c->nbf = nbf; c->ngeo = ngeo; c->vdim = vdim;
   c->have_cv = true;
   c->tab_dirty = true;
+  ++c->aux_version;
   return FEM_OK;
 }
 
@@ -334,6 +335,7 @@ int32_t fem_cellparam_upload(fem_ctx* c, int64_t ncells, const double* param) {
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   c->have_param = true;
+  ++c->aux_version;
   return FEM_OK;
 }
 

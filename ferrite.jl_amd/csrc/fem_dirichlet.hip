@@ -181,6 +181,7 @@ int32_t fem_constraints_upload(fem_ctx* c, int64_t np, const int64_t* prescribed
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   c->np = np;
   c->any_inhom = inhom;
+  ++c->aux_version;
   return FEM_OK;
 }
 

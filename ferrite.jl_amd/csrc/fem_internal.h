@@ -71,6 +71,7 @@ struct fem_ctx {
   int32_t* d_diag = nullptr;           // position of K[j,j] in column j (-1 if absent)
   double* d_nzval = nullptr;
   uint64_t nzval_version = 1;          // bumped whenever nzval changes (solver operator copies track it)
+  uint64_t aux_version = 1;            // bumped when cell parameters, cell values or constraints change
 
   // colors: cells permuted into color-major order on the device
   int ncolors = 0;

@@ -122,13 +122,58 @@ struct LatOp {
   bool unit32 = false, unit64 = false; // ... and its diagonal all ones
   unsigned long long* cnt = nullptr;   // [4] fill counters
   int grid = 0;
+  std::vector<int32_t> hmap;   // host copy of map
+};
+
+// Cell-coefficient form of Â on the lattice (the element-by-element operator).  When every
+// cell is a congruent copy of cell 0, K = Σ_c k_c K¹ (K¹ cell 0's element matrix for a unit
+// coefficient, k_c the per-cell parameter) with the Dirichlet rows and columns zeroed, so
+//   (Â x)_P = t_P Σ_{cells c ∋ P} k_c Σ_b K¹[a_c(P)][b] t_{Q_c(b)} x_{Q_c(b)},
+// t = S on free rows and 0 on prescribed rows and padding; a prescribed row keeps Â's unit
+// diagonal, (Â x)_P = x_P.  The solver uses this form only after k_cell_check found every
+// entry of the scaled K equal to it; it then reads one coefficient per cell and the scaling
+// instead of the value planes.  K¹ is row-major over Ferrite's local corner order.
+struct CellKref {
+  double k[64];
+};
+// The SpMV's K¹ for box cells: K¹[a][b] = f[|Δx| + 2 |Δy| + 4 |Δz|] of the corners' offsets
+// (eight values in 3D, four in 2D; the host checks K¹ has this form).
+template <typename X>
+struct CellKrefT {
+  X f[8];
+};
+
+struct CellOp {
+  int state = 0;               // 0 not tried, 1 cell origins placed, −1 not for this mesh
+  int dim = 0;
+  int32_t* corg = nullptr;     // [ncells] padded row of each cell's first corner
+  double* kc64 = nullptr;      // [npad] k_c at the origin row of cell c, 0 where no cell starts
+  float* kc32 = nullptr;
+  double* t64 = nullptr;       // [npad] s on free rows, 0 on prescribed rows and padding
+  float* t32 = nullptr;
+  double x0[8][3] = {};        // corners of cell 0
+  CellKref kref{};             // K¹ of the last check
+  double f[8] = {};            // its values per corner-offset class (box cells)
+  uint64_t ver32 = 0, ver64 = 0, aux32 = 0, aux64 = 0;   // versions the last check saw (+1)
+  int jac32 = -1, jac64 = -1;
+  bool ok32 = false, ok64 = false;   // the last check passed
+  unsigned long long* cnt = nullptr; // [4] check counters
+  int lds32 = 0, lds64 = 0;    // dynamic LDS of the SpMV (bytes)
+  int ne = 0;                  // window entries per thread of the SpMV
+  // 3D: the z-marching SpMV (zne > 0): entries per thread, in-plane tiles; per precision the
+  // blocks, planes per chunk and work items
+  int zne = 0, ntx = 0;
+  int grid32 = 0, grid64 = 0, lz32 = 0, lz64 = 0;
+  int64_t nitems32 = 0, nitems64 = 0;
 };
 
 struct SolverWork {
   int64_t n = 0;
   SellOp sell;
   LatOp lat;
-  bool use_lat = false;        // the current solve runs on the lattice operator
+  CellOp cells;
+  bool use_lat = false;        // the current solve runs on a lattice operator (padded vectors)
+  bool use_cells = false;      // ... in its cell-coefficient form
   int64_t cap32 = 0, cap64 = 0;   // length of the recurrence vectors
   int sell_grid = 0;           // blocks of the SELL SpMV (fixed: its partial count)
   SpmvShape sh64;              // CSR-stream FP64 residual
@@ -167,6 +212,9 @@ void free_solver(fem_ctx* c) {
   LatOp& l = w->lat;
   dev_free(&l.map); dev_free(&l.dofp); dev_free(&l.val32); dev_free(&l.val64); dev_free(&l.cnt);
   dev_free(&l.mir32); dev_free(&l.mir64);
+  CellOp& cl = w->cells;
+  dev_free(&cl.corg); dev_free(&cl.kc64); dev_free(&cl.kc32); dev_free(&cl.t64); dev_free(&cl.t32);
+  dev_free(&cl.cnt);
   delete w;
   c->solver = nullptr;
 }
@@ -655,6 +703,354 @@ __global__ __launch_bounds__(kBlock) void k_lat_spmv(int64_t r_lo, int64_t nrows
     for (int k = 1; k < 2 * NS - 1; ++k) acc += (A)v[k] * (A)xv[k];
     y[r] = (X)acc;
     d += (double)xv[0] * (double)(X)acc;
+  }
+  const double b = block_sum(d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
+// ---- cell-coefficient form of Â on the lattice ----
+// Local index of the corner at offset (ex, ey, ez) ∈ {0,1}³ from a cell's first corner, in
+// Ferrite's corner order ((0,0,0) (1,0,0) (1,1,0) (0,1,0), then the same at z = 1).
+__host__ __device__ constexpr int cell_lid(int ex, int ey, int ez) {
+  return ez * 4 + (ey ? 3 - ex : ex);
+}
+
+// kc[corg[e]] = param[e] (FP64 and FP32 copies); rows where no cell starts keep 0.
+__global__ void k_cell_coef(int64_t ncells, const int32_t* __restrict__ corg, const double* __restrict__ param,
+                            double* __restrict__ kc64, float* __restrict__ kc32) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < ncells; e += (int64_t)gridDim.x * blockDim.x) {
+    const double k = param[e];
+    kc64[corg[e]] = k;
+    kc32[corg[e]] = (float)k;
+  }
+}
+
+// t[P] = s of the row's dof, 0 on prescribed dofs (isc) and on padding.
+template <typename X>
+__global__ void k_cell_scale(int64_t npad, const int32_t* __restrict__ dofp, const double* __restrict__ sc,
+                             const uint8_t* __restrict__ isc, X* __restrict__ t) {
+  for (int64_t P = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; P < npad; P += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t j = dofp[P];
+    t[P] = (j >= 0 && !(isc && isc[j])) ? (X)sc[j] : X(0);
+  }
+}
+
+// Entry-by-entry check of the scaled K against the cell form, in FP64, kLatLanes lanes per
+// padded row (column j of the CSC arrays, as the plane fill).  Â's diagonal is 1, so tol is
+// relative to it.  cnt[0] entries that differ by more than tol, cnt[1] lattice-neighbour
+// entries in the pattern, cnt[2] lattice neighbours of the rows (the cell form couples every
+// two corners of a cell, so each must be a pattern entry); other entries must hold zero.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nrows, const int32_t* __restrict__ dofp,
+                                                       const int32_t* __restrict__ colptr,
+                                                       const int32_t* __restrict__ rowval,
+                                                       const double* __restrict__ nzval, const double* __restrict__ sc,
+                                                       const uint8_t* __restrict__ isc, const int32_t* __restrict__ map,
+                                                       LatGeom g, const double* __restrict__ kc, CellKref kr,
+                                                       double tol, unsigned long long* __restrict__ cnt) {
+  __shared__ double sh[kBlock / 64];
+  constexpr int NB = 1 << DIM;
+  const int lane = threadIdx.x % kLatLanes;
+  const int64_t groups = (int64_t)gridDim.x * (kBlock / kLatLanes);
+  const int64_t px = g.px, pxy = g.pxy;
+  double bad = 0.0, seen = 0.0, want = 0.0;
+  for (int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kLatLanes; q < nrows; q += groups) {
+    const int64_t P = r_lo + q;
+    const int32_t j = dofp[P];
+    if (j < 0) continue;
+    if (lane == 0)
+      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) want += dofp[P + dx + px * dy + pxy * dz] >= 0 ? 1.0 : 0.0;
+    const double sj = sc[j];
+    const bool pj = isc && isc[j];
+    const int32_t p1 = colptr[j + 1];
+    for (int32_t p = colptr[j] + lane; p < p1; p += kLatLanes) {
+      const int32_t i = rowval[p];
+      const double a = nzval[p] * sj * sc[i];
+      const int code = lat_code((int64_t)map[i] - P, g);
+      if (code < 0 || (DIM == 2 && code / 9 != 1)) {
+        if (!(fabs(a) <= tol)) bad += 1.0;
+        continue;
+      }
+      seen += 1.0;
+      const int dx = code % 3 - 1, dy = (code / 3) % 3 - 1, dz = code / 9 - 1;
+      double m = 0.0;
+      if (pj || (isc && isc[i])) {
+        m = i == j ? 1.0 : 0.0;
+      } else {
+        for (int e = 0; e < NB; ++e) {   // the cells with origin P − e that hold both corners
+          const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+          const int bx = ex + dx, by = ey + dy, bz = ez + dz;
+          if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+          m += kc[P - ex - px * ey - pxy * ez] * kr.k[cell_lid(ex, ey, ez) * NB + cell_lid(bx, by, bz)];
+        }
+        m *= sj * sc[i];
+      }
+      if (!(fabs(a - m) <= tol)) bad += 1.0;
+    }
+  }
+  count2(bad, seen, cnt, 0, sh);
+  count2(want, 0.0, cnt, 2, sh);
+}
+
+// w = Â x in the cell form, rows [r_lo, r_lo + nrows) of the padded numbering, one row per
+// thread in tiles of kBlock consecutive rows; partial x·w per block.  Per tile the block
+// stages in LDS, for each z-plane offset, the window of t·x its rows' neighbours span
+// (kBlock + 2 px + 2 entries) and the windows of the coefficients of the cells around them
+// (kBlock + px + 1 entries per cell layer), so every row reads its 27 (9) neighbours and 8 (4)
+// cells from LDS.  The windows are one index range of E entries, at most NE per thread; a
+// thread issues all its loads of a tile before it stores any of them, and K¹ is read from LDS
+// (so that the row arithmetic keeps few registers and many waves stay resident to hide the
+// loads' latency).  MAP 1 (gridDim.x a multiple of 8): the blocks of one XCD (b ≡ x mod 8)
+// sweep one contiguous eighth of the tiles in order, so the window of the plane below was
+// loaded into that XCD's
+// L2 shortly before.  Fixed summation order (deterministic).
+template <typename X, int DIM, int MAP, int NE>
+__global__ __launch_bounds__(kBlock) void k_cell_spmv(int64_t r_lo, int64_t nrows, int64_t npad, int32_t px,
+                                                      int32_t pxy, const X* __restrict__ kc, const X* __restrict__ t,
+                                                      const X* __restrict__ x, X* __restrict__ y,
+                                                      double* __restrict__ part, const Scal* __restrict__ s,
+                                                      CellKrefT<X> kr) {
+  if (s->done || s->fresh) return;
+  extern __shared__ __align__(16) unsigned char cell_lds[];
+  __shared__ double sh[kBlock / 64];
+  constexpr int NB = 1 << DIM;
+  constexpr int NZW = DIM == 3 ? 3 : 1;    // t·x windows: planes z − 1, z, z + 1
+  constexpr int NKW = DIM == 3 ? 2 : 1;    // coefficient windows: cell layers z, z − 1
+  X* zw = reinterpret_cast<X*>(cell_lds);
+  const int W = kBlock + 2 * px + 2, WK = kBlock + px + 1;
+  const int EZ = NZW * W, E = EZ + NKW * WK;   // E ≤ NE kBlock (checked by the host)
+  X* kw = zw + EZ;
+  // offsets from the tile's first row of this thread's window entries (tile independent)
+  int32_t rel[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int idx = threadIdx.x + k * kBlock;
+    if (idx < EZ) {
+      const int w = idx / W;
+      rel[k] = (DIM == 3 ? (w - 1) * pxy : 0) - px - 1 + (idx - w * W);
+    } else {
+      const int i2 = idx - EZ, w = i2 / WK;
+      rel[k] = -w * pxy - px - 1 + (i2 - w * WK);
+    }
+  }
+  const int64_t ntiles = (nrows + kBlock - 1) / kBlock;
+  int64_t t0, t1, tstep;
+  if (MAP == 1) {
+    const int64_t xcd = blockIdx.x % 8;
+    t0 = xcd * ntiles / 8 + blockIdx.x / 8;
+    t1 = (xcd + 1) * ntiles / 8;
+    tstep = gridDim.x / 8;
+  } else {
+    t0 = blockIdx.x;
+    t1 = ntiles;
+    tstep = gridDim.x;
+  }
+  double d = 0.0;
+  // rows and byte offsets are 32-bit (8 npad < 2^32: checked by the host), loads take a
+  // scalar base and a 32-bit offset (no 64-bit address per load in VGPRs)
+  const uint32_t np32 = (uint32_t)npad, rend32 = (uint32_t)(r_lo + nrows);
+  for (int64_t tile = t0; tile < t1; tile += tstep) {
+    const uint32_t R0 = (uint32_t)(r_lo + tile * kBlock);   // every window starts at or after row 0
+    X va[NE], vb[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int idx = threadIdx.x + k * kBlock;
+      const uint32_t q = R0 + (uint32_t)rel[k];
+      va[k] = X(0);
+      vb[k] = X(1);
+      if (idx < E && q < np32) {
+        if (idx < EZ) {
+          va[k] = at_byte(t, q * (uint32_t)sizeof(X));
+          vb[k] = at_byte(x, q * (uint32_t)sizeof(X));
+        } else {
+          va[k] = at_byte(kc, q * (uint32_t)sizeof(X));
+        }
+      }
+    }
+    const uint32_t P = R0 + threadIdx.x;
+    const X tc = P < rend32 ? at_byte(t, P * (uint32_t)sizeof(X)) : X(0);
+    const X xc = P < rend32 ? at_byte(x, P * (uint32_t)sizeof(X)) : X(0);
+    __syncthreads();   // the previous tile's LDS reads are done
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int idx = threadIdx.x + k * kBlock;
+      if (idx < E) zw[idx] = va[k] * vb[k];
+    }
+    __syncthreads();
+    if (P < rend32) {
+      const int l = threadIdx.x + px + 1;   // row P in the windows
+      X zn[DIM == 3 ? 27 : 9];              // t·x of the neighbours, (dz, dy, dx) row-major
+#pragma unroll
+      for (int dz = 0; dz < (DIM == 3 ? 3 : 1); ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+          for (int dx = 0; dx < 3; ++dx) zn[dz * 9 + dy * 3 + dx] = zw[dz * W + l + (dx - 1) + px * (dy - 1)];
+      X acc = X(0);
+#pragma unroll
+      for (int e = 0; e < NB; ++e) {        // cell with origin P − e; P is its corner e
+        const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+        const X kcell = kw[ez * WK + l - ex - px * ey];
+        X sub = X(0);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {      // corner b of that cell, at offset b − e from P
+          const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
+          sub += kr.f[b ^ e] * zn[(bz - ez + (DIM == 3 ? 1 : 0)) * 9 + (by - ey + 1) * 3 + (bx - ex + 1)];
+        }
+        acc += kcell * sub;
+      }
+      const X yv = tc != X(0) ? tc * acc : xc;
+      y[P] = yv;
+      d += (double)xc * (double)yv;
+    }
+  }
+  const double b = block_sum(d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
+// 3D cell form marching in z: work item (j, c) = in-plane tile j (kBlock consecutive rows of
+// a plane, in-plane offsets j kBlock ..) over the planes [z0, z1) of chunk c.  The block keeps
+// a ring of three t·x windows (planes z − 1, z, z + 1; kBlock + 2 px + 2 entries each) and two
+// coefficient windows (cell layers z, z − 1; kBlock + px + 1 entries) in LDS and loads one new
+// window of each per plane: step p brings the t·x window of plane p and the coefficients of
+// layer p − 1, and its loads (with t, x of the rows of plane p − 1) are issued before the
+// rows of plane p − 2 are computed, so they overlap that arithmetic.  Padding rows of the
+// marched planes are computed too (t = 0 there, so y = x = 0).  Items are ordered chunk-major;
+// MAP 1: the blocks of one XCD sweep one contiguous eighth of the items, so neighbouring tiles
+// (which share their halo rows) run on the same XCD at the same time.
+template <typename X, int MAP, int NE, int D>
+__global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px, int32_t pxy, int32_t pz,
+                                                        int32_t ntx, int32_t lz, int64_t nitems,
+                                                        const X* __restrict__ kc, const X* __restrict__ t,
+                                                        const X* __restrict__ x, X* __restrict__ y,
+                                                        double* __restrict__ part, const Scal* __restrict__ s,
+                                                        CellKrefT<X> kr) {
+  if (s->done || s->fresh) return;
+  extern __shared__ __align__(16) unsigned char cell_lds[];
+  __shared__ double sh[kBlock / 64];
+  const int W = kBlock + 2 * px + 2, WK = kBlock + px + 1;
+  X* zw = reinterpret_cast<X*>(cell_lds);   // [3][W]
+  X* kw = zw + 3 * W;                        // [2][WK]
+  const int E = W + WK;                      // entries of one step, ≤ NE kBlock (checked by the host)
+  // this thread's entries of a step: window offset from the tile's first row of the plane
+  int32_t rel[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int idx = threadIdx.x + k * kBlock;
+    rel[k] = idx < W ? idx - px - 1 : (idx - W) - px - 1 - pxy;   // k entries: layer p − 1
+  }
+  const uint32_t np32 = (uint32_t)npad;
+  int64_t i0, i1, istep;
+  if (MAP == 1) {
+    const int64_t xcd = blockIdx.x % 8;
+    i0 = xcd * nitems / 8 + blockIdx.x / 8;
+    i1 = (xcd + 1) * nitems / 8;
+    istep = gridDim.x / 8;
+  } else {
+    i0 = blockIdx.x;
+    i1 = nitems;
+    istep = gridDim.x;
+  }
+  // D register stages, each holding the loads of one step: stage u serves the steps p with
+  // p − z0 − 1 ≡ u (mod D), so D steps' loads are in flight while a plane is computed
+  X va[D][NE], vb[D][NE], tn[D], xn[D];
+  // step p of tile offset o0 (in-plane) into stage u: t·x window of plane p, coefficients of
+  // layer p − 1, t and x of row o0 + threadIdx.x of plane p − 1 (q wraps below 0 to beyond
+  // npad: zero)
+  auto fetch = [&](int u, int32_t p, int32_t o0) {
+    const uint32_t base = (uint32_t)p * (uint32_t)pxy + (uint32_t)o0;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int idx = threadIdx.x + k * kBlock;
+      const uint32_t q = base + (uint32_t)rel[k];
+      va[u][k] = X(0);
+      vb[u][k] = X(1);
+      if (idx < E && q < np32) {
+        if (idx < W) {
+          va[u][k] = at_byte(t, q * (uint32_t)sizeof(X));
+          vb[u][k] = at_byte(x, q * (uint32_t)sizeof(X));
+        } else {
+          va[u][k] = at_byte(kc, q * (uint32_t)sizeof(X));
+        }
+      }
+    }
+    const uint32_t P = base - (uint32_t)pxy + threadIdx.x;   // wraps beyond npad for p = 0
+    const bool in = o0 + (int32_t)threadIdx.x < pxy && P < np32;
+    tn[u] = in ? at_byte(t, P * (uint32_t)sizeof(X)) : X(0);
+    xn[u] = in ? at_byte(x, P * (uint32_t)sizeof(X)) : X(0);
+  };
+  auto store = [&](int u, int32_t p) {   // z slot p mod 3, coefficient slot (p − 1) mod 2
+    X* zs = zw + (p % 3) * W;
+    X* ks = kw + ((p - 1) & 1) * WK;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int idx = threadIdx.x + k * kBlock;
+      if (idx < W) zs[idx] = va[u][k] * vb[u][k];
+      else if (idx < E) ks[idx - W] = va[u][k];
+    }
+  };
+  double d = 0.0;
+  for (int64_t item = i0; item < i1; item += istep) {
+    const int32_t c = (int32_t)(item / ntx), j = (int32_t)(item - (int64_t)c * ntx);
+    const int32_t z0 = 1 + c * lz, z1 = min(z0 + lz, pz - 1);
+    const int32_t o0 = j * kBlock;
+    fetch(0, z0 - 1, o0);
+    __syncthreads();   // the previous item's LDS reads are done
+    store(0, z0 - 1);
+    fetch(0, z0, o0);
+    store(0, z0);
+#pragma unroll
+    for (int u = 0; u < D; ++u)
+      if (z0 + 1 + u <= z1) fetch(u, z0 + 1 + u, o0);
+    for (int32_t zb = z0; zb < z1; zb += D) {
+#pragma unroll
+      for (int u = 0; u < D; ++u) {
+        const int32_t zp = zb + u;
+        if (zp >= z1) break;
+        const X tc = tn[u], xc = xn[u];   // row of plane zp
+        store(u, zp + 1);
+        __syncthreads();
+        if (zp + 1 + D <= z1) fetch(u, zp + 1 + D, o0);
+        if (o0 + (int32_t)threadIdx.x < pxy) {
+          const int l = threadIdx.x + px + 1;   // the row in the windows
+          const X* zm = zw + ((zp + 2) % 3) * W;   // plane zp − 1
+          const X* z0w = zw + (zp % 3) * W;
+          const X* zpw = zw + ((zp + 1) % 3) * W;
+          X zn[27];
+#pragma unroll
+          for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+              const int o = l + (dx - 1) + px * (dy - 1);
+              zn[dy * 3 + dx] = zm[o];
+              zn[9 + dy * 3 + dx] = z0w[o];
+              zn[18 + dy * 3 + dx] = zpw[o];
+            }
+          const X* k0 = kw + (zp & 1) * WK;         // layer zp
+          const X* k1 = kw + ((zp - 1) & 1) * WK;   // layer zp − 1
+          X acc = X(0);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {             // cell with origin P − e; P is its corner e
+            const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+            const X kcell = (ez ? k1 : k0)[l - ex - px * ey];
+            X sub = X(0);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {           // corner b of that cell, at offset b − e from P
+              const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
+              sub += kr.f[b ^ e] * zn[(bz - ez + 1) * 9 + (by - ey + 1) * 3 + (bx - ex + 1)];
+            }
+            acc += kcell * sub;
+          }
+          const X yv = tc != X(0) ? tc * acc : xc;
+          const uint32_t P = (uint32_t)zp * (uint32_t)pxy + (uint32_t)(o0 + threadIdx.x);
+          *reinterpret_cast<X*>(reinterpret_cast<char*>(y) + P * (uint32_t)sizeof(X)) = yv;
+          d += (double)xc * (double)yv;
+        }
+        __syncthreads();   // this plane's reads are done before the next store
+      }
+    }
   }
   const double b = block_sum(d, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
@@ -1211,6 +1607,7 @@ static int ensure_lattice(fem_ctx* c, SolverWork* w) {
   gr = std::min<int64_t>(gr, (l.nrows + kBlock - 1) / kBlock);
   if (gr >= 8) gr -= gr % 8;
   l.grid = (int)std::max<int64_t>(1, gr);
+  l.hmap = std::move(map);
   l.state = 1;
   return FEM_OK;
 }
@@ -1288,6 +1685,246 @@ static void launch_lat(fem_ctx* c, const LatOp& l, const X* x, X* y, double* par
     if (l.ns == 14) launch_lat_ns<double, double, 14>(c, l, l.val64, x, y, part, sc);
     else launch_lat_ns<double, double, 5>(c, l, l.val64, x, y, part, sc);
   }
+}
+
+// K¹: cell 0's element matrix of the Poisson element for a unit coefficient, from the
+// uploaded CellValues tables, in the element kernels' arithmetic (J_ab = Σ_j x_j[a] dM_j[b],
+// ∇N = dN · J⁻¹, K¹_ij = Σ_q ∇N_i·∇N_j det J w_q).  False when the tables do not describe a
+// Q1 scalar field on Q1 geometry of the lattice's dimension.
+static bool cell_kref(const fem_ctx* c, const double (*x0)[3], int dim, CellKref* out) {
+  const int nb = 1 << dim;
+  if (!c->have_cv || c->rdim != dim || c->nbf != nb || c->ngeo != nb || c->vdim != 1) return false;
+  *out = CellKref{};
+  const RefTables& R = c->ref;
+  for (int q = 0; q < c->nqp; ++q) {
+    double J[3][3] = {}, Ji[3][3] = {};
+    for (int j = 0; j < nb; ++j)
+      for (int a = 0; a < dim; ++a)
+        for (int b = 0; b < dim; ++b) J[a][b] += x0[j][a] * R.dM[q][j][b];
+    double det;
+    if (dim == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      Ji[0][0] = J[1][1] / det; Ji[0][1] = -J[0][1] / det;
+      Ji[1][0] = -J[1][0] / det; Ji[1][1] = J[0][0] / det;
+    } else {
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                   c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      Ji[0][0] = c00 / det; Ji[1][0] = c01 / det; Ji[2][0] = c02 / det;
+      Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+      Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+      Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+      Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+      Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+      Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    }
+    if (!(det > 0.0)) return false;
+    double gN[8][3] = {};
+    for (int i = 0; i < nb; ++i)
+      for (int b = 0; b < dim; ++b)
+        for (int a = 0; a < dim; ++a) gN[i][b] += R.dN[q][i][a] * Ji[a][b];
+    const double dV = det * R.w[q];
+    for (int i = 0; i < nb; ++i)
+      for (int j = i; j < nb; ++j) {
+        double g = 0.0;
+        for (int b = 0; b < dim; ++b) g += gN[i][b] * gN[j][b];
+        out->k[i * nb + j] += g * dV;
+      }
+  }
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < i; ++j) out->k[i * nb + j] = out->k[j * nb + i];
+  return true;
+}
+
+// Launch shape of the cell-form SpMV for values of `bytes` bytes.  Measured at 128³ (in the
+// solve, z-marching, XCD ranges): FP32 768 blocks 22.4 µs, 1024 21.0, 1280 25.0, 1536 23.0,
+// 2048 22.9; FP64 768 29.5 µs, 1024 38.6, 1280 35.9, 1536 33.3, 2048 36.9.  Z-marching: about
+// one work item per block (the planes per chunk follow from the block count).
+static void cell_shape(const fem_ctx* c, const LatOp& l, CellOp* cl, int bytes) {
+  const int64_t want = c->tune.cell_blocks ? c->tune.cell_blocks : (bytes == 4 ? 1024 : 768);
+  int& grid = bytes == 4 ? cl->grid32 : cl->grid64;
+  if (cl->zne > 0) {
+    const int64_t pz = l.npad / l.pxy;
+    int& lz = bytes == 4 ? cl->lz32 : cl->lz64;
+    int64_t& nitems = bytes == 4 ? cl->nitems32 : cl->nitems64;
+    lz = (int)std::max<int64_t>(2, (cl->ntx * (pz - 2) + want - 1) / want);
+    nitems = (int64_t)cl->ntx * ((pz - 2 + lz - 1) / lz);
+    grid = (int)std::max<int64_t>(8, std::min<int64_t>(want, (nitems + 7) / 8 * 8));
+  } else {
+    const int64_t ntiles = (l.nrows + kBlock - 1) / kBlock;
+    int64_t gr = std::min<int64_t>(want, ntiles);
+    if (gr >= 8) gr -= gr % 8;
+    grid = (int)std::max<int64_t>(1, gr);
+  }
+}
+
+// Box cells: K¹[a][b] depends only on the axes along which corners a and b differ.  f[m] =
+// K¹ of corner 0 and the corner at offset bits m; false unless every entry matches its class to
+// 1e-12 of the largest.  K¹ is then replaced by the class form the SpMV applies.
+static bool cell_classes(int dim, CellKref* kr, double* f) {
+  const int nb = 1 << dim;
+  double big = 0.0;
+  for (int m = 0; m < 8; ++m) f[m] = 0.0;
+  for (int m = 0; m < nb; ++m) {
+    f[m] = kr->k[cell_lid(m & 1, (m >> 1) & 1, m >> 2)];   // row of corner 0 = offset (0,0,0)
+    big = std::max(big, std::fabs(f[m]));
+  }
+  for (int a = 0; a < nb; ++a)
+    for (int b = 0; b < nb; ++b) {
+      double& v = kr->k[cell_lid(a & 1, (a >> 1) & 1, a >> 2) * nb + cell_lid(b & 1, (b >> 1) & 1, b >> 2)];
+      if (!(std::fabs(v - f[a ^ b]) <= 1e-12 * big)) return false;
+      v = f[a ^ b];
+    }
+  return true;
+}
+
+// Cell form of Â for the solver in X precision (needs the lattice operator's numbering and
+// the Jacobi scaling): the cell origins once per pattern, then whenever K, the cell
+// parameters, the tables or the constraints changed, the coefficients and the entry check.
+// *ok = true when the check passed (to 2⁻²² of Â's unit diagonal for the FP32 solver, 1e-12
+// for the FP64 one); t, the masked scaling, is refreshed on every call.
+template <typename X>
+static int ensure_cells(fem_ctx* c, SolverWork* w, int jacobi, bool* ok) {
+  *ok = false;
+  CellOp& cl = w->cells;
+  const LatOp& l = w->lat;
+  int rc;
+  if (cl.state == -1 || !jacobi || l.state != 1 || !c->have_param || !c->d_param_cell) return FEM_OK;
+  if (cl.state == 0) {
+    cl.state = -1;
+    const int dim = l.ns == 14 ? 3 : 2;
+    const int nb = 1 << dim;
+    if (c->npc != nb || (int64_t)c->h_cell_dofs.size() != c->ncells * nb) return FEM_OK;
+    // LDS windows of the SpMV: at most 24 entries per thread (FP64: 48 KiB per block)
+    const int64_t words = (dim == 3 ? 3 : 1) * (kBlock + 2 * l.px + 2) + (dim == 3 ? 2 : 1) * (kBlock + l.px + 1);
+    if (words > 24 * kBlock) return FEM_OK;
+    std::vector<int32_t> corg(c->ncells);
+    for (int64_t e = 0; e < c->ncells; ++e) corg[e] = l.hmap[c->h_cell_dofs[e * nb]];
+    for (int a = 0; a < nb; ++a) {
+      const int32_t nd = c->h_cell_nodes[a];
+      for (int k = 0; k < dim; ++k)
+        FEM_HIP(c, hipMemcpy(&cl.x0[a][k], c->d_x[k] + nd, sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if ((rc = dev_alloc(c, &cl.corg, c->ncells)) || (rc = dev_alloc(c, &cl.kc64, l.npad)) ||
+        (rc = dev_alloc(c, &cl.kc32, l.npad)) || (rc = dev_alloc(c, &cl.cnt, 4)))
+      return rc;
+    FEM_HIP(c, hipMemcpyAsync(cl.corg, corg.data(), 4 * c->ncells, hipMemcpyHostToDevice, c->stream));
+    FEM_HIP(c, hipMemsetAsync(cl.kc64, 0, 8 * l.npad, c->stream));
+    FEM_HIP(c, hipMemsetAsync(cl.kc32, 0, 4 * l.npad, c->stream));
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    cl.dim = dim;
+    cl.ne = (int)((words + kBlock - 1) / kBlock);
+    cl.lds32 = (int)(4 * words);
+    cl.lds64 = (int)(8 * words);
+    const int64_t step_words = (kBlock + 2 * l.px + 2) + (kBlock + l.px + 1);
+    if (dim == 3 && step_words <= 12 * kBlock && c->tune.cell_march != -1) {
+      cl.zne = (int)((step_words + kBlock - 1) / kBlock);
+      cl.ntx = (int)((l.pxy + kBlock - 1) / kBlock);
+    }
+    cell_shape(c, l, &cl, 4);
+    cell_shape(c, l, &cl, 8);
+    cl.state = 1;
+  }
+  X*& t = sizeof(X) == 4 ? reinterpret_cast<X*&>(cl.t32) : reinterpret_cast<X*&>(cl.t64);
+  if (!t && (rc = dev_alloc(c, &t, l.npad))) return rc;
+  k_cell_scale<X><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(l.npad, l.dofp, w->sc, c->np > 0 ? c->d_isc : nullptr, t);
+  uint64_t& ver = sizeof(X) == 4 ? cl.ver32 : cl.ver64;
+  uint64_t& aux = sizeof(X) == 4 ? cl.aux32 : cl.aux64;
+  int& jac = sizeof(X) == 4 ? cl.jac32 : cl.jac64;
+  bool& pass = sizeof(X) == 4 ? cl.ok32 : cl.ok64;
+  if (ver != c->nzval_version + 1 || aux != c->aux_version + 1 || jac != jacobi) {
+    pass = false;
+    if (cell_kref(c, cl.x0, cl.dim, &cl.kref) && cell_classes(cl.dim, &cl.kref, cl.f)) {
+      k_cell_coef<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->ncells, cl.corg, c->d_param_cell, cl.kc64, cl.kc32);
+      const LatGeom g{(uint32_t)l.px, (uint32_t)l.pxy, (uint32_t)(1 + l.px + l.pxy), l.ns, l.npad};
+      const int64_t rows_per_block = kBlock / kLatLanes;
+      const int fb = (int)std::min<int64_t>(16384, (l.nrows + rows_per_block - 1) / rows_per_block);
+      const double tol = sizeof(X) == 4 ? 1.0 / (1 << 22) : 1e-12;
+      const uint8_t* isc = c->np > 0 ? c->d_isc : nullptr;
+      FEM_HIP(c, hipMemsetAsync(cl.cnt, 0, 4 * sizeof(unsigned long long), c->stream));
+      if (cl.dim == 3)
+        k_cell_check<3><<<fb, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.dofp, c->d_colptr, c->d_rowval, c->d_nzval,
+                                                      w->sc, isc, l.map, g, cl.kc64, cl.kref, tol, cl.cnt);
+      else
+        k_cell_check<2><<<fb, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.dofp, c->d_colptr, c->d_rowval, c->d_nzval,
+                                                      w->sc, isc, l.map, g, cl.kc64, cl.kref, tol, cl.cnt);
+      unsigned long long h[4] = {};
+      FEM_HIP(c, hipMemcpyAsync(h, cl.cnt, sizeof h, hipMemcpyDeviceToHost, c->stream));
+      FEM_HIP(c, hipStreamSynchronize(c->stream));
+      pass = h[0] == 0 && h[1] == h[2];
+    }
+    ver = c->nzval_version + 1;
+    aux = c->aux_version + 1;
+    jac = jacobi;
+  }
+  *ok = pass;
+  return FEM_OK;
+}
+
+template <typename X, int DIM, int NE>
+static void launch_cells_ne(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc) {
+  const LatOp& l = w->lat;
+  const CellOp& cl = w->cells;
+  CellKrefT<X> kr;
+  for (int i = 0; i < 8; ++i) kr.f[i] = (X)cl.f[i];
+  const X* kc = sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.kc32) : reinterpret_cast<const X*>(cl.kc64);
+  const X* t = sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.t32) : reinterpret_cast<const X*>(cl.t64);
+  const int lds = sizeof(X) == 4 ? cl.lds32 : cl.lds64;
+  const int grid = sizeof(X) == 4 ? cl.grid32 : cl.grid64;
+  if (c->tune.sell_map != 1 && grid % 8 == 0)
+    k_cell_spmv<X, DIM, 1, NE><<<grid, kBlock, lds, c->stream>>>(l.r_lo, l.nrows, l.npad, (int32_t)l.px,
+                                                                 (int32_t)l.pxy, kc, t, x, y, part, sc, kr);
+  else
+    k_cell_spmv<X, DIM, 0, NE><<<grid, kBlock, lds, c->stream>>>(l.r_lo, l.nrows, l.npad, (int32_t)l.px,
+                                                                 (int32_t)l.pxy, kc, t, x, y, part, sc, kr);
+}
+
+template <typename X, int NE>
+static void launch_cells_z(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc) {
+  const LatOp& l = w->lat;
+  const CellOp& cl = w->cells;
+  CellKrefT<X> kr;
+  for (int i = 0; i < 8; ++i) kr.f[i] = (X)cl.f[i];
+  const X* kc = sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.kc32) : reinterpret_cast<const X*>(cl.kc64);
+  const X* t = sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.t32) : reinterpret_cast<const X*>(cl.t64);
+  const int lds = sizeof(X) == 4 ? cl.lds32 : cl.lds64;
+  const int32_t pz = (int32_t)(l.npad / l.pxy);
+  constexpr int D = sizeof(X) == 4 ? 3 : 2;   // steps in flight (register stages)
+  const int grid = sizeof(X) == 4 ? cl.grid32 : cl.grid64, lz = sizeof(X) == 4 ? cl.lz32 : cl.lz64;
+  const int64_t nitems = sizeof(X) == 4 ? cl.nitems32 : cl.nitems64;
+  if (c->tune.sell_map != 1 && grid % 8 == 0)
+    k_cell_spmv_z<X, 1, NE, D><<<grid, kBlock, lds, c->stream>>>(l.npad, (int32_t)l.px, (int32_t)l.pxy, pz, cl.ntx,
+                                                                 lz, nitems, kc, t, x, y, part, sc, kr);
+  else
+    k_cell_spmv_z<X, 0, NE, D><<<grid, kBlock, lds, c->stream>>>(l.npad, (int32_t)l.px, (int32_t)l.pxy, pz, cl.ntx,
+                                                                 lz, nitems, kc, t, x, y, part, sc, kr);
+}
+
+// NE: window entries per thread, the smallest instantiated bound of ⌈E / kBlock⌉
+template <typename X, int DIM>
+static void launch_cells_dim(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc) {
+  if (DIM == 3 && w->cells.zne > 0) {
+    const int zne = w->cells.zne;
+    if (zne <= 4) launch_cells_z<X, 4>(c, w, x, y, part, sc);
+    else if (zne <= 6) launch_cells_z<X, 6>(c, w, x, y, part, sc);
+    else if (zne <= 8) launch_cells_z<X, 8>(c, w, x, y, part, sc);
+    else launch_cells_z<X, 12>(c, w, x, y, part, sc);
+    return;
+  }
+  const int ne = w->cells.ne;
+  if (ne <= 4) launch_cells_ne<X, DIM, 4>(c, w, x, y, part, sc);
+  else if (ne <= 6) launch_cells_ne<X, DIM, 6>(c, w, x, y, part, sc);
+  else if (ne <= 8) launch_cells_ne<X, DIM, 8>(c, w, x, y, part, sc);
+  else if (ne <= 10) launch_cells_ne<X, DIM, 10>(c, w, x, y, part, sc);
+  else if (ne <= 12) launch_cells_ne<X, DIM, 12>(c, w, x, y, part, sc);
+  else if (ne <= 16) launch_cells_ne<X, DIM, 16>(c, w, x, y, part, sc);
+  else launch_cells_ne<X, DIM, 24>(c, w, x, y, part, sc);
+}
+
+template <typename X>
+static void launch_cells(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc) {
+  if (w->cells.dim == 3) launch_cells_dim<X, 3>(c, w, x, y, part, sc);
+  else launch_cells_dim<X, 2>(c, w, x, y, part, sc);
 }
 
 // SELL structure (once per pattern): widths/bases on the device, offsets on the host.
@@ -1412,6 +2049,10 @@ static void launch_sell(fem_ctx* c, const SellOp& o, int g, const V* val, const 
 // number of partials written.
 template <typename X>
 static int spmv_hat(fem_ctx* c, SolverWork* w, const X* x, X* y, const Scal* sc) {
+  if (w->use_cells) {
+    launch_cells<X>(c, w, x, y, w->part_s, sc);
+    return sizeof(X) == 4 ? w->cells.grid32 : w->cells.grid64;
+  }
   if (w->use_lat) {
     launch_lat<X>(c, w->lat, x, y, w->part_s, sc);
     return w->lat.grid;
@@ -1437,10 +2078,17 @@ static int prepare_operator(fem_ctx* c, SolverWork* w, int jacobi, int64_t* nv) 
   int rc;
   k_scale<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_diag, c->d_nzval, jacobi, w->sc);
   w->use_lat = false;
+  w->use_cells = false;
   const bool want = sizeof(X) == 4 ? c->tune.lattice != -1 : c->tune.lattice == 1;
-  if (want) {
+  const bool want_cells = sizeof(X) == 4 ? c->tune.cell_operator != -1 : c->tune.cell_operator == 1;
+  if (want || want_cells) {
     if ((rc = ensure_lattice(c, w))) return rc;
-    if (w->lat.state == 1) {
+    if (want_cells && w->lat.state == 1) {
+      bool ok = false;
+      if ((rc = ensure_cells<X>(c, w, jacobi, &ok))) return rc;
+      w->use_cells = w->use_lat = ok;
+    }
+    if (want && !w->use_cells && w->lat.state == 1) {
       bool sym = false;
       if ((rc = ensure_lat_vals<X>(c, w, jacobi, &sym))) return rc;
       w->use_lat = sym;
@@ -1522,10 +2170,13 @@ static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int
 template <typename X>
 static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache& gc) {
   static_assert(kBatch % 2 == 0, "batch must end on scalar record 0");
-  const void* val = w->use_lat ? (sizeof(X) == 4 ? (const void*)w->lat.val32 : (const void*)w->lat.val64)
-                               : (sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64);
-  // the lattice SpMV variant depends on the values (a unit diagonal), not only on the operands
-  const uintptr_t variant = w->use_lat ? 1 + (sizeof(X) == 4 ? w->lat.unit32 : w->lat.unit64) : 0;
+  const void* val = w->use_cells ? (sizeof(X) == 4 ? (const void*)w->cells.kc32 : (const void*)w->cells.kc64)
+                    : w->use_lat ? (sizeof(X) == 4 ? (const void*)w->lat.val32 : (const void*)w->lat.val64)
+                                 : (sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64);
+  // the lattice SpMV variant depends on the values (a unit diagonal), not only on the operands;
+  // the cell form's K¹ is a kernel argument, so its graph is keyed on the check's version
+  const uintptr_t variant = w->use_cells ? 3 + 2 * (sizeof(X) == 4 ? w->cells.ver32 : w->cells.ver64)
+                            : w->use_lat ? 1 + (sizeof(X) == 4 ? w->lat.unit32 : w->lat.unit64) : 0;
   const void* key[9] = {val, v.x, v.r, v.p, v.w, v.sv, w->s, w->use_lat ? (const void*)w->lat.map : w->sell.off,
                         reinterpret_cast<const void*>(variant)};
   if (c->tune.time_kernels) {   // kernels launched directly (launch_batch), SpMVs bracketed by events
@@ -1694,7 +2345,7 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   if (rc && rc != FEM_ERR_NOCONV) return rc;
   out.iterations = iters;
   out.outer_iterations = handovers;
-  out.operator_format = !w->use_lat ? 0 : (mixed ? w->lat.unit32 : w->lat.unit64) ? 2 : 1;
+  out.operator_format = w->use_cells ? 3 : !w->use_lat ? 0 : (mixed ? w->lat.unit32 : w->lat.unit64) ? 2 : 1;
   if ((rc = read_scal(c, w, &h))) return rc;   // s->rr: the true residual of the returned u
   out.relres = std::sqrt(h.rr / bnorm2);
   c->timings.solve_ms = cg_ms;
@@ -1758,7 +2409,10 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
       !(t->lattice == 0 || t->lattice == 1 || t->lattice == -1) ||
       !(t->lattice_blocks == 0 || (t->lattice_blocks >= 8 && t->lattice_blocks <= kMaxVecBlocks &&
                                    t->lattice_blocks % 8 == 0)) ||
-      !(t->cg_split == 0 || t->cg_split == 1 || t->cg_split == -1))
+      !(t->cg_split == 0 || t->cg_split == 1 || t->cg_split == -1) ||
+      !(t->cell_operator == 0 || t->cell_operator == 1 || t->cell_operator == -1) ||
+      !(t->cell_blocks == 0 || (t->cell_blocks >= 8 && t->cell_blocks <= kMaxVecBlocks && t->cell_blocks % 8 == 0)) ||
+      !(t->cell_march == 0 || t->cell_march == -1))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

@@ -61,7 +61,8 @@ class Tuning(C.Structure):
                 ("sell_unroll", C.c_int32), ("sell_map", C.c_int32), ("gather_group", C.c_int32),
                 ("cg_elems", C.c_int32), ("gather_chunks", C.c_int32), ("sell_nt", C.c_int32),
                 ("affine", C.c_int32), ("lattice", C.c_int32), ("lattice_blocks", C.c_int32),
-                ("cg_split", C.c_int32)]
+                ("cg_split", C.c_int32), ("cell_operator", C.c_int32), ("cell_blocks", C.c_int32),
+                ("cell_march", C.c_int32)]
 
 
 class Timings(C.Structure):

@@ -224,7 +224,9 @@ class SolveResult:
         self.iterations = info.iterations
         self.outer_iterations = info.outer_iterations
         self.relres = info.relres
-        self.operator = "lattice" if info.operator_format in (1, 2) else "sell"
+        # the solver's copy of K: "sell" (sliced ELL), "lattice" (value planes per neighbour)
+        # or "cells" (per-cell coefficients times cell 0's unit-coefficient element matrix)
+        self.operator = {1: "lattice", 2: "lattice", 3: "cells"}.get(info.operator_format, "sell")
         self.unit_diagonal = info.operator_format == 2
         self.u = u
 

@@ -26,6 +26,7 @@ def main():
                     help="owner-computes assembly: cells loaded per column lane before adding")
     ap.add_argument("--ir-sweep", action="store_true",
                     help="mixed solver: reliable-update factor δ (inner_rtol) vs iterations and loop time")
+    ap.add_argument("--cell-sweep", action="store_true", help="cell-form SpMV: XCD map x block counts")
     ap.add_argument("--lattice-sweep", action="store_true",
                     help="lattice vs sliced-ELL solver operator x block mapping: isolated and in-solve SpMV")
     ap.add_argument("--sell-sweep", action="store_true",
@@ -150,6 +151,27 @@ def main():
                   f"{t['spmv_ms'] / max(t['spmv_launches'], 1) * 1e3:.1f} us; graph loop "
                   f"{fa.timings(K)['solve_ms']:.1f} ms ({r.iterations} its)", flush=True)
         fa.set_tuning(K, lattice_blocks=0)
+    if args.cell_sweep:
+        for march, smap, mname in ((0, 0, "z-march, XCD ranges"), (-1, 0, "tiles, XCD ranges")):
+            for blocks in (768, 1024, 1280, 1536, 2048):
+                fa.set_tuning(K, cell_operator=1, cell_blocks=blocks, sell_map=smap, cell_march=march)
+                a64 = fa.spmv_bench(K, mixed=False, reps=20)
+                a32 = fa.spmv_bench(K, mixed=True, reps=20)
+                fa.set_tuning(K, cell_operator=1, cell_blocks=blocks, sell_map=smap, time_kernels=1)
+                line = []
+                for mode in ("cg", "mixed"):
+                    r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
+                    t = fa.timings(K)
+                    line.append(f"{mode} [{r.operator}] {t['spmv_ms'] / max(t['spmv_launches'], 1) * 1e3:.1f} us x "
+                                f"{t['spmv_launches']} ({r.iterations} its)")
+                fa.set_tuning(K, cell_operator=1, cell_blocks=blocks, sell_map=smap, time_kernels=0)
+                loops = []
+                for mode in ("cg", "mixed"):
+                    r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
+                    loops.append(f"{mode} {fa.timings(K)['solve_ms']:.1f} ms")
+                print(f"cell operator {mname} {blocks} blocks: isolated fp64 {a64 * 1e3:.1f} us, fp32 {a32 * 1e3:.1f} us; "
+                      f"in-solve {'; '.join(line)}; graph loops {', '.join(loops)}", flush=True)
+        fa.set_tuning(K, cell_operator=0, cell_blocks=0, sell_map=0, cell_march=0)
     if args.sell_sweep:
         for smap, mname, unrolls in ((1, "grid-stride, ordinary loads", (4, 8, 16)), (2, "contiguous/XCD", (4, 8)),
                                      (3, "grid-stride, non-temporal loads", (4, 8))):

@@ -134,7 +134,8 @@ typedef struct {
   double relres;             /* final true relative residual ‖f − K u‖/‖f‖ in FP64 */
   int32_t operator_format;   /* the CG operator copy used: 0 sliced ELL, 1 lattice planes,
                                 2 lattice planes whose (scaled) diagonal is all ones, so the
-                                SpMV does not read the centre plane */
+                                SpMV does not read the centre plane, 3 cell coefficients
+                                (tuning cell_operator) */
   int32_t reserved;
 } fem_solve_info;
 
@@ -192,6 +193,17 @@ typedef struct {
   int32_t cg_split;                  /* CG step after the SpMV: 0 (default) or 1 = a one-block scalar
                                         kernel, then the vector move; -1 = one kernel whose every
                                         block reduces the partials itself */
+  int32_t cell_operator;             /* solver operator on a lattice of congruent cells whose K is
+                                        the sum of k_e times cell 0's unit-coefficient element
+                                        matrix (checked entry by entry against K): applied from
+                                        the per-cell parameter, 4 bytes per cell instead of value
+                                        planes (0 = default: mixed solver only; 1: FP64 solver too;
+                                        -1: never) */
+  int32_t cell_blocks;               /* blocks of that SpMV, a multiple of 8 (0 = default: 1024 FP32,
+                                        768 FP64) */
+  int32_t cell_march;                /* 3D cell form: 0 (default) = each block marches a tile up
+                                        through a chunk of planes, loading one plane per step;
+                                        -1 = independent tiles of rows (three planes per tile) */
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
 int32_t fem_get_tuning(const fem_ctx* ctx, fem_tuning* t);

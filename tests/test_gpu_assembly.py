@@ -64,12 +64,18 @@ def test_heat_equation_kat(precision, tol):
     assert abs(np.linalg.norm(res.u) - 3.307743912641305) / 3.307743912641305 < (1e-10 if precision == "fp64" else 1e-5)
     assert _relerr(res.u, u_ref) < (1e-10 if precision == "fp64" else 1e-5)
     assert m > 0
-    # the mixed solver on the lattice operator (FP32 planes, FP64 reliable updates) meets the
-    # same KAT
+    # the mixed solver (FP32 operator, FP64 reliable updates) meets the same KAT on the cell
+    # form of K (k_e times one element matrix: the FP64-assembled K passes the entry check; the
+    # FP32-element one may fall back to the planes) and on the lattice planes
     mix = fa.solve(K, f, mode="mixed", rtol=1e-13)
-    assert mix.operator == "lattice"
+    assert mix.operator == "cells" if precision == "fp64" else mix.operator in ("cells", "lattice")
     assert _relerr(mix.u, res.u) < 1e-10
     assert abs(np.linalg.norm(mix.u) - 3.307743912641305) / 3.307743912641305 < (1e-10 if precision == "fp64" else 1e-5)
+    fa.set_tuning(K, cell_operator=-1)
+    planes = fa.solve(K, f, mode="mixed", rtol=1e-13)
+    fa.set_tuning(K, cell_operator=0)
+    assert planes.operator == "lattice"
+    assert _relerr(planes.u, res.u) < 1e-10
 
 
 _cache = {}

@@ -1,11 +1,14 @@
-"""The solver's lattice operator (value planes per neighbour, no column indices) against the
-oracle's direct solve and against the sliced-ELL operator on the same system.
+"""The solver's lattice operators against the oracle's direct solve and against the
+sliced-ELL operator on the same system: the value planes per neighbour (no column indices)
+and the cell form (k_e times cell 0's unit-coefficient element matrix, used only after an
+entry-by-entry check against K).
 
 Bar: u within 1e-8 of the oracle (CG to 1e-12), the lattice and sliced operators' solutions
 within 1e-8 of each other; the operator actually used is read back from fem_solve_info.
 Meshes with non-cube dimensions (13x9x7 hexes, 17x11 quads) so that an axis mix-up in the
 padded numbering cannot cancel out; one mesh with a cell whose local numbering is rotated
-(a valid hex the lattice walk cannot place) must fall back to the sliced operator.
+(a valid hex the lattice walk cannot place) must fall back to the sliced operator; a K that
+differs from the cell form (one entry changed, a distorted cell) must fall back to the planes.
 """
 import numpy as np
 import pytest
@@ -21,7 +24,9 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def _system(ct, nel, right, k, cells_hook=None):
+def _system(ct, nel, right, k, cells_hook=None, cell_operator=-1):
+    """Assembled and constrained system; the cell form is off unless asked for, so that the
+    plane tests exercise the planes."""
     g = fa.generate_grid(ct, nel, tuple(0.0 for _ in nel), right)
     if cells_hook is not None:
         cells_hook(g)
@@ -29,6 +34,7 @@ def _system(ct, nel, right, k, cells_hook=None):
     cv = fa.CellValues(fa.QuadratureRule(g.refshape, 2), ip)
     dh = fa.DofHandler(g).add("u", ip).close()
     K = fa.allocate_matrix(fa.MI355XCSC, dh)
+    fa.set_tuning(K, cell_operator=cell_operator)
     f = fa.allocate_vector(K)
     fa.assemble_global(fa.start_assemble(K, f), cv, fa.create_coloring(g), fa.PoissonElement(k=k, source=1.0))
     ch = fa.ConstraintHandler(dh)
@@ -120,3 +126,81 @@ def test_lattice_symmetry_check_falls_back():
     K.set_nzval(nz)
     res = fa.solve(K, f, mode="mixed", rtol=1e-12)
     assert res.operator == "lattice"
+
+
+@pytest.mark.parametrize("ct,nel,right", [("Hexahedron", (13, 9, 7), (1.3, 0.9, 0.7)),
+                                          ("Quadrilateral", (17, 11), (1.7, 1.1))])
+def test_cell_operator_parity(ct, nel, right):
+    """The default mixed solver on congruent cells applies K from k_e and one element matrix."""
+    print("case cell operator", ct, nel, flush=True)
+    k = poisson_conductivity(nel, seed=14)
+    g, dh, K, f = _system(ct, nel, right, k, cell_operator=0)
+    u_ref = _oracle_u(ct, nel, right, k)
+    cells = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert cells.operator == "cells"
+    assert cells.relres <= 1e-12 and _rel(cells.u, u_ref) < 1e-8
+    again = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert again.operator == "cells" and np.array_equal(again.u, cells.u)   # deterministic
+    plain = fa.solve(K, f, mode="mixed", rtol=1e-12, precond="none")        # needs the Jacobi scaling
+    assert plain.operator == "lattice"
+    assert plain.relres <= 1e-12 and _rel(plain.u, u_ref) < 1e-8
+    fa.set_tuning(K, cell_operator=1)
+    c64 = fa.solve(K, f, mode="cg", rtol=1e-12)
+    assert c64.operator == "cells"
+    assert c64.relres <= 1e-12 and _rel(c64.u, u_ref) < 1e-8
+    assert fa.spmv_bench(K, mixed=False, reps=3) > 0.0
+    fa.set_tuning(K, cell_operator=-1)
+    sell64 = fa.solve(K, f, mode="cg", rtol=1e-12)
+    assert sell64.operator == "sell" and _rel(c64.u, sell64.u) < 1e-9
+    planes = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert planes.operator == "lattice" and _rel(cells.u, planes.u) < 1e-8
+    fa.set_tuning(K, cell_operator=0, time_kernels=1)        # direct launches, timed SpMVs
+    timed = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert timed.operator == "cells" and np.array_equal(timed.u, cells.u)
+    assert fa.spmv_bench(K, mixed=True, reps=3) > 0.0
+
+
+def test_cell_operator_check_falls_back():
+    """K changed in place so that it is no longer k_e times one element matrix (a symmetric
+    pair of off-diagonal entries scaled by 1 + 1e-4, well above the check's 2^-22 of the unit
+    diagonal): the mixed solver must use the planes; with K restored, the cell form again."""
+    print("case cell operator check", flush=True)
+    nel = (7, 6, 5)
+    k = poisson_conductivity(nel, seed=15)
+    g, dh, K, f = _system("Hexahedron", nel, (1.0, 1.0, 1.0), k, cell_operator=0)
+    ref = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert ref.operator == "cells"
+    nz = K.nzval()
+    rows, cols = K.rowval - 1, np.repeat(np.arange(K.n), np.diff(K.colptr))
+    off = np.nonzero((rows != cols) & (np.abs(nz) > 1e-3 * np.abs(nz).max()))[0]
+    e = int(off[len(off) // 2])
+    i, j = rows[e], cols[e]
+    e2 = int(np.nonzero((rows == j) & (cols == i))[0][0])
+    bad = nz.copy()
+    bad[e] *= 1.0 + 1e-4
+    bad[e2] *= 1.0 + 1e-4
+    K.set_nzval(bad)
+    res = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert res.operator == "lattice" and res.relres <= 1e-12
+    K.set_nzval(nz)
+    back = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert back.operator == "cells" and np.array_equal(back.u, ref.u)
+
+
+def test_cell_operator_distorted_mesh_falls_back():
+    """One interior node moved: the cells are no longer congruent, the check rejects the cell
+    form and the planes give the same solution as the FP64 sliced solve."""
+    print("case cell operator distorted", flush=True)
+    nel = (6, 5, 4)
+    k = poisson_conductivity(nel, seed=16)
+
+    def shift(g):
+        x = g.nodes
+        i = int(np.argmin(np.linalg.norm(x - np.array([0.5, 0.6, 0.5]), axis=1)))
+        x[i] += np.array([0.03, -0.02, 0.01])
+    g, dh, K, f = _system("Hexahedron", nel, (1.0, 1.0, 1.0), k, shift, cell_operator=0)
+    res = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert res.operator == "lattice"
+    fa.set_tuning(K, cell_operator=-1, lattice=-1)
+    ref = fa.solve(K, f, mode="cg", rtol=1e-12)
+    assert ref.operator == "sell" and _rel(res.u, ref.u) < 1e-8
