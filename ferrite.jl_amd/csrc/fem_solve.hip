@@ -160,10 +160,10 @@ struct CellOp {
   unsigned long long* cnt = nullptr; // [4] check counters
   int lds32 = 0, lds64 = 0;    // dynamic LDS of the SpMV (bytes)
   int ne = 0;                  // window entries per thread of the SpMV
-  // 3D: the z-marching SpMV (zne > 0): entries per thread, in-plane tiles; per precision the
-  // blocks, planes per chunk and work items
-  int zne = 0, ntx = 0;
-  int grid32 = 0, grid64 = 0, lz32 = 0, lz64 = 0;
+  // 3D: the z-marching SpMV (zn > 0): t·x and coefficient window entries per thread, in-plane
+  // tiles; per precision the blocks, planes per chunk, work items and LDS bytes
+  int zn = 0, kn = 0, ntx = 0;
+  int grid32 = 0, grid64 = 0, lz32 = 0, lz64 = 0, ldsz32 = 0, ldsz64 = 0;
   int64_t nitems32 = 0, nitems64 = 0;
 };
 
@@ -735,6 +735,55 @@ __global__ void k_cell_scale(int64_t npad, const int32_t* __restrict__ dofp, con
   }
 }
 
+// Workgroup barrier that orders LDS only: waits for this wave's LDS operations, then meets the
+// other waves.  __syncthreads() also fences global memory, which waits for every load in
+// flight — including loads issued ahead for later steps, which this barrier leaves in flight.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// One row of the cell form from the t·x of its neighbours zn ((dz, dy, dx) row-major, offsets
+// −1..1) and the coefficients k[e] of the cells around it (cell with origin P − e, e = ex + 2 ey
+// + 4 ez): the neighbour at offset d lies in the cells whose e has e_a = 0 where d_a = +1 and
+// e_a = 1 where d_a = −1 (either where d_a = 0), with K¹ value f[|dx| + 2 |dy| + 4 |dz|], so
+//   y = Σ_m f[m] Σ_{|d| = m} (Σ of k over those cells) zn[d];
+// the sums of k over cell pairs and quadruples are shared between neighbours (54 operations
+// instead of 8 × 8 + 8).
+template <typename X, int DIM>
+__device__ __forceinline__ X cell_row(const X* zn, const X* k, const X* f) {
+  if constexpr (DIM == 3) {
+    auto Z = [&](int dx, int dy, int dz) { return zn[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)]; };
+    const X z00 = k[0] + k[4], z10 = k[1] + k[5], z01 = k[2] + k[6], z11 = k[3] + k[7];   // over ez: [ex][ey]
+    const X y00 = k[0] + k[2], y10 = k[1] + k[3], y01 = k[4] + k[6], y11 = k[5] + k[7];   // over ey: [ex][ez]
+    const X x00 = k[0] + k[1], x10 = k[2] + k[3], x01 = k[4] + k[5], x11 = k[6] + k[7];   // over ex: [ey][ez]
+    const X yz0 = z00 + z01, yz1 = z10 + z11;   // over ey, ez: [ex]
+    const X xz0 = z00 + z10, xz1 = z01 + z11;   // over ex, ez: [ey]
+    const X xy0 = x00 + x10, xy1 = x01 + x11;   // over ex, ey: [ez]
+    const X t0 = (yz0 + yz1) * Z(0, 0, 0);
+    const X t1 = yz0 * Z(1, 0, 0) + yz1 * Z(-1, 0, 0);
+    const X t2 = xz0 * Z(0, 1, 0) + xz1 * Z(0, -1, 0);
+    const X t4 = xy0 * Z(0, 0, 1) + xy1 * Z(0, 0, -1);
+    const X t3 = z00 * Z(1, 1, 0) + z10 * Z(-1, 1, 0) + z01 * Z(1, -1, 0) + z11 * Z(-1, -1, 0);
+    const X t5 = y00 * Z(1, 0, 1) + y10 * Z(-1, 0, 1) + y01 * Z(1, 0, -1) + y11 * Z(-1, 0, -1);
+    const X t6 = x00 * Z(0, 1, 1) + x10 * Z(0, -1, 1) + x01 * Z(0, 1, -1) + x11 * Z(0, -1, -1);
+    X t7 = X(0);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) t7 += k[e] * Z(1 - 2 * (e & 1), 1 - 2 * ((e >> 1) & 1), 1 - 2 * (e >> 2));
+    return f[0] * t0 + f[1] * t1 + f[2] * t2 + f[3] * t3 + f[4] * t4 + f[5] * t5 + f[6] * t6 + f[7] * t7;
+  } else {
+    auto Z = [&](int dx, int dy) { return zn[(dy + 1) * 3 + (dx + 1)]; };
+    const X y0 = k[0] + k[2], y1 = k[1] + k[3];   // over ey: [ex]
+    const X x0 = k[0] + k[1], x1 = k[2] + k[3];   // over ex: [ey]
+    const X t0 = (y0 + y1) * Z(0, 0);
+    const X t1 = y0 * Z(1, 0) + y1 * Z(-1, 0);
+    const X t2 = x0 * Z(0, 1) + x1 * Z(0, -1);
+    X t3 = X(0);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) t3 += k[e] * Z(1 - 2 * (e & 1), 1 - 2 * (e >> 1));
+    return f[0] * t0 + f[1] * t1 + f[2] * t2 + f[3] * t3;
+  }
+}
+
 // Entry-by-entry check of the scaled K against the cell form, in FP64, kLatLanes lanes per
 // padded row (column j of the CSC arrays, as the plane fill).  Â's diagonal is 1, so tol is
 // relative to it.  cnt[0] entries that differ by more than tol, cnt[1] lattice-neighbour
@@ -888,19 +937,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv(int64_t r_lo, int64_t nrow
         for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
           for (int dx = 0; dx < 3; ++dx) zn[dz * 9 + dy * 3 + dx] = zw[dz * W + l + (dx - 1) + px * (dy - 1)];
-      X acc = X(0);
+      X kn[NB];                             // cell with origin P − e: P is its corner e
 #pragma unroll
-      for (int e = 0; e < NB; ++e) {        // cell with origin P − e; P is its corner e
-        const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
-        const X kcell = kw[ez * WK + l - ex - px * ey];
-        X sub = X(0);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {      // corner b of that cell, at offset b − e from P
-          const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
-          sub += kr.f[b ^ e] * zn[(bz - ez + (DIM == 3 ? 1 : 0)) * 9 + (by - ey + 1) * 3 + (bx - ex + 1)];
-        }
-        acc += kcell * sub;
-      }
+      for (int e = 0; e < NB; ++e) kn[e] = kw[(e >> 2) * WK + l - (e & 1) - px * ((e >> 1) & 1)];
+      const X acc = cell_row<X, DIM>(zn, kn, kr.f);
       const X yv = tc != X(0) ? tc * acc : xc;
       y[P] = yv;
       d += (double)xc * (double)yv;
@@ -912,15 +952,19 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv(int64_t r_lo, int64_t nrow
 
 // 3D cell form marching in z: work item (j, c) = in-plane tile j (kBlock consecutive rows of
 // a plane, in-plane offsets j kBlock ..) over the planes [z0, z1) of chunk c.  The block keeps
-// a ring of three t·x windows (planes z − 1, z, z + 1; kBlock + 2 px + 2 entries each) and two
-// coefficient windows (cell layers z, z − 1; kBlock + px + 1 entries) in LDS and loads one new
-// window of each per plane: step p brings the t·x window of plane p and the coefficients of
-// layer p − 1, and its loads (with t, x of the rows of plane p − 1) are issued before the
-// rows of plane p − 2 are computed, so they overlap that arithmetic.  Padding rows of the
-// marched planes are computed too (t = 0 there, so y = x = 0).  Items are ordered chunk-major;
-// MAP 1: the blocks of one XCD sweep one contiguous eighth of the items, so neighbouring tiles
-// (which share their halo rows) run on the same XCD at the same time.
-template <typename X, int MAP, int NE, int D>
+// a ring of three t·x windows (planes z − 1, z, z + 1; kBlock + 2 px + 2 entries each, ZN per
+// thread) and two coefficient windows (cell layers z, z − 1; kBlock + px + 1 entries, KN per
+// thread) in LDS and loads one new window of each per plane: step p brings the t·x window of
+// plane p, the coefficients of layer p − 1 and t, x of the rows of plane p − 1.  D register
+// stages hold the loads of D steps, issued D planes ahead of their use.  The step loop is
+// straight-line code (every load issued, out-of-range rows read as padding row 0, which
+// holds zeros; the rows of a plane go to an LDS buffer and are written once per item; the
+// planes per chunk are rounded up to a multiple of D, the extra steps' rows discarded) and
+// its barriers order LDS only, so the loads of later steps stay in flight across them.
+// Padding rows of the marched planes are computed too (t = 0 there, so y = x = 0).  Items are
+// ordered chunk-major; MAP 1: the blocks of one XCD sweep one contiguous eighth of the items,
+// so neighbouring tiles (which share their halo rows) run on the same XCD at the same time.
+template <typename X, int MAP, int ZN, int KN, int D>
 __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px, int32_t pxy, int32_t pz,
                                                         int32_t ntx, int32_t lz, int64_t nitems,
                                                         const X* __restrict__ kc, const X* __restrict__ t,
@@ -930,16 +974,26 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
   if (s->done || s->fresh) return;
   extern __shared__ __align__(16) unsigned char cell_lds[];
   __shared__ double sh[kBlock / 64];
-  const int W = kBlock + 2 * px + 2, WK = kBlock + px + 1;
+  const int W = kBlock + 2 * px + 2, WK = kBlock + px + 1;   // W ≤ ZN kBlock, WK ≤ KN kBlock (host)
+  const int nsteps = (lz + D - 1) / D * D;
   X* zw = reinterpret_cast<X*>(cell_lds);   // [3][W]
   X* kw = zw + 3 * W;                        // [2][WK]
-  const int E = W + WK;                      // entries of one step, ≤ NE kBlock (checked by the host)
-  // this thread's entries of a step: window offset from the tile's first row of the plane
-  int32_t rel[NE];
+  X* yb = kw + 2 * WK;                       // [nsteps][kBlock] rows of the item's planes
+  // this thread's window entries: offsets from the tile's first row of plane p (t·x) and of
+  // layer p − 1 (coefficients); entries past a window read row 0
+  int32_t relz[ZN], relk[KN];
+  bool inz[ZN], ink[KN];
 #pragma unroll
-  for (int k = 0; k < NE; ++k) {
+  for (int k = 0; k < ZN; ++k) {
     const int idx = threadIdx.x + k * kBlock;
-    rel[k] = idx < W ? idx - px - 1 : (idx - W) - px - 1 - pxy;   // k entries: layer p − 1
+    inz[k] = idx < W;
+    relz[k] = idx - px - 1;
+  }
+#pragma unroll
+  for (int k = 0; k < KN; ++k) {
+    const int idx = threadIdx.x + k * kBlock;
+    ink[k] = idx < WK;
+    relk[k] = idx - px - 1 - pxy;
   }
   const uint32_t np32 = (uint32_t)npad;
   int64_t i0, i1, istep;
@@ -953,104 +1007,90 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
     i1 = nitems;
     istep = gridDim.x;
   }
-  // D register stages, each holding the loads of one step: stage u serves the steps p with
-  // p − z0 − 1 ≡ u (mod D), so D steps' loads are in flight while a plane is computed
-  X va[D][NE], vb[D][NE], tn[D], xn[D];
-  // step p of tile offset o0 (in-plane) into stage u: t·x window of plane p, coefficients of
-  // layer p − 1, t and x of row o0 + threadIdx.x of plane p − 1 (q wraps below 0 to beyond
-  // npad: zero)
+  X vt[D][ZN], vx[D][ZN], vk[D][KN], tn[D], xn[D];
+  // step p of tile offset o0 into stage u (q wraps below 0 to beyond npad: row 0)
   auto fetch = [&](int u, int32_t p, int32_t o0) {
     const uint32_t base = (uint32_t)p * (uint32_t)pxy + (uint32_t)o0;
 #pragma unroll
-    for (int k = 0; k < NE; ++k) {
-      const int idx = threadIdx.x + k * kBlock;
-      const uint32_t q = base + (uint32_t)rel[k];
-      va[u][k] = X(0);
-      vb[u][k] = X(1);
-      if (idx < E && q < np32) {
-        if (idx < W) {
-          va[u][k] = at_byte(t, q * (uint32_t)sizeof(X));
-          vb[u][k] = at_byte(x, q * (uint32_t)sizeof(X));
-        } else {
-          va[u][k] = at_byte(kc, q * (uint32_t)sizeof(X));
-        }
-      }
+    for (int k = 0; k < ZN; ++k) {
+      const uint32_t q = base + (uint32_t)relz[k];
+      const uint32_t qq = (inz[k] && q < np32) ? q : 0u;
+      vt[u][k] = at_byte(t, qq * (uint32_t)sizeof(X));
+      vx[u][k] = at_byte(x, qq * (uint32_t)sizeof(X));
     }
-    const uint32_t P = base - (uint32_t)pxy + threadIdx.x;   // wraps beyond npad for p = 0
-    const bool in = o0 + (int32_t)threadIdx.x < pxy && P < np32;
-    tn[u] = in ? at_byte(t, P * (uint32_t)sizeof(X)) : X(0);
-    xn[u] = in ? at_byte(x, P * (uint32_t)sizeof(X)) : X(0);
+#pragma unroll
+    for (int k = 0; k < KN; ++k) {
+      const uint32_t q = base + (uint32_t)relk[k];
+      const uint32_t qq = (ink[k] && q < np32) ? q : 0u;
+      vk[u][k] = at_byte(kc, qq * (uint32_t)sizeof(X));
+    }
+    const uint32_t P = base - (uint32_t)pxy + threadIdx.x;
+    const uint32_t PP = P < np32 ? P : 0u;
+    tn[u] = at_byte(t, PP * (uint32_t)sizeof(X));
+    xn[u] = at_byte(x, PP * (uint32_t)sizeof(X));
   };
   auto store = [&](int u, int32_t p) {   // z slot p mod 3, coefficient slot (p − 1) mod 2
     X* zs = zw + (p % 3) * W;
     X* ks = kw + ((p - 1) & 1) * WK;
 #pragma unroll
-    for (int k = 0; k < NE; ++k) {
-      const int idx = threadIdx.x + k * kBlock;
-      if (idx < W) zs[idx] = va[u][k] * vb[u][k];
-      else if (idx < E) ks[idx - W] = va[u][k];
-    }
+    for (int k = 0; k < ZN; ++k)
+      if (inz[k]) zs[threadIdx.x + k * kBlock] = vt[u][k] * vx[u][k];
+#pragma unroll
+    for (int k = 0; k < KN; ++k)
+      if (ink[k]) ks[threadIdx.x + k * kBlock] = vk[u][k];
   };
   double d = 0.0;
   for (int64_t item = i0; item < i1; item += istep) {
     const int32_t c = (int32_t)(item / ntx), j = (int32_t)(item - (int64_t)c * ntx);
     const int32_t z0 = 1 + c * lz, z1 = min(z0 + lz, pz - 1);
     const int32_t o0 = j * kBlock;
+    const bool col = o0 + (int32_t)threadIdx.x < pxy;   // this thread's row exists in the plane
     fetch(0, z0 - 1, o0);
-    __syncthreads();   // the previous item's LDS reads are done
+    lds_barrier();     // the previous item's LDS reads are done
     store(0, z0 - 1);
     fetch(0, z0, o0);
     store(0, z0);
 #pragma unroll
-    for (int u = 0; u < D; ++u)
-      if (z0 + 1 + u <= z1) fetch(u, z0 + 1 + u, o0);
-    for (int32_t zb = z0; zb < z1; zb += D) {
+    for (int u = 0; u < D; ++u) fetch(u, z0 + 1 + u, o0);
+    for (int32_t sb = 0; sb < nsteps; sb += D) {
 #pragma unroll
       for (int u = 0; u < D; ++u) {
-        const int32_t zp = zb + u;
-        if (zp >= z1) break;
+        const int32_t zp = z0 + sb + u;
         const X tc = tn[u], xc = xn[u];   // row of plane zp
         store(u, zp + 1);
-        __syncthreads();
-        if (zp + 1 + D <= z1) fetch(u, zp + 1 + D, o0);
-        if (o0 + (int32_t)threadIdx.x < pxy) {
-          const int l = threadIdx.x + px + 1;   // the row in the windows
-          const X* zm = zw + ((zp + 2) % 3) * W;   // plane zp − 1
-          const X* z0w = zw + (zp % 3) * W;
-          const X* zpw = zw + ((zp + 1) % 3) * W;
-          X zn[27];
+        lds_barrier();
+        fetch(u, zp + 1 + D, o0);
+        const int l = threadIdx.x + px + 1;   // the row in the windows
+        const X* zm = zw + ((zp + 2) % 3) * W;   // plane zp − 1
+        const X* z0w = zw + (zp % 3) * W;
+        const X* zpw = zw + ((zp + 1) % 3) * W;
+        X zn[27];
 #pragma unroll
-          for (int dy = 0; dy < 3; ++dy)
+        for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-              const int o = l + (dx - 1) + px * (dy - 1);
-              zn[dy * 3 + dx] = zm[o];
-              zn[9 + dy * 3 + dx] = z0w[o];
-              zn[18 + dy * 3 + dx] = zpw[o];
-            }
-          const X* k0 = kw + (zp & 1) * WK;         // layer zp
-          const X* k1 = kw + ((zp - 1) & 1) * WK;   // layer zp − 1
-          X acc = X(0);
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {             // cell with origin P − e; P is its corner e
-            const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
-            const X kcell = (ez ? k1 : k0)[l - ex - px * ey];
-            X sub = X(0);
-#pragma unroll
-            for (int b = 0; b < 8; ++b) {           // corner b of that cell, at offset b − e from P
-              const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
-              sub += kr.f[b ^ e] * zn[(bz - ez + 1) * 9 + (by - ey + 1) * 3 + (bx - ex + 1)];
-            }
-            acc += kcell * sub;
+          for (int dx = 0; dx < 3; ++dx) {
+            const int o = l + (dx - 1) + px * (dy - 1);
+            zn[dy * 3 + dx] = zm[o];
+            zn[9 + dy * 3 + dx] = z0w[o];
+            zn[18 + dy * 3 + dx] = zpw[o];
           }
-          const X yv = tc != X(0) ? tc * acc : xc;
-          const uint32_t P = (uint32_t)zp * (uint32_t)pxy + (uint32_t)(o0 + threadIdx.x);
-          *reinterpret_cast<X*>(reinterpret_cast<char*>(y) + P * (uint32_t)sizeof(X)) = yv;
-          d += (double)xc * (double)yv;
-        }
-        __syncthreads();   // this plane's reads are done before the next store
+        const X* k0 = kw + (zp & 1) * WK;         // layer zp
+        const X* k1 = kw + ((zp - 1) & 1) * WK;   // layer zp − 1
+        X kn[8];                                  // cell with origin P − e: P is its corner e
+#pragma unroll
+        for (int e = 0; e < 8; ++e) kn[e] = (e >> 2 ? k1 : k0)[l - (e & 1) - px * ((e >> 1) & 1)];
+        const X acc = cell_row<X, 3>(zn, kn, kr.f);
+        const X yv = tc != X(0) ? tc * acc : xc;
+        yb[(sb + u) * kBlock + threadIdx.x] = yv;
+        d += (col && zp < z1) ? (double)xc * (double)yv : 0.0;
+        lds_barrier();     // this plane's reads are done before the next store
       }
     }
+    if (col)
+      for (int32_t pl = 0; pl < z1 - z0; ++pl) {
+        const uint32_t P = (uint32_t)(z0 + pl) * (uint32_t)pxy + (uint32_t)(o0 + threadIdx.x);
+        *reinterpret_cast<X*>(reinterpret_cast<char*>(y) + P * (uint32_t)sizeof(X)) = yb[pl * kBlock + threadIdx.x];
+      }
   }
   const double b = block_sum(d, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
@@ -1737,19 +1777,22 @@ static bool cell_kref(const fem_ctx* c, const double (*x0)[3], int dim, CellKref
 }
 
 // Launch shape of the cell-form SpMV for values of `bytes` bytes.  Measured at 128³ (in the
-// solve, z-marching, XCD ranges): FP32 768 blocks 22.4 µs, 1024 21.0, 1280 25.0, 1536 23.0,
-// 2048 22.9; FP64 768 29.5 µs, 1024 38.6, 1280 35.9, 1536 33.3, 2048 36.9.  Z-marching: about
+// solve, z-marching, XCD ranges): FP32 768 blocks 19.0 µs, 1024 22.2, 1280 25.6, 1536 21.1,
+// 2048 24.9; FP64 768 32.2 µs, 1024 33.8, 1280 40.2, 1536 34.2, 2048 39.1.  Z-marching: about
 // one work item per block (the planes per chunk follow from the block count).
 static void cell_shape(const fem_ctx* c, const LatOp& l, CellOp* cl, int bytes) {
-  const int64_t want = c->tune.cell_blocks ? c->tune.cell_blocks : (bytes == 4 ? 1024 : 768);
+  const int64_t want = c->tune.cell_blocks ? c->tune.cell_blocks : 768;
   int& grid = bytes == 4 ? cl->grid32 : cl->grid64;
-  if (cl->zne > 0) {
+  if (cl->zn > 0) {
     const int64_t pz = l.npad / l.pxy;
+    const int D = bytes == 4 ? 3 : 2;   // register stages of the kernel
     int& lz = bytes == 4 ? cl->lz32 : cl->lz64;
     int64_t& nitems = bytes == 4 ? cl->nitems32 : cl->nitems64;
-    lz = (int)std::max<int64_t>(2, (cl->ntx * (pz - 2) + want - 1) / want);
+    lz = (int)std::min<int64_t>(16, std::max<int64_t>(2, (cl->ntx * (pz - 2) + want - 1) / want));
     nitems = (int64_t)cl->ntx * ((pz - 2 + lz - 1) / lz);
     grid = (int)std::max<int64_t>(8, std::min<int64_t>(want, (nitems + 7) / 8 * 8));
+    const int64_t words = 3 * (kBlock + 2 * l.px + 2) + 2 * (kBlock + l.px + 1) + (lz + D - 1) / D * D * kBlock;
+    (bytes == 4 ? cl->ldsz32 : cl->ldsz64) = (int)(bytes * words);
   } else {
     const int64_t ntiles = (l.nrows + kBlock - 1) / kBlock;
     int64_t gr = std::min<int64_t>(want, ntiles);
@@ -1816,9 +1859,11 @@ static int ensure_cells(fem_ctx* c, SolverWork* w, int jacobi, bool* ok) {
     cl.ne = (int)((words + kBlock - 1) / kBlock);
     cl.lds32 = (int)(4 * words);
     cl.lds64 = (int)(8 * words);
-    const int64_t step_words = (kBlock + 2 * l.px + 2) + (kBlock + l.px + 1);
-    if (dim == 3 && step_words <= 12 * kBlock && c->tune.cell_march != -1) {
-      cl.zne = (int)((step_words + kBlock - 1) / kBlock);
+    // z-marching windows per thread: (ZN, KN) = (2, 2) up to px = 127, (3, 2) up to 255, (4, 3)
+    // up to 383; wider lattices use independent tiles
+    if (dim == 3 && l.px <= 383 && c->tune.cell_march != -1) {
+      cl.zn = l.px <= 127 ? 2 : l.px <= 255 ? 3 : 4;
+      cl.kn = l.px <= 255 ? 2 : 3;
       cl.ntx = (int)((l.pxy + kBlock - 1) / kBlock);
     }
     cell_shape(c, l, &cl, 4);
@@ -1879,7 +1924,7 @@ static void launch_cells_ne(fem_ctx* c, const SolverWork* w, const X* x, X* y, d
                                                                  (int32_t)l.pxy, kc, t, x, y, part, sc, kr);
 }
 
-template <typename X, int NE>
+template <typename X, int ZN, int KN>
 static void launch_cells_z(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc) {
   const LatOp& l = w->lat;
   const CellOp& cl = w->cells;
@@ -1887,28 +1932,27 @@ static void launch_cells_z(fem_ctx* c, const SolverWork* w, const X* x, X* y, do
   for (int i = 0; i < 8; ++i) kr.f[i] = (X)cl.f[i];
   const X* kc = sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.kc32) : reinterpret_cast<const X*>(cl.kc64);
   const X* t = sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.t32) : reinterpret_cast<const X*>(cl.t64);
-  const int lds = sizeof(X) == 4 ? cl.lds32 : cl.lds64;
+  const int lds = sizeof(X) == 4 ? cl.ldsz32 : cl.ldsz64;
   const int32_t pz = (int32_t)(l.npad / l.pxy);
   constexpr int D = sizeof(X) == 4 ? 3 : 2;   // steps in flight (register stages)
   const int grid = sizeof(X) == 4 ? cl.grid32 : cl.grid64, lz = sizeof(X) == 4 ? cl.lz32 : cl.lz64;
   const int64_t nitems = sizeof(X) == 4 ? cl.nitems32 : cl.nitems64;
   if (c->tune.sell_map != 1 && grid % 8 == 0)
-    k_cell_spmv_z<X, 1, NE, D><<<grid, kBlock, lds, c->stream>>>(l.npad, (int32_t)l.px, (int32_t)l.pxy, pz, cl.ntx,
+    k_cell_spmv_z<X, 1, ZN, KN, D><<<grid, kBlock, lds, c->stream>>>(l.npad, (int32_t)l.px, (int32_t)l.pxy, pz, cl.ntx,
                                                                  lz, nitems, kc, t, x, y, part, sc, kr);
   else
-    k_cell_spmv_z<X, 0, NE, D><<<grid, kBlock, lds, c->stream>>>(l.npad, (int32_t)l.px, (int32_t)l.pxy, pz, cl.ntx,
+    k_cell_spmv_z<X, 0, ZN, KN, D><<<grid, kBlock, lds, c->stream>>>(l.npad, (int32_t)l.px, (int32_t)l.pxy, pz, cl.ntx,
                                                                  lz, nitems, kc, t, x, y, part, sc, kr);
 }
 
 // NE: window entries per thread, the smallest instantiated bound of ⌈E / kBlock⌉
 template <typename X, int DIM>
 static void launch_cells_dim(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc) {
-  if (DIM == 3 && w->cells.zne > 0) {
-    const int zne = w->cells.zne;
-    if (zne <= 4) launch_cells_z<X, 4>(c, w, x, y, part, sc);
-    else if (zne <= 6) launch_cells_z<X, 6>(c, w, x, y, part, sc);
-    else if (zne <= 8) launch_cells_z<X, 8>(c, w, x, y, part, sc);
-    else launch_cells_z<X, 12>(c, w, x, y, part, sc);
+  if (DIM == 3 && w->cells.zn > 0) {
+    const int zn = w->cells.zn;
+    if (zn == 2) launch_cells_z<X, 2, 2>(c, w, x, y, part, sc);
+    else if (zn == 3) launch_cells_z<X, 3, 2>(c, w, x, y, part, sc);
+    else launch_cells_z<X, 4, 3>(c, w, x, y, part, sc);
     return;
   }
   const int ne = w->cells.ne;

@@ -199,8 +199,7 @@ typedef struct {
                                         the per-cell parameter, 4 bytes per cell instead of value
                                         planes (0 = default: mixed solver only; 1: FP64 solver too;
                                         -1: never) */
-  int32_t cell_blocks;               /* blocks of that SpMV, a multiple of 8 (0 = default: 1024 FP32,
-                                        768 FP64) */
+  int32_t cell_blocks;               /* blocks of that SpMV, a multiple of 8 (0 = default 768) */
   int32_t cell_march;                /* 3D cell form: 0 (default) = each block marches a tile up
                                         through a chunk of planes, loading one plane per step;
                                         -1 = independent tiles of rows (three planes per tile) */
