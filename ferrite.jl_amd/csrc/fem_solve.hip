@@ -125,9 +125,12 @@ struct LatOp {
   std::vector<int32_t> hmap;   // host copy of map
 };
 
-// Cell-coefficient form of Â on the lattice (the element-by-element operator).  When every
-// cell is a congruent copy of cell 0, K = Σ_c k_c K¹ (K¹ cell 0's element matrix for a unit
-// coefficient, k_c the per-cell parameter) with the Dirichlet rows and columns zeroed, so
+// Cell-coefficient form of Â on the lattice: the element-assembly operator of
+// docs/src/literate-howto/gpu_assembly.jl:266-305 (K x as small dense products with one
+// element matrix per cell), here with Dirichlet rows and columns and one number per cell.
+// When every cell is a congruent copy of cell 0, K = Σ_c k_c K¹ (K¹ cell 0's element matrix
+// for a unit coefficient, k_c the per-cell parameter) with the Dirichlet rows and columns
+// zeroed, so
 //   (Â x)_P = t_P Σ_{cells c ∋ P} k_c Σ_b K¹[a_c(P)][b] t_{Q_c(b)} x_{Q_c(b)},
 // t = S on free rows and 0 on prescribed rows and padding; a prescribed row keeps Â's unit
 // diagonal, (Â x)_P = x_P.  The solver uses this form only after k_cell_check found every
@@ -571,13 +574,15 @@ struct LatGeom {
 // it is not a lattice neighbour.  d + ofs read as the digits (dz + 1, dy + 1, dx + 1) in the
 // mixed radix (pxy, px, 1); as every real node lies at least one padding node away from the
 // lattice edge, digits ≤ 2 identify a neighbour uniquely.  In 2D (pz = 1) dz + 1 is always 1.
+// The digits come from compares, not divisions: for 0 ≤ t ≤ 2 ofs the z digit is at most 2
+// (py ≥ 3), a y digit above 2 leaves an x digit ≥ px > 2, and both are rejected alike.
 __device__ __forceinline__ int lat_code(int64_t d, const LatGeom& g) {
   const int64_t t = d + g.ofs;
   if (t < 0 || t > 2 * (int64_t)g.ofs) return -1;
   const uint32_t u = (uint32_t)t;
-  const uint32_t cz = u / g.pxy, rz = u - cz * g.pxy;
-  const uint32_t cy = rz / g.px, cx = rz - cy * g.px;
-  if (cz > 2 || cy > 2 || cx > 2) return -1;
+  const uint32_t cz = (u >= g.pxy) + (u >= 2 * g.pxy), rz = u - cz * g.pxy;
+  const uint32_t cy = (rz >= g.px) + (rz >= 2 * g.px), cx = rz - cy * g.px;
+  if (cx > 2) return -1;
   return (int)(cx + 3 * cy + 9 * cz);
 }
 
@@ -784,11 +789,14 @@ __device__ __forceinline__ X cell_row(const X* zn, const X* k, const X* f) {
   }
 }
 
-// Entry-by-entry check of the scaled K against the cell form, in FP64, kLatLanes lanes per
-// padded row (column j of the CSC arrays, as the plane fill).  Â's diagonal is 1, so tol is
-// relative to it.  cnt[0] entries that differ by more than tol, cnt[1] lattice-neighbour
-// entries in the pattern, cnt[2] lattice neighbours of the rows (the cell form couples every
-// two corners of a cell, so each must be a pattern entry); other entries must hold zero.
+// Entry-by-entry check of the scaled K against the cell form, in FP64, kCheckLanes lanes per
+// padded row (column j of the CSC arrays), each lane up to four entries whose loads are issued
+// together (a lattice column has at most 27), and the row's eight cell coefficients loaded
+// once.  Â's diagonal is 1, so tol is relative to it.  cnt[0] entries that differ by more
+// than tol, cnt[1] lattice-neighbour entries in the pattern, cnt[2] lattice neighbours of
+// the rows (the cell form couples every two corners of a cell, so each must be a pattern
+// entry); other entries must hold zero.
+constexpr int kCheckLanes = 8;
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nrows, const int32_t* __restrict__ dofp,
                                                        const int32_t* __restrict__ colptr,
@@ -799,44 +807,63 @@ __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nro
                                                        double tol, unsigned long long* __restrict__ cnt) {
   __shared__ double sh[kBlock / 64];
   constexpr int NB = 1 << DIM;
-  const int lane = threadIdx.x % kLatLanes;
-  const int64_t groups = (int64_t)gridDim.x * (kBlock / kLatLanes);
+  constexpr int U = 4;   // entries per lane per pass
+  const int lane = threadIdx.x % kCheckLanes;
+  const int64_t groups = (int64_t)gridDim.x * (kBlock / kCheckLanes);
   const int64_t px = g.px, pxy = g.pxy;
   double bad = 0.0, seen = 0.0, want = 0.0;
-  for (int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kLatLanes; q < nrows; q += groups) {
+  for (int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kCheckLanes; q < nrows; q += groups) {
     const int64_t P = r_lo + q;
     const int32_t j = dofp[P];
     if (j < 0) continue;
-    if (lane == 0)
-      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-          for (int dx = -1; dx <= 1; ++dx) want += dofp[P + dx + px * dy + pxy * dz] >= 0 ? 1.0 : 0.0;
+    if (lane == 0) {   // the lattice fills its box: the neighbours are a product over the axes
+      const double nx = 1.0 + (dofp[P - 1] >= 0) + (dofp[P + 1] >= 0);
+      const double ny = 1.0 + (dofp[P - px] >= 0) + (dofp[P + px] >= 0);
+      const double nz = DIM == 3 ? 1.0 + (dofp[P - pxy] >= 0) + (dofp[P + pxy] >= 0) : 1.0;
+      want += nx * ny * nz;
+    }
+    double k8[NB];   // cell with origin P − e
+#pragma unroll
+    for (int e = 0; e < NB; ++e) k8[e] = kc[P - (e & 1) - px * ((e >> 1) & 1) - pxy * (e >> 2)];
     const double sj = sc[j];
     const bool pj = isc && isc[j];
-    const int32_t p1 = colptr[j + 1];
-    for (int32_t p = colptr[j] + lane; p < p1; p += kLatLanes) {
-      const int32_t i = rowval[p];
-      const double a = nzval[p] * sj * sc[i];
-      const int code = lat_code((int64_t)map[i] - P, g);
-      if (code < 0 || (DIM == 2 && code / 9 != 1)) {
-        if (!(fabs(a) <= tol)) bad += 1.0;
-        continue;
+    const int32_t p0 = colptr[j], p1 = colptr[j + 1];
+    for (int32_t pb = p0; pb < p1; pb += U * kCheckLanes) {
+      int32_t iv[U];
+      double a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t p = pb + lane + u * kCheckLanes;
+        iv[u] = p < p1 ? rowval[p] : -1;
+        a[u] = p < p1 ? nzval[p] : 0.0;
       }
-      seen += 1.0;
-      const int dx = code % 3 - 1, dy = (code / 3) % 3 - 1, dz = code / 9 - 1;
-      double m = 0.0;
-      if (pj || (isc && isc[i])) {
-        m = i == j ? 1.0 : 0.0;
-      } else {
-        for (int e = 0; e < NB; ++e) {   // the cells with origin P − e that hold both corners
-          const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
-          const int bx = ex + dx, by = ey + dy, bz = ez + dz;
-          if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
-          m += kc[P - ex - px * ey - pxy * ez] * kr.k[cell_lid(ex, ey, ez) * NB + cell_lid(bx, by, bz)];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t i = iv[u];
+        if (i < 0) continue;
+        const double av = a[u] * sj * sc[i];
+        const int code = lat_code((int64_t)map[i] - P, g);
+        if (code < 0 || (DIM == 2 && code / 9 != 1)) {
+          if (!(fabs(av) <= tol)) bad += 1.0;
+          continue;
         }
-        m *= sj * sc[i];
+        seen += 1.0;
+        const int dx = code % 3 - 1, dy = (code / 3) % 3 - 1, dz = code / 9 - 1;
+        double m = 0.0;
+        if (pj || (isc && isc[i])) {
+          m = i == j ? 1.0 : 0.0;
+        } else {
+#pragma unroll
+          for (int e = 0; e < NB; ++e) {   // the cells with origin P − e that hold both corners
+            const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+            const int bx = ex + dx, by = ey + dy, bz = ez + dz;
+            const bool both = bx >= 0 && bx <= 1 && by >= 0 && by <= 1 && bz >= 0 && bz <= 1;
+            m += both ? k8[e] * kr.k[cell_lid(ex, ey, ez) * NB + cell_lid(bx & 1, by & 1, bz & 1)] : 0.0;
+          }
+          m *= sj * sc[i];
+        }
+        if (!(fabs(av - m) <= tol)) bad += 1.0;
       }
-      if (!(fabs(a - m) <= tol)) bad += 1.0;
     }
   }
   count2(bad, seen, cnt, 0, sh);
@@ -1882,7 +1909,7 @@ static int ensure_cells(fem_ctx* c, SolverWork* w, int jacobi, bool* ok) {
     if (cell_kref(c, cl.x0, cl.dim, &cl.kref) && cell_classes(cl.dim, &cl.kref, cl.f)) {
       k_cell_coef<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->ncells, cl.corg, c->d_param_cell, cl.kc64, cl.kc32);
       const LatGeom g{(uint32_t)l.px, (uint32_t)l.pxy, (uint32_t)(1 + l.px + l.pxy), l.ns, l.npad};
-      const int64_t rows_per_block = kBlock / kLatLanes;
+      const int64_t rows_per_block = kBlock / kCheckLanes;
       const int fb = (int)std::min<int64_t>(16384, (l.nrows + rows_per_block - 1) / rows_per_block);
       const double tol = sizeof(X) == 4 ? 1.0 / (1 << 22) : 1e-12;
       const uint8_t* isc = c->np > 0 ? c->d_isc : nullptr;

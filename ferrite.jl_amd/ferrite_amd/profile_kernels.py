@@ -71,9 +71,13 @@ def main():
     print(f"phase spmv: fp64 {ms64:.4f} ms, fp32 {ms32:.4f} ms per launch (nnz {K.nnz}, n {dh.ndofs})",
           flush=True)
     if args.solve:
-        for mode in ("cg", "mixed"):
+        # default operators (FP64: sliced ELL; mixed: cell form), then the mixed solver on the
+        # lattice planes and the FP64 solver on the cell form
+        for mode, tune in (("cg", {}), ("mixed", {}), ("mixed", {"cell_operator": -1}), ("cg", {"cell_operator": 1})):
+            fa.set_tuning(K, **tune)
             r = fa.solve(K, f, mode=mode, rtol=1e-10, download=False)
-            print(f"phase solve {mode}: {r.iterations} iterations, relres {r.relres:.2e}", flush=True)
+            print(f"phase solve {mode} [{r.operator}]: {r.iterations} iterations, relres {r.relres:.2e}", flush=True)
+        fa.set_tuning(K, cell_operator=0)
     if args.cg_sweep:
         for split, ce in ((-1, 8), (0, 4), (0, 8), (0, 16)):
             fa.set_tuning(K, cg_elems=ce, cg_split=split)
