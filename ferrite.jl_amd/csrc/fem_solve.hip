@@ -168,6 +168,13 @@ struct CellOp {
   int zn = 0, kn = 0, ntx = 0;
   int grid32 = 0, grid64 = 0, lz32 = 0, lz64 = 0, ldsz32 = 0, ldsz64 = 0;
   int64_t nitems32 = 0, nitems64 = 0;
+  // FP64 residual f − K u on the cell form: allowed when the last check found every entry
+  // within kCellTolR (versions it saw +1); u and K u in the padded numbering, the free-row mask,
+  // a scalar record with done = fresh = 0 for the SpMV's early-exit test
+  bool okr = false;
+  uint64_t verr = 0, auxr = 0, auxm = 0;
+  double *up64 = nullptr, *yp64 = nullptr, *tm64 = nullptr;
+  Scal* s0 = nullptr;
 };
 
 struct SolverWork {
@@ -217,7 +224,7 @@ void free_solver(fem_ctx* c) {
   dev_free(&l.mir32); dev_free(&l.mir64);
   CellOp& cl = w->cells;
   dev_free(&cl.corg); dev_free(&cl.kc64); dev_free(&cl.kc32); dev_free(&cl.t64); dev_free(&cl.t32);
-  dev_free(&cl.cnt);
+  dev_free(&cl.cnt); dev_free(&cl.up64); dev_free(&cl.yp64); dev_free(&cl.tm64); dev_free(&cl.s0);
   delete w;
   c->solver = nullptr;
 }
@@ -740,6 +747,54 @@ __global__ void k_cell_scale(int64_t npad, const int32_t* __restrict__ dofp, con
   }
 }
 
+// FP64 residual f − K u on the cell form, allowed when every entry of the scaled K matched it
+// to kCellTolR (a few FP64 roundings of Â's unit diagonal): u is spread into the padded
+// numbering, the FP64 cell SpMV with the free-row mask for t gives (K u)_P on free rows, and
+// prescribed rows use K_jj u_j from nzval.
+constexpr double kCellTolR = 1e-14;
+
+// up[map[j]] = u[j]; padding entries of up keep 0.
+__global__ void k_to_pad(int64_t n, const int32_t* __restrict__ map, const double* __restrict__ u,
+                         double* __restrict__ up) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+    up[map[j]] = u[j];
+}
+
+// m[P] = 1 on free rows, 0 on prescribed rows and padding.
+__global__ void k_cell_mask(int64_t npad, const int32_t* __restrict__ dofp, const uint8_t* __restrict__ isc,
+                            double* __restrict__ m) {
+  for (int64_t P = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; P < npad; P += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t j = dofp[P];
+    m[P] = (j >= 0 && !(isc && isc[j])) ? 1.0 : 0.0;
+  }
+}
+
+// res_j = f_j − (K u)_j: y[map[j]] on free rows, K_jj u_j on prescribed rows (apply! left only
+// the diagonal there); partial Σ res² per block.
+__global__ __launch_bounds__(kBlock) void k_res_cells(int64_t n, const int32_t* __restrict__ map,
+                                                      const double* __restrict__ yp, const double* __restrict__ f,
+                                                      const double* __restrict__ u, const uint8_t* __restrict__ isc,
+                                                      const int32_t* __restrict__ diag,
+                                                      const double* __restrict__ nzval, double* __restrict__ res,
+                                                      double* __restrict__ part) {
+  __shared__ double sh[kBlock / 64];
+  double acc = 0.0;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += (int64_t)gridDim.x * kBlock) {
+    double ku;
+    if (isc && isc[j]) {
+      const int32_t p = diag[j];
+      ku = (p >= 0 ? nzval[p] : 0.0) * u[j];
+    } else {
+      ku = yp[map[j]];
+    }
+    const double r = f[j] - ku;
+    res[j] = r;
+    acc += r * r;
+  }
+  const double b = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
 // Workgroup barrier that orders LDS only: waits for this wave's LDS operations, then meets the
 // other waves.  __syncthreads() also fences global memory, which waits for every load in
 // flight — including loads issued ahead for later steps, which this barrier leaves in flight.
@@ -795,7 +850,8 @@ __device__ __forceinline__ X cell_row(const X* zn, const X* k, const X* f) {
 // once.  Â's diagonal is 1, so tol is relative to it.  cnt[0] entries that differ by more
 // than tol, cnt[1] lattice-neighbour entries in the pattern, cnt[2] lattice neighbours of
 // the rows (the cell form couples every two corners of a cell, so each must be a pattern
-// entry); other entries must hold zero.
+// entry), cnt[3] entries that differ by more than tolr (the residual's bar); other entries
+// must hold zero.
 constexpr int kCheckLanes = 8;
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nrows, const int32_t* __restrict__ dofp,
@@ -804,14 +860,15 @@ __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nro
                                                        const double* __restrict__ nzval, const double* __restrict__ sc,
                                                        const uint8_t* __restrict__ isc, const int32_t* __restrict__ map,
                                                        LatGeom g, const double* __restrict__ kc, CellKref kr,
-                                                       double tol, unsigned long long* __restrict__ cnt) {
+                                                       double tol, double tolr,
+                                                       unsigned long long* __restrict__ cnt) {
   __shared__ double sh[kBlock / 64];
   constexpr int NB = 1 << DIM;
   constexpr int U = 4;   // entries per lane per pass
   const int lane = threadIdx.x % kCheckLanes;
   const int64_t groups = (int64_t)gridDim.x * (kBlock / kCheckLanes);
   const int64_t px = g.px, pxy = g.pxy;
-  double bad = 0.0, seen = 0.0, want = 0.0;
+  double bad = 0.0, badr = 0.0, seen = 0.0, want = 0.0;
   for (int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kCheckLanes; q < nrows; q += groups) {
     const int64_t P = r_lo + q;
     const int32_t j = dofp[P];
@@ -845,6 +902,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nro
         const int code = lat_code((int64_t)map[i] - P, g);
         if (code < 0 || (DIM == 2 && code / 9 != 1)) {
           if (!(fabs(av) <= tol)) bad += 1.0;
+          if (!(fabs(av) <= tolr)) badr += 1.0;
           continue;
         }
         seen += 1.0;
@@ -863,11 +921,12 @@ __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nro
           m *= sj * sc[i];
         }
         if (!(fabs(av - m) <= tol)) bad += 1.0;
+        if (!(fabs(av - m) <= tolr)) badr += 1.0;
       }
     }
   }
   count2(bad, seen, cnt, 0, sh);
-  count2(want, 0.0, cnt, 2, sh);
+  count2(want, badr, cnt, 2, sh);
 }
 
 // w = Â x in the cell form, rows [r_lo, r_lo + nrows) of the padded numbering, one row per
@@ -1916,14 +1975,30 @@ static int ensure_cells(fem_ctx* c, SolverWork* w, int jacobi, bool* ok) {
       FEM_HIP(c, hipMemsetAsync(cl.cnt, 0, 4 * sizeof(unsigned long long), c->stream));
       if (cl.dim == 3)
         k_cell_check<3><<<fb, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.dofp, c->d_colptr, c->d_rowval, c->d_nzval,
-                                                      w->sc, isc, l.map, g, cl.kc64, cl.kref, tol, cl.cnt);
+                                                      w->sc, isc, l.map, g, cl.kc64, cl.kref, tol, kCellTolR,
+                                                      cl.cnt);
       else
         k_cell_check<2><<<fb, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.dofp, c->d_colptr, c->d_rowval, c->d_nzval,
-                                                      w->sc, isc, l.map, g, cl.kc64, cl.kref, tol, cl.cnt);
+                                                      w->sc, isc, l.map, g, cl.kc64, cl.kref, tol, kCellTolR,
+                                                      cl.cnt);
       unsigned long long h[4] = {};
       FEM_HIP(c, hipMemcpyAsync(h, cl.cnt, sizeof h, hipMemcpyDeviceToHost, c->stream));
       FEM_HIP(c, hipStreamSynchronize(c->stream));
       pass = h[0] == 0 && h[1] == h[2];
+      cl.okr = h[3] == 0 && h[1] == h[2];
+      cl.verr = c->nzval_version + 1;
+      cl.auxr = c->aux_version + 1;
+      if (cl.okr && !cl.up64) {
+        if ((rc = dev_alloc(c, &cl.up64, l.npad)) || (rc = dev_alloc(c, &cl.yp64, l.npad)) ||
+            (rc = dev_alloc(c, &cl.tm64, l.npad)) || (rc = dev_alloc(c, &cl.s0, 1)))
+          return rc;
+        FEM_HIP(c, hipMemsetAsync(cl.up64, 0, 8 * l.npad, c->stream));
+        FEM_HIP(c, hipMemsetAsync(cl.s0, 0, sizeof(Scal), c->stream));
+      }
+      if (cl.okr && cl.auxm != c->aux_version + 1) {
+        k_cell_mask<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(l.npad, l.dofp, isc, cl.tm64);
+        cl.auxm = c->aux_version + 1;
+      }
     }
     ver = c->nzval_version + 1;
     aux = c->aux_version + 1;
@@ -1934,13 +2009,15 @@ static int ensure_cells(fem_ctx* c, SolverWork* w, int jacobi, bool* ok) {
 }
 
 template <typename X, int DIM, int NE>
-static void launch_cells_ne(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc) {
+static void launch_cells_ne(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc,
+                            const X* tover = nullptr) {
   const LatOp& l = w->lat;
   const CellOp& cl = w->cells;
   CellKrefT<X> kr;
   for (int i = 0; i < 8; ++i) kr.f[i] = (X)cl.f[i];
   const X* kc = sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.kc32) : reinterpret_cast<const X*>(cl.kc64);
-  const X* t = sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.t32) : reinterpret_cast<const X*>(cl.t64);
+  const X* t = tover ? tover
+                     : sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.t32) : reinterpret_cast<const X*>(cl.t64);
   const int lds = sizeof(X) == 4 ? cl.lds32 : cl.lds64;
   const int grid = sizeof(X) == 4 ? cl.grid32 : cl.grid64;
   if (c->tune.sell_map != 1 && grid % 8 == 0)
@@ -1952,13 +2029,15 @@ static void launch_cells_ne(fem_ctx* c, const SolverWork* w, const X* x, X* y, d
 }
 
 template <typename X, int ZN, int KN>
-static void launch_cells_z(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc) {
+static void launch_cells_z(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc,
+                           const X* tover = nullptr) {
   const LatOp& l = w->lat;
   const CellOp& cl = w->cells;
   CellKrefT<X> kr;
   for (int i = 0; i < 8; ++i) kr.f[i] = (X)cl.f[i];
   const X* kc = sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.kc32) : reinterpret_cast<const X*>(cl.kc64);
-  const X* t = sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.t32) : reinterpret_cast<const X*>(cl.t64);
+  const X* t = tover ? tover
+                     : sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.t32) : reinterpret_cast<const X*>(cl.t64);
   const int lds = sizeof(X) == 4 ? cl.ldsz32 : cl.ldsz64;
   const int32_t pz = (int32_t)(l.npad / l.pxy);
   constexpr int D = sizeof(X) == 4 ? 3 : 2;   // steps in flight (register stages)
@@ -1974,28 +2053,30 @@ static void launch_cells_z(fem_ctx* c, const SolverWork* w, const X* x, X* y, do
 
 // NE: window entries per thread, the smallest instantiated bound of ⌈E / kBlock⌉
 template <typename X, int DIM>
-static void launch_cells_dim(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc) {
+static void launch_cells_dim(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc,
+                             const X* tover = nullptr) {
   if (DIM == 3 && w->cells.zn > 0) {
     const int zn = w->cells.zn;
-    if (zn == 2) launch_cells_z<X, 2, 2>(c, w, x, y, part, sc);
-    else if (zn == 3) launch_cells_z<X, 3, 2>(c, w, x, y, part, sc);
-    else launch_cells_z<X, 4, 3>(c, w, x, y, part, sc);
+    if (zn == 2) launch_cells_z<X, 2, 2>(c, w, x, y, part, sc, tover);
+    else if (zn == 3) launch_cells_z<X, 3, 2>(c, w, x, y, part, sc, tover);
+    else launch_cells_z<X, 4, 3>(c, w, x, y, part, sc, tover);
     return;
   }
   const int ne = w->cells.ne;
-  if (ne <= 4) launch_cells_ne<X, DIM, 4>(c, w, x, y, part, sc);
-  else if (ne <= 6) launch_cells_ne<X, DIM, 6>(c, w, x, y, part, sc);
-  else if (ne <= 8) launch_cells_ne<X, DIM, 8>(c, w, x, y, part, sc);
-  else if (ne <= 10) launch_cells_ne<X, DIM, 10>(c, w, x, y, part, sc);
-  else if (ne <= 12) launch_cells_ne<X, DIM, 12>(c, w, x, y, part, sc);
-  else if (ne <= 16) launch_cells_ne<X, DIM, 16>(c, w, x, y, part, sc);
-  else launch_cells_ne<X, DIM, 24>(c, w, x, y, part, sc);
+  if (ne <= 4) launch_cells_ne<X, DIM, 4>(c, w, x, y, part, sc, tover);
+  else if (ne <= 6) launch_cells_ne<X, DIM, 6>(c, w, x, y, part, sc, tover);
+  else if (ne <= 8) launch_cells_ne<X, DIM, 8>(c, w, x, y, part, sc, tover);
+  else if (ne <= 10) launch_cells_ne<X, DIM, 10>(c, w, x, y, part, sc, tover);
+  else if (ne <= 12) launch_cells_ne<X, DIM, 12>(c, w, x, y, part, sc, tover);
+  else if (ne <= 16) launch_cells_ne<X, DIM, 16>(c, w, x, y, part, sc, tover);
+  else launch_cells_ne<X, DIM, 24>(c, w, x, y, part, sc, tover);
 }
 
 template <typename X>
-static void launch_cells(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc) {
-  if (w->cells.dim == 3) launch_cells_dim<X, 3>(c, w, x, y, part, sc);
-  else launch_cells_dim<X, 2>(c, w, x, y, part, sc);
+static void launch_cells(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc,
+                         const X* tover = nullptr) {
+  if (w->cells.dim == 3) launch_cells_dim<X, 3>(c, w, x, y, part, sc, tover);
+  else launch_cells_dim<X, 2>(c, w, x, y, part, sc, tover);
 }
 
 // SELL structure (once per pattern): widths/bases on the device, offsets on the host.
@@ -2301,6 +2382,17 @@ static void reseed(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, const double* 
 
 // res = f − K u and s->rr = ||res||² with the unscaled FP64 K (CSC arrays, fixed order).
 static void residual(fem_ctx* c, SolverWork* w) {
+  const CellOp& cl = w->cells;
+  if (w->use_cells && cl.okr && cl.up64 && cl.verr == c->nzval_version + 1 && cl.auxr == c->aux_version + 1 &&
+      cl.auxm == c->aux_version + 1) {   // on the cell form (k_cell_check passed at kCellTolR)
+    const uint8_t* isc = c->np > 0 ? c->d_isc : nullptr;
+    k_to_pad<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, w->lat.map, c->d_u, cl.up64);
+    launch_cells<double>(c, w, cl.up64, cl.yp64, w->part_s, cl.s0, cl.tm64);
+    k_res_cells<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, w->lat.map, cl.yp64, c->d_f, c->d_u, isc, c->d_diag,
+                                                         c->d_nzval, w->res, w->part_s);
+    k_sum_to<<<1, kWide, 0, c->stream>>>(kMaxVecBlocks, w->part_s, w->s, 0);
+    return;
+  }
   launch_spmv<double, double, 1>(c, w->sh64, c->d_nzval, c->d_u, c->d_f, w->res, w->part_s, w->s);
   k_sum_to<<<1, kWide, 0, c->stream>>>(w->sh64.blocks, w->part_s, w->s, 0);
 }
