@@ -192,7 +192,7 @@ def main():
 
     spmv_ms = results["mixed"]["spmv_avg_ms"]               # inside the CG loop of the timed steps
     if results["mixed"]["operator"] == "cells":
-        SPMV32_KERNEL = "k_cell_spmv_z<float, 1, 3, 2, 3>"
+        SPMV32_KERNEL = "k_cell_spmv_z<float, 1, 4, 3, 2, 2>"   # two rows per thread
         spmv_bytes = 3 * 4 * npad + 4 * (px - 2) * px * px
     elif results["mixed"]["operator"] == "lattice":
         SPMV32_KERNEL, spmv_bytes, _ = lattice_roofline(results["mixed"]["unit_diagonal"], spmv_ms)

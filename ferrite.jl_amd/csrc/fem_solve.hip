@@ -163,9 +163,10 @@ struct CellOp {
   unsigned long long* cnt = nullptr; // [4] check counters
   int lds32 = 0, lds64 = 0;    // dynamic LDS of the SpMV (bytes)
   int ne = 0;                  // window entries per thread of the SpMV
-  // 3D: the z-marching SpMV (zn > 0): t·x and coefficient window entries per thread, in-plane
-  // tiles; per precision the blocks, planes per chunk, work items and LDS bytes
-  int zn = 0, kn = 0, ntx = 0;
+  // 3D: the z-marching SpMV, per precision [FP32, FP64] (zn > 0): t·x and coefficient window
+  // entries per thread, rows per thread, in-plane tiles; the blocks, planes per chunk, work
+  // items and LDS bytes
+  int zn[2] = {}, kn[2] = {}, rpt[2] = {}, ntx[2] = {};
   int grid32 = 0, grid64 = 0, lz32 = 0, lz64 = 0, ldsz32 = 0, ldsz64 = 0;
   int64_t nitems32 = 0, nitems64 = 0;
   // FP64 residual f − K u on the cell form: allowed when the last check found every entry
@@ -1050,7 +1051,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv(int64_t r_lo, int64_t nrow
 // Padding rows of the marched planes are computed too (t = 0 there, so y = x = 0).  Items are
 // ordered chunk-major; MAP 1: the blocks of one XCD sweep one contiguous eighth of the items,
 // so neighbouring tiles (which share their halo rows) run on the same XCD at the same time.
-template <typename X, int MAP, int ZN, int KN, int D>
+template <typename X, int MAP, int ZN, int KN, int D, int RPT>
 __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px, int32_t pxy, int32_t pz,
                                                         int32_t ntx, int32_t lz, int64_t nitems,
                                                         const X* __restrict__ kc, const X* __restrict__ t,
@@ -1060,11 +1061,12 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
   if (s->done || s->fresh) return;
   extern __shared__ __align__(16) unsigned char cell_lds[];
   __shared__ double sh[kBlock / 64];
-  const int W = kBlock + 2 * px + 2, WK = kBlock + px + 1;   // W ≤ ZN kBlock, WK ≤ KN kBlock (host)
+  constexpr int T = RPT * kBlock;            // rows of a tile: RPT per thread, kBlock apart
+  const int W = T + 2 * px + 2, WK = T + px + 1;   // W ≤ ZN kBlock, WK ≤ KN kBlock (host)
   const int nsteps = (lz + D - 1) / D * D;
   X* zw = reinterpret_cast<X*>(cell_lds);   // [3][W]
   X* kw = zw + 3 * W;                        // [2][WK]
-  X* yb = kw + 2 * WK;                       // [nsteps][kBlock] rows of the item's planes
+  X* yb = kw + 2 * WK;                       // [nsteps][T] rows of the item's planes
   // this thread's window entries: offsets from the tile's first row of plane p (t·x) and of
   // layer p − 1 (coefficients); entries past a window read row 0
   int32_t relz[ZN], relk[KN];
@@ -1093,7 +1095,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
     i1 = nitems;
     istep = gridDim.x;
   }
-  X vt[D][ZN], vx[D][ZN], vk[D][KN], tn[D], xn[D];
+  X vt[D][ZN], vx[D][ZN], vk[D][KN], tn[D][RPT], xn[D][RPT];
   // step p of tile offset o0 into stage u (q wraps below 0 to beyond npad: row 0)
   auto fetch = [&](int u, int32_t p, int32_t o0) {
     const uint32_t base = (uint32_t)p * (uint32_t)pxy + (uint32_t)o0;
@@ -1110,10 +1112,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
       const uint32_t qq = (ink[k] && q < np32) ? q : 0u;
       vk[u][k] = at_byte(kc, qq * (uint32_t)sizeof(X));
     }
-    const uint32_t P = base - (uint32_t)pxy + threadIdx.x;
-    const uint32_t PP = P < np32 ? P : 0u;
-    tn[u] = at_byte(t, PP * (uint32_t)sizeof(X));
-    xn[u] = at_byte(x, PP * (uint32_t)sizeof(X));
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const uint32_t P = base - (uint32_t)pxy + (uint32_t)(r * kBlock) + threadIdx.x;
+      const uint32_t PP = P < np32 ? P : 0u;
+      tn[u][r] = at_byte(t, PP * (uint32_t)sizeof(X));
+      xn[u][r] = at_byte(x, PP * (uint32_t)sizeof(X));
+    }
   };
   auto store = [&](int u, int32_t p) {   // z slot p mod 3, coefficient slot (p − 1) mod 2
     X* zs = zw + (p % 3) * W;
@@ -1129,8 +1134,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
   for (int64_t item = i0; item < i1; item += istep) {
     const int32_t c = (int32_t)(item / ntx), j = (int32_t)(item - (int64_t)c * ntx);
     const int32_t z0 = 1 + c * lz, z1 = min(z0 + lz, pz - 1);
-    const int32_t o0 = j * kBlock;
-    const bool col = o0 + (int32_t)threadIdx.x < pxy;   // this thread's row exists in the plane
+    const int32_t o0 = j * T;
     fetch(0, z0 - 1, o0);
     lds_barrier();     // the previous item's LDS reads are done
     store(0, z0 - 1);
@@ -1142,41 +1146,53 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
 #pragma unroll
       for (int u = 0; u < D; ++u) {
         const int32_t zp = z0 + sb + u;
-        const X tc = tn[u], xc = xn[u];   // row of plane zp
+        X tc[RPT], xc[RPT];   // rows of plane zp
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          tc[r] = tn[u][r];
+          xc[r] = xn[u][r];
+        }
         store(u, zp + 1);
         lds_barrier();
         fetch(u, zp + 1 + D, o0);
-        const int l = threadIdx.x + px + 1;   // the row in the windows
         const X* zm = zw + ((zp + 2) % 3) * W;   // plane zp − 1
         const X* z0w = zw + (zp % 3) * W;
         const X* zpw = zw + ((zp + 1) % 3) * W;
-        X zn[27];
-#pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-          for (int dx = 0; dx < 3; ++dx) {
-            const int o = l + (dx - 1) + px * (dy - 1);
-            zn[dy * 3 + dx] = zm[o];
-            zn[9 + dy * 3 + dx] = z0w[o];
-            zn[18 + dy * 3 + dx] = zpw[o];
-          }
         const X* k0 = kw + (zp & 1) * WK;         // layer zp
         const X* k1 = kw + ((zp - 1) & 1) * WK;   // layer zp − 1
-        X kn[8];                                  // cell with origin P − e: P is its corner e
 #pragma unroll
-        for (int e = 0; e < 8; ++e) kn[e] = (e >> 2 ? k1 : k0)[l - (e & 1) - px * ((e >> 1) & 1)];
-        const X acc = cell_row<X, 3>(zn, kn, kr.f);
-        const X yv = tc != X(0) ? tc * acc : xc;
-        yb[(sb + u) * kBlock + threadIdx.x] = yv;
-        d += (col && zp < z1) ? (double)xc * (double)yv : 0.0;
+        for (int r = 0; r < RPT; ++r) {
+          const int l = r * kBlock + threadIdx.x + px + 1;   // the row in the windows
+          X zn[27];
+#pragma unroll
+          for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+              const int o = l + (dx - 1) + px * (dy - 1);
+              zn[dy * 3 + dx] = zm[o];
+              zn[9 + dy * 3 + dx] = z0w[o];
+              zn[18 + dy * 3 + dx] = zpw[o];
+            }
+          X kn[8];                                  // cell with origin P − e: P is its corner e
+#pragma unroll
+          for (int e = 0; e < 8; ++e) kn[e] = (e >> 2 ? k1 : k0)[l - (e & 1) - px * ((e >> 1) & 1)];
+          const X acc = cell_row<X, 3>(zn, kn, kr.f);
+          const X yv = tc[r] != X(0) ? tc[r] * acc : xc[r];
+          yb[(sb + u) * T + r * kBlock + threadIdx.x] = yv;
+          const bool col = o0 + r * kBlock + (int32_t)threadIdx.x < pxy;
+          d += (col && zp < z1) ? (double)xc[r] * (double)yv : 0.0;
+        }
         lds_barrier();     // this plane's reads are done before the next store
       }
     }
-    if (col)
-      for (int32_t pl = 0; pl < z1 - z0; ++pl) {
-        const uint32_t P = (uint32_t)(z0 + pl) * (uint32_t)pxy + (uint32_t)(o0 + threadIdx.x);
-        *reinterpret_cast<X*>(reinterpret_cast<char*>(y) + P * (uint32_t)sizeof(X)) = yb[pl * kBlock + threadIdx.x];
-      }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      if (o0 + r * kBlock + (int32_t)threadIdx.x < pxy)
+        for (int32_t pl = 0; pl < z1 - z0; ++pl) {
+          const uint32_t P = (uint32_t)(z0 + pl) * (uint32_t)pxy + (uint32_t)(o0 + r * kBlock + threadIdx.x);
+          *reinterpret_cast<X*>(reinterpret_cast<char*>(y) + P * (uint32_t)sizeof(X)) =
+              yb[pl * T + r * kBlock + threadIdx.x];
+        }
   }
   const double b = block_sum(d, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
@@ -1862,22 +1878,46 @@ static bool cell_kref(const fem_ctx* c, const double (*x0)[3], int dim, CellKref
   return true;
 }
 
+// Steps in flight (register stages) of the z-marching cell SpMV: three for FP32 with one row
+// per thread, otherwise two (register pressure).
+constexpr int cell_stages(int bytes, int rpt) { return bytes == 4 && rpt == 1 ? 3 : 2; }
+
 // Launch shape of the cell-form SpMV for values of `bytes` bytes.  Measured at 128³ (in the
-// solve, z-marching, XCD ranges): FP32 768 blocks 19.0 µs, 1024 22.2, 1280 25.6, 1536 21.1,
-// 2048 24.9; FP64 768 32.2 µs, 1024 33.8, 1280 40.2, 1536 34.2, 2048 39.1.  Z-marching: about
-// one work item per block (the planes per chunk follow from the block count).
+// solve, z-marching, XCD ranges): FP32 two rows per thread 512 blocks 20.6 µs, 768 17.7, 1024
+// 23.8, 1536 22.9; one row per thread 512 32.2, 768 19.5, 1024 22.1, 1536 20.8; FP64 (one row)
+// 512 37.5, 768 32.3, 1024 33.9, 1536 34.0.  Z-marching: about one work item per block (the
+// planes per chunk follow from the block count).
 static void cell_shape(const fem_ctx* c, const LatOp& l, CellOp* cl, int bytes) {
   const int64_t want = c->tune.cell_blocks ? c->tune.cell_blocks : 768;
+  const int pi = bytes == 4 ? 0 : 1;
   int& grid = bytes == 4 ? cl->grid32 : cl->grid64;
-  if (cl->zn > 0) {
+  cl->zn[pi] = 0;
+  if (cl->dim == 3 && c->tune.cell_march != -1) {
+    // z-marching: rows per thread (two only in FP32: the FP64 windows would not fit in LDS),
+    // then the window entries per thread (ZN, KN) of an instantiated kernel
+    int rpt = bytes == 4 ? (c->tune.cell_rows ? c->tune.cell_rows : 2) : 1;
+    if (rpt == 2 && l.px > 255) rpt = 1;
+    const int64_t T = rpt * kBlock;
+    const int zn = (int)((T + 2 * l.px + 2 + kBlock - 1) / kBlock), kn = (int)((T + l.px + 1 + kBlock - 1) / kBlock);
+    const bool inst = rpt == 1 ? ((zn == 2 && kn == 2) || (zn == 3 && kn == 2) || (zn == 4 && kn == 3))
+                               : ((zn == 3 && kn == 3) || (zn == 4 && kn == 3));
+    if (inst) {
+      cl->zn[pi] = zn;
+      cl->kn[pi] = kn;
+      cl->rpt[pi] = rpt;
+      cl->ntx[pi] = (int)((l.pxy + T - 1) / T);
+    }
+  }
+  if (cl->zn[pi] > 0) {
     const int64_t pz = l.npad / l.pxy;
-    const int D = bytes == 4 ? 3 : 2;   // register stages of the kernel
+    const int D = cell_stages(bytes, cl->rpt[pi]);   // register stages of the kernel
+    const int64_t T = cl->rpt[pi] * kBlock;
     int& lz = bytes == 4 ? cl->lz32 : cl->lz64;
     int64_t& nitems = bytes == 4 ? cl->nitems32 : cl->nitems64;
-    lz = (int)std::min<int64_t>(16, std::max<int64_t>(2, (cl->ntx * (pz - 2) + want - 1) / want));
-    nitems = (int64_t)cl->ntx * ((pz - 2 + lz - 1) / lz);
+    lz = (int)std::min<int64_t>(16, std::max<int64_t>(2, (cl->ntx[pi] * (pz - 2) + want - 1) / want));
+    nitems = (int64_t)cl->ntx[pi] * ((pz - 2 + lz - 1) / lz);
     grid = (int)std::max<int64_t>(8, std::min<int64_t>(want, (nitems + 7) / 8 * 8));
-    const int64_t words = 3 * (kBlock + 2 * l.px + 2) + 2 * (kBlock + l.px + 1) + (lz + D - 1) / D * D * kBlock;
+    const int64_t words = 3 * (T + 2 * l.px + 2) + 2 * (T + l.px + 1) + (lz + D - 1) / D * D * T;
     (bytes == 4 ? cl->ldsz32 : cl->ldsz64) = (int)(bytes * words);
   } else {
     const int64_t ntiles = (l.nrows + kBlock - 1) / kBlock;
@@ -1945,13 +1985,6 @@ static int ensure_cells(fem_ctx* c, SolverWork* w, int jacobi, bool* ok) {
     cl.ne = (int)((words + kBlock - 1) / kBlock);
     cl.lds32 = (int)(4 * words);
     cl.lds64 = (int)(8 * words);
-    // z-marching windows per thread: (ZN, KN) = (2, 2) up to px = 127, (3, 2) up to 255, (4, 3)
-    // up to 383; wider lattices use independent tiles
-    if (dim == 3 && l.px <= 383 && c->tune.cell_march != -1) {
-      cl.zn = l.px <= 127 ? 2 : l.px <= 255 ? 3 : 4;
-      cl.kn = l.px <= 255 ? 2 : 3;
-      cl.ntx = (int)((l.pxy + kBlock - 1) / kBlock);
-    }
     cell_shape(c, l, &cl, 4);
     cell_shape(c, l, &cl, 8);
     cl.state = 1;
@@ -2028,7 +2061,7 @@ static void launch_cells_ne(fem_ctx* c, const SolverWork* w, const X* x, X* y, d
                                                                  (int32_t)l.pxy, kc, t, x, y, part, sc, kr);
 }
 
-template <typename X, int ZN, int KN>
+template <typename X, int ZN, int KN, int RPT>
 static void launch_cells_z(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc,
                            const X* tover = nullptr) {
   const LatOp& l = w->lat;
@@ -2040,26 +2073,35 @@ static void launch_cells_z(fem_ctx* c, const SolverWork* w, const X* x, X* y, do
                      : sizeof(X) == 4 ? reinterpret_cast<const X*>(cl.t32) : reinterpret_cast<const X*>(cl.t64);
   const int lds = sizeof(X) == 4 ? cl.ldsz32 : cl.ldsz64;
   const int32_t pz = (int32_t)(l.npad / l.pxy);
-  constexpr int D = sizeof(X) == 4 ? 3 : 2;   // steps in flight (register stages)
+  constexpr int D = cell_stages(sizeof(X), RPT);   // steps in flight (register stages)
   const int grid = sizeof(X) == 4 ? cl.grid32 : cl.grid64, lz = sizeof(X) == 4 ? cl.lz32 : cl.lz64;
   const int64_t nitems = sizeof(X) == 4 ? cl.nitems32 : cl.nitems64;
+  const int ntx = cl.ntx[sizeof(X) == 4 ? 0 : 1];
   if (c->tune.sell_map != 1 && grid % 8 == 0)
-    k_cell_spmv_z<X, 1, ZN, KN, D><<<grid, kBlock, lds, c->stream>>>(l.npad, (int32_t)l.px, (int32_t)l.pxy, pz, cl.ntx,
-                                                                 lz, nitems, kc, t, x, y, part, sc, kr);
+    k_cell_spmv_z<X, 1, ZN, KN, D, RPT><<<grid, kBlock, lds, c->stream>>>(l.npad, (int32_t)l.px, (int32_t)l.pxy, pz,
+                                                                          ntx, lz, nitems, kc, t, x, y, part, sc, kr);
   else
-    k_cell_spmv_z<X, 0, ZN, KN, D><<<grid, kBlock, lds, c->stream>>>(l.npad, (int32_t)l.px, (int32_t)l.pxy, pz, cl.ntx,
-                                                                 lz, nitems, kc, t, x, y, part, sc, kr);
+    k_cell_spmv_z<X, 0, ZN, KN, D, RPT><<<grid, kBlock, lds, c->stream>>>(l.npad, (int32_t)l.px, (int32_t)l.pxy, pz,
+                                                                          ntx, lz, nitems, kc, t, x, y, part, sc, kr);
 }
 
 // NE: window entries per thread, the smallest instantiated bound of ⌈E / kBlock⌉
 template <typename X, int DIM>
 static void launch_cells_dim(fem_ctx* c, const SolverWork* w, const X* x, X* y, double* part, const Scal* sc,
                              const X* tover = nullptr) {
-  if (DIM == 3 && w->cells.zn > 0) {
-    const int zn = w->cells.zn;
-    if (zn == 2) launch_cells_z<X, 2, 2>(c, w, x, y, part, sc, tover);
-    else if (zn == 3) launch_cells_z<X, 3, 2>(c, w, x, y, part, sc, tover);
-    else launch_cells_z<X, 4, 3>(c, w, x, y, part, sc, tover);
+  const int pi = sizeof(X) == 4 ? 0 : 1;
+  if (DIM == 3 && w->cells.zn[pi] > 0) {
+    const int zn = w->cells.zn[pi];
+    if (w->cells.rpt[pi] == 2) {   // FP32 only
+      if constexpr (sizeof(X) == 4) {
+        if (zn == 3) launch_cells_z<X, 3, 3, 2>(c, w, x, y, part, sc, tover);
+        else launch_cells_z<X, 4, 3, 2>(c, w, x, y, part, sc, tover);
+      }
+    } else {
+      if (zn == 2) launch_cells_z<X, 2, 2, 1>(c, w, x, y, part, sc, tover);
+      else if (zn == 3) launch_cells_z<X, 3, 2, 1>(c, w, x, y, part, sc, tover);
+      else launch_cells_z<X, 4, 3, 1>(c, w, x, y, part, sc, tover);
+    }
     return;
   }
   const int ne = w->cells.ne;
@@ -2575,7 +2617,7 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
       !(t->cg_split == 0 || t->cg_split == 1 || t->cg_split == -1) ||
       !(t->cell_operator == 0 || t->cell_operator == 1 || t->cell_operator == -1) ||
       !(t->cell_blocks == 0 || (t->cell_blocks >= 8 && t->cell_blocks <= kMaxVecBlocks && t->cell_blocks % 8 == 0)) ||
-      !(t->cell_march == 0 || t->cell_march == -1))
+      !(t->cell_march == 0 || t->cell_march == -1) || !(t->cell_rows >= 0 && t->cell_rows <= 2))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

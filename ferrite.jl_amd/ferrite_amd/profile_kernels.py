@@ -156,9 +156,11 @@ def main():
                   f"{fa.timings(K)['solve_ms']:.1f} ms ({r.iterations} its)", flush=True)
         fa.set_tuning(K, lattice_blocks=0)
     if args.cell_sweep:
-        for march, smap, mname in ((0, 0, "z-march, XCD ranges"), (-1, 0, "tiles, XCD ranges")):
-            for blocks in (768, 1024, 1280, 1536, 2048):
-                fa.set_tuning(K, cell_operator=1, cell_blocks=blocks, sell_map=smap, cell_march=march)
+        for march, rows, mname in ((0, 2, "z-march, 2 rows/thread"), (0, 1, "z-march, 1 row/thread"),
+                                   (-1, 0, "tiles")):
+            for blocks in (512, 768, 1024, 1536):
+                smap = 0
+                fa.set_tuning(K, cell_operator=1, cell_blocks=blocks, sell_map=smap, cell_march=march, cell_rows=rows)
                 a64 = fa.spmv_bench(K, mixed=False, reps=20)
                 a32 = fa.spmv_bench(K, mixed=True, reps=20)
                 fa.set_tuning(K, cell_operator=1, cell_blocks=blocks, sell_map=smap, time_kernels=1)
@@ -175,7 +177,7 @@ def main():
                     loops.append(f"{mode} {fa.timings(K)['solve_ms']:.1f} ms")
                 print(f"cell operator {mname} {blocks} blocks: isolated fp64 {a64 * 1e3:.1f} us, fp32 {a32 * 1e3:.1f} us; "
                       f"in-solve {'; '.join(line)}; graph loops {', '.join(loops)}", flush=True)
-        fa.set_tuning(K, cell_operator=0, cell_blocks=0, sell_map=0, cell_march=0)
+        fa.set_tuning(K, cell_operator=0, cell_blocks=0, sell_map=0, cell_march=0, cell_rows=0)
     if args.sell_sweep:
         for smap, mname, unrolls in ((1, "grid-stride, ordinary loads", (4, 8, 16)), (2, "contiguous/XCD", (4, 8)),
                                      (3, "grid-stride, non-temporal loads", (4, 8))):

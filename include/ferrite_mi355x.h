@@ -203,6 +203,8 @@ typedef struct {
   int32_t cell_march;                /* 3D cell form: 0 (default) = each block marches a tile up
                                         through a chunk of planes, loading one plane per step;
                                         -1 = independent tiles of rows (three planes per tile) */
+  int32_t cell_rows;                 /* 3D cell form, FP32: rows per thread of the marching SpMV,
+                                        1 or 2 (0 = default 2; FP64 always 1) */
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
 int32_t fem_get_tuning(const fem_ctx* ctx, fem_tuning* t);
