@@ -221,7 +221,8 @@ typedef struct {
 } fem_timings;
 int32_t fem_get_timings(const fem_ctx* ctx, fem_timings* t);
 /* Average duration (HIP events) of `reps` back-to-back SpMV launches y = K x of the solver,
- * FP64 (mixed = 0) or FP32 values and vectors (mixed = 1).  The solver's dominant kernel. */
+ * FP64 (mixed = 0) or FP32 values and vectors (mixed = 1), on the operator copy the solver
+ * would use (cell form, lattice planes or sliced ELL).  The solver's dominant kernel. */
 int32_t fem_spmv_bench(fem_ctx* ctx, int32_t mixed, int32_t reps, double* avg_ms);
 
 /* ---- compliance adjoint (no counterpart in the reference; the compliance of

@@ -24,7 +24,7 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def _system(ct, nel, right, k, cells_hook=None, cell_operator=-1):
+def _system(ct, nel, right, k, cells_hook=None, cell_operator=-1, bc_sets=("left", "right")):
     """Assembled and constrained system; the cell form is off unless asked for, so that the
     plane tests exercise the planes."""
     g = fa.generate_grid(ct, nel, tuple(0.0 for _ in nel), right)
@@ -38,7 +38,7 @@ def _system(ct, nel, right, k, cells_hook=None, cell_operator=-1):
     f = fa.allocate_vector(K)
     fa.assemble_global(fa.start_assemble(K, f), cv, fa.create_coloring(g), fa.PoissonElement(k=k, source=1.0))
     ch = fa.ConstraintHandler(dh)
-    ch.add(fa.Dirichlet("u", g.facetsets["left"] + g.facetsets["right"], lambda x, t: 0.0)).close()
+    ch.add(fa.Dirichlet("u", [fc for b in bc_sets for fc in g.facetsets[b]], lambda x, t: 0.0)).close()
     fa.apply(K, f, ch)
     return g, dh, K, f
 
@@ -225,3 +225,22 @@ def test_cell_operator_launch_variants_40cube():
         fa.set_tuning(K, cell_operator=1)
         r64 = fa.solve(K, f, mode="cg", rtol=1e-12)
         assert r64.operator == "cells" and _rel(r64.u, ref.u) < 1e-9, tune
+
+
+@pytest.mark.parametrize("nel", [(300, 3, 2), (400, 3, 2)])
+def test_cell_operator_wide_lattices(nel):
+    """Long lattices: px = 303 marches with one row per thread and (4, 3) window entries per
+    thread (two rows per thread need px ≤ 255), px = 403 uses independent tiles; both
+    against the FP64 sliced solve.  All six faces are constrained (most rows prescribed),
+    which also keeps the problem well conditioned."""
+    print("case cell operator wide lattice", nel, flush=True)
+    k = poisson_conductivity(nel, seed=18)
+    g, dh, K, f = _system("Hexahedron", nel, (3.0, 0.03, 0.02), k, cell_operator=-1,
+                          bc_sets=("left", "right", "front", "back", "bottom", "top"))
+    fa.set_tuning(K, lattice=-1)
+    ref = fa.solve(K, f, mode="cg", rtol=1e-12)
+    fa.set_tuning(K, lattice=0, cell_operator=1)
+    for mode in ("mixed", "cg"):
+        res = fa.solve(K, f, mode=mode, rtol=1e-12)
+        assert res.operator == "cells", mode
+        assert res.relres <= 1e-12 and _rel(res.u, ref.u) < 1e-8, mode
