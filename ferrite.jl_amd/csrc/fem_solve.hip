@@ -2368,9 +2368,19 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache
                     : w->use_lat ? (sizeof(X) == 4 ? (const void*)w->lat.val32 : (const void*)w->lat.val64)
                                  : (sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64);
   // the lattice SpMV variant depends on the values (a unit diagonal), not only on the operands;
-  // the cell form's K¹ is a kernel argument, so its graph is keyed on the check's version
-  const uintptr_t variant = w->use_cells ? 3 + 2 * (sizeof(X) == 4 ? w->cells.ver32 : w->cells.ver64)
-                            : w->use_lat ? 1 + (sizeof(X) == 4 ? w->lat.unit32 : w->lat.unit64) : 0;
+  // the cell form's K¹ class values are kernel arguments, so its graph is keyed on their bits
+  // (a new K with new k_e reuses the graph: k_e is read from memory)
+  uintptr_t variant = w->use_lat ? 1 + (sizeof(X) == 4 ? w->lat.unit32 : w->lat.unit64) : 0;
+  if (w->use_cells) {
+    uint64_t h = 1469598103934665603ull;   // FNV-1a over the eight values in X precision
+    for (int i = 0; i < 8; ++i) {
+      const X v = (X)w->cells.f[i];
+      uint64_t bits = 0;
+      std::memcpy(&bits, &v, sizeof(X));
+      h = (h ^ bits) * 1099511628211ull;
+    }
+    variant = (uintptr_t)(h << 2) | 3;
+  }
   const void* key[9] = {val, v.x, v.r, v.p, v.w, v.sv, w->s, w->use_lat ? (const void*)w->lat.map : w->sell.off,
                         reinterpret_cast<const void*>(variant)};
   if (c->tune.time_kernels) {   // kernels launched directly (launch_batch), SpMVs bracketed by events
