@@ -156,9 +156,10 @@ def main():
                   f"{fa.timings(K)['solve_ms']:.1f} ms ({r.iterations} its)", flush=True)
         fa.set_tuning(K, lattice_blocks=0)
     if args.cell_sweep:
-        for march, rows, mname in ((0, 2, "z-march, 2 rows/thread"), (0, 1, "z-march, 1 row/thread"),
-                                   (-1, 0, "tiles")):
-            for blocks in (512, 768, 1024, 1536):
+        for march, rows, mname, counts in ((0, 2, "z-march, 2 rows/thread", (512, 640, 704, 768, 832, 896, 1024)),
+                                           (0, 1, "z-march, 1 row/thread", (512, 768, 1024, 1536)),
+                                           (-1, 0, "tiles", (512, 768, 1024, 1536))):
+            for blocks in counts:
                 smap = 0
                 fa.set_tuning(K, cell_operator=1, cell_blocks=blocks, sell_map=smap, cell_march=march, cell_rows=rows)
                 a64 = fa.spmv_bench(K, mixed=False, reps=20)
