@@ -16,6 +16,8 @@ from .mesh import Grid, Hexahedron, Quadrilateral, generate_grid
 from .refel import (FacetQuadratureRule, Lagrange, QuadratureRule, RefHexahedron, RefLine, RefQuadrilateral,
                     VectorizedInterpolation)
 from .values import CellValues, FacetValues
+from .vtk import (VTKGridFile, write_cell_colors, write_cell_data, write_node_data, write_nodeset,
+                  write_solution)
 
 __all__ = [
     "FemArgumentError", "FemConvergenceError", "FemError", "FemPatternError", "LIB_PATH", "exported_symbols",
@@ -26,4 +28,5 @@ __all__ = [
     "create_sparsity_pattern", "Grid", "Hexahedron", "Quadrilateral", "generate_grid", "FacetQuadratureRule",
     "Lagrange", "QuadratureRule", "RefHexahedron", "RefLine", "RefQuadrilateral", "VectorizedInterpolation",
     "CellValues", "FacetValues", "BatchedPoisson", "batch_sizes", "make_batch",
+    "VTKGridFile", "write_solution", "write_node_data", "write_cell_data", "write_nodeset", "write_cell_colors",
 ]
