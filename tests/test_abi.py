@@ -31,8 +31,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_ctx_create_without_gpu_fails_cleanly():
+    # count devices without initialising torch's own HIP runtime: a second runtime initialised
+    # in this process before the library's would leave the library without a device
     import torch
-    if torch.cuda.is_available():
+    if torch.cuda.device_count() > 0:
         return
     try:
         fa.Context(0)
