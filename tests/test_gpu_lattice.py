@@ -277,3 +277,22 @@ def test_cell_operator_inhomogeneous_dirichlet(ct, nel, right):
     assert res.relres <= 1e-12 and _rel(res.u, u_ref) < 1e-8
     pd = np.asarray(ch.prescribed_dofs) - 1
     assert np.allclose(res.u[pd], u_ref[pd], rtol=1e-10, atol=1e-12)
+
+
+def test_cell_operator_new_parameters_without_reassembly():
+    """New per-cell parameters uploaded after K was assembled (K unchanged): the cell form would
+    be built from coefficients K does not hold, so the entry check must fail and the solve
+    use the planes; the solution is still K's."""
+    print("case cell operator stale parameters", flush=True)
+    nel = (8, 6, 5)
+    k = poisson_conductivity(nel, seed=20)
+    g, dh, K, f = _system("Hexahedron", nel, (1.0, 1.0, 1.0), k, cell_operator=0)
+    ref = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert ref.operator == "cells"
+    fa.set_cell_parameters(K, 2.0 * k)            # K keeps the old k
+    res = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert res.operator == "lattice"
+    assert _rel(res.u, ref.u) < 1e-8
+    fa.set_cell_parameters(K, k)                  # parameters that match K again
+    back = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert back.operator == "cells" and np.array_equal(back.u, ref.u)
