@@ -188,7 +188,8 @@ typedef struct {
                                         tensor-product lattice and K is symmetric: value planes
                                         per neighbour, no column indices, half the values
                                         (0 = default: mixed solver only; 1: FP64 solver too;
-                                        -1: never, the sliced ELL copy) */
+                                        -1: never, the sliced ELL copy); the cell form
+                                        (cell_operator) is preferred where it applies */
   int32_t lattice_blocks;            /* blocks of the lattice SpMV, a multiple of 8 (0 = default) */
   int32_t cg_split;                  /* CG step after the SpMV: 0 (default) or 1 = a one-block scalar
                                         kernel, then the vector move; -1 = one kernel whose every
