@@ -137,6 +137,9 @@ int32_t fem_ctx_create(int32_t device, fem_ctx** out) {
     return FEM_ERR_OOM;
   }
   c->npartials = kReduceBlocks * 4;
+  // the allocation-free apply!'s scratch address on this device (once per device); a failure
+  // only disables that entry point, which then reports it
+  (void)noalloc_scratch_init(device);
   *out = c;
   return FEM_OK;
 }

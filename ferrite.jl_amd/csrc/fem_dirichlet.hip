@@ -12,6 +12,8 @@
 // f is bitwise identical to the reference loop's for the same K.
 #include "fem_internal.h"
 
+#include <mutex>
+
 namespace fem {
 
 template <int BLOCK>
@@ -87,10 +89,14 @@ __global__ void k_check_symmetric(int64_t n, const int32_t* __restrict__ colptr,
 
 // f[r] -= g_d K[r,d] for prescribed d (ascending d), rows found through the
 // structurally symmetric pattern: row r's prescribed neighbours d are the rows
-// of column r; K[r,d] is located in column d by binary search.
+// of column r; K[r,d] is located in column d by binary search.  g_d is read from a dense
+// by-dof copy (DENSE_G) or found by binary search in the sorted prescribed list; both give
+// the same value, so f is the same bit for bit.
+template <bool DENSE_G>
 __global__ void k_add_inhom(int64_t n, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
                             const double* __restrict__ nzval, const uint8_t* __restrict__ isc,
-                            const double* __restrict__ gfull, double* __restrict__ f) {
+                            const double* __restrict__ gfull, int64_t np, const int32_t* __restrict__ pres,
+                            const double* __restrict__ g, double* __restrict__ f) {
 #pragma clang fp contract(off)
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     double fr = f[r];
@@ -98,7 +104,17 @@ __global__ void k_add_inhom(int64_t n, const int32_t* __restrict__ colptr, const
     for (int32_t p = colptr[r]; p < colptr[r + 1]; ++p) {
       const int32_t d = rowval[p];
       if (!isc[d]) continue;
-      const double v = gfull[d];
+      double v;
+      if constexpr (DENSE_G) {
+        v = gfull[d];
+      } else {
+        int64_t lo = 0, hi = np;   // isc[d] set: d is in pres[0, np)
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (pres[mid] < d) lo = mid + 1; else hi = mid;
+        }
+        v = g[lo];
+      }
       if (v == 0.0) continue;
       int32_t lo = colptr[d], hi = colptr[d + 1];
       while (lo < hi) {
@@ -143,11 +159,51 @@ __global__ void k_scatter_g(int64_t np, const int32_t* __restrict__ pres, const 
     gfull[pres[i]] = g[i];
 }
 
+__global__ void k_set_int(int* __restrict__ p, int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed device scratch of the allocation-free apply! (fem_apply_dirichlet_noalloc): the
+// 1024 per-block partial sums of |K_jj|, meandiag(K) and the pattern-symmetry flag.  One
+// copy per GPU, held until the process ends; every call on a GPU shares it, so calls that
+// overlap on one GPU race.  Each use overwrites what it reads (every block writes its
+// partial, the mean kernel writes the mean, a fill kernel sets the flag before the check),
+// so nothing has to be cleared between calls.
+// ---------------------------------------------------------------------------
+constexpr int kScratchMean = kReduceBlocks;       // index of meandiag(K)
+constexpr int kScratchFlag = kReduceBlocks + 1;   // the flag, an int stored in a double slot
+constexpr int kScratchDoubles = kReduceBlocks + 2;
+constexpr size_t kScratchBytes = kScratchDoubles * sizeof(double);   // 8,208 bytes
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};   // device address of `scratch` on each GPU
+
 static int grid_of(int64_t n, int block = 256) {
   int64_t g = (n + block - 1) / block;
   if (g < 1) g = 1;
   if (g > 8192) g = 8192;
   return (int)g;
+}
+
+// Device address of `scratch` for the current device (the caller has made `device` current).
+// Host code passes this address, never the array's name, to kernels and copies.
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return FEM_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return FEM_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FEM_ERR_HIP;
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return FEM_OK;
+}
+
+static double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 }  // namespace fem
@@ -215,7 +271,8 @@ int32_t fem_apply_dirichlet(fem_ctx* c, int32_t applyzero, double* m_out) {
     if (rc) return rc;
     FEM_HIP(c, hipMemsetAsync(gfull, 0, sizeof(double) * n, c->stream));
     k_scatter_g<<<grid_of(c->np), 256, 0, c->stream>>>(c->np, c->d_pres, c->d_g, gfull);
-    k_add_inhom<<<grid_of(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_isc, gfull, c->d_f);
+    k_add_inhom<true><<<grid_of(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_isc, gfull,
+                                                          0, nullptr, nullptr, c->d_f);
     FEM_HIP(c, hipStreamSynchronize(c->stream));
     dev_free(&gfull);
   }
@@ -233,6 +290,54 @@ int32_t fem_apply_dirichlet(fem_ctx* c, int32_t applyzero, double* m_out) {
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
   c->timings.apply_ms = event_ms(c);
   if (m_out) FEM_HIP(c, hipMemcpy(m_out, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost));
+  ++c->nzval_version;
+  return FEM_OK;
+}
+
+int32_t fem_apply_dirichlet_noalloc(fem_ctx* c, int32_t applyzero, double* m_out) {
+  if (!c || c->nnz == 0) return set_error(c, FEM_ERR_STATE, "no matrix to apply constraints to");
+  if (!c->d_isc) return set_error(c, FEM_ERR_STATE, "upload constraints first");
+  double* S = noalloc_scratch(c->device);
+  if (!S) return set_error(c, FEM_ERR_STATE, "fem_apply_dirichlet_noalloc: no device scratch address for this GPU");
+  // The partials, the mean and the flag are all the scratch this call uses, whatever the
+  // problem size (the partial count is fixed), so no size can exceed it.
+  static_assert(kScratchBytes >= (kReduceBlocks + 2) * sizeof(double), "scratch too small");
+  FEM_HIP(c, hipSetDevice(c->device));
+  const int64_t n = c->n;
+  if (c->pattern_sym < 0) {   // once per pattern
+    int* flag = reinterpret_cast<int*>(S + kScratchFlag);
+    int h = 0;
+    k_set_int<<<1, 64, 0, c->stream>>>(flag, 1);
+    k_check_symmetric<<<grid_of(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, flag);
+    FEM_HIP(c, hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+    c->pattern_sym = h;
+  }
+  if (!applyzero && c->any_inhom && c->pattern_sym != 1)
+    return set_error(c, FEM_ERR_ARG, "apply! with inhomogeneous Dirichlet values needs a structurally symmetric pattern");
+  FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+  k_absdiag_partial<<<kReduceBlocks, 256, 0, c->stream>>>(n, c->d_diag, c->d_nzval, S);
+  k_finish_mean<<<1, 256, 0, c->stream>>>(kReduceBlocks, S, n, S + kScratchMean);
+  if (!applyzero && c->any_inhom)
+    k_add_inhom<false><<<grid_of(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, c->d_nzval, c->d_isc,
+                                                           nullptr, c->np, c->d_pres, c->d_g, c->d_f);
+  if (c->pattern_sym == 1) {
+    if (c->np > 0)
+      k_zero_prescribed<<<grid_of(c->np), 256, 0, c->stream>>>(c->np, c->d_pres, c->d_colptr, c->d_rowval,
+                                                                 c->d_nzval);
+  } else {
+    k_zero_rows_cols<<<grid_of(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, c->d_isc, c->d_nzval);
+  }
+  if (c->np > 0)
+    k_set_diag<<<grid_of(c->np), 256, 0, c->stream>>>(c->np, c->d_pres, c->d_diag, c->d_g, S + kScratchMean,
+                                                        applyzero, c->d_nzval, c->d_f);
+  FEM_HIP(c, hipGetLastError());
+  FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
+  c->timings.apply_ms = event_ms(c);
+  if (m_out) {
+    FEM_HIP(c, hipMemcpyAsync(m_out, S + kScratchMean, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FEM_HIP(c, hipStreamSynchronize(c->stream));
+  }
   ++c->nzval_version;
   return FEM_OK;
 }

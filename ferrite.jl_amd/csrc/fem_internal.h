@@ -188,6 +188,7 @@ void dev_free(T** p) {
 float event_ms(fem_ctx* c);
 
 int build_tables(fem_ctx* c);
+int noalloc_scratch_init(int device);   // fem_dirichlet.hip: address of its device scratch
 void free_solver(fem_ctx* c);
 int build_gather_tables(fem_ctx* c);
 bool gather_supported(const fem_ctx* c);

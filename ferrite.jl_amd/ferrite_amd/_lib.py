@@ -102,6 +102,7 @@ SIGNATURES = {
     "fem_point_load": (I32, [P, I64, PI64, PD]),
     "fem_constraints_upload": (I32, [P, I64, PI64, PD]),
     "fem_apply_dirichlet": (I32, [P, I32, PD]),
+    "fem_apply_dirichlet_noalloc": (I32, [P, I32, PD]),
     "fem_solve": (I32, [P, C.POINTER(SolveCfg), C.POINTER(SolveInfo)]),
     "fem_matrix_download": (I32, [P, PD, PD]),
     "fem_matrix_upload": (I32, [P, PD, PD]),

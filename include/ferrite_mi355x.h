@@ -111,6 +111,15 @@ int32_t fem_constraints_upload(fem_ctx* ctx, int64_t np, const int64_t* prescrib
 /* apply!(K, f, ch) (applyzero = 0) / apply_zero!(K, f, ch) (applyzero = 1).  Deterministic,
  * no atomics.  Returns meandiag(K) (ConstraintHandler.jl:952-958) in *m_out if non-NULL. */
 int32_t fem_apply_dirichlet(fem_ctx* ctx, int32_t applyzero, double* m_out);
+/* The same apply! / apply_zero!, with the same results bit for bit, allocating nothing: no
+ * counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc) once the matrix and
+ * the constraints are uploaded.  Its reduction partials, meandiag and the pattern-symmetry
+ * flag live in one fixed 8,208-byte device array per GPU, held until the process ends, instead
+ * of buffers allocated by the context or the call (fem_apply_dirichlet allocates a dense g
+ * vector per inhomogeneous call); no problem size needs more.  Every call on a GPU shares that
+ * array: calls that overlap on one GPU (two contexts on two host threads) race. */
+int32_t fem_apply_dirichlet_noalloc(fem_ctx* ctx, int32_t applyzero, double* m_out);
 
 /* ---- solve (replaces u = K \ f, heat_equation.jl:218; the reference has no CG) ---- */
 enum fem_solve_mode {
