@@ -146,3 +146,52 @@ def test_batch_problem_setup_matches_oracle():
         assert np.array_equal(cp, oK.colptr) and np.array_equal(rv, oK.rowval)
         assert np.array_equal(ch.prescribed_dofs, np.asarray(och.prescribed_dofs))
         assert sorted(np.concatenate(colors).tolist()) == list(range(1, g.ncells + 1))
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_q2_hex_dofs_single_valued_geometry(impl):
+    """Q2 hex dof numbering pinned by geometry (no reference table of hex Q2 celldofs exists).
+
+    On a grid with moved interior nodes, every local dof of every cell is placed at its
+    physical point x(ξ_a) (Q1 geometry at the Q2 reference coordinates).  A global dof must
+    be the same point from every cell that holds it, distinct dofs distinct points, and each
+    cell's edge / face / volume dofs must sit at the midpoint of Ferrite's edge (grid.jl:218-
+    229) / the centre of its face / the cell centre, which the trilinear map preserves.  A
+    misread edge or face key (dofs shared between the wrong cells) or a local entity order
+    that disagrees with the reference coordinates fails here."""
+    from oracle.refel import REF_EDGES, REF_FACES
+    nel = (3, 2, 2)
+    og = o_generate_grid("Hexahedron", nel, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    X = og.nodes.copy()
+    inner = np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1)
+    X[inner] += np.random.default_rng(3).uniform(-0.05, 0.05, (int(inner.sum()), 3))
+    oip = OLagrange("RefHexahedron", 2) ** 3
+    if impl == "oracle":
+        cd = np.array(ODofHandler(og).add("u", oip).close().cell_dofs).reshape(og.ncells, 81)
+        ndofs = int(cd.max())
+    else:
+        g = fa.generate_grid("Hexahedron", nel, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+        dh = fa.DofHandler(g).add("u", fa.Lagrange("RefHexahedron", 2) ** 3).close()
+        cd, ndofs = np.asarray(dh.cell_dofs), dh.ndofs
+    geo = OLagrange("RefHexahedron", 1)
+    refc = OLagrange("RefHexahedron", 2).reference_coordinates()
+    M = np.array([[geo.shape_value(xi, j) for j in range(1, 9)] for xi in refc])   # (27, 8)
+    pos = {}
+    for c in range(og.ncells):
+        X8 = X[np.array(og.cells[c]) - 1]
+        P = M @ X8
+        for a in range(27):
+            d = int(cd[c, 3 * a])
+            assert (d - 1) % 3 == 0 and cd[c, 3 * a + 1] == d + 1 and cd[c, 3 * a + 2] == d + 2
+            if d in pos:
+                assert np.abs(pos[d] - P[a]).max() <= 1e-14
+            else:
+                pos[d] = P[a]
+        for k, (v1, v2) in enumerate(REF_EDGES["RefHexahedron"]):
+            assert np.abs(P[8 + k] - 0.5 * (X8[v1 - 1] + X8[v2 - 1])).max() <= 1e-14
+        for k, face in enumerate(REF_FACES["RefHexahedron"]):
+            assert np.abs(P[20 + k] - X8[np.array(face) - 1].mean(axis=0)).max() <= 1e-14
+        assert np.abs(P[26] - X8.mean(axis=0)).max() <= 1e-14
+    assert len(pos) * 3 == ndofs
+    pts = np.array(list(pos.values()))
+    assert len(np.unique(np.round(pts, 10), axis=0)) == len(pos)     # distinct dofs, distinct points
