@@ -295,6 +295,7 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
   FEM_HIP(c, hipSetDevice(c->device));
   c->err_cell = c->err_row = c->err_col = -1;
   c->timings.assemble_launches = 0;
+  c->timings.element_ms = c->timings.scatter_ms = 0.0;
   k_reset_err<<<1, 1, 0, c->stream>>>(c->d_err);
   const bool mixed = cfg->precision == FEM_MIXED;
   // Scalar Q1 Poisson: owner-computes gather (each nzval/f entry written once, no fill
@@ -312,6 +313,16 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
     }
     if (c->tune.asm_mode == 2)
       return set_error(c, FEM_ERR_ARG, "gather assembly needs a scalar Q1 quad/hex with 2x2(x2) Gauss");
+  }
+  if (cfg->kernel == FEM_KERNEL_ELASTICITY) {
+    // vector Q2 hex: owner-computes (element matrices, then a gather per dof node)
+    bool done = false;
+    int rq = assemble_elasticity_gather(c, cfg, &done);
+    if (rq) return rq;
+    if (done) {
+      ++c->nzval_version;
+      return check_assembly_errors(c);
+    }
   }
   int rc = build_tables(c);
   if (rc) return rc;

@@ -131,6 +131,7 @@ int32_t fem_ctx_create(int32_t device, fem_ctx** out) {
   }
   (void)hipEventCreate(&c->ev0);
   (void)hipEventCreate(&c->ev1);
+  (void)hipEventCreate(&c->evm);
   if (dev_alloc(c, &c->d_err, 1) || dev_alloc(c, &c->d_scalar, 16) ||
       dev_alloc(c, &c->d_partials, kReduceBlocks * 4)) {
     fem_ctx_destroy(c);
@@ -157,6 +158,7 @@ void fem_ctx_destroy(fem_ctx* c) {
   dev_free(&c->d_gcnodes); dev_free(&c->d_d2c_ptr); dev_free(&c->d_d2c); dev_free(&c->d_gslot);
   dev_free(&c->d_keb64); dev_free(&c->d_feb64); dev_free(&c->d_keb32); dev_free(&c->d_feb32);
   dev_free(&c->d_gblist);
+  free_q2_tables(c);
   for (hipEvent_t e : c->gchunk_ev) (void)hipEventDestroy(e);
   if (c->stream2) {
     (void)hipStreamSynchronize(c->stream2);
@@ -168,6 +170,7 @@ void fem_ctx_destroy(fem_ctx* c) {
   dev_free(&c->d_aff); dev_free(&c->d_afff);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->evm) (void)hipEventDestroy(c->evm);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -204,6 +207,8 @@ int32_t fem_mesh_upload(fem_ctx* c, int32_t sdim, int64_t nnodes, const double* 
     FEM_HIP(c, hipStreamSynchronize(c->stream));
   }
   c->sdim = sdim; c->nnodes = nnodes; c->ncells = ncells; c->npc = npc;
+  free_solver(c);   // the solver's operator copies cache geometry (cell corners, lattice map)
+  free_q2_tables(c);
   c->tables_ready = false;
   c->gather_ready = false;
   c->gchunks_built = 0;
@@ -229,6 +234,8 @@ int32_t fem_dofs_upload(fem_ctx* c, int64_t ndofs, int32_t ndpc, const int64_t* 
   FEM_HIP(c, hipMemsetAsync(c->d_f, 0, sizeof(double) * ndofs, c->stream));
   FEM_HIP(c, hipMemsetAsync(c->d_u, 0, sizeof(double) * ndofs, c->stream));
   FEM_HIP(c, hipStreamSynchronize(c->stream));
+  free_solver(c);   // operator copies are built on the dof numbering
+  free_q2_tables(c);
   c->tables_ready = false;
   c->gather_ready = false;
   c->gchunks_built = 0;
@@ -272,6 +279,7 @@ int32_t fem_pattern_upload(fem_ctx* c, int64_t n, const int64_t* colptr, const i
   c->max_row_nnz = 0;
   for (int64_t j = 0; j < n; ++j) c->max_row_nnz = std::max<int>(c->max_row_nnz, cp[j + 1] - cp[j]);
   free_solver(c);
+  free_q2_tables(c);
   c->tables_ready = false;
   c->gather_ready = false;
   c->gchunks_built = 0;

@@ -15,6 +15,8 @@
 // the FP64 path uses register-blocked VALU FMAs.
 #include "fem_internal.h"
 
+#include <algorithm>
+
 namespace fem {
 
 template <int DIM, int NG, int NB, int NQ>
@@ -190,6 +192,15 @@ __device__ __forceinline__ int pidx(int i, int j) {  // (i <= j) -> 0..5
   return i == 0 ? j : (i == 1 ? 2 + j : 5);
 }
 
+// Stored element matrices of the owner-computes path: per cell and local node b the 81 x 3
+// column block Ke[(a, i), (b, j)] as [(i, j)][a] with 28 rows a (row 27 is padding), so the
+// element kernel stores runs of four a straight from its MFMA accumulators and the gather
+// reads one contiguous 252-value block per (cell, b).
+constexpr int kKS = 9 * kKP;        // 252 values per (cell, b)
+constexpr int kKC = kQN * kKS;      // 6804 values per cell
+
+// Colored scatter: s = colour slot s0 + block (cnodes/cparam in colour order, ccell maps the
+// slot to its cell).
 template <typename T>
 __global__ __launch_bounds__(64) void k_elast_q2hex(
     int64_t s0, const double* __restrict__ X0, const double* __restrict__ X1, const double* __restrict__ X2,
@@ -308,10 +319,10 @@ __global__ __launch_bounds__(64) void k_elast_q2hex(
   }
   __syncthreads();
 
-  // Ke entries in CSC order (entry e = J*81 + I: column J, row I) and the scatter.
   const T* sP = sm + Q2Smem<T>::oP;
   const T E = (T)cparam[s];
   const T lE = lam * E, mE = mu * E;
+  // Ke entries in CSC order (entry e = J*81 + I: column J, row I) and the scatter.
   const int32_t* __restrict__ nzt = cnz + s * (kQD * kQD);
   constexpr int CH = 8;
   constexpr int TOT = kQD * kQD;
@@ -344,6 +355,267 @@ __global__ __launch_bounds__(64) void k_elast_q2hex(
   }
 }
 
+__global__ void k_fill_zero_f(double* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Q2 hex^3, owner-computes path.
+//   phase 1  k_elast_q2hex_ke<float> (FP32 MFMA) / k_elast_q2hex_ke<double> (FP64 MFMA):
+//            one 4-wave workgroup per cell in cell-id order, the element matrix stored as
+//            above.
+//   phase 2  k_gather_q2: one wave per dof node m (dofs 3m, 3m+1, 3m+2).  The three columns
+//            share one row set made of triples (checked once per pattern), so the wave keeps
+//            the three columns in LDS, adds the 81 x 3 block of every cell holding m in
+//            ascending cell id (the order of Ferrite's serial loop), and writes the three
+//            columns, which are contiguous in nzval, once.  No nzval entry is read-modify-
+//            written per cell and no per-entry position table is read: a cell's local node a
+//            lands at row triple nrank[e][a] of the node's columns.
+// ---------------------------------------------------------------------------
+
+// MFMA 16x16x4 in FP32 (v_mfma_f32_16x16x4f32) or FP64 (v_mfma_f64_16x16x4f64).  Lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; it holds D[row(l / 16, e)][l % 16], with
+// row = 4 (l / 16) + e for FP32 and (l / 16) + 4 e for FP64 (measured on gfx950).
+template <typename T> struct Mfma16;
+template <> struct Mfma16<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static constexpr __device__ int row(int kq, int e) { return 4 * kq + e; }
+};
+template <> struct Mfma16<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static constexpr __device__ int row(int kq, int e) { return kq + 4 * e; }
+};
+
+// Element matrices of the owner-computes path, one cell per 256-thread block.  All nine
+// products P_ij = A_i A_jᵀ (not only i <= j) are formed, so a lane holds P_ij[a][b] and
+// P_ji[a][b] for the same (a, b) and forms Ke_ij = E (λ P_ij + μ P_ji + μ δ_ij Σ_k P_kk) in
+// registers.  A_i[a][q] = √dV_q g_q[a, i] carries the quadrature weight on both operands, so
+// P_ji[b][a] multiplies the same two factors as P_ij[a][b] in the same order: Ke is symmetric
+// bit for bit, as Ferrite's Ke is, and CG sees a symmetric K.  Wave w computes the 16 x 16
+// output tile (ti, tj) = (w / 2, w % 2) of the 32 x 32 padded products; the 27 x 27 (q, a)
+// gradient pairs are spread over the block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_elast_q2hex_ke(
+    const double* __restrict__ X0, const double* __restrict__ X1, const double* __restrict__ X2,
+    const int32_t* __restrict__ cnodes, const double* __restrict__ param, const T* __restrict__ tab, T lam, T mu,
+    T* __restrict__ keb, DevError* __restrict__ err) {
+  using L = TabL<3, 8, kQN, kQQ>;
+  using V = typename Mfma16<T>::v4;
+  __shared__ T sA[3 * kMP * kKP];   // [i][a][q]; rows a >= 27 and column q = 27 stay zero
+  __shared__ T sJi[kKP][9];
+  __shared__ T sdq[kKP];            // √(det J w_q)
+  const int tid = threadIdx.x;
+  const int64_t cell = blockIdx.x;
+  // this thread's (q, a) pairs t = tid + 256 m of the gradient phase: their table entries are
+  // loaded first, so the loads overlap the Jacobian phase
+  constexpr int NQA = (kQQ * kQN + 255) / 256;
+  T dnv[NQA][3];
+#pragma unroll
+  for (int m = 0; m < NQA; ++m) {
+    const int t = tid + 256 * m;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dnv[m][d] = t < kQQ * kQN ? tab[L::odN + t * 3 + d] : T(0);
+  }
+  for (int t = tid; t < 3 * kMP * kKP; t += 256) sA[t] = T(0);
+  if (tid < kQQ) {   // Jacobian at point q from the 8 corners
+    const int q = tid;
+    T J[3][3] = {{T(0), T(0), T(0)}, {T(0), T(0), T(0)}, {T(0), T(0), T(0)}};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      const int32_t nd = cnodes[cell * 8 + a];
+      const T xa[3] = {(T)X0[nd], (T)X1[nd], (T)X2[nd]};
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) J[i][j] += xa[i] * tab[L::odM + (q * 8 + a) * 3 + j];
+    }
+    T Ji[3][3];
+    const T detJ = det_inv_e<T, 3>(J, Ji);
+    if (!(detJ > T(0))) atomicMin(&err->detj_cell, (int)cell);
+    sdq[q] = sqrt(detJ * tab[L::ow + q]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) sJi[q][3 * i + j] = Ji[i][j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < NQA; ++m) {   // √dV ∇N_a at point q: g_i = Σ_d dN_d Ji[d][i]
+    const int t = tid + 256 * m;
+    if (t >= kQQ * kQN) break;
+    const int q = t / kQN, a = t - q * kQN;
+    const T d0 = dnv[m][0], d1 = dnv[m][1], d2 = dnv[m][2], sq = sdq[q];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      sA[(i * kMP + a) * kKP + q] = sq * (d0 * sJi[q][i] + d1 * sJi[q][3 + i] + d2 * sJi[q][6 + i]);
+  }
+  __syncthreads();
+  const T E = (T)param[cell];
+  const T lE = lam * E, mE = mu * E;
+  const int lane = tid & 63, w = tid >> 6;
+  const int ti = w >> 1, tj = w & 1;
+  const int r = lane & 15, kq = lane >> 4;
+  V P[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) P[i][j] = V{T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int kk = 0; kk < kKP / 4; ++kk) {
+    const int k = 4 * kk + kq;
+    T av[3], bv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      av[i] = sA[(i * kMP + 16 * ti + r) * kKP + k];
+      bv[i] = sA[(i * kMP + 16 * tj + r) * kKP + k];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) P[i][j] = Mfma16<T>::op(av[i], bv[j], P[i][j]);
+  }
+  // lane holds rows a = 16 ti + row(kq, e) (e = 0..3) of column b = 16 tj + r; rows up to 27
+  // are stored (27 is the zero padding row)
+  const int b = 16 * tj + r;
+  if (b < kQN) {
+    T* __restrict__ out = keb + cell * (int64_t)kKC + b * kKS + 16 * ti;
+    const V tr = P[0][0] + P[1][1] + P[2][2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        V v = lE * P[i][j] + mE * P[j][i];
+        if (i == j) v += mE * tr;
+        T* o = out + (i * 3 + j) * kKP;
+        if constexpr (sizeof(T) == 4) {   // four consecutive rows: one 16-byte store
+          if (16 * ti + 4 * kq < kKP) *reinterpret_cast<V*>(o + 4 * kq) = v;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (16 * ti + Mfma16<T>::row(kq, e) < kKP) o[Mfma16<T>::row(kq, e)] = v[e];
+        }
+      }
+  }
+}
+
+// flag = 0 unless every dof node m has three equal-length columns 3m..3m+2 whose rows are
+// the same triples (3k, 3k+1, 3k+2), at most 254 of them.
+__global__ void k_check_node_blocks(int64_t nnode, const int32_t* __restrict__ colptr,
+                                    const int32_t* __restrict__ rowval, int* __restrict__ flag) {
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < nnode; m += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c0 = colptr[3 * m], L = colptr[3 * m + 1] - c0;
+    bool ok = L % 3 == 0 && L / 3 <= 254 && colptr[3 * m + 2] - colptr[3 * m + 1] == L &&
+              colptr[3 * m + 3] - colptr[3 * m + 2] == L;
+    for (int32_t k = 0; ok && k < L; k += 3) {
+      const int32_t r0 = rowval[c0 + k];
+      ok = r0 % 3 == 0;
+      for (int col = 0; col < 3; ++col)
+        for (int i = 0; i < 3; ++i) ok = ok && rowval[c0 + col * L + k + i] == r0 + i;
+    }
+    if (!ok) *flag = 0;   // every writer stores the same 0
+  }
+}
+
+// nrank[e][a] = row triple of local node a of list entry e's cell inside its node's columns
+// (255 when absent from the pattern).
+__global__ void k_build_nrank(int64_t nnode, const int32_t* __restrict__ colptr, const int32_t* __restrict__ rowval,
+                              const int32_t* __restrict__ n2c_ptr, const int32_t* __restrict__ n2c,
+                              const int32_t* __restrict__ cdofs, uint8_t* __restrict__ nrank) {
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < nnode; m += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c0 = colptr[3 * m], c1 = colptr[3 * m + 1];
+    for (int32_t e = n2c_ptr[m]; e < n2c_ptr[m + 1]; ++e) {
+      const int64_t cell = n2c[e] / kQN;
+      for (int a = 0; a < kQN; ++a) {
+        const int32_t row = cdofs[cell * kQD + 3 * a];
+        int32_t lo = c0, hi = c1;
+        while (lo < hi) {
+          const int32_t mid = (lo + hi) >> 1;
+          if (rowval[mid] < row) lo = mid + 1; else hi = mid;
+        }
+        const bool found = lo < c1 && rowval[lo] == row;
+        nrank[(int64_t)e * kQN + a] = found ? (uint8_t)((lo - c0) / 3) : (uint8_t)255;
+      }
+    }
+  }
+}
+
+// One wave per dof node; G list entries have their loads in flight before they are added.
+template <typename T, int G>
+__global__ __launch_bounds__(64) void k_gather_q2(int64_t nnode, const int32_t* __restrict__ colptr,
+                                                  const int32_t* __restrict__ n2c_ptr, const int32_t* __restrict__ n2c,
+                                                  const uint8_t* __restrict__ nrank, const T* __restrict__ keb,
+                                                  double* __restrict__ nzval, int fillzero,
+                                                  DevError* __restrict__ err) {
+  extern __shared__ double acc[];   // the node's three columns, 3 L values
+  constexpr int S = (kKS + 63) / 64;   // 4 values of the 252-value block per lane
+  const int lane = threadIdx.x;
+  // workgroups are dealt round-robin to the 8 XCDs: each XCD takes one contiguous range of
+  // nodes, whose cells (numbered close together by the first-visit dof numbering) then
+  // share that XCD's L2.  gridDim.x is a multiple of 8.
+  const int64_t per = gridDim.x / 8;
+  const int64_t m = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (m >= nnode) return;
+  const int32_t c0 = colptr[3 * m], L = colptr[3 * m + 1] - c0, tot = 3 * L;
+  for (int q = lane; q < tot; q += 64) acc[q] = fillzero ? 0.0 : nzval[c0 + q];
+  // block slot t = lane + 64 s -> (i, j) = t / 28, row a = t % 28 (a = 27: padding)
+  int dst[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int t = lane + 64 * s;
+    const int ij = t / kKP, a = t - ij * kKP;
+    const int i = ij / 3, j = ij - 3 * i;
+    dst[s] = (t < kKS && a < kQN) ? (j * L + i) : -1;   // + 3 rank
+  }
+  __syncthreads();
+  const int32_t e0 = n2c_ptr[m], e1 = n2c_ptr[m + 1];
+  for (int32_t eb = e0; eb < e1; eb += G) {
+    T v[G][S];
+    uint8_t rk[G][S];
+    int32_t cl[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int32_t e = eb + g;
+      const bool in = e < e1;
+      const int32_t ent = in ? n2c[e] : 0;
+      const int64_t cell = ent / kQN;
+      const int b = ent - (int)cell * kQN;
+      cl[g] = (int32_t)cell;
+      const T* blk = keb + cell * (int64_t)kKC + b * kKS;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int t = lane + 64 * s;
+        const bool ok = in && dst[s] >= 0;
+        v[g][s] = ok ? blk[t] : T(0);
+        rk[g][s] = ok ? nrank[(int64_t)e * kQN + (t % kKP)] : (uint8_t)255;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (eb + g >= e1) break;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        if (dst[s] < 0) continue;
+        if (rk[g][s] != 255) {
+          // LDS add without a return value: the adds of one instruction hit distinct slots and a
+          // wave's LDS operations complete in order, so each slot still sums its entries in
+          // ascending cell order, rounded as acc + v
+          atomicAdd(&acc[dst[s] + 3 * rk[g][s]], (double)v[g][s]);
+        } else if (v[g][s] != T(0)) {
+          atomicMin(&err->pattern_cell, cl[g]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = lane; q < tot; q += 64) nzval[c0 + q] = acc[q];
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -374,6 +646,129 @@ static int upload_tab(fem_ctx* c) {
   FEM_HIP(c, hipMemcpyAsync(c->d_tabf, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   c->tab_dirty = true;  // packed for this kernel family; the Poisson path repacks
+  return FEM_OK;
+}
+
+void free_q2_tables(fem_ctx* c) {
+  dev_free(&c->d_n2c_ptr); dev_free(&c->d_n2c); dev_free(&c->d_nrank);
+  dev_free(&c->d_keq32); dev_free(&c->d_keq64);
+  c->keq32_cells = c->keq64_cells = 0;
+  c->q2_ready = -1;
+}
+
+static int gridq(int64_t n, int b = 256) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + b - 1) / b, 16384)); }
+
+// Whether this is vector Q2 hex elasticity (27 nodes, 3x3x3 Gauss, Q1 geometry).
+static bool is_q2hex(const fem_ctx* c) {
+  return c->rdim == 3 && c->sdim == 3 && c->ngeo == 8 && c->nbf == kQN && c->nqp == kQQ && c->npc == 8 &&
+         c->vdim == 3 && c->ndpc == kQD;
+}
+
+// Node lists and rank table of the owner-computes Q2 path, once per mesh/dofs/pattern.
+// Sets q2_ready = 0 (colored scatter instead) when a cell's dofs are not node triples
+// 3m, 3m+1, 3m+2 in local order or the pattern is not made of such triples.
+static int build_q2_tables(fem_ctx* c) {
+  if (c->q2_ready >= 0) return FEM_OK;
+  c->q2_ready = 0;
+  const int64_t nc = c->ncells, nd = c->ndofs;
+  if (!is_q2hex(c) || nd % 3 || c->n != nd || nc * kQN >= INT32_MAX) return FEM_OK;
+  const int64_t nnode = nd / 3;
+  const int32_t* cd = c->h_cell_dofs.data();
+  for (int64_t i = 0; i < nc * kQN; ++i) {
+    const int32_t d = cd[3 * i];
+    if (d % 3 || cd[3 * i + 1] != d + 1 || cd[3 * i + 2] != d + 2) return FEM_OK;
+  }
+  // node -> (cell, local node), ascending cell id per node (counting sort over the cell loop)
+  std::vector<int32_t> ptr(nnode + 1, 0), ent(nc * kQN);
+  for (int64_t i = 0; i < nc * kQN; ++i) ++ptr[cd[3 * i] / 3 + 1];
+  for (int64_t m = 0; m < nnode; ++m) ptr[m + 1] += ptr[m];
+  std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t cell = 0; cell < nc; ++cell)
+    for (int b = 0; b < kQN; ++b) ent[fill[cd[(cell * kQN + b) * 3] / 3]++] = (int32_t)(cell * kQN + b);
+  int rc;
+  int* flag = nullptr;
+  int32_t* dcd = nullptr;
+  if ((rc = dev_alloc(c, &flag, 1))) return rc;
+  const int one = 1;
+  int h = 0;
+  FEM_HIP(c, hipMemcpyAsync(flag, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  k_check_node_blocks<<<gridq(nnode), 256, 0, c->stream>>>(nnode, c->d_colptr, c->d_rowval, flag);
+  FEM_HIP(c, hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  dev_free(&flag);
+  if (!h) return FEM_OK;
+  if (!c->d_gcnodes) {
+    if ((rc = dev_alloc(c, &c->d_gcnodes, nc * c->npc))) return rc;
+    FEM_HIP(c, hipMemcpyAsync(c->d_gcnodes, c->h_cell_nodes.data(), 4 * nc * c->npc, hipMemcpyHostToDevice,
+                              c->stream));
+  }
+  if ((rc = dev_alloc(c, &c->d_n2c_ptr, nnode + 1)) || (rc = dev_alloc(c, &c->d_n2c, nc * kQN)) ||
+      (rc = dev_alloc(c, &c->d_nrank, nc * kQN * (int64_t)kQN)) || (rc = dev_alloc(c, &dcd, nc * kQD)))
+    return rc;
+  FEM_HIP(c, hipMemcpyAsync(c->d_n2c_ptr, ptr.data(), 4 * (nnode + 1), hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipMemcpyAsync(c->d_n2c, ent.data(), 4 * nc * kQN, hipMemcpyHostToDevice, c->stream));
+  FEM_HIP(c, hipMemcpyAsync(dcd, cd, 4 * nc * kQD, hipMemcpyHostToDevice, c->stream));
+  k_build_nrank<<<gridq(nnode), 256, 0, c->stream>>>(nnode, c->d_colptr, c->d_rowval, c->d_n2c_ptr, c->d_n2c, dcd,
+                                                      c->d_nrank);
+  FEM_HIP(c, hipGetLastError());
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  dev_free(&dcd);
+  c->q2_ready = 1;
+  return FEM_OK;
+}
+
+int assemble_elasticity_gather(fem_ctx* c, const fem_assemble_cfg* cfg, bool* done) {
+  *done = false;
+  if (!is_q2hex(c) || c->tune.asm_mode == 1) return FEM_OK;
+  const double nu = cfg->nu;
+  if (!(nu > -1.0 && nu < 0.5)) return set_error(c, FEM_ERR_ARG, "Poisson's ratio must be in (-1, 0.5)");
+  int rc;
+  if ((rc = build_q2_tables(c))) return rc;
+  if (c->q2_ready != 1) return FEM_OK;
+  const double lam = nu / ((1 + nu) * (1 - 2 * nu)), mu = 1.0 / (2 * (1 + nu));
+  const bool mixed = cfg->precision == FEM_MIXED;
+  const int64_t nc = c->ncells, nnode = c->ndofs / 3;
+  if (mixed && (!c->d_keq32 || c->keq32_cells != nc)) {
+    if ((rc = dev_alloc(c, &c->d_keq32, nc * (int64_t)kKC))) return rc;
+    c->keq32_cells = nc;
+  }
+  if (!mixed && (!c->d_keq64 || c->keq64_cells != nc)) {
+    if ((rc = dev_alloc(c, &c->d_keq64, nc * (int64_t)kKC))) return rc;
+    c->keq64_cells = nc;
+  }
+  if ((rc = upload_tab<3, 8, 27, 27>(c))) return rc;
+  const size_t lds = sizeof(double) * 3 * (size_t)c->max_row_nnz;
+  if (lds > 64 * 1024) return FEM_OK;   // longest column beyond one workgroup's LDS: colored scatter
+  *done = true;
+  FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
+  if (cfg->fillzero) k_fill_zero_f<<<512, 256, 0, c->stream>>>(c->d_f, c->ndofs);   // nzval is written whole
+  if (mixed)
+    k_elast_q2hex_ke<float><<<(unsigned)nc, 256, 0, c->stream>>>(c->d_x[0], c->d_x[1], c->d_x[2], c->d_gcnodes,
+                                                                 c->d_param_cell, c->d_tabf, (float)lam, (float)mu,
+                                                                 c->d_keq32, c->d_err);
+  else
+    k_elast_q2hex_ke<double><<<(unsigned)nc, 256, 0, c->stream>>>(c->d_x[0], c->d_x[1], c->d_x[2], c->d_gcnodes,
+                                                                  c->d_param_cell, c->d_tab, lam, mu, c->d_keq64,
+                                                                  c->d_err);
+  FEM_HIP(c, hipEventRecord(c->evm, c->stream));
+  const unsigned nblk = (unsigned)((nnode + 7) / 8 * 8);
+  const int fz = cfg->fillzero ? 1 : 0;
+  if (mixed)
+    k_gather_q2<float, 8><<<nblk, 64, lds, c->stream>>>(nnode, c->d_colptr, c->d_n2c_ptr, c->d_n2c, c->d_nrank,
+                                                        c->d_keq32, c->d_nzval, fz, c->d_err);
+  else
+    k_gather_q2<double, 8><<<nblk, 64, lds, c->stream>>>(nnode, c->d_colptr, c->d_n2c_ptr, c->d_n2c, c->d_nrank,
+                                                         c->d_keq64, c->d_nzval, fz, c->d_err);
+  FEM_HIP(c, hipGetLastError());
+  FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
+  c->timings.assemble_kernel_ms = event_ms(c);
+  float e1 = 0.f, e2 = 0.f;
+  FEM_HIP(c, hipEventElapsedTime(&e1, c->ev0, c->evm));
+  FEM_HIP(c, hipEventElapsedTime(&e2, c->evm, c->ev1));
+  c->timings.element_ms = e1;
+  c->timings.scatter_ms = e2;
+  c->timings.fill_ms = 0.0;
+  c->timings.assemble_launches = 2 + (cfg->fillzero ? 1 : 0);
   return FEM_OK;
 }
 

@@ -465,6 +465,7 @@ static int run_gather(fem_ctx* c, const T* tab, double src, bool fillzero) {
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   if (!piped) {
     elem(0, nc, c->stream);
+    FEM_HIP(c, hipEventRecord(c->evm, c->stream));
     gather((int)(((c->n + kGatherBlock - 1) / kGatherBlock + 7) / 8 * 8), c->stream, nullptr);
     c->timings.assemble_launches = 2;
   } else {
@@ -485,6 +486,13 @@ static int run_gather(fem_ctx* c, const T* tab, double src, bool fillzero) {
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
   c->timings.assemble_kernel_ms = event_ms(c);
+  if (!piped) {
+    float e1 = 0.f, e2 = 0.f;
+    FEM_HIP(c, hipEventElapsedTime(&e1, c->ev0, c->evm));
+    FEM_HIP(c, hipEventElapsedTime(&e2, c->evm, c->ev1));
+    c->timings.element_ms = e1;
+    c->timings.scatter_ms = e2;
+  }
   return FEM_OK;
 }
 

@@ -100,6 +100,18 @@ struct fem_ctx {
   float* d_feb32 = nullptr;
   int64_t keb64_cells = 0, keb32_cells = 0;
   bool gather_ready = false;
+  // owner-computes assembly of vector Q2 hex elasticity: the three dofs 3m..3m+2 of a dof node
+  // m share one row set made of such triples; node -> (cell, local node) lists n2c (packed
+  // cell*27 + b, ascending cell), the rank of each local node's row triple among column 3m's
+  // triples (one byte per list entry and local node a, 255 = absent), and the element matrices
+  // [cell][b][(i, j)][a] (28 rows a per block, row 27 padding) in FP32 / FP64
+  int q2_ready = -1;                   // 1 tables built, 0 not applicable, -1 not checked
+  int32_t* d_n2c_ptr = nullptr;
+  int32_t* d_n2c = nullptr;
+  uint8_t* d_nrank = nullptr;
+  float* d_keq32 = nullptr;
+  double* d_keq64 = nullptr;
+  int64_t keq32_cells = 0, keq64_cells = 0;
   // pipelined gather: cells in nchunk contiguous chunks; column blocks listed by the chunk
   // after which all their cells are computed (each chunk's list padded to a multiple of 8
   // with −1); a second stream runs the gather of chunk c while the element kernel of c + 1
@@ -150,6 +162,7 @@ struct fem_ctx {
   fem_timings timings{};
   fem_tuning tune{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evm = nullptr;            // between the two phases of an owner-computes assembly
 };
 
 namespace fem {
@@ -191,6 +204,9 @@ int build_tables(fem_ctx* c);
 int noalloc_scratch_init(int device);   // fem_dirichlet.hip: address of its device scratch
 void free_solver(fem_ctx* c);
 int build_gather_tables(fem_ctx* c);
+void free_q2_tables(fem_ctx* c);
+// Q2 hex elasticity through the owner-computes path when its tables apply (*done = true).
+int assemble_elasticity_gather(fem_ctx* c, const fem_assemble_cfg* cfg, bool* done);
 bool gather_supported(const fem_ctx* c);
 int gather_poisson(fem_ctx* c, bool mixed, double src, bool fillzero, const double* tab64, const float* tab32);
 

@@ -68,7 +68,7 @@ class Tuning(C.Structure):
 class Timings(C.Structure):
     _fields_ = [("assemble_kernel_ms", C.c_double), ("fill_ms", C.c_double), ("apply_ms", C.c_double),
                 ("spmv_ms", C.c_double), ("spmv_launches", C.c_int32), ("assemble_launches", C.c_int32),
-                ("solve_ms", C.c_double)]
+                ("solve_ms", C.c_double), ("element_ms", C.c_double), ("scatter_ms", C.c_double)]
 
 
 class BatchCfg(C.Structure):

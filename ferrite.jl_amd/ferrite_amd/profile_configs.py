@@ -7,9 +7,12 @@
 elasticity  Q2 hex^3 small-strain elasticity on generate_grid(Hexahedron, (n, n, n)) over [0,1]^3,
             3x3x3 Gauss, nu = 0.3, E_e ~ U[0.5, 2.0] (seed 0), clamped 'left', traction (0,0,-1)
             on 'right' (3x3 facet rule).  Times the FP64 element path and the FP32 MFMA element
-            path (both with FP64 scatter), reports Ke / u differences between them and the
-            element contraction rate (dense B^T D B count, 2,283,228 flop/cell, and the six
-            27x27x27 products the kernel performs).
+            path, each through the owner-computes path (element phase and gather phase timed
+            apart with HIP events) and through the colored scatter; reports K differences
+            between them, the element contraction rate (dense B^T D B count, 2,283,228
+            flop/cell, and the MFMA work issued), the gather's bytes and GB/s, and the u error
+            of the FP32 element path split into element rounding (backward error of the FP32
+            K at the FP64 solution) and CG tolerance (both solved to rtol and to 5e-11).
 cantilever  Plane-strain Q1^2 cantilever on (nx, ny) cells over [0,2]x[0,1], E = E_min +
             theta^3 (1 - E_min), theta ~ U[0.2, 1.0] (seed 0), clamped at x = 0, point load
             (0, -1) at (2, 0.5).  Times J (device CG) and the adjoint kernel, then checks `fd`
@@ -29,8 +32,14 @@ import sys
 import time
 
 DENSE_BTDB_FLOP_PER_CELL = 2_283_228          # 27 q x 2 x (6*6*81 + 81*6*81)
-PRODUCT_FLOP_PER_CELL = 6 * 27 ** 3 * 2       # six 27x27 products over 27 points
+PRODUCT_FLOP_PER_CELL = 6 * 27 ** 3 * 2       # six 27x27 products over 27 points (FP64 VALU kernel)
+# FP32 MFMA kernel: nine products, each 4 tiles of 16x16 over 7 steps of v_mfma_f32_16x16x4f32
+MFMA_ISSUED_FLOP_PER_CELL = 9 * 4 * 7 * 16 * 16 * 4 * 2
 MFMA_FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+# tight CG tolerance of the C3 error split (FP64 Jacobi CG on the 48^3 system stagnates
+# between 1e-11 and 4e-11, depending on the rounding of K)
+TIGHT_RTOL = 5e-11
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -38,8 +47,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def q2_gather_bytes(ncells, nnode, nnz, ke_bytes):
+    """Algorithmic bytes of the owner-computes Q2 gather (phase 2): every cell's stored element
+    matrix read once (27 blocks of 252 values), its 27 x 27 rank bytes and 27 list entries,
+    two colptr values and the list bounds per dof node, and K.nzval written once."""
+    return ncells * (27 * 252 * ke_bytes + 27 * 27 + 27 * 4) + nnode * (8 + 4) + nnz * 8
+
+
 def elasticity(args):
     import numpy as np
+    import scipy.sparse as sp
     import ferrite_amd as fa
     n = args.n
     t0 = time.perf_counter()
@@ -57,33 +74,84 @@ def elasticity(args):
     ch.add(fa.Dirichlet("u", g.facetsets["left"], lambda x, t: [0.0, 0.0, 0.0])).close()
     log(f"setup {time.perf_counter() - t0:.1f}s: {g.ncells} cells, {dh.ndofs} dofs, nnz {K.nnz}, "
         f"{len(colors)} colors")
+    nc, nnode = g.ncells, dh.ndofs // 3
     out = {"workload": f"Q2 hex^3 elasticity {n}^3, 3x3x3 Gauss, E ~ U[0.5,2], nu 0.3",
-           "cells": g.ncells, "dofs": dh.ndofs, "nnz": K.nnz, "colors": len(colors)}
-    sols, nzs = {}, {}
+           "cells": nc, "dofs": dh.ndofs, "nnz": K.nnz, "colors": len(colors),
+           "units": {"dense_btdb_flop_per_cell": DENSE_BTDB_FLOP_PER_CELL,
+                     "mfma_issued_flop_per_cell": MFMA_ISSUED_FLOP_PER_CELL,
+                     "product_flop_per_cell_fp64": PRODUCT_FLOP_PER_CELL}}
+    nzs = {}
     for prec in ("fp64", "mixed"):
-        ms = []
-        for _ in range(args.reps):
-            fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=prec)
-            ms.append(fa.timings(K)["assemble_kernel_ms"])
-        best = min(ms)
-        nzs[prec] = K.nzval()
+        for asm in ("gather", "scatter"):
+            fa.set_tuning(K, asm_mode=0 if asm == "gather" else 1)
+            runs = []
+            for _ in range(args.reps):
+                fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=prec)
+                runs.append(fa.timings(K))
+            best = min(runs, key=lambda t: t["assemble_kernel_ms"])
+            ms = best["assemble_kernel_ms"]
+            rec = {"assembly_ms": ms, "assembly_ms_all": [t["assemble_kernel_ms"] for t in runs],
+                   "cells_per_s": nc / (ms * 1e-3),
+                   "dense_btdb_tflops": DENSE_BTDB_FLOP_PER_CELL * nc / (ms * 1e-3) / 1e12}
+            if asm == "gather":
+                e_ms, s_ms = best["element_ms"], best["scatter_ms"]
+                kb = 4 if prec == "mixed" else 8
+                gb = q2_gather_bytes(nc, nnode, K.nnz, kb)
+                rec.update({"element_ms": e_ms, "scatter_ms": s_ms,
+                            "element_dense_btdb_tflops": DENSE_BTDB_FLOP_PER_CELL * nc / (e_ms * 1e-3) / 1e12,
+                            "element_ke_write_GBs": nc * 6804 * kb / (e_ms * 1e-3) / 1e9,
+                            "scatter_bytes": gb, "scatter_GBs": gb / (s_ms * 1e-3) / 1e9,
+                            "scatter_frac_hbm": gb / (s_ms * 1e-3) / 1e9 / HBM_PEAK_GBS})
+                if prec == "mixed":
+                    rec["element_mfma_issued_tflops"] = MFMA_ISSUED_FLOP_PER_CELL * nc / (e_ms * 1e-3) / 1e12
+                    rec["element_mfma_frac_issued"] = rec["element_mfma_issued_tflops"] / MFMA_FP32_PEAK_TFLOPS
+                else:
+                    rec["element_fp64_product_tflops"] = PRODUCT_FLOP_PER_CELL * nc / (e_ms * 1e-3) / 1e12
+                log(f"{prec} {asm}: assembly {ms:.3f} ms (element {e_ms:.3f} ms, gather {s_ms:.3f} ms "
+                    f"= {rec['scatter_GBs']:.0f} GB/s)")
+            else:
+                log(f"{prec} {asm}: assembly {ms:.3f} ms")
+            out[f"{prec}_{asm}"] = rec
+            nzs[(prec, asm)] = K.nzval()
+    for prec in ("fp64", "mixed"):
+        a, b = nzs[(prec, "scatter")], nzs[(prec, "gather")]
+        out[f"{prec}_K_rel_diff_scatter_vs_gather"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    del nzs[("fp64", "scatter")], nzs[("mixed", "scatter")]
+    fa.set_tuning(K, asm_mode=0)
+    a, b = nzs[("mixed", "gather")], nzs[("fp64", "gather")]
+    out["K_rel_err_mixed_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    # u error of the FP32 element path, split into the element rounding and the CG tolerance:
+    # both systems solved to args.rtol and to TIGHT_RTOL, and the backward error of the FP32-element
+    # matrix at the FP64 solution, ||(K32 - K64) u64|| / ||f||, which is what the rounding of Ke
+    # alone contributes to the residual
+    sols, kc = {}, {}
+    for prec in ("fp64", "mixed"):
+        fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=prec)
         fa.apply_traction(K, f, fv, g.facetsets["right"], (0.0, 0.0, -1.0))
         fa.apply(K, f, ch)
-        t1 = time.perf_counter()
-        res = fa.solve(K, f, mode="cg", rtol=args.rtol)
-        ts = time.perf_counter() - t1
-        sols[prec] = res.u
-        out[prec] = {"assembly_ms": best, "assembly_ms_all": ms, "cells_per_s": g.ncells / (best * 1e-3),
-                     "dense_btdb_tflops": DENSE_BTDB_FLOP_PER_CELL * g.ncells / (best * 1e-3) / 1e12,
-                     "product_tflops": PRODUCT_FLOP_PER_CELL * g.ncells / (best * 1e-3) / 1e12,
-                     "solve_s": ts, "cg_iterations": res.iterations, "relres": res.relres}
-        log(f"{prec}: assembly {best:.3f} ms, CG {res.iterations} its {ts:.2f}s relres {res.relres:.2e}")
-    a, b = nzs["mixed"], nzs["fp64"]
-    out["K_rel_err_mixed_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    out["u_rel_err_mixed_vs_fp64"] = float(np.linalg.norm(sols["mixed"] - sols["fp64"]) /
-                                           np.linalg.norm(sols["fp64"]))
-    out["mfma_fp32_frac_product_count"] = out["mixed"]["product_tflops"] / MFMA_FP32_PEAK_TFLOPS
-    out["fp64_frac_product_count"] = out["fp64"]["product_tflops"] / FP64_PEAK_TFLOPS
+        kc[prec] = K.nzval()
+        fvec = f.to_numpy()
+        for rtol in (args.rtol, TIGHT_RTOL):
+            t1 = time.perf_counter()
+            res = fa.solve(K, f, mode="cg", rtol=rtol, maxit=20000)
+            ts = time.perf_counter() - t1
+            sols[(prec, rtol)] = res.u
+            out[f"{prec}_gather"][f"solve_rtol_{rtol:g}"] = {"solve_s": ts, "cg_iterations": res.iterations,
+                                                             "relres": res.relres}
+            log(f"{prec}: CG to {rtol:g}: {res.iterations} its {ts:.2f}s relres {res.relres:.2e}")
+    for rtol in (args.rtol, TIGHT_RTOL):
+        u32, u64 = sols[("mixed", rtol)], sols[("fp64", rtol)]
+        out[f"u_rel_err_mixed_vs_fp64_rtol_{rtol:g}"] = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
+    u64 = sols[("fp64", TIGHT_RTOL)]
+    cp = np.asarray(K.colptr, dtype=np.int64) - 1
+    rv = np.asarray(K.rowval, dtype=np.int64) - 1
+    D = sp.csc_matrix((kc["mixed"] - kc["fp64"], rv, cp), shape=(dh.ndofs, dh.ndofs))
+    out["backward_err_fp32_Ke_at_u64"] = float(np.linalg.norm(D @ u64) / np.linalg.norm(fvec))
+    out["K_rel_err_mixed_vs_fp64_constrained"] = float(np.linalg.norm(kc["mixed"] - kc["fp64"]) /
+                                                       np.linalg.norm(kc["fp64"]))
+    log(f"u error mixed vs fp64: {out[f'u_rel_err_mixed_vs_fp64_rtol_{args.rtol:g}']:.2e} at rtol {args.rtol:g}, "
+        f"{out[f'u_rel_err_mixed_vs_fp64_rtol_{TIGHT_RTOL:g}']:.2e} at {TIGHT_RTOL:g}; backward error of the FP32 Ke "
+        f"{out['backward_err_fp32_Ke_at_u64']:.2e}")
     return out
 
 

@@ -228,6 +228,9 @@ typedef struct {
   int32_t spmv_launches;       /* number of those launches (0 without time_kernels) */
   int32_t assemble_launches;
   double solve_ms;             /* the CG loop of the last solve (HIP events) */
+  double element_ms;           /* two-phase (owner-computes) assembly: the element-matrix phase */
+  double scatter_ms;           /* ... and the phase that sums them into nzval/f (0 for the
+                                  colored scatter, whose kernels do both) */
 } fem_timings;
 int32_t fem_get_timings(const fem_ctx* ctx, fem_timings* t);
 /* Average duration (HIP events) of `reps` back-to-back SpMV launches y = K x of the solver,

@@ -34,14 +34,21 @@ def _device_vector(ct, nel, left, right, order, q, colors="workstream"):
     return g, ip, cv, dh, K, f, fa.create_coloring(g, colors)
 
 
+ASM = {"gather": 0, "scatter": 1}
+
+
+@pytest.mark.parametrize("asm", ["gather", "scatter"])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
-def test_q2_hex_elasticity_parity(precision, tol):
-    print("case q2 hex elasticity", precision, flush=True)
+def test_q2_hex_elasticity_parity(precision, tol, asm):
+    print("case q2 hex elasticity", precision, asm, flush=True)
     nel = (3, 2, 2)
     rng = np.random.default_rng(5)
     E = rng.uniform(0.5, 2.0, int(np.prod(nel)))
     g, ip, cv, dh, K, f, colors = _device_vector("Hexahedron", nel, (0, 0, 0), (1, 1, 1), 2, 3)
+    fa.set_tuning(K, asm_mode=ASM[asm])
     fa.assemble_global(fa.start_assemble(K, f), cv, colors, fa.ElasticityElement(E=E, nu=0.3), precision=precision)
+    t = fa.timings(K)
+    assert (t["element_ms"] > 0 and t["scatter_ms"] > 0) == (asm == "gather")
     grid, ocv, odh, OK, of = setup_vector("Hexahedron", nel, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 2, 3)
     assemble_elasticity(grid, ocv, odh, OK, of, E, 0.3)
     assert np.array_equal(K.colptr, OK.colptr) and np.array_equal(K.rowval, OK.rowval)
@@ -67,6 +74,42 @@ def test_q2_hex_elasticity_parity(precision, tol):
     res = fa.solve(K, f, mode="cg", rtol=1e-13)
     u = OC.solve(OK, of)
     assert _rel(res.u, u) < (1e-8 if precision == "fp64" else 1e-4)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
+def test_q2_hex_gather_parity_distorted(precision, tol):
+    """Owner-computes Q2 path on a 6x5x4 mesh (vertex nodes in 8 cells) with moved interior
+    nodes: K vs the oracle, start_assemble(fillzero=false) accumulation, run-twice bitwise,
+    and the colored scatter within rounding."""
+    print("case q2 hex gather distorted", precision, flush=True)
+    nel = (6, 5, 4)
+    rng = np.random.default_rng(11)
+    E = rng.uniform(0.5, 2.0, int(np.prod(nel)))
+    g, ip, cv, dh, K, f, colors = _device_vector("Hexahedron", nel, (0, 0, 0), (1, 1, 1), 2, 3)
+    grid, ocv, odh, OK, of = setup_vector("Hexahedron", nel, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 2, 3)
+    # move interior nodes (the same on both sides); boundary nodes stay on the box
+    X = g.nodes.copy()
+    inner = np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1)
+    X[inner] += rng.uniform(-0.04, 0.04, (int(inner.sum()), 3))
+    g.nodes[:] = X
+    grid.nodes[:] = X
+    K = fa.allocate_matrix(fa.MI355XCSC, dh)
+    f = fa.allocate_vector(K)
+    el = fa.ElasticityElement(E=E, nu=0.3)
+    fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=precision)
+    assemble_elasticity(grid, ocv, odh, OK, of, E, 0.3)
+    nz = K.nzval()
+    assert _rel(nz, OK.nzval) <= tol
+    assert np.abs(nz - OK.nzval).max() <= tol * np.abs(OK.nzval).max()
+    S = K.to_scipy()
+    assert abs(S - S.T).max() == 0.0        # symmetric bit for bit, like Ferrite's K
+    fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=precision)
+    assert np.array_equal(K.nzval(), nz)
+    fa.assemble_global(fa.start_assemble(K, f, fillzero=False), cv, colors, el, precision=precision)
+    assert _rel(K.nzval(), 2 * OK.nzval) <= tol
+    fa.set_tuning(K, asm_mode=1)
+    fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=precision)
+    assert _rel(K.nzval(), nz) <= (1e-14 if precision == "fp64" else 1e-6)
 
 
 def _cantilever(nx, ny, seed):
