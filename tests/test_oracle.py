@@ -252,3 +252,24 @@ def test_apply_inhomogeneous_matches_elimination():
     uf = np.linalg.solve(A[np.ix_(free, free)], b[free] - A[np.ix_(free, pres)] @ g)
     assert np.allclose(u[pres], g, atol=1e-14)
     assert np.allclose(u[free], uf, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("ct,nel,distort", [("Quadrilateral", (5, 4), 0.0), ("Hexahedron", (4, 3, 3), 0.0),
+                                            ("Hexahedron", (3, 3, 2), 0.06)])
+def test_vectorised_poisson_matches_serial_loop(ct, nel, distort):
+    """oracle.fem_vec (batched over cells) against the per-cell serial loop of oracle.problems:
+    nzval and f to ≤1e-14 (different order of the batched products only), same pattern."""
+    from oracle.fem_vec import assemble_poisson_vec
+    from oracle.problems import assemble_poisson, poisson_conductivity, setup_poisson
+    grid, cv, dh, K, f, _ = setup_poisson(ct, nel)
+    if distort:
+        X = grid.nodes
+        inner = np.all((X > -1 + 1e-12) & (X < 1 - 1e-12), axis=1)
+        X[inner] += np.random.default_rng(2).uniform(-distort, distort, (int(inner.sum()), X.shape[1]))
+    k = poisson_conductivity(nel, seed=4)
+    assemble_poisson(grid, cv, dh, K, f, k_cell=k)
+    cd = np.array(dh.cell_dofs).reshape(grid.ncells, -1)
+    nzv, fv = assemble_poisson_vec(grid, cv, cd, K.colptr, K.rowval, k_cell=k)
+    assert np.linalg.norm(nzv - K.nzval) <= 1e-14 * np.linalg.norm(K.nzval)
+    assert np.abs(nzv - K.nzval).max() <= 1e-14 * np.abs(K.nzval).max()
+    assert np.linalg.norm(fv - f) <= 1e-14 * np.linalg.norm(f)
