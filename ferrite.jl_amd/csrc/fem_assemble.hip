@@ -309,6 +309,7 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
       rg = gather_poisson(c, mixed, cfg->source, cfg->fillzero != 0, c->d_tab, c->d_tabf);
       if (rg) return rg;
       ++c->nzval_version;
+      elem_op_note_assembly(c, 1, !mixed, cfg->fillzero != 0);
       return check_assembly_errors(c);
     }
     if (c->tune.asm_mode == 2)
@@ -321,6 +322,7 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
     if (rq) return rq;
     if (done) {
       ++c->nzval_version;
+      elem_op_note_assembly(c, 3, !mixed, cfg->fillzero != 0);
       return check_assembly_errors(c);
     }
   }
@@ -363,5 +365,8 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
   }
   if (rc) return rc;
   ++c->nzval_version;
+  // the Q1 quad^2 elasticity kernel stores its element matrices as planes as well
+  const bool q1v = cfg->kernel == FEM_KERNEL_ELASTICITY && c->rdim == 2 && c->ndpc == 8 && c->ep_cells == c->ncells;
+  elem_op_note_assembly(c, q1v ? 2 : 0, !mixed, cfg->fillzero != 0);
   return check_assembly_errors(c);
 }

@@ -159,6 +159,7 @@ void fem_ctx_destroy(fem_ctx* c) {
   dev_free(&c->d_keb64); dev_free(&c->d_feb64); dev_free(&c->d_keb32); dev_free(&c->d_feb32);
   dev_free(&c->d_gblist);
   free_q2_tables(c);
+  dev_free(&c->d_gcdofs); dev_free(&c->d_ep64); dev_free(&c->d_ep32);
   for (hipEvent_t e : c->gchunk_ev) (void)hipEventDestroy(e);
   if (c->stream2) {
     (void)hipStreamSynchronize(c->stream2);
@@ -209,6 +210,8 @@ int32_t fem_mesh_upload(fem_ctx* c, int32_t sdim, int64_t nnodes, const double* 
   c->sdim = sdim; c->nnodes = nnodes; c->ncells = ncells; c->npc = npc;
   free_solver(c);   // the solver's operator copies cache geometry (cell corners, lattice map)
   free_q2_tables(c);
+  dev_free(&c->d_d2c_ptr); dev_free(&c->d_d2c); dev_free(&c->d_gcdofs);
+  c->ea_kind = 0;
   c->tables_ready = false;
   c->gather_ready = false;
   c->gchunks_built = 0;
@@ -236,6 +239,8 @@ int32_t fem_dofs_upload(fem_ctx* c, int64_t ndofs, int32_t ndpc, const int64_t* 
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   free_solver(c);   // operator copies are built on the dof numbering
   free_q2_tables(c);
+  dev_free(&c->d_d2c_ptr); dev_free(&c->d_d2c); dev_free(&c->d_gcdofs);
+  c->ea_kind = 0;
   c->tables_ready = false;
   c->gather_ready = false;
   c->gchunks_built = 0;
@@ -280,6 +285,7 @@ int32_t fem_pattern_upload(fem_ctx* c, int64_t n, const int64_t* colptr, const i
   for (int64_t j = 0; j < n; ++j) c->max_row_nnz = std::max<int>(c->max_row_nnz, cp[j + 1] - cp[j]);
   free_solver(c);
   free_q2_tables(c);
+  c->ea_kind = 0;
   c->tables_ready = false;
   c->gather_ready = false;
   c->gchunks_built = 0;

@@ -290,6 +290,7 @@ int32_t fem_apply_dirichlet(fem_ctx* c, int32_t applyzero, double* m_out) {
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
   c->timings.apply_ms = event_ms(c);
   if (m_out) FEM_HIP(c, hipMemcpy(m_out, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost));
+  elem_op_note_apply(c);
   ++c->nzval_version;
   return FEM_OK;
 }
@@ -338,6 +339,7 @@ int32_t fem_apply_dirichlet_noalloc(fem_ctx* c, int32_t applyzero, double* m_out
     FEM_HIP(c, hipMemcpyAsync(m_out, S + kScratchMean, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     FEM_HIP(c, hipStreamSynchronize(c->stream));
   }
+  elem_op_note_apply(c);
   ++c->nzval_version;
   return FEM_OK;
 }

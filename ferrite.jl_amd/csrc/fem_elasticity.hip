@@ -63,7 +63,8 @@ __global__ __launch_bounds__(BLOCK) void k_elast_small(
     int64_t s0, int64_t s1, const double* __restrict__ X0, const double* __restrict__ X1,
     const double* __restrict__ X2, const int32_t* __restrict__ cnodes, const int32_t* __restrict__ cdofs,
     const int32_t* __restrict__ cnz, const int32_t* __restrict__ ccell, const double* __restrict__ cparam,
-    const T* __restrict__ tab, T lam, T mu, double* __restrict__ nzval, DevError* __restrict__ err) {
+    const T* __restrict__ tab, T lam, T mu, double* __restrict__ nzval, DevError* __restrict__ err,
+    T* __restrict__ planes, int64_t ncells) {
   constexpr int NB = NG * DIM;  // dofs (function ip == geometry ip, vectorised)
   constexpr int TRI = NB * (NB + 1) / 2;
   using L = TabL<DIM, NG, NG, NQ>;
@@ -130,6 +131,11 @@ __global__ __launch_bounds__(BLOCK) void k_elast_small(
     if (bad) atomicMin(&err->detj_cell, ccell[s]);
 #pragma unroll
     for (int i = 0; i < TRI; ++i) s_ke[lc * TRI + i] = ke[i];
+    if (planes) {   // the element matrix for the element-assembly solver operator: Ke[t][cell]
+      const int64_t cell = ccell[s];
+#pragma unroll
+      for (int i = 0; i < TRI; ++i) planes[i * ncells + cell] = ke[i];
+    }
   }
   __syncthreads();
   constexpr int E2 = NB * NB;
@@ -650,7 +656,7 @@ static int upload_tab(fem_ctx* c) {
 }
 
 void free_q2_tables(fem_ctx* c) {
-  dev_free(&c->d_n2c_ptr); dev_free(&c->d_n2c); dev_free(&c->d_nrank);
+  dev_free(&c->d_n2c_ptr); dev_free(&c->d_n2c); dev_free(&c->d_nrank); dev_free(&c->d_q2cnode);
   dev_free(&c->d_keq32); dev_free(&c->d_keq64);
   c->keq32_cells = c->keq64_cells = 0;
   c->q2_ready = -1;
@@ -711,6 +717,11 @@ static int build_q2_tables(fem_ctx* c) {
   k_build_nrank<<<gridq(nnode), 256, 0, c->stream>>>(nnode, c->d_colptr, c->d_rowval, c->d_n2c_ptr, c->d_n2c, dcd,
                                                       c->d_nrank);
   FEM_HIP(c, hipGetLastError());
+  // dof node of every local node (element-assembly solver operator)
+  std::vector<int32_t> cn(nc * kQN);
+  for (int64_t i = 0; i < nc * kQN; ++i) cn[i] = cd[3 * i] / 3;
+  if ((rc = dev_alloc(c, &c->d_q2cnode, nc * kQN))) return rc;
+  FEM_HIP(c, hipMemcpyAsync(c->d_q2cnode, cn.data(), 4 * nc * kQN, hipMemcpyHostToDevice, c->stream));
   FEM_HIP(c, hipStreamSynchronize(c->stream));
   dev_free(&dcd);
   c->q2_ready = 1;
@@ -783,6 +794,19 @@ int assemble_elasticity(fem_ctx* c, const fem_assemble_cfg* cfg) {
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   if (c->rdim == 2 && c->ngeo == 4 && c->nbf == 4 && c->nqp == 4 && c->npc == 4) {
     if ((rc = upload_tab<2, 4, 4, 4>(c))) return rc;
+    // element matrices kept as planes (upper triangle, 36 per cell) for the element-assembly
+    // solver operator, with the row lists it walks
+    constexpr int TRI = 36;
+    if (c->ep_cells != c->ncells) {
+      dev_free(&c->d_ep64);
+      dev_free(&c->d_ep32);
+    }
+    if (mixed ? !c->d_ep32 : !c->d_ep64) {
+      if ((rc = mixed ? dev_alloc(c, &c->d_ep32, TRI * c->ncells) : dev_alloc(c, &c->d_ep64, TRI * c->ncells)))
+        return rc;
+    }
+    c->ep_cells = c->ncells;
+    if ((rc = build_ea_rows_tables(c))) return rc;
     FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
     for (int col = 0; col < c->ncolors; ++col) {
       const int64_t s0 = c->color_ptr[col], s1 = c->color_ptr[col + 1];
@@ -791,10 +815,12 @@ int assemble_elasticity(fem_ctx* c, const fem_assemble_cfg* cfg) {
       const unsigned nb = (unsigned)((s1 - s0 + B - 1) / B);
       if (mixed)
         k_elast_small<float, 2, 4, 4, B><<<nb, B, 0, c->stream>>>(s0, s1, c->d_x[0], c->d_x[1], c->d_x[2], c->d_cnodes,
-            c->d_cdofs, c->d_cnz, c->d_ccell, c->d_cparam, c->d_tabf, (float)lam, (float)mu, c->d_nzval, c->d_err);
+            c->d_cdofs, c->d_cnz, c->d_ccell, c->d_cparam, c->d_tabf, (float)lam, (float)mu, c->d_nzval, c->d_err,
+            c->d_ep32, c->ncells);
       else
         k_elast_small<double, 2, 4, 4, B><<<nb, B, 0, c->stream>>>(s0, s1, c->d_x[0], c->d_x[1], c->d_x[2], c->d_cnodes,
-            c->d_cdofs, c->d_cnz, c->d_ccell, c->d_cparam, c->d_tab, lam, mu, c->d_nzval, c->d_err);
+            c->d_cdofs, c->d_cnz, c->d_ccell, c->d_cparam, c->d_tab, lam, mu, c->d_nzval, c->d_err, c->d_ep64,
+            c->ncells);
       c->timings.assemble_launches++;
     }
   } else if (c->rdim == 3 && c->ngeo == 8 && c->nbf == 27 && c->nqp == 27 && c->npc == 8) {

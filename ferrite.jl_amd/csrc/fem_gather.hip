@@ -282,10 +282,11 @@ int build_gather_tables(fem_ctx* c) {
       const int32_t d = c->h_cell_dofs[cell * nb + b];
       ent[fill[d]++] = (int32_t)(cell * nb + b);
     }
-  int32_t* dcd = nullptr;
+  // cell-order dofs stay resident: the element-assembly solver operator reads them
   if ((rc = dev_alloc(c, &c->d_d2c_ptr, nd + 1)) || (rc = dev_alloc(c, &c->d_d2c, nc * nb)) ||
-      (rc = dev_alloc(c, &c->d_gslot, nc * (int64_t)nb * nb)) || (rc = dev_alloc(c, &dcd, nc * nb)))
+      (rc = dev_alloc(c, &c->d_gslot, nc * (int64_t)nb * nb)) || (rc = dev_alloc(c, &c->d_gcdofs, nc * nb)))
     return rc;
+  int32_t* dcd = c->d_gcdofs;
   FEM_HIP(c, hipMemcpyAsync(c->d_d2c_ptr, ptr.data(), 4 * (nd + 1), hipMemcpyHostToDevice, c->stream));
   FEM_HIP(c, hipMemcpyAsync(c->d_d2c, ent.data(), 4 * nc * nb, hipMemcpyHostToDevice, c->stream));
   FEM_HIP(c, hipMemcpyAsync(dcd, c->h_cell_dofs.data(), 4 * nc * nb, hipMemcpyHostToDevice, c->stream));
@@ -296,12 +297,10 @@ int build_gather_tables(fem_ctx* c) {
     k_build_slots<4><<<gridn(nd), 256, 0, c->stream>>>(nd, c->d_colptr, c->d_rowval, c->d_d2c_ptr, c->d_d2c, dcd,
                                                         c->d_gslot);
   else {
-    dev_free(&dcd);
     return set_error(c, FEM_ERR_ARG, "gather assembly supports 4 or 8 dofs per cell");
   }
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipStreamSynchronize(c->stream));
-  dev_free(&dcd);
   c->gather_ready = true;
   return FEM_OK;
 }

@@ -43,6 +43,25 @@ struct DevError {
 
 struct SolverWork;
 
+// CG scalars of the scaled system (hat quantities): γ = r̂·r̂, δ = r̂·Â r̂.
+struct Scal {
+  double rz;      // γ of the current residual
+  double pq;      // p·Â p at a reseed
+  double rr;      // true residual ||f − K u||² after residual() (unscaled)
+  double alpha;   // step length of the last step
+  double beta;
+  double bnorm2;
+  double gstop;   // hand the recurrence over (done = 1) once γ <= gstop
+  double rzp;     // γ of the previous step
+  double gam;     // (r̂·p)/(r̂·r̂) at a reseed: p is made orthogonal to the new residual
+  int done;       // 1: recurrence stopped (claims convergence or fell by δ), or broke down
+  int iters;
+  int restart;    // breakdown happened: the next reseed restarts p = r̂
+  int fresh;      // p, s = Â p and α were just set by a reseed: the next step only moves x̂, r̂
+  int sfresh;     // (record written by k_cg_scal) the step k_cg_move runs now is such a step
+  int pad;
+};
+
 }  // namespace fem
 
 struct fem_ctx {
@@ -163,6 +182,21 @@ struct fem_ctx {
   fem_tuning tune{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evm = nullptr;            // between the two phases of an owner-computes assembly
+
+  // element matrices that K equals (the element-assembly solver operator, fem_elemop.hip):
+  // ea_kind 1 scalar Q1 planes Ke[t][cell] of the strict upper triangle (diagonal from the zero
+  // row sums), 2 planes of the upper triangle with the diagonal (Q1 quad^2), 3 Q2 node blocks;
+  // valid while nzval_version == ea_version, i.e. K is the last fillzero assembly, followed
+  // by apply! (ea_constrained) with the constraints of aux version ea_aux
+  int ea_kind = 0;
+  bool ea_fp64 = false;                // which planes hold them: the FP64 or the FP32 copy
+  uint64_t ea_version = 0, ea_aux = 0;
+  bool ea_constrained = false;
+  int32_t* d_gcdofs = nullptr;         // [ncells][ndpc] cell dofs in cell-id order
+  int32_t* d_q2cnode = nullptr;        // [ncells][27] dof nodes of each Q2 cell
+  double* d_ep64 = nullptr;            // Q1 quad^2 planes (36 per cell), FP64 / FP32
+  float* d_ep32 = nullptr;
+  int64_t ep_cells = 0;
 };
 
 namespace fem {
@@ -204,6 +238,17 @@ int build_tables(fem_ctx* c);
 int noalloc_scratch_init(int device);   // fem_dirichlet.hip: address of its device scratch
 void free_solver(fem_ctx* c);
 int build_gather_tables(fem_ctx* c);
+// element-assembly solver operator (fem_elemop.hip)
+struct ElemOp;
+bool elem_op_valid(const fem_ctx* c);
+int elem_op_prepare(fem_ctx* c, ElemOp** op, const double* sc, bool fp32, int* grid);
+int elem_op_launch(fem_ctx* c, const ElemOp* op, const float* x, float* y, double* part, const Scal* s);
+int elem_op_launch(fem_ctx* c, const ElemOp* op, const double* x, double* y, double* part, const Scal* s);
+void elem_op_free(ElemOp** op);
+const void* elem_op_key(const fem_ctx* c, const ElemOp* op, bool fp32);
+void elem_op_note_apply(fem_ctx* c);
+void elem_op_note_assembly(fem_ctx* c, int kind, bool fp64, bool fillzero);
+int build_ea_rows_tables(fem_ctx* c);
 void free_q2_tables(fem_ctx* c);
 // Q2 hex elasticity through the owner-computes path when its tables apply (*done = true).
 int assemble_elasticity_gather(fem_ctx* c, const fem_assemble_cfg* cfg, bool* done);

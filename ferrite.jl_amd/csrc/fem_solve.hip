@@ -41,24 +41,6 @@ constexpr int kBlock = 256;
 constexpr int kMaxVecBlocks = 2048;
 constexpr int kBatch = 16;           // CG iterations per graph replay
 
-// CG scalars of the scaled system (hat quantities): γ = r̂·r̂, δ = r̂·Â r̂.
-struct Scal {
-  double rz;      // γ of the current residual
-  double pq;      // p·Â p at a reseed
-  double rr;      // true residual ||f − K u||² after residual() (unscaled)
-  double alpha;   // step length of the last step
-  double beta;
-  double bnorm2;
-  double gstop;   // hand the recurrence over (done = 1) once γ <= gstop
-  double rzp;     // γ of the previous step
-  double gam;     // (r̂·p)/(r̂·r̂) at a reseed: p is made orthogonal to the new residual
-  int done;       // 1: recurrence stopped (claims convergence or fell by δ), or broke down
-  int iters;
-  int restart;    // breakdown happened: the next reseed restarts p = r̂
-  int fresh;      // p, s = Â p and α were just set by a reseed: the next step only moves x̂, r̂
-  int sfresh;     // (record written by k_cg_scal) the step k_cg_move runs now is such a step
-  int pad;
-};
 
 struct SpmvShape {
   int R = 0;          // rows per block
@@ -185,6 +167,9 @@ struct SolverWork {
   CellOp cells;
   bool use_lat = false;        // the current solve runs on a lattice operator (padded vectors)
   bool use_cells = false;      // ... in its cell-coefficient form
+  bool use_elem = false;       // the current solve runs on the stored element matrices
+  ElemOp* eop = nullptr;       // their per-solve vectors (fem_elemop.hip)
+  int elem_grid = 0;
   int64_t cap32 = 0, cap64 = 0;   // length of the recurrence vectors
   int sell_grid = 0;           // blocks of the SELL SpMV (fixed: its partial count)
   SpmvShape sh64;              // CSR-stream FP64 residual
@@ -223,6 +208,7 @@ void free_solver(fem_ctx* c) {
   LatOp& l = w->lat;
   dev_free(&l.map); dev_free(&l.dofp); dev_free(&l.val32); dev_free(&l.val64); dev_free(&l.cnt);
   dev_free(&l.mir32); dev_free(&l.mir64);
+  elem_op_free(&w->eop);
   CellOp& cl = w->cells;
   dev_free(&cl.corg); dev_free(&cl.kc64); dev_free(&cl.kc32); dev_free(&cl.t64); dev_free(&cl.t32);
   dev_free(&cl.cnt); dev_free(&cl.up64); dev_free(&cl.yp64); dev_free(&cl.tm64); dev_free(&cl.s0);
@@ -2243,6 +2229,7 @@ static void launch_sell(fem_ctx* c, const SellOp& o, int g, const V* val, const 
 // number of partials written.
 template <typename X>
 static int spmv_hat(fem_ctx* c, SolverWork* w, const X* x, X* y, const Scal* sc) {
+  if (w->use_elem) return elem_op_launch(c, w->eop, x, y, w->part_s, sc);
   if (w->use_cells) {
     launch_cells<X>(c, w, x, y, w->part_s, sc);
     return sizeof(X) == 4 ? w->cells.grid32 : w->cells.grid64;
@@ -2273,6 +2260,7 @@ static int prepare_operator(fem_ctx* c, SolverWork* w, int jacobi, int64_t* nv) 
   k_scale<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, c->d_diag, c->d_nzval, jacobi, w->sc);
   w->use_lat = false;
   w->use_cells = false;
+  w->use_elem = false;
   const bool want = sizeof(X) == 4 ? c->tune.lattice != -1 : c->tune.lattice == 1;
   const bool want_cells = sizeof(X) == 4 ? c->tune.cell_operator != -1 : c->tune.cell_operator == 1;
   if (want || want_cells) {
@@ -2288,9 +2276,16 @@ static int prepare_operator(fem_ctx* c, SolverWork* w, int jacobi, int64_t* nv) 
       w->use_lat = sym;
     }
   }
+  // the stored element matrices where no lattice form applies (any mesh; the mixed solver by
+  // default, the FP64 solver with tuning elem_operator = 1)
+  const bool want_elem = sizeof(X) == 4 ? c->tune.elem_operator != -1 : c->tune.elem_operator == 1;
+  if (!w->use_lat && want_elem && elem_op_valid(c)) {
+    if ((rc = elem_op_prepare(c, &w->eop, w->sc, sizeof(X) == 4, &w->elem_grid))) return rc;
+    w->use_elem = true;
+  }
   *nv = w->use_lat ? w->lat.npad : c->n;
   if ((rc = ensure_vecs<X>(c, w, *nv))) return rc;
-  if (w->use_lat) return FEM_OK;
+  if (w->use_lat || w->use_elem) return FEM_OK;
   if ((rc = ensure_sell(c, w))) return rc;
   return ensure_sell_vals<X>(c, w, jacobi);
 }
@@ -2364,13 +2359,15 @@ static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int
 template <typename X>
 static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache& gc) {
   static_assert(kBatch % 2 == 0, "batch must end on scalar record 0");
-  const void* val = w->use_cells ? (sizeof(X) == 4 ? (const void*)w->cells.kc32 : (const void*)w->cells.kc64)
+  const void* val = w->use_elem ? elem_op_key(c, w->eop, sizeof(X) == 4)
+                    : w->use_cells ? (sizeof(X) == 4 ? (const void*)w->cells.kc32 : (const void*)w->cells.kc64)
                     : w->use_lat ? (sizeof(X) == 4 ? (const void*)w->lat.val32 : (const void*)w->lat.val64)
                                  : (sizeof(X) == 4 ? (const void*)w->sell.val32 : (const void*)w->sell.val64);
   // the lattice SpMV variant depends on the values (a unit diagonal), not only on the operands;
   // the cell form's K¹ class values are kernel arguments, so its graph is keyed on their bits
   // (a new K with new k_e reuses the graph: k_e is read from memory)
-  uintptr_t variant = w->use_lat ? 1 + (sizeof(X) == 4 ? w->lat.unit32 : w->lat.unit64) : 0;
+  uintptr_t variant = w->use_lat ? 1 + (sizeof(X) == 4 ? w->lat.unit32 : w->lat.unit64)
+                      : w->use_elem ? 8 + c->ea_kind : 0;
   if (w->use_cells) {
     uint64_t h = 1469598103934665603ull;   // FNV-1a over the eight values in X precision
     for (int i = 0; i < 8; ++i) {
@@ -2381,7 +2378,10 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache
     }
     variant = (uintptr_t)(h << 2) | 3;
   }
-  const void* key[9] = {val, v.x, v.r, v.p, v.w, v.sv, w->s, w->use_lat ? (const void*)w->lat.map : w->sell.off,
+  const void* key[9] = {val, v.x, v.r, v.p, v.w, v.sv, w->s,
+                        w->use_lat ? (const void*)w->lat.map
+                        : w->use_elem ? (c->ea_kind == 3 ? (const void*)c->d_n2c : (const void*)c->d_d2c)
+                                      : w->sell.off,
                         reinterpret_cast<const void*>(variant)};
   if (c->tune.time_kernels) {   // kernels launched directly (launch_batch), SpMVs bracketed by events
     for (hipEvent_t& e : w->tev)
@@ -2560,7 +2560,8 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   if (rc && rc != FEM_ERR_NOCONV) return rc;
   out.iterations = iters;
   out.outer_iterations = handovers;
-  out.operator_format = w->use_cells ? 3 : !w->use_lat ? 0 : (mixed ? w->lat.unit32 : w->lat.unit64) ? 2 : 1;
+  out.operator_format = w->use_elem ? 4 : w->use_cells ? 3 : !w->use_lat ? 0
+                        : (mixed ? w->lat.unit32 : w->lat.unit64) ? 2 : 1;
   if ((rc = read_scal(c, w, &h))) return rc;   // s->rr: the true residual of the returned u
   out.relres = std::sqrt(h.rr / bnorm2);
   c->timings.solve_ms = cg_ms;
@@ -2627,7 +2628,8 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
       !(t->cg_split == 0 || t->cg_split == 1 || t->cg_split == -1) ||
       !(t->cell_operator == 0 || t->cell_operator == 1 || t->cell_operator == -1) ||
       !(t->cell_blocks == 0 || (t->cell_blocks >= 8 && t->cell_blocks <= kMaxVecBlocks && t->cell_blocks % 8 == 0)) ||
-      !(t->cell_march == 0 || t->cell_march == -1) || !(t->cell_rows >= 0 && t->cell_rows <= 2))
+      !(t->cell_march == 0 || t->cell_march == -1) || !(t->cell_rows >= 0 && t->cell_rows <= 2) ||
+      !(t->elem_operator >= -1 && t->elem_operator <= 1))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);
   c->tune = *t;

@@ -62,7 +62,7 @@ class Tuning(C.Structure):
                 ("cg_elems", C.c_int32), ("gather_chunks", C.c_int32), ("sell_nt", C.c_int32),
                 ("affine", C.c_int32), ("lattice", C.c_int32), ("lattice_blocks", C.c_int32),
                 ("cg_split", C.c_int32), ("cell_operator", C.c_int32), ("cell_blocks", C.c_int32),
-                ("cell_march", C.c_int32), ("cell_rows", C.c_int32)]
+                ("cell_march", C.c_int32), ("cell_rows", C.c_int32), ("elem_operator", C.c_int32)]
 
 
 class Timings(C.Structure):

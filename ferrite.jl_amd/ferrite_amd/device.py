@@ -226,7 +226,7 @@ class SolveResult:
         self.relres = info.relres
         # the solver's copy of K: "sell" (sliced ELL), "lattice" (value planes per neighbour)
         # or "cells" (per-cell coefficients times cell 0's unit-coefficient element matrix)
-        self.operator = {1: "lattice", 2: "lattice", 3: "cells"}.get(info.operator_format, "sell")
+        self.operator = {1: "lattice", 2: "lattice", 3: "cells", 4: "elements"}.get(info.operator_format, "sell")
         self.unit_diagonal = info.operator_format == 2
         self.u = u
 

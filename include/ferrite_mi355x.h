@@ -144,7 +144,8 @@ typedef struct {
   int32_t operator_format;   /* the CG operator copy used: 0 sliced ELL, 1 lattice planes,
                                 2 lattice planes whose (scaled) diagonal is all ones, so the
                                 SpMV does not read the centre plane, 3 cell coefficients
-                                (tuning cell_operator) */
+                                (tuning cell_operator), 4 the element matrices of the last
+                                assembly (tuning elem_operator) */
   int32_t reserved;
 } fem_solve_info;
 
@@ -215,6 +216,12 @@ typedef struct {
                                         -1 = independent tiles of rows (three planes per tile) */
   int32_t cell_rows;                 /* 3D cell form, FP32: rows per thread of the marching SpMV,
                                         1 or 2 (0 = default 2; FP64 always 1) */
+  int32_t elem_operator;             /* solver operator from the element matrices the last owner-
+                                        computes assembly stored (scalar Q1, Q1 quad^2, Q2 hex^3;
+                                        any mesh), applied as per-cell products, when K is still
+                                        that assembly followed by apply!: 0 = default: mixed
+                                        solver only, where no lattice form applies; 1: FP64 solver
+                                        too; -1: never */
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
 int32_t fem_get_tuning(const fem_ctx* ctx, fem_tuning* t);
