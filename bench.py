@@ -9,7 +9,11 @@ in HBM.  Two forms are timed back to back on the same FP64-assembled K:
             the operator the structured mesh of congruent cells allows (per-cell k_e times one
             element matrix, used after an entry-by-entry check against K)
 Timed too and reported beside: the mixed solver on the lattice value planes (`mixed_lattice`),
-the FP64 solver on the planes (`fp64_lattice`) and on the cell form (`fp64_cells`).
+the FP64 solver on the planes (`fp64_lattice`) and on the cell form (`fp64_cells`), both solvers
+on the element matrices of the assembly (`mixed_elements`, `fp64_elements`) and the mixed solver on
+the sliced copy (`mixed_sell`), so `mixed_speedup_same_operator` compares the two solvers on each
+operator form; and (`distorted`) the same workload on a mesh with moved interior nodes, where only
+the sliced copy and the element matrices apply.
 `value` is the mixed form's cells/s (the headline); the FP64 form's cells/s, their
 ratio and ‖u_mixed − u_fp64‖/‖u_fp64‖ are reported beside it.  The FP32-element-math
 assembly variant is timed separately with its K and u errors (an FP32 Ke moves u by
@@ -148,7 +152,7 @@ def main():
         dtc = (time.perf_counter() - t1) / args.steps
         fa.set_tuning(K, time_kernels=1)
         _, _, ts = step(mode)
-        fa.set_tuning(K, lattice=0, cell_operator=0, time_kernels=0)
+        fa.set_tuning(K, lattice=0, cell_operator=0, elem_operator=0, time_kernels=0)
         log(f"{name}: {dtc * 1e3:.1f} ms/step, {r.iterations} iterations, operator {r.operator}, SpMV "
             f"{ts['spmv_ms'] / max(ts['spmv_launches'], 1) * 1e3:.1f} us")
         return {"cells_per_s": g.ncells / dtc, "ms_per_step": dtc * 1e3, "operator": r.operator,
@@ -158,6 +162,12 @@ def main():
     mixed_lattice = context("mixed on the lattice planes", "mixed", cell_operator=-1)
     fp64_lattice = context("fp64 on the lattice planes", "cg", lattice=1, cell_operator=-1)
     fp64_cells = context("fp64 on the cell form", "cg", cell_operator=1)
+    # like for like on the general operator forms, which any mesh gets: the sliced ELL copy of K
+    # and the element matrices of the owner-computes assembly
+    mixed_sell = context("mixed on the sliced copy", "mixed", lattice=-1, cell_operator=-1, elem_operator=-1)
+    mixed_elements = context("mixed on the element matrices", "mixed", lattice=-1, cell_operator=-1, elem_operator=1)
+    fp64_elements = context("fp64 on the element matrices", "cg", lattice=-1, cell_operator=-1, elem_operator=1)
+    fa.set_tuning(K, lattice=0, cell_operator=0, elem_operator=0)
 
     # FP32 element math with FP64 accumulation: K error and the u it leads to (reported beside)
     fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision="fp64")
@@ -207,6 +217,8 @@ def main():
     asm_ms = results["fp64"]["asm_kernel_ms"]
     asm_gbs = ASSEMBLY_BYTES_PER_CELL * g.ncells / (asm_ms * 1e-3) / 1e9
 
+    distorted = distorted_run(args, fa, np, torch, g, ip, cv, dh, colors, k)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, g, dh, k, colors, K, ch)
@@ -241,6 +253,16 @@ def main():
         "fp64_cells": fp64_cells,
         "mixed_speedup_vs_fp64_lattice": fp64_lattice["ms_per_step"] / results["mixed"]["ms"],
         "mixed_speedup_vs_fp64_cells": fp64_cells["ms_per_step"] / results["mixed"]["ms"],
+        "mixed_sell": mixed_sell,
+        "mixed_elements": mixed_elements,
+        "fp64_elements": fp64_elements,
+        # mixed vs FP64 on the same operator form (FP64 on the sliced copy = the "fp64" line)
+        "mixed_speedup_same_operator": {
+            "cells": fp64_cells["ms_per_step"] / results["mixed"]["ms"],
+            "lattice": fp64_lattice["ms_per_step"] / mixed_lattice["ms_per_step"],
+            "sell": results["fp64"]["ms"] / mixed_sell["ms_per_step"],
+            "elements": fp64_elements["ms_per_step"] / mixed_elements["ms_per_step"]},
+        "distorted": distorted,
         "u_rel_err_mixed_vs_fp64": uerr,
         "fp32_element_assembly": {"kernel_ms": asm32_ms, "K_rel_err_vs_fp64": kerr32,
                                   "u_rel_err_vs_fp64": uerr32},
@@ -264,6 +286,72 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def distorted_run(args, fa, np, torch, g0, ip, cv, dh0, colors, k):
+    """The same workload on a distorted mesh: interior nodes moved by up to 0.3 h (seeded), so
+    cells are general trilinear hexahedra and the cell form does not apply; the lattice planes
+    are switched off as well, so the solvers run on the operators any (unstructured) mesh gets:
+    the sliced ELL copy of K and the element matrices.  Assembly takes the trilinear geometry
+    form.  Returns per-solver cells/s, iterations and in-solve SpMV durations."""
+    n = args.n
+    g = fa.generate_grid("Hexahedron", (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    X = g.nodes
+    inner = np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1)
+    X[inner] += np.random.default_rng(args.seed + 1).uniform(-0.3 / n, 0.3 / n, (int(inner.sum()), 3))
+    dh = fa.DofHandler(g).add("u", ip).close()
+    K = fa.allocate_matrix(fa.MI355XCSC, dh, device=0)
+    f = fa.allocate_vector(K)
+    el = fa.PoissonElement(k=k, source=1.0)
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.Dirichlet("u", g.facetsets["left"] + g.facetsets["right"], lambda x, t: 0.0)).close()
+    out = {"workload": f"as above on a distorted {n}^3 mesh (interior nodes moved up to 0.3 h), "
+                       "lattice forms off", "seed": args.seed + 1}
+
+    def step(mode):
+        fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision="fp64")
+        t_asm = fa.timings(K)["assemble_kernel_ms"]
+        fa.apply(K, f, ch)
+        return fa.solve(K, f, mode=mode, rtol=args.rtol, download=False), t_asm
+
+    us = {}
+    for name, mode, elem in (("fp64_sell", "cg", -1), ("mixed_sell", "mixed", -1),
+                             ("fp64_elements", "cg", 1), ("mixed_elements", "mixed", 1)):
+        fa.set_tuning(K, lattice=-1, cell_operator=-1, elem_operator=elem)
+        for _ in range(args.warmup):
+            step(mode)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            r, t_asm = step(mode)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t1) / args.steps
+        us[name] = fa.device.DeviceVector(K, 2).to_numpy()
+        fa.set_tuning(K, lattice=-1, cell_operator=-1, elem_operator=elem, time_kernels=1)
+        _ = step(mode)
+        ts = fa.timings(K)
+        spmv = ts["spmv_ms"] / max(ts["spmv_launches"], 1)
+        out[name] = {"cells_per_s": g.ncells / dt, "ms_per_step": dt * 1e3, "operator": r.operator,
+                     "cg_iterations": r.iterations, "relres": r.relres, "spmv_avg_ms": spmv,
+                     "assembly_kernel_ms": t_asm}
+        log(f"distorted {name}: {dt * 1e3:.1f} ms/step, {r.iterations} iterations, operator {r.operator}, "
+            f"SpMV {spmv * 1e3:.1f} us, assembly {t_asm:.3f} ms")
+    ref = us["fp64_sell"]
+    for name in ("mixed_sell", "fp64_elements", "mixed_elements"):
+        out[name]["u_rel_err_vs_fp64_sell"] = float(np.linalg.norm(us[name] - ref) / np.linalg.norm(ref))
+    out["mixed_speedup_same_operator"] = {
+        "sell": out["fp64_sell"]["ms_per_step"] / out["mixed_sell"]["ms_per_step"],
+        "elements": out["fp64_elements"]["ms_per_step"] / out["mixed_elements"]["ms_per_step"]}
+    # FP32 sliced-ELL SpMV roofline on this mesh (the same CSR byte count as on the box)
+    nnz, nd = K.nnz, dh.ndofs
+    b32 = nnz * (4 + 4) + 4 * (nd + 1) + 4 * nd + 4 * nd
+    ms32 = out["mixed_sell"]["spmv_avg_ms"]
+    out["roofline_sell32"] = {"bound": "hbm", "kernel": "k_sell<float, float, 0, true, 8, false, true>",
+                              "achieved": b32 / (ms32 * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": b32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": b32,
+                              "avg_launch_ms": ms32}
+    del K, f
+    return out
 
 
 def cpu_baseline(args, g, dh, k, colors, K, ch):

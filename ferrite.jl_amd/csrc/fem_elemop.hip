@@ -24,7 +24,8 @@
 //   kind 3  Q2 hex³ elasticity: per cell and local node b the 81 x 3 column block as
 //           [(i, j)][a] with 28 rows (row 27 padding), 252 values.
 // Every row sums its cells in ascending cell id and each cell's terms in a fixed order, so
-// the result is bitwise reproducible.
+// the result is bitwise reproducible.  Both run in two passes: the cell products yc = Ke t,
+// then the row sums of the cell values.
 #include <algorithm>
 #include <vector>
 
@@ -40,6 +41,8 @@ struct ElemOp {
   float *sm32 = nullptr, *pd32 = nullptr;
   double *sm64 = nullptr, *pd64 = nullptr;
   float* keb32 = nullptr;          // FP32 copy of FP64 element matrices (mixed solver)
+  float* yc32 = nullptr;           // cell products yc[a][cell] of the two-pass form (kinds 1, 2)
+  double* yc64 = nullptr;
   uint64_t conv_ver = 0;           // ea_version the copy was made from (+1)
   int grid = kEaGrid;
 };
@@ -87,46 +90,100 @@ __global__ void k_ea_to32(int64_t m, const P* __restrict__ a, float* __restrict_
     b[i] = (float)a[i];
 }
 
-// Scalar / small vector elements: one lane per row, walking the row's (cell, local index)
-// list d2c in ascending cell id.  ROWSUM: strict-upper planes, diagonal from the row sum
-// (kind 1); otherwise upper planes with the diagonal (kind 2).
+// Scalar / small vector elements, two passes.
+//   k_ea_cells: one lane per cell: t_b = sm[dof_b] x[dof_b] of its NB dofs, yc = Ke t (NB x NB
+//               product), stored as planes yc[a][cell].  ROWSUM: strict-upper planes and the
+//               diagonal from the row sum (kind 1); otherwise upper planes with the diagonal
+//               (kind 2).
+//   k_ea_sum:   one lane per row: the row's cell values yc[a][cell] in ascending cell id (its
+//               (cell, local index) list d2c), scaled; partial x·y per block.
 template <typename X, typename P, int NB, bool ROWSUM>
-__global__ __launch_bounds__(kEaBlock) void k_ea_rows(int64_t n, int64_t ncells, const int32_t* __restrict__ d2c_ptr,
-                                                      const int32_t* __restrict__ d2c, const int32_t* __restrict__ cdofs,
-                                                      const P* __restrict__ planes, const X* __restrict__ sm,
-                                                      const X* __restrict__ pd, const X* __restrict__ x,
-                                                      X* __restrict__ y, double* __restrict__ part,
-                                                      const Scal* __restrict__ s) {
+__global__ __launch_bounds__(kEaBlock) void k_ea_cells(int64_t ncells, const int32_t* __restrict__ cdofs,
+                                                       const P* __restrict__ planes, const X* __restrict__ sm,
+                                                       const X* __restrict__ x, X* __restrict__ yc,
+                                                       const Scal* __restrict__ s) {
   if (s->done || s->fresh) return;   // the recurrence stopped, or this step only moves x̂, r̂
+  for (int64_t cell = blockIdx.x * (int64_t)kEaBlock + threadIdx.x; cell < ncells;
+       cell += (int64_t)gridDim.x * kEaBlock) {
+    int32_t dof[NB];
+    if constexpr (NB == 8) {
+      const int4* p = reinterpret_cast<const int4*>(cdofs + cell * 8);
+      const int4 u0 = p[0], u1 = p[1];
+      dof[0] = u0.x; dof[1] = u0.y; dof[2] = u0.z; dof[3] = u0.w;
+      dof[4] = u1.x; dof[5] = u1.y; dof[6] = u1.z; dof[7] = u1.w;
+    } else {
+      const int4 u0 = *reinterpret_cast<const int4*>(cdofs + cell * 4);
+      dof[0] = u0.x; dof[1] = u0.y; dof[2] = u0.z; dof[3] = u0.w;
+    }
+    X t[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) t[b] = sm[dof[b]] * x[dof[b]];
+    X y[NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) y[a] = X(0);
+    if constexpr (ROWSUM) {
+      // Ke[a,b] (a < b) from the plane; Ke[a,a] = −Σ_{b≠a} Ke[a,b] in ascending b
+      X dg[NB];
+#pragma unroll
+      for (int a = 0; a < NB; ++a) dg[a] = X(0);
+      X v[NB * (NB - 1) / 2];
+#pragma unroll
+      for (int q = 0; q < NB * (NB - 1) / 2; ++q) v[q] = (X)planes[(int64_t)q * ncells + cell];
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if (a != b) {
+            const X kab = v[ea_off_idx(a < b ? a : b, a < b ? b : a, NB)];
+            dg[a] -= kab;
+            y[a] += kab * t[b];
+          }
+#pragma unroll
+      for (int a = 0; a < NB; ++a) y[a] += dg[a] * t[a];
+    } else {
+      X v[NB * (NB + 1) / 2];
+#pragma unroll
+      for (int q = 0; q < NB * (NB + 1) / 2; ++q) v[q] = (X)planes[(int64_t)q * ncells + cell];
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) y[a] += v[tri_idx(a < b ? a : b, a < b ? b : a, NB)] * t[b];
+    }
+#pragma unroll
+    for (int a = 0; a < NB; ++a) yc[(int64_t)a * ncells + cell] = y[a];
+  }
+}
+
+template <typename X, int NB>
+__global__ __launch_bounds__(kEaBlock) void k_ea_sum(int64_t n, int64_t ncells, const int32_t* __restrict__ d2c_ptr,
+                                                     const int32_t* __restrict__ d2c, const X* __restrict__ yc,
+                                                     const X* __restrict__ sm, const X* __restrict__ pd,
+                                                     const X* __restrict__ x, X* __restrict__ y,
+                                                     double* __restrict__ part, const Scal* __restrict__ s) {
+  if (s->done || s->fresh) return;
   __shared__ double sh[kEaBlock / 64];
   double d = 0.0;
   for (int64_t row = blockIdx.x * (int64_t)kEaBlock + threadIdx.x; row < n; row += (int64_t)gridDim.x * kEaBlock) {
     X acc = X(0);
     const int32_t e0 = d2c_ptr[row], e1 = d2c_ptr[row + 1];
-    for (int32_t e = e0; e < e1; ++e) {
+    int32_t e = e0;
+    for (; e + 4 <= e1; e += 4) {   // four cells' values in flight, added in ascending order
+      int32_t pk[4];
+      X v[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) pk[g] = d2c[e + g];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int64_t cell = pk[g] / NB;
+        v[g] = yc[(int64_t)(pk[g] - (int)cell * NB) * ncells + cell];
+      }
+#pragma unroll
+      for (int g = 0; g < 4; ++g) acc += v[g];
+    }
+    for (; e < e1; ++e) {
       const int32_t pk = d2c[e];
       const int64_t cell = pk / NB;
-      const int a = pk - (int)cell * NB;
-      int32_t dof[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) dof[b] = cdofs[cell * NB + b];
-      X v[NB];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const int lo = a < b ? a : b, hi = a < b ? b : a;
-        if constexpr (ROWSUM) v[b] = a == b ? X(0) : (X)planes[(int64_t)ea_off_idx(lo, hi, NB) * ncells + cell];
-        else v[b] = (X)planes[(int64_t)tri_idx(lo, hi, NB) * ncells + cell];
-      }
-      if constexpr (ROWSUM) {
-        X dg = X(0);
-#pragma unroll
-        for (int b = 0; b < NB; ++b) dg -= v[b];   // v[a] is 0 here
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-          if (b == a) v[b] = dg;
-      }
-#pragma unroll
-      for (int b = 0; b < NB; ++b) acc += v[b] * (sm[dof[b]] * x[dof[b]]);
+      acc += yc[(int64_t)(pk - (int)cell * NB) * ncells + cell];
     }
     const X xr = x[row];
     const X yv = sm[row] * acc + pd[row] * xr;
@@ -137,64 +194,79 @@ __global__ __launch_bounds__(kEaBlock) void k_ea_rows(int64_t n, int64_t ncells,
   if (threadIdx.x == 0) part[blockIdx.x] = bsum;
 }
 
-// Q2 hex³: one wave per dof node m (rows 3m + j), walking the node's (cell, local node b)
-// list n2c.  Lane slot t = lane + 64 k of the node's 252-value block holds Ke[(a, i), (b, j)]
-// with (i, j) = t / 28, a = t % 28, i.e. (Ke is symmetric) the coupling of row (b, j) with
-// column (a, i): it multiplies t_{3 node(a) + i} into the partial sum of component j.
+// Q2 hex³, two passes.
+//   k_ea_q2_cells: three cells per 256-thread block, one thread per row r = (a, i) of a cell's
+//                  81 x 81 Ke: yc[cell][r] = Σ_b Σ_j Ke[(a, i), (b, j)] t_(b, j), reading the
+//                  stored column blocks (Ke is symmetric: row (a, i) = column (a, i)); t in LDS.
+//                  The 81 threads of a cell read each block as runs of 27 consecutive values.
+//   k_ea_q2_sum:   one lane per row 3m + j: Σ yc[cell][3b + j] over the node's (cell, b) list.
+constexpr int kQ2Cells = 3;   // cells per block of k_ea_q2_cells (243 of 256 threads busy)
+
 template <typename X, typename P>
-__global__ __launch_bounds__(kEaBlock) void k_ea_q2(int64_t nnode, const int32_t* __restrict__ n2c_ptr,
-                                                    const int32_t* __restrict__ n2c, const int32_t* __restrict__ cnode,
-                                                    const P* __restrict__ keb, const X* __restrict__ sm,
-                                                    const X* __restrict__ pd, const X* __restrict__ x,
-                                                    X* __restrict__ y, double* __restrict__ part,
-                                                    const Scal* __restrict__ s) {
+__global__ __launch_bounds__(kEaBlock) void k_ea_q2_cells(int64_t ncells, const int32_t* __restrict__ cnode,
+                                                          const P* __restrict__ keb, const X* __restrict__ sm,
+                                                          const X* __restrict__ x, X* __restrict__ yc,
+                                                          const Scal* __restrict__ s) {
   if (s->done || s->fresh) return;
-  constexpr int kKS = 252, kKC = 27 * kKS, S4 = 4;
-  __shared__ double sh[kEaBlock / 64];
-  const int lane = threadIdx.x & 63;
-  const int64_t waves = (int64_t)gridDim.x * (kEaBlock / 64);
-  int sa[S4], si[S4], sj[S4];
-#pragma unroll
-  for (int k = 0; k < S4; ++k) {
-    const int t = lane + 64 * k;
-    const int ij = t / 28, a = t - 28 * ij;
-    sa[k] = (t < kKS && a < 27) ? a : -1;
-    si[k] = ij / 3;
-    sj[k] = ij - 3 * (ij / 3);
+  constexpr int kKS = 252, kKC = 27 * kKS;
+  __shared__ X st[kQ2Cells][81];
+  const int lc = threadIdx.x / 81, r = threadIdx.x - 81 * lc;
+  const int64_t cell = (int64_t)blockIdx.x * kQ2Cells + lc;
+  const bool act = lc < kQ2Cells && cell < ncells;
+  if (act) {   // t of this cell: the thread of row r loads entry r (node r / 3, component r % 3)
+    const int64_t dof = 3 * (int64_t)cnode[cell * 27 + r / 3] + (r - 3 * (r / 3));
+    st[lc][r] = sm[dof] * x[dof];
   }
+  __syncthreads();
+  if (!act) return;
+  const int a = r / 3, i = r - 3 * a;
+  const P* blk = keb + cell * (int64_t)kKC + (i * 3) * 28 + a;
+  X acc = X(0);
+#pragma unroll 3
+  for (int b = 0; b < 27; ++b) {
+    const P* bb = blk + b * kKS;
+    const X v0 = (X)bb[0], v1 = (X)bb[28], v2 = (X)bb[56];   // (i, j = 0, 1, 2)
+    acc += v0 * st[lc][3 * b] + v1 * st[lc][3 * b + 1] + v2 * st[lc][3 * b + 2];
+  }
+  yc[cell * 81 + r] = acc;
+}
+
+template <typename X>
+__global__ __launch_bounds__(kEaBlock) void k_ea_q2_sum(int64_t n, const int32_t* __restrict__ n2c_ptr,
+                                                        const int32_t* __restrict__ n2c, const X* __restrict__ yc,
+                                                        const X* __restrict__ sm, const X* __restrict__ pd,
+                                                        const X* __restrict__ x, X* __restrict__ y,
+                                                        double* __restrict__ part, const Scal* __restrict__ s) {
+  if (s->done || s->fresh) return;
+  __shared__ double sh[kEaBlock / 64];
   double d = 0.0;
-  for (int64_t m = blockIdx.x * (int64_t)(kEaBlock / 64) + (threadIdx.x >> 6); m < nnode; m += waves) {
-    X acc0 = X(0), acc1 = X(0), acc2 = X(0);
+  for (int64_t row = blockIdx.x * (int64_t)kEaBlock + threadIdx.x; row < n; row += (int64_t)gridDim.x * kEaBlock) {
+    const int64_t m = row / 3;
+    const int j = (int)(row - 3 * m);
+    X acc = X(0);
     const int32_t e0 = n2c_ptr[m], e1 = n2c_ptr[m + 1];
-    for (int32_t e = e0; e < e1; ++e) {
-      const int32_t ent = n2c[e];
-      const int64_t cell = ent / 27;
-      const int b = ent - (int)cell * 27;
-      const P* blk = keb + cell * (int64_t)kKC + b * kKS;
+    int32_t e = e0;
+    for (; e + 4 <= e1; e += 4) {   // four cells' values in flight, added in ascending order
+      int32_t ent[4];
+      X v[4];
 #pragma unroll
-      for (int k = 0; k < S4; ++k) {
-        if (sa[k] < 0) continue;
-        const int64_t col = 3 * (int64_t)cnode[cell * 27 + sa[k]] + si[k];
-        const X c = (X)blk[lane + 64 * k] * (sm[col] * x[col]);
-        acc0 += sj[k] == 0 ? c : X(0);
-        acc1 += sj[k] == 1 ? c : X(0);
-        acc2 += sj[k] == 2 ? c : X(0);
-      }
-    }
-    acc0 = ea_wave_sum(acc0);
-    acc1 = ea_wave_sum(acc1);
-    acc2 = ea_wave_sum(acc2);
-    if (lane == 0) {
-      const X av[3] = {acc0, acc1, acc2};
+      for (int g = 0; g < 4; ++g) ent[g] = n2c[e + g];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const int64_t row = 3 * m + j;
-        const X xr = x[row];
-        const X yv = sm[row] * av[j] + pd[row] * xr;
-        y[row] = yv;
-        d += (double)xr * (double)yv;
+      for (int g = 0; g < 4; ++g) {
+        const int64_t cell = ent[g] / 27;
+        v[g] = yc[cell * 81 + 3 * (ent[g] - (int)cell * 27) + j];
       }
+#pragma unroll
+      for (int g = 0; g < 4; ++g) acc += v[g];
     }
+    for (; e < e1; ++e) {
+      const int64_t cell = n2c[e] / 27;
+      acc += yc[cell * 81 + 3 * (n2c[e] - (int)cell * 27) + j];
+    }
+    const X xr = x[row];
+    const X yv = sm[row] * acc + pd[row] * xr;
+    y[row] = yv;
+    d += (double)xr * (double)yv;
   }
   const double bsum = ea_block_sum(d, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = bsum;
@@ -239,6 +311,7 @@ void elem_op_free(ElemOp** op) {
   if (!*op) return;
   ElemOp* o = *op;
   dev_free(&o->sm32); dev_free(&o->pd32); dev_free(&o->sm64); dev_free(&o->pd64); dev_free(&o->keb32);
+  dev_free(&o->yc32); dev_free(&o->yc64);
   delete o;
   *op = nullptr;
 }
@@ -258,11 +331,14 @@ int elem_op_prepare(fem_ctx* c, ElemOp** opp, const double* sc, bool fp32, int* 
   int rc;
   if (o->n != n) {
     dev_free(&o->sm32); dev_free(&o->pd32); dev_free(&o->sm64); dev_free(&o->pd64);
+    dev_free(&o->yc32); dev_free(&o->yc64);
     o->n = n;
   }
   const uint8_t* isc = c->ea_constrained ? c->d_isc : nullptr;
+  const int64_t ycn = c->ncells * (int64_t)c->ndpc;   // one value per cell and local dof
   if (fp32) {
     if (!o->sm32 && ((rc = dev_alloc(c, &o->sm32, n)) || (rc = dev_alloc(c, &o->pd32, n)))) return rc;
+    if (!o->yc32 && (rc = dev_alloc(c, &o->yc32, ycn))) return rc;
     k_ea_prep<float><<<gride(n), 256, 0, c->stream>>>(n, sc, isc, c->d_diag, c->d_nzval, o->sm32, o->pd32);
     if (c->ea_fp64 && o->conv_ver != c->ea_version + 1) {
       const int64_t m = ea_plane_count(c);
@@ -273,6 +349,7 @@ int elem_op_prepare(fem_ctx* c, ElemOp** opp, const double* sc, bool fp32, int* 
     }
   } else {
     if (!o->sm64 && ((rc = dev_alloc(c, &o->sm64, n)) || (rc = dev_alloc(c, &o->pd64, n)))) return rc;
+    if (!o->yc64 && (rc = dev_alloc(c, &o->yc64, ycn))) return rc;
     k_ea_prep<double><<<gride(n), 256, 0, c->stream>>>(n, sc, isc, c->d_diag, c->d_nzval, o->sm64, o->pd64);
   }
   FEM_HIP(c, hipGetLastError());
@@ -282,32 +359,35 @@ int elem_op_prepare(fem_ctx* c, ElemOp** opp, const double* sc, bool fp32, int* 
 
 template <typename X, typename P>
 static void launch_kind(fem_ctx* c, const ElemOp* o, const P* planes, const X* sm, const X* pd, const X* x, X* y,
-                        double* part, const Scal* s) {
+                        X* yc, double* part, const Scal* s) {
   const int g = o->grid;
   const int nb = c->ndpc;
   if (c->ea_kind == 3) {
-    k_ea_q2<X, P><<<g, kEaBlock, 0, c->stream>>>(c->n / 3, c->d_n2c_ptr, c->d_n2c, c->d_q2cnode, planes, sm, pd, x, y,
-                                                 part, s);
-  } else if (c->ea_kind == 1) {
-    if (nb == 8)
-      k_ea_rows<X, P, 8, true><<<g, kEaBlock, 0, c->stream>>>(c->n, c->ncells, c->d_d2c_ptr, c->d_d2c, c->d_gcdofs,
-                                                              planes, sm, pd, x, y, part, s);
-    else
-      k_ea_rows<X, P, 4, true><<<g, kEaBlock, 0, c->stream>>>(c->n, c->ncells, c->d_d2c_ptr, c->d_d2c, c->d_gcdofs,
-                                                              planes, sm, pd, x, y, part, s);
-  } else {
-    k_ea_rows<X, P, 8, false><<<g, kEaBlock, 0, c->stream>>>(c->n, c->ncells, c->d_d2c_ptr, c->d_d2c, c->d_gcdofs,
-                                                             planes, sm, pd, x, y, part, s);
+    const int64_t gq = (c->ncells + kQ2Cells - 1) / kQ2Cells;
+    k_ea_q2_cells<X, P><<<(unsigned)gq, kEaBlock, 0, c->stream>>>(c->ncells, c->d_q2cnode, planes, sm, x, yc, s);
+    k_ea_q2_sum<X><<<g, kEaBlock, 0, c->stream>>>(c->n, c->d_n2c_ptr, c->d_n2c, yc, sm, pd, x, y, part, s);
+    return;
   }
+  const int gc = gride(c->ncells, kEaBlock);
+  if (c->ea_kind == 1 && nb == 8)
+    k_ea_cells<X, P, 8, true><<<gc, kEaBlock, 0, c->stream>>>(c->ncells, c->d_gcdofs, planes, sm, x, yc, s);
+  else if (c->ea_kind == 1)
+    k_ea_cells<X, P, 4, true><<<gc, kEaBlock, 0, c->stream>>>(c->ncells, c->d_gcdofs, planes, sm, x, yc, s);
+  else
+    k_ea_cells<X, P, 8, false><<<gc, kEaBlock, 0, c->stream>>>(c->ncells, c->d_gcdofs, planes, sm, x, yc, s);
+  if (nb == 8)
+    k_ea_sum<X, 8><<<g, kEaBlock, 0, c->stream>>>(c->n, c->ncells, c->d_d2c_ptr, c->d_d2c, yc, sm, pd, x, y, part, s);
+  else
+    k_ea_sum<X, 4><<<g, kEaBlock, 0, c->stream>>>(c->n, c->ncells, c->d_d2c_ptr, c->d_d2c, yc, sm, pd, x, y, part, s);
 }
 
 // y = Â x on the element matrices (partial x·y per block); returns the partial count.
 int elem_op_launch(fem_ctx* c, const ElemOp* o, const float* x, float* y, double* part, const Scal* s) {
   if (c->ea_fp64) {
-    launch_kind<float, float>(c, o, o->keb32, o->sm32, o->pd32, x, y, part, s);
+    launch_kind<float, float>(c, o, o->keb32, o->sm32, o->pd32, x, y, o->yc32, part, s);
   } else {
     const float* pl = c->ea_kind == 1 ? c->d_keb32 : c->ea_kind == 2 ? c->d_ep32 : c->d_keq32;
-    launch_kind<float, float>(c, o, pl, o->sm32, o->pd32, x, y, part, s);
+    launch_kind<float, float>(c, o, pl, o->sm32, o->pd32, x, y, o->yc32, part, s);
   }
   return o->grid;
 }
@@ -315,10 +395,10 @@ int elem_op_launch(fem_ctx* c, const ElemOp* o, const float* x, float* y, double
 int elem_op_launch(fem_ctx* c, const ElemOp* o, const double* x, double* y, double* part, const Scal* s) {
   if (c->ea_fp64) {
     const double* pl = c->ea_kind == 1 ? c->d_keb64 : c->ea_kind == 2 ? c->d_ep64 : c->d_keq64;
-    launch_kind<double, double>(c, o, pl, o->sm64, o->pd64, x, y, part, s);
+    launch_kind<double, double>(c, o, pl, o->sm64, o->pd64, x, y, o->yc64, part, s);
   } else {
     const float* pl = c->ea_kind == 1 ? c->d_keb32 : c->ea_kind == 2 ? c->d_ep32 : c->d_keq32;
-    launch_kind<double, float>(c, o, pl, o->sm64, o->pd64, x, y, part, s);
+    launch_kind<double, float>(c, o, pl, o->sm64, o->pd64, x, y, o->yc64, part, s);
   }
   return o->grid;
 }

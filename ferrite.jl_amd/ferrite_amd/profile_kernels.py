@@ -11,8 +11,9 @@ runs, in order:
   4. with --solve: one FP64 and one mixed solve on their default operators, then the mixed
      solver on the lattice planes and the FP64 solver on the cell form.
 --q2: builds the Q2 hex³ elasticity n³ system, runs the copy calibration, then `reps`
-assemblies through the owner-computes path in FP32 (MFMA) and FP64 element precision, and
-one colored-scatter assembly of each.
+assemblies through the owner-computes path in FP32 (MFMA) and FP64 element precision, `reps`
+solver SpMVs on the element matrices and on the sliced copy, and one colored-scatter assembly
+of each precision.
 Each phase prints a line, so a counter run can be matched to its kernels by name.
 """
 import argparse
@@ -85,7 +86,14 @@ def q2(args):
         t = fa.timings(K)
         print(f"phase q2 gather {prec}: {args.reps} x {g.ncells} cells, last {t['assemble_kernel_ms']:.3f} ms "
               f"(element {t['element_ms']:.3f} ms, gather {t['scatter_ms']:.3f} ms)", flush=True)
-    fa.set_tuning(K, asm_mode=1)
+    # solver SpMVs on the FP64-assembled K: the element matrices against the sliced copy
+    fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision="fp64")
+    for elem, name in ((1, "element matrices"), (-1, "sliced copy")):
+        fa.set_tuning(K, elem_operator=elem)
+        ms32 = fa.spmv_bench(K, mixed=True, reps=args.reps)
+        ms64 = fa.spmv_bench(K, mixed=False, reps=args.reps)
+        print(f"phase q2 spmv on the {name}: fp32 {ms32:.4f} ms, fp64 {ms64:.4f} ms per launch", flush=True)
+    fa.set_tuning(K, elem_operator=0, asm_mode=1)
     for prec in ("mixed", "fp64"):
         fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=prec)
         t = fa.timings(K)

@@ -219,9 +219,10 @@ typedef struct {
   int32_t elem_operator;             /* solver operator from the element matrices the last owner-
                                         computes assembly stored (scalar Q1, Q1 quad^2, Q2 hex^3;
                                         any mesh), applied as per-cell products, when K is still
-                                        that assembly followed by apply!: 0 = default: mixed
-                                        solver only, where no lattice form applies; 1: FP64 solver
-                                        too; -1: never */
+                                        that assembly followed by apply! and no lattice form
+                                        applies: 0 = default: both solvers, for the Q2 node blocks
+                                        (fewer bytes per SpMV than the sliced copy); 1: for every
+                                        kind; -1: never */
 } fem_tuning;
 int32_t fem_set_tuning(fem_ctx* ctx, const fem_tuning* t);
 int32_t fem_get_tuning(const fem_ctx* ctx, fem_tuning* t);

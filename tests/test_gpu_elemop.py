@@ -74,7 +74,7 @@ def test_distorted_q1_hex_poisson():
     # the operator is dropped once K is no longer that assembly (+ apply!) ...
     nz = K.nzval()
     K.set_nzval(nz)
-    fa.set_tuning(K, lattice=-1)
+    fa.set_tuning(K, lattice=-1, elem_operator=1)
     assert fa.solve(K, f, mode="mixed", rtol=1e-12).operator == "sell"
     # ... or after an accumulating assembly
     fa.assemble_global(fa.start_assemble(K, f, fillzero=False), cv, colors, el)
@@ -88,10 +88,10 @@ def test_distorted_q1_hex_poisson():
 
 @pytest.mark.parametrize("stiffness", ["simp", "mild"])
 def test_cantilever_q1_quad_vector(stiffness):
-    """The plane-strain cantilever.  With the SIMP stiffness of the differentiation workload
-    (E = 1e-3 + θ³(1 − 1e-3), θ ~ U[0.2, 1]) cond(K) is beyond what an FP32 copy of K can
-    carry (the mixed solver stops with FEM_ERR_NOCONV on any FP32 operator), so the FP64
-    solver runs on the element operator there; with E ~ U[0.5, 1] both solvers do."""
+    """The plane-strain cantilever, with the SIMP stiffness of the differentiation workload
+    (E = 1e-3 + θ³(1 − 1e-3), θ ~ U[0.2, 1]) and with E ~ U[0.5, 1].  cond(K) is large enough
+    here that the mixed solver's FP32 recurrence claims its drop for several hand-overs in a
+    row without moving the true residual much before it resumes (the stall rule allows six)."""
     print("case elemop cantilever q1^2", stiffness, flush=True)
     nel = (24, 12)
     rng = np.random.default_rng(3)
@@ -124,8 +124,8 @@ def test_cantilever_q1_quad_vector(stiffness):
     och.add(OC.Dirichlet("u", grid.facetsets["left"], lambda x, t: [0.0, 0.0])).close()
     OC.apply(OK, of, och)
     u_ref = OC.solve(OK, of)
-    out, again = _solves(K, f, u_ref, modes=("cg",) if stiffness == "simp" else ("mixed", "cg"))
-    assert again.operator == ("sell" if stiffness == "simp" else "elements")   # FP64 default: sliced
+    out, again = _solves(K, f, u_ref)
+    assert again.operator == "sell"         # default: the planes only when asked for
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -152,7 +152,8 @@ def test_q2_hex_elasticity(precision):
     import scipy.sparse.linalg as spla
     u_ref = spla.spsolve(K.to_scipy().tocsc(), f.to_numpy())
     out, again = _solves(K, f, u_ref)
-    assert again.operator == "elements"
+    assert again.operator == "elements"     # default for the Q2 node blocks, both solvers
+    assert fa.solve(K, f, mode="cg", rtol=1e-12).operator == "elements"
     if precision == "fp64":
         grid, ocv, odh, OK, of = setup_vector("Hexahedron", nel, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 2, 3)
         grid.nodes[:] = g.nodes
