@@ -88,8 +88,10 @@ def main():
     g = fa.generate_grid("Hexahedron", nel, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
     ip = fa.Lagrange("RefHexahedron", 1)
     cv = fa.CellValues(fa.QuadratureRule("RefHexahedron", 2), ip)
+    t_host = time.perf_counter()
     dh = fa.DofHandler(g).add("u", ip).close()
     K = fa.allocate_matrix(fa.MI355XCSC, dh, device=0)
+    t_host = time.perf_counter() - t_host
     f = fa.allocate_vector(K)
     colors = fa.create_coloring(g, args.coloring)
     rng = np.random.default_rng(args.seed)
@@ -98,6 +100,20 @@ def main():
     ch = fa.ConstraintHandler(dh)
     ch.add(fa.Dirichlet("u", g.facetsets["left"] + g.facetsets["right"], lambda x, t: 0.0)).close()
     log(f"setup {time.time() - t0:.1f}s: {g.ncells} cells, {dh.ndofs} dofs, nnz {K.nnz}, {len(colors)} colors")
+    # close!(dh) + allocate_matrix(dh) on the device (fem_dofs_distribute, fem_pattern_build),
+    # copied out for the host mirror, against the host restatement just run
+    torch.cuda.synchronize()
+    t_dev = time.perf_counter()
+    dh_d = fa.DofHandler(g).add("u", ip).close(device=0)
+    K_d = fa.allocate_matrix(fa.MI355XCSC, dh_d)
+    t_dev = time.perf_counter() - t_dev
+    same = bool(np.array_equal(dh_d.cell_dofs, dh.cell_dofs) and np.array_equal(K_d.colptr, K.colptr)
+                and np.array_equal(K_d.rowval, K.rowval))
+    setup = {"host_s": t_host, "device_s": t_dev, "device_equals_host": same,
+             "what": "close!(dh) + allocate_matrix(dh) pattern + device upload; the device path also "
+                     "copies cell_dofs, colptr and rowval back for the host mirror"}
+    log(f"setup: host {t_host:.3f} s, device {t_dev:.3f} s, identical {same}")
+    del K_d, dh_d
 
     def step(mode, asm_precision="fp64"):
         A = fa.start_assemble(K, f)
@@ -263,6 +279,7 @@ def main():
             "sell": results["fp64"]["ms"] / mixed_sell["ms_per_step"],
             "elements": fp64_elements["ms_per_step"] / mixed_elements["ms_per_step"]},
         "distorted": distorted,
+        "setup": setup,
         "u_rel_err_mixed_vs_fp64": uerr,
         "fp32_element_assembly": {"kernel_ms": asm32_ms, "K_rel_err_vs_fp64": kerr32,
                                   "u_rel_err_vs_fp64": uerr32},

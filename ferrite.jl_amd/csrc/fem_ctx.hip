@@ -111,6 +111,31 @@ int build_tables(fem_ctx* c) {
   return FEM_OK;
 }
 
+// The context's pattern bookkeeping once colptr / rowval / nzval / diag are on the device
+// (fem_pattern_upload, fem_pattern_build): diagonal positions (Ferrite's patterns always store
+// the diagonal; apply! writes K[d,d]), sizes, and the tables built on the old pattern dropped.
+int pattern_finish(fem_ctx* c, int64_t n, int64_t nnz, const int32_t* hcolptr) {
+  k_diag_pos<<<grid_for(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, c->d_diag);
+  FEM_HIP(c, hipGetLastError());
+  std::vector<int32_t> dg(n);
+  FEM_HIP(c, hipMemcpyAsync(dg.data(), c->d_diag, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  FEM_HIP(c, hipStreamSynchronize(c->stream));
+  for (int64_t j = 0; j < n; ++j)
+    if (dg[j] < 0) return set_error(c, FEM_ERR_ARG, "pattern lacks a diagonal entry");
+  c->n = n; c->nnz = nnz;
+  c->pattern_sym = -1;
+  c->max_row_nnz = 0;
+  for (int64_t j = 0; j < n; ++j) c->max_row_nnz = std::max<int>(c->max_row_nnz, hcolptr[j + 1] - hcolptr[j]);
+  free_solver(c);
+  free_q2_tables(c);
+  c->ea_kind = 0;
+  c->tables_ready = false;
+  c->gather_ready = false;
+  c->gchunks_built = 0;
+  c->affine_mesh = -1;
+  return FEM_OK;
+}
+
 }  // namespace fem
 
 using namespace fem;
@@ -269,28 +294,7 @@ int32_t fem_pattern_upload(fem_ctx* c, int64_t n, const int64_t* colptr, const i
   FEM_HIP(c, hipMemcpyAsync(c->d_colptr, cp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, c->stream));
   FEM_HIP(c, hipMemcpyAsync(c->d_rowval, rv.data(), sizeof(int32_t) * nnz, hipMemcpyHostToDevice, c->stream));
   FEM_HIP(c, hipMemsetAsync(c->d_nzval, 0, sizeof(double) * nnz, c->stream));
-  k_diag_pos<<<grid_for(n), 256, 0, c->stream>>>(n, c->d_colptr, c->d_rowval, c->d_diag);
-  FEM_HIP(c, hipGetLastError());
-  FEM_HIP(c, hipStreamSynchronize(c->stream));
-  // Ferrite's patterns always store the diagonal; apply! writes K[d,d]
-  for (int64_t j = 0; j < n; ++j) {
-    int32_t lo = cp[j], hi = cp[j + 1];
-    bool found = false;
-    for (int32_t p = lo; p < hi; ++p) found |= (rv[p] == (int32_t)j);
-    if (!found) return set_error(c, FEM_ERR_ARG, "pattern lacks a diagonal entry");
-  }
-  c->n = n; c->nnz = nnz;
-  c->pattern_sym = -1;
-  c->max_row_nnz = 0;
-  for (int64_t j = 0; j < n; ++j) c->max_row_nnz = std::max<int>(c->max_row_nnz, cp[j + 1] - cp[j]);
-  free_solver(c);
-  free_q2_tables(c);
-  c->ea_kind = 0;
-  c->tables_ready = false;
-  c->gather_ready = false;
-  c->gchunks_built = 0;
-  c->affine_mesh = -1;
-  return FEM_OK;
+  return pattern_finish(c, n, nnz, cp.data());
 }
 
 int32_t fem_colors_upload(fem_ctx* c, int32_t ncolors, const int64_t* color_ptr, const int64_t* color_cells) {

@@ -235,6 +235,7 @@ void dev_free(T** p) {
 float event_ms(fem_ctx* c);
 
 int build_tables(fem_ctx* c);
+int pattern_finish(fem_ctx* c, int64_t n, int64_t nnz, const int32_t* hcolptr);
 int noalloc_scratch_init(int device);   // fem_dirichlet.hip: address of its device scratch
 void free_solver(fem_ctx* c);
 int build_gather_tables(fem_ctx* c);

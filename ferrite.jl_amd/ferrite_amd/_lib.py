@@ -95,6 +95,10 @@ SIGNATURES = {
     "fem_dofs_upload": (I32, [P, I64, I32, PI64]),
     "fem_pattern_upload": (I32, [P, I64, PI64, PI64]),
     "fem_colors_upload": (I32, [P, I32, PI64, PI64]),
+    "fem_dofs_distribute": (I32, [P, I32, I32, PI64, PI32]),
+    "fem_dofs_download": (I32, [P, PI64]),
+    "fem_pattern_build": (I32, [P, PI64]),
+    "fem_pattern_download": (I32, [P, PI64, PI64]),
     "fem_cellvalues_upload": (I32, [P, I32, I32, I32, I32, I32, PD, PD, PD, PD]),
     "fem_cellparam_upload": (I32, [P, I64, PD]),
     "fem_assemble": (I32, [P, C.POINTER(AssembleCfg)]),

@@ -110,6 +110,17 @@ def allocate_matrix(mtype, dh, device=0, ctx=None):
     """allocate_matrix(MI355XCSC, dh): upload mesh, dofs and the pattern; nzval zeroed in HBM."""
     if mtype is not MI355XCSC:
         raise TypeError("only MI355XCSC is provided by this package")
+    dctx = getattr(dh, "_ctx", None)
+    if dctx is not None and ctx is None:
+        # dofs distributed on the device (DofHandler.close(device=...)): mesh and dofs are in
+        # that context already; the pattern is built there too, and copied out for the host
+        dh._ctx = None
+        nnz = C.c_int64(0)
+        dctx.call("fem_pattern_build", C.byref(nnz))
+        colptr = np.zeros(dh.ndofs + 1, dtype=np.int64)
+        rowval = np.zeros(nnz.value, dtype=np.int64)
+        dctx.call("fem_pattern_download", L.ptr(colptr, C.c_int64), L.ptr(rowval, C.c_int64))
+        return DeviceMatrix(dh, dctx, colptr, rowval)
     colptr, rowval = create_sparsity_pattern(dh)
     ctx = ctx or Context(device)
     g = dh.grid

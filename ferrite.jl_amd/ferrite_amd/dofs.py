@@ -30,7 +30,12 @@ class DofHandler:
         self.field_ips.append(ip)
         return self
 
-    def close(self):
+    def close(self, device=None):
+        """close!(dh).  With `device` (a GPU index) the numbering is computed on that GPU
+        (fem_dofs_distribute, one field of quadrilaterals or hexahedra); the context it ran
+        in is kept for allocate_matrix, which then builds the pattern there as well."""
+        if device is not None:
+            return self._close_device(device)
         g = self.grid
         nf = len(self.field_ips)
         order = np.array([base_interpolation(ip).order for ip in self.field_ips], dtype=np.int32)
@@ -57,6 +62,30 @@ class DofHandler:
             o += ip.nbasefunctions
         self.field_offsets = offs
         self.closed = True
+        return self
+
+    def _close_device(self, device):
+        from .device import Context
+        if len(self.field_ips) != 1:
+            raise ValueError("close(device=...) distributes one field")
+        g = self.grid
+        ip = self.field_ips[0]
+        ctx = Context(device)
+        xyz = L.as_f64(g.nodes)
+        cells = L.as_i64(g.cells)
+        ctx.call("fem_mesh_upload", g.sdim, g.nnodes, L.ptr(xyz, L.C.c_double), g.ncells, cells.shape[1],
+                 L.ptr(cells, L.C.c_int64))
+        nd, ndpc = L.C.c_int64(0), L.C.c_int32(0)
+        ctx.call("fem_dofs_distribute", base_interpolation(ip).order, n_components(ip), L.C.byref(nd),
+                 L.C.byref(ndpc))
+        cd = np.zeros(g.ncells * ndpc.value, dtype=np.int64)
+        ctx.call("fem_dofs_download", L.ptr(cd, L.C.c_int64))
+        self.ndofs_per_cell = ndpc.value
+        self.cell_dofs = cd.reshape(g.ncells, ndpc.value)
+        self.ndofs = nd.value
+        self.field_offsets = [0]
+        self.closed = True
+        self._ctx = ctx
         return self
 
     def celldofs(self, cell):

@@ -63,8 +63,18 @@ def elasticity(args):
     g = fa.generate_grid("Hexahedron", (n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
     ip = fa.Lagrange("RefHexahedron", 2) ** 3
     cv = fa.CellValues(fa.QuadratureRule("RefHexahedron", 3), ip)
+    t_host = time.perf_counter()
     dh = fa.DofHandler(g).add("u", ip).close()
     K = fa.allocate_matrix(fa.MI355XCSC, dh)
+    t_host = time.perf_counter() - t_host
+    t_dev = time.perf_counter()
+    dh_d = fa.DofHandler(g).add("u", ip).close(device=0)
+    K_d = fa.allocate_matrix(fa.MI355XCSC, dh_d)
+    t_dev = time.perf_counter() - t_dev
+    same = bool(np.array_equal(dh_d.cell_dofs, dh.cell_dofs) and np.array_equal(K_d.colptr, K.colptr)
+                and np.array_equal(K_d.rowval, K.rowval))
+    log(f"setup: host {t_host:.2f} s, device {t_dev:.2f} s, identical {same}")
+    del K_d, dh_d
     f = fa.allocate_vector(K)
     colors = fa.create_coloring(g)
     E = np.random.default_rng(args.seed).uniform(0.5, 2.0, g.ncells)
@@ -77,6 +87,7 @@ def elasticity(args):
     nc, nnode = g.ncells, dh.ndofs // 3
     out = {"workload": f"Q2 hex^3 elasticity {n}^3, 3x3x3 Gauss, E ~ U[0.5,2], nu 0.3",
            "cells": nc, "dofs": dh.ndofs, "nnz": K.nnz, "colors": len(colors),
+           "setup": {"host_s": t_host, "device_s": t_dev, "device_equals_host": same},
            "units": {"dense_btdb_flop_per_cell": DENSE_BTDB_FLOP_PER_CELL,
                      "mfma_issued_flop_per_cell": MFMA_ISSUED_FLOP_PER_CELL}}
     nzs = {}

@@ -69,6 +69,19 @@ int32_t fem_dofs_upload(fem_ctx* ctx, int64_t ndofs, int32_t ndofs_per_cell, con
 /* allocate_matrix(dh) pattern (sparsity_pattern.jl:628-645): SparseMatrixCSC colptr/rowval.
  * Allocates K.nzval (FP64) on the device. */
 int32_t fem_pattern_upload(fem_ctx* ctx, int64_t n, const int64_t* colptr, const int64_t* rowval);
+/* close!(dh) on the device for one field Lagrange{order}^vdim on the uploaded quadrilateral or
+ * hexahedral mesh (DofHandler.jl:576-800: first-visit numbering, cells ascending, vertices ->
+ * edges -> faces -> interior per cell; the same numbering as fem_setup_distribute_dofs).  Sets
+ * the context's dofs as fem_dofs_upload would; *ndofs and *ndofs_per_cell out. */
+int32_t fem_dofs_distribute(fem_ctx* ctx, int32_t order, int32_t vdim, int64_t* ndofs, int32_t* ndofs_per_cell);
+/* dh.cell_dofs of the context (1-based Int64, ncells x ndofs_per_cell). */
+int32_t fem_dofs_download(fem_ctx* ctx, int64_t* cell_dofs);
+/* allocate_matrix(dh)'s pattern built on the device from the context's dofs
+ * (sparsity_pattern.jl:223-253, :1136-1249, :951-1016); installs it as fem_pattern_upload
+ * does (nzval allocated, zero).  *nnz out. */
+int32_t fem_pattern_build(fem_ctx* ctx, int64_t* nnz);
+/* The pattern as SparseMatrixCSC's colptr[n+1] / rowval[nnz] (1-based Int64). */
+int32_t fem_pattern_download(fem_ctx* ctx, int64_t* colptr, int64_t* rowval);
 /* create_coloring(grid) (coloring.jl:308-317) flattened: color c holds
  * color_cells[color_ptr[c] .. color_ptr[c+1]-1] (0-based offsets, 1-based cell ids). */
 int32_t fem_colors_upload(fem_ctx* ctx, int32_t ncolors, const int64_t* color_ptr, const int64_t* color_cells);
