@@ -191,6 +191,8 @@ void fem_ctx_destroy(fem_ctx* c) {
     (void)hipStreamDestroy(c->stream2);
   }
   dev_free(&c->d_pres); dev_free(&c->d_g); dev_free(&c->d_isc);
+  dev_free(&c->d_affidx); dev_free(&c->d_afd); dev_free(&c->d_acp); dev_free(&c->d_acd); dev_free(&c->d_acv);
+  dev_free(&c->d_acb); dev_free(&c->d_slp); dev_free(&c->d_sld); dev_free(&c->d_slv);
   dev_free(&c->d_scalar); dev_free(&c->d_partials); dev_free(&c->d_err);
   dev_free(&c->d_tab); dev_free(&c->d_tabf); dev_free(&c->d_qc); dev_free(&c->d_qcf);
   dev_free(&c->d_aff); dev_free(&c->d_afff);

@@ -415,6 +415,10 @@ const void* elem_op_key(const fem_ctx* c, const ElemOp* o, bool fp32) {
 // and columns are masked in the operator); called before apply! bumps nzval_version.
 void elem_op_note_apply(fem_ctx* c) {
   if (c->ea_kind == 0) return;
+  if (c->naff > 0) {   // K := Cᵀ K C couples the masters beyond the element matrices
+    c->ea_kind = 0;
+    return;
+  }
   if (c->ea_version != c->nzval_version || (c->ea_constrained && c->ea_aux != c->aux_version)) {
     c->ea_kind = 0;
     return;

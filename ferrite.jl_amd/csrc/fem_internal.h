@@ -171,6 +171,17 @@ struct fem_ctx {
   double* d_g = nullptr;
   uint8_t* d_isc = nullptr;
   bool any_inhom = false;
+  // affine constraints u_c = Σ a u_m + b among the prescribed dofs (apply! condenses them)
+  int64_t naff = 0;
+  int32_t* d_affidx = nullptr;         // [n] index of each dof's affine constraint, −1 if none
+  int32_t* d_afd = nullptr;            // [naff] the affinely constrained dofs, ascending
+  int32_t* d_acp = nullptr;            // [naff + 1] coefficient ranges
+  int32_t* d_acd = nullptr;            // master dofs
+  double* d_acv = nullptr;             // coefficients
+  double* d_acb = nullptr;             // [naff] b of each constraint
+  int32_t* d_slp = nullptr;            // [n + 1] per master dof: its slaves (ascending), with
+  int32_t* d_sld = nullptr;            //   the slave dof
+  double* d_slv = nullptr;             //   and the slave's coefficient for this master
   double* d_scalar = nullptr;          // small device scratch for reductions (meandiag etc.)
   double* d_partials = nullptr;
   int npartials = 0;

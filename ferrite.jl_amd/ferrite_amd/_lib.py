@@ -107,6 +107,8 @@ SIGNATURES = {
     "fem_constraints_upload": (I32, [P, I64, PI64, PD]),
     "fem_apply_dirichlet": (I32, [P, I32, PD]),
     "fem_apply_dirichlet_noalloc": (I32, [P, I32, PD]),
+    "fem_affine_upload": (I32, [P, I64, PI64, PI64, PI64, PD, PD]),
+    "fem_apply_solution": (I32, [P, I32]),
     "fem_solve": (I32, [P, C.POINTER(SolveCfg), C.POINTER(SolveInfo)]),
     "fem_matrix_download": (I32, [P, PD, PD]),
     "fem_matrix_upload": (I32, [P, PD, PD]),

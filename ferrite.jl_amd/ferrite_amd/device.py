@@ -19,7 +19,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .dofs import create_sparsity_pattern
+from .dofs import add_constraint_entries, create_sparsity_pattern
 
 
 class MI355XCSC:
@@ -106,10 +106,15 @@ class DeviceVector:
         return out
 
 
-def allocate_matrix(mtype, dh, device=0, ctx=None):
-    """allocate_matrix(MI355XCSC, dh): upload mesh, dofs and the pattern; nzval zeroed in HBM."""
+def allocate_matrix(mtype, dh, ch=None, device=0, ctx=None):
+    """allocate_matrix(MI355XCSC, dh[, ch]): upload mesh, dofs and the pattern; nzval zeroed in
+    HBM.  With a closed ConstraintHandler holding affine constraints the pattern also gets the
+    entries their condensation fills (allocate_matrix(dh, ch), sparsity_pattern.jl:782-843)."""
     if mtype is not MI355XCSC:
         raise TypeError("only MI355XCSC is provided by this package")
+    if ch is not None and not ch.closed:
+        raise RuntimeError("close! the ConstraintHandler first")
+    affine = ch is not None and ch.has_affine
     dctx = getattr(dh, "_ctx", None)
     if dctx is not None and ctx is None:
         # dofs distributed on the device (DofHandler.close(device=...)): mesh and dofs are in
@@ -120,8 +125,13 @@ def allocate_matrix(mtype, dh, device=0, ctx=None):
         colptr = np.zeros(dh.ndofs + 1, dtype=np.int64)
         rowval = np.zeros(nnz.value, dtype=np.int64)
         dctx.call("fem_pattern_download", L.ptr(colptr, C.c_int64), L.ptr(rowval, C.c_int64))
+        if affine:
+            colptr, rowval = add_constraint_entries(colptr, rowval, ch)
+            dctx.call("fem_pattern_upload", dh.ndofs, L.ptr(colptr, C.c_int64), L.ptr(rowval, C.c_int64))
         return DeviceMatrix(dh, dctx, colptr, rowval)
     colptr, rowval = create_sparsity_pattern(dh)
+    if affine:
+        colptr, rowval = add_constraint_entries(colptr, rowval, ch)
     ctx = ctx or Context(device)
     g = dh.grid
     xyz = L.as_f64(g.nodes)
@@ -216,11 +226,21 @@ def assemble_global(assembler, cv, colors, element, precision="fp64"):
     return K, assembler.f
 
 
-def apply(K, f, ch, applyzero=False):
-    """apply!(K, f, ch) on the device; returns meandiag(K)."""
+def _upload_constraints(K, ch):
     p = L.as_i64(ch.prescribed_dofs)
     g = L.as_f64(ch.inhomogeneities)
     K.ctx.call("fem_constraints_upload", len(p), L.ptr(p, C.c_int64), L.ptr(g, C.c_double))
+    if getattr(ch, "has_affine", False):
+        dofs, cptr, cdofs, cvals, b = ch.affine_arrays()
+        K.ctx.call("fem_affine_upload", len(dofs), L.ptr(dofs, C.c_int64), L.ptr(cptr, C.c_int64),
+                   L.ptr(cdofs, C.c_int64), L.ptr(cvals, C.c_double), L.ptr(b, C.c_double))
+
+
+def apply(K, f, ch, applyzero=False):
+    """apply!(K, f, ch) on the device; returns meandiag(K).  Affine constraints of ch are
+    condensed into K and f (K := CᵀKC, f := Cᵀf in the serial column order of _condense!,
+    ConstraintHandler.jl:809-867) before the prescribed rows and columns are cleared."""
+    _upload_constraints(K, ch)
     m = C.c_double(0.0)
     K.ctx.call("fem_apply_dirichlet", 1 if applyzero else 0, C.byref(m))
     return m.value
@@ -228,6 +248,14 @@ def apply(K, f, ch, applyzero=False):
 
 def apply_zero(K, f, ch):
     return apply(K, f, ch, applyzero=True)
+
+
+def apply_solution(K, ch, applyzero=False, download=True):
+    """apply!(u, ch) / apply_zero!(u, ch) (ConstraintHandler.jl:686-700) on the device solution
+    u of the last solve: u[prescribed] = g, then u_i = b_i + Σ a_ij u_j for affine dofs."""
+    _upload_constraints(K, ch)
+    K.ctx.call("fem_apply_solution", 1 if applyzero else 0)
+    return DeviceVector(K, 2).to_numpy() if download else None
 
 
 class SolveResult:

@@ -125,6 +125,31 @@ def create_sparsity_pattern(dh):
     return colptr, rowval
 
 
+def add_constraint_entries(colptr, rowval, ch):
+    """_add_constraint_entries!(sp, ch; keep_constrained = true) (sparsity_pattern.jl:782-843):
+    every entry (r, c) of the pattern adds (r', c') for r' in E(r), c' in E(c), where E maps an
+    affinely constrained dof to its masters and any other dof to itself; with the boolean
+    expansion B (B[d, e] = 1 iff e in E(d)) the result is P ∪ BᵀPB."""
+    import scipy.sparse as sp
+    if not getattr(ch, "has_affine", False):
+        return colptr, rowval
+    n = len(colptr) - 1
+    dofs, cptr, cdofs, _, _ = ch.affine_arrays()
+    keep = np.ones(n, dtype=bool)
+    keep[dofs - 1] = False
+    ki = np.nonzero(keep)[0]
+    ri = np.concatenate([ki, np.repeat(dofs - 1, np.diff(cptr))])
+    ci = np.concatenate([ki, cdofs - 1])
+    B = sp.csr_matrix((np.ones(ri.size, dtype=np.int8), (ri, ci)), shape=(n, n))
+    B.data[:] = 1
+    P = sp.csc_matrix((np.ones(rowval.size, dtype=np.int8), rowval - 1,
+                       colptr - 1), shape=(n, n))
+    Q = ((B.T @ P @ B) + P).tocsc()
+    Q.sum_duplicates()
+    Q.sort_indices()
+    return Q.indptr.astype(np.int64) + 1, Q.indices.astype(np.int64) + 1
+
+
 def create_coloring(grid, alg="workstream"):
     """create_coloring(grid; alg = WorkStream | Greedy): list of ascending 1-based cell-id arrays."""
     a = {"workstream": L.FEM_COLORING_WORKSTREAM, "greedy": L.FEM_COLORING_GREEDY}[alg.lower()]

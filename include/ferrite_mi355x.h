@@ -134,6 +134,19 @@ int32_t fem_apply_dirichlet(fem_ctx* ctx, int32_t applyzero, double* m_out);
  * array: calls that overlap on one GPU (two contexts on two host threads) race. */
 int32_t fem_apply_dirichlet_noalloc(fem_ctx* ctx, int32_t applyzero, double* m_out);
 
+/* Affine constraints u_c = Σ_k a_ck u_mk + b_c (AffineConstraint, ConstraintHandler.jl:140-150,
+ * :368-402; PeriodicDirichlet's mirror = image pairs, :1134-1294), after fem_constraints_upload
+ * whose prescribed list holds their dofs (sorted, 1-based) with the effective inhomogeneities
+ * of update! (:521-540).  coef_ptr[naff+1] are 0-based offsets into coef_dofs (1-based masters,
+ * none of them affinely constrained) / coef_vals; b holds each constraint's own b.  apply!
+ * then condenses K := Cᵀ K C, f := Cᵀ f (:809-867) before zeroing the prescribed rows and
+ * columns; the pattern needs allocate_matrix(dh, ch)'s constraint entries
+ * (sparsity_pattern.jl:782-843).  naff = 0 drops them. */
+int32_t fem_affine_upload(fem_ctx* ctx, int64_t naff, const int64_t* dofs, const int64_t* coef_ptr,
+                          const int64_t* coef_dofs, const double* coef_vals, const double* b);
+/* apply!(u, ch) / apply_zero!(u, ch) on the resident u (ConstraintHandler.jl:686-700). */
+int32_t fem_apply_solution(fem_ctx* ctx, int32_t applyzero);
+
 /* ---- solve (replaces u = K \ f, heat_equation.jl:218; the reference has no CG) ---- */
 enum fem_solve_mode {
   FEM_SOLVE_CG_FP64 = 0,  /* CG, FP64 SpMV and vectors */

@@ -1,0 +1,164 @@
+"""Host side of affine and periodic constraints (no GPU): the ConstraintHandler's bookkeeping
+and allocate_matrix(dh, ch)'s pattern.
+
+  * the pattern with constraint entries equals the oracle's restatement of
+    _add_constraint_entries! (sparsity_pattern.jl:782-843) entry for entry;
+  * PeriodicDirichlet on the unit square / cube (periodic in one or two directions): the
+    constrained dofs are exactly the dofs of the mirror facets that are not also images,
+    every mirror maps straight to an unconstrained image at the same position modulo the
+    period, and the constraint classes are the torus' identification classes (brute force
+    over all dof coordinates);
+  * update! gives an affine dof the effective inhomogeneity b + Σ a g over Dirichlet masters;
+    a later Dirichlet or affine constraint on a dof replaces the earlier one; nested affine
+    constraints are rejected at close!.
+"""
+import numpy as np
+import pytest
+
+import ferrite_amd as fa
+from oracle.affine import AffineConstraint as OAC
+from oracle.affine import AffineHandler, add_constraint_entries
+
+
+def oracle_handler(ch):
+    """The oracle's AffineHandler holding the product handler's constraints."""
+    aff = set(int(d) for d in ch.affine_dofs)
+    dirichlet = {int(d): float(v) for d, v in zip(ch.prescribed_dofs, ch.inhomogeneities) if int(d) not in aff}
+    oh = AffineHandler(ch.dh.ndofs, dirichlet)
+    dofs, ptr, cd, cv, b = ch.affine_arrays()
+    for i, d in enumerate(dofs):
+        oh.add(OAC(d, list(zip(cd[ptr[i]:ptr[i + 1]], cv[ptr[i]:ptr[i + 1]])), b[i]))
+    return oh.close()
+
+
+def dof_coords(dh):
+    """Coordinates of every dof (Lagrange nodes mapped by the Q1 geometry); brute force."""
+    g = dh.grid
+    ip = dh.field_ips[0]
+    base = fa.refel.base_interpolation(ip)
+    vdim = fa.refel.n_components(ip)
+    geo = fa.Lagrange(base.shape, 1)
+    M, _ = geo.values_and_gradients(np.asarray(base.reference_coordinates()))
+    X = np.zeros((dh.ndofs, g.sdim))
+    comp = np.zeros(dh.ndofs, dtype=np.int64)
+    for c in range(1, g.ncells + 1):
+        x = M @ g.getcoordinates(c)
+        for a in range(x.shape[0]):
+            for d in range(vdim):
+                dof = dh.cell_dofs[c - 1][a * vdim + d]
+                X[dof - 1] = x[a]
+                comp[dof - 1] = d
+    return X, comp
+
+
+def periodic_handler(ct, nel, order, axes, vdim=1):
+    g = fa.generate_grid(ct, nel, (0.0,) * len(nel), (1.0,) * len(nel))
+    ip = fa.Lagrange(g.refshape, order)
+    if vdim > 1:
+        ip = ip ** vdim
+    dh = fa.DofHandler(g).add("u", ip).close()
+    ch = fa.ConstraintHandler(dh)
+    # (image at coordinate 0, mirror at coordinate 1) per axis
+    names = ({0: ("left", "right"), 1: ("bottom", "top")} if len(nel) == 2 else
+             {0: ("left", "right"), 1: ("front", "back"), 2: ("bottom", "top")})
+    # one PeriodicDirichlet over all directions' pairs (as the reference's examples append the
+    # facet maps), so that the edges and corners untangle into one class each
+    pairs = []
+    for ax in axes:
+        pairs += fa.collect_periodic_facets(g, names[ax][1], names[ax][0])
+    ch.add(fa.PeriodicDirichlet("u", pairs))
+    ch.close()
+    return g, dh, ch
+
+
+@pytest.mark.parametrize("ct,nel,order,axes,vdim", [
+    ("Quadrilateral", (5, 4), 1, (0,), 1),
+    ("Quadrilateral", (4, 3), 2, (0, 1), 1),
+    ("Quadrilateral", (3, 3), 1, (0, 1), 2),
+    ("Hexahedron", (3, 2, 2), 1, (0,), 1),
+    ("Hexahedron", (2, 2, 2), 2, (0, 1, 2), 1),
+])
+def test_periodic_bookkeeping_is_the_torus(ct, nel, order, axes, vdim):
+    g, dh, ch = periodic_handler(ct, nel, order, axes, vdim)
+    X, comp = dof_coords(dh)
+    dofs, ptr, cd, cv, b = ch.affine_arrays()
+    assert np.all(np.diff(ptr) == 1) and np.all(cv == 1.0) and np.all(b == 0.0)
+    # torus classes: positions equal modulo 1 along the periodic axes
+    key = X.copy()
+    for ax in axes:
+        key[:, ax] = np.where(np.isclose(key[:, ax], 1.0), 0.0, key[:, ax])
+    classes = {}
+    for d in range(dh.ndofs):
+        classes.setdefault((tuple(np.round(key[d], 9)), comp[d]), []).append(d + 1)
+    image = dict(zip(dofs.tolist(), cd.tolist()))
+    assert not (set(image) & set(image.values()))
+    for members in classes.values():
+        roots = [m for m in members if m not in image]
+        assert len(roots) == 1
+        for m in members:
+            if m != roots[0]:
+                assert image[m] == roots[0]
+    # the mirrors sit on the mirror side (x = 1 along some periodic axis)
+    on_mirror = np.zeros(dh.ndofs, dtype=bool)
+    for ax in axes:
+        on_mirror |= np.isclose(X[:, ax], 1.0)
+    assert set(dofs.tolist()) <= set((np.nonzero(on_mirror)[0] + 1).tolist())
+
+
+@pytest.mark.parametrize("ct,nel,order,axes", [
+    ("Quadrilateral", (5, 4), 1, (0,)),
+    ("Quadrilateral", (4, 3), 2, (0, 1)),
+    ("Hexahedron", (3, 2, 2), 1, (0, 2)),
+])
+def test_pattern_with_constraint_entries(ct, nel, order, axes):
+    g, dh, ch = periodic_handler(ct, nel, order, axes)
+    cp0, rv0 = fa.create_sparsity_pattern(dh)
+    cp, rv = fa.add_constraint_entries(cp0, rv0, ch)
+    ocp, orv = add_constraint_entries(cp0, rv0, oracle_handler(ch))
+    assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
+    assert len(rv) > len(rv0)
+
+
+def test_pattern_general_affine():
+    g = fa.generate_grid("Quadrilateral", (4, 4), (0.0, 0.0), (1.0, 1.0))
+    ip = fa.Lagrange(g.refshape, 1)
+    dh = fa.DofHandler(g).add("u", ip).close()
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.Dirichlet("u", g.facetsets["left"], lambda x, t: 1.0 + x[1]))
+    ch.add(fa.AffineConstraint(7, [(13, 0.5), (20, 0.25), (1, 2.0)], 0.1))
+    ch.add(fa.AffineConstraint(24, [(3, -1.0)], 0.0))
+    ch.close()
+    cp0, rv0 = fa.create_sparsity_pattern(dh)
+    cp, rv = fa.add_constraint_entries(cp0, rv0, ch)
+    ocp, orv = add_constraint_entries(cp0, rv0, oracle_handler(ch))
+    assert np.array_equal(cp, ocp) and np.array_equal(rv, orv)
+
+
+def test_effective_inhomogeneity_and_override():
+    g = fa.generate_grid("Quadrilateral", (3, 3), (0.0, 0.0), (1.0, 1.0))
+    dh = fa.DofHandler(g).add("u", fa.Lagrange(g.refshape, 1)).close()
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.Dirichlet("u", g.facetsets["left"], lambda x, t: 3.0 + x[1]))
+    left = sorted(ch._entries)
+    # an affine constraint on a Dirichlet dof replaces it; its masters include a Dirichlet dof
+    ch.add(fa.AffineConstraint(left[0], [(left[1], 2.0), (16, 0.5)], 0.25))
+    ch.close()
+    X, _ = dof_coords(dh)
+    i = {int(d): k for k, d in enumerate(ch.prescribed_dofs)}
+    assert ch.inhomogeneities[i[left[0]]] == 0.25 + 2.0 * (3.0 + X[left[1] - 1, 1])
+    assert ch.inhomogeneities[i[left[1]]] == 3.0 + X[left[1] - 1, 1]
+    assert list(ch.affine_dofs) == [left[0]]
+    # a later Dirichlet on an affine dof replaces the affine constraint
+    ch2 = fa.ConstraintHandler(dh)
+    ch2.add(fa.AffineConstraint(left[0], [(16, 1.0)], 0.0))
+    ch2.add(fa.Dirichlet("u", g.facetsets["left"], lambda x, t: 0.0)).close()
+    assert not ch2.has_affine
+
+
+def test_nested_affine_rejected():
+    g = fa.generate_grid("Quadrilateral", (2, 2), (0.0, 0.0), (1.0, 1.0))
+    dh = fa.DofHandler(g).add("u", fa.Lagrange(g.refshape, 1)).close()
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.AffineConstraint(1, [(2, 1.0)], 0.0)).add(fa.AffineConstraint(2, [(3, 1.0)], 0.0))
+    with pytest.raises(ValueError, match="nested"):
+        ch.close()
