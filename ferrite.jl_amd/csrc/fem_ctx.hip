@@ -131,7 +131,6 @@ int pattern_finish(fem_ctx* c, int64_t n, int64_t nnz, const int32_t* hcolptr) {
   c->ea_kind = 0;
   c->tables_ready = false;
   c->gather_ready = false;
-  c->gchunks_built = 0;
   c->affine_mesh = -1;
   return FEM_OK;
 }
@@ -182,14 +181,8 @@ void fem_ctx_destroy(fem_ctx* c) {
   dev_free(&c->d_cparam); dev_free(&c->d_param_cell); dev_free(&c->d_f); dev_free(&c->d_u);
   dev_free(&c->d_gcnodes); dev_free(&c->d_d2c_ptr); dev_free(&c->d_d2c); dev_free(&c->d_gslot);
   dev_free(&c->d_keb64); dev_free(&c->d_feb64); dev_free(&c->d_keb32); dev_free(&c->d_feb32);
-  dev_free(&c->d_gblist);
   free_q2_tables(c);
   dev_free(&c->d_gcdofs); dev_free(&c->d_ep64); dev_free(&c->d_ep32);
-  for (hipEvent_t e : c->gchunk_ev) (void)hipEventDestroy(e);
-  if (c->stream2) {
-    (void)hipStreamSynchronize(c->stream2);
-    (void)hipStreamDestroy(c->stream2);
-  }
   dev_free(&c->d_pres); dev_free(&c->d_g); dev_free(&c->d_isc);
   dev_free(&c->d_affidx); dev_free(&c->d_afd); dev_free(&c->d_acp); dev_free(&c->d_acd); dev_free(&c->d_acv);
   dev_free(&c->d_acb); dev_free(&c->d_slp); dev_free(&c->d_sld); dev_free(&c->d_slv);
@@ -241,7 +234,6 @@ int32_t fem_mesh_upload(fem_ctx* c, int32_t sdim, int64_t nnodes, const double* 
   c->ea_kind = 0;
   c->tables_ready = false;
   c->gather_ready = false;
-  c->gchunks_built = 0;
   c->affine_mesh = -1;
   return FEM_OK;
 }
@@ -270,7 +262,6 @@ int32_t fem_dofs_upload(fem_ctx* c, int64_t ndofs, int32_t ndpc, const int64_t* 
   c->ea_kind = 0;
   c->tables_ready = false;
   c->gather_ready = false;
-  c->gchunks_built = 0;
   c->affine_mesh = -1;
   return FEM_OK;
 }

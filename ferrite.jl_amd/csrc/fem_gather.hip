@@ -123,14 +123,13 @@ __global__ __launch_bounds__(256) void k_elem_poisson(int64_t ncells, int64_t cb
 // lane walks its column's cells G at a time: the list entries, slot bytes and element
 // values of G cells are loaded before any of them is added (loads in flight), then added
 // in ascending cell order.  ecol[q] names the column of the block's q-th nzval entry, so
-// the coalesced read-in / write-out of the block's nzval range needs no search.  With a
-// block list, the launch covers the listed column blocks only (−1 entries are padding).
+// the coalesced read-in / write-out of the block's nzval range needs no search.
 template <typename T, int NB, int MAXL, int G>
 __global__ __launch_bounds__(kGatherBlock) void k_gather_cols(
     int64_t n, int64_t ncells, const int32_t* __restrict__ colptr, const int32_t* __restrict__ d2c_ptr,
     const int32_t* __restrict__ d2c, const uint8_t* __restrict__ slot, const T* __restrict__ keb,
     const T* __restrict__ feb, double* __restrict__ nzval, double* __restrict__ f, int fillzero,
-    const int32_t* __restrict__ blist, DevError* __restrict__ err) {
+    DevError* __restrict__ err) {
   static_assert(kGatherBlock <= 256, "column index of a block entry is one byte");
   __shared__ double acc[MAXL * kGatherBlock];
   __shared__ int32_t scp[kGatherBlock + 1];
@@ -140,9 +139,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_cols(
   // one contiguous range of column blocks, so neighbouring columns, which read the same
   // cells' element planes, share an L2.  gridDim.x is a multiple of 8.
   const int64_t per = gridDim.x / 8;
-  const int64_t lb = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-  const int64_t cb = blist ? (int64_t)blist[lb] : lb;
-  if (cb < 0) return;
+  const int64_t cb = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
   const int64_t j0 = cb * kGatherBlock;
   if (j0 >= n) return;
   const int ncol = (int)min<int64_t>(kGatherBlock, n - j0);
@@ -305,51 +302,6 @@ int build_gather_tables(fem_ctx* c) {
   return FEM_OK;
 }
 
-// Column-block lists of the pipelined gather for `nchunk` cell chunks (host; once per
-// tables/chunk count).  A block is listed under the chunk of the largest cell any of its
-// columns holds, so its gather may run as soon as that chunk's element planes exist.
-static int build_gather_chunks(fem_ctx* c, int nchunk) {
-  if (c->gchunks_built == nchunk && c->d_gblist) return FEM_OK;
-  const int64_t nc = c->ncells, n = c->n;
-  const int nb = c->ndpc;
-  std::vector<int64_t> cb(nchunk + 1);
-  for (int k = 0; k <= nchunk; ++k) cb[k] = nc * k / nchunk;
-  // last (largest) cell of every dof: the cell loop visits cells in ascending order
-  std::vector<int32_t> last(n, 0);
-  for (int64_t cell = 0; cell < nc; ++cell)
-    for (int b = 0; b < nb; ++b) last[c->h_cell_dofs[cell * nb + b]] = (int32_t)cell;
-  const int64_t nblk = (n + kGatherBlock - 1) / kGatherBlock;
-  std::vector<std::vector<int32_t>> lists(nchunk);
-  for (int64_t blk = 0; blk < nblk; ++blk) {
-    int32_t mx = 0;
-    for (int64_t j = blk * kGatherBlock; j < std::min<int64_t>(n, (blk + 1) * kGatherBlock); ++j) mx = std::max(mx, last[j]);
-    const int k = (int)(std::upper_bound(cb.begin(), cb.end(), (int64_t)mx) - cb.begin()) - 1;
-    lists[std::min(std::max(k, 0), nchunk - 1)].push_back((int32_t)blk);
-  }
-  std::vector<int32_t> flat;
-  std::vector<int64_t> off(nchunk + 1, 0);
-  for (int k = 0; k < nchunk; ++k) {
-    off[k] = (int64_t)flat.size();
-    flat.insert(flat.end(), lists[k].begin(), lists[k].end());
-    while ((flat.size() - off[k]) % 8) flat.push_back(-1);   // a multiple of 8 per chunk
-  }
-  off[nchunk] = (int64_t)flat.size();
-  int rc = dev_alloc(c, &c->d_gblist, flat.size());
-  if (rc) return rc;
-  FEM_HIP(c, hipMemcpyAsync(c->d_gblist, flat.data(), 4 * flat.size(), hipMemcpyHostToDevice, c->stream));
-  FEM_HIP(c, hipStreamSynchronize(c->stream));
-  if (!c->stream2) FEM_HIP(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
-  while ((int)c->gchunk_ev.size() < nchunk + 1) {
-    hipEvent_t e = nullptr;
-    FEM_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    c->gchunk_ev.push_back(e);
-  }
-  c->gchunk_cell = cb;
-  c->gchunk_blk = off;
-  c->gchunks_built = nchunk;
-  return FEM_OK;
-}
-
 // flag = 0 when some cell's trilinear map is not a parallelepiped (FP64, tol 1e-14).
 __global__ __launch_bounds__(256) void k_check_affine(int64_t ncells, const double* __restrict__ X0,
                                                       const double* __restrict__ X1, const double* __restrict__ X2,
@@ -406,29 +358,22 @@ static int run_gather(fem_ctx* c, const T* tab, double src, bool fillzero) {
     if ((rc = dev_alloc(c, &keb, nc * NOFF)) || (rc = dev_alloc(c, &feb, nc * NB))) return rc;
     cap = nc;
   }
-  // default: no overlap.  Measured at 128³ (FP64): 1 chunk 0.661 ms, 2: 0.703, 8: 0.843 —
-  // the element kernel (222 VGPRs, 2 waves/SIMD) and the gather (31.6 KB LDS per block)
-  // cannot share a CU, so chunking only adds launches
-  const int nchunk = c->tune.gather_chunks ? c->tune.gather_chunks : 1;
-  const bool piped = nchunk > 1 && nc >= (int64_t)nchunk * 4096;
-  if (piped && (rc = build_gather_chunks(c, nchunk))) return rc;
   const int grp = c->tune.gather_group ? c->tune.gather_group : 4;
-#define FEM_GATHER(L, GG, NBLK, ST, BL)                                                                        \
-  k_gather_cols<T, NB, L, GG><<<NBLK, kGatherBlock, 0, ST>>>(c->n, nc, c->d_colptr, c->d_d2c_ptr, c->d_d2c,    \
-                                                             c->d_gslot, keb, feb, c->d_nzval, c->d_f,         \
-                                                             fillzero ? 1 : 0, BL, c->d_err)
-#define FEM_GATHER_G(L, NBLK, ST, BL)                            \
-  do {                                                           \
-    if (grp == 1) FEM_GATHER(L, 1, NBLK, ST, BL);                \
-    else if (grp == 2) FEM_GATHER(L, 2, NBLK, ST, BL);           \
-    else if (grp == 8) FEM_GATHER(L, 8, NBLK, ST, BL);           \
-    else FEM_GATHER(L, 4, NBLK, ST, BL);                         \
+#define FEM_GATHER(L, GG, NBLK)                                                                                \
+  k_gather_cols<T, NB, L, GG><<<NBLK, kGatherBlock, 0, c->stream>>>(c->n, nc, c->d_colptr, c->d_d2c_ptr,       \
+                                                                    c->d_d2c, c->d_gslot, keb, feb, c->d_nzval, \
+                                                                    c->d_f, fillzero ? 1 : 0, c->d_err)
+#define FEM_GATHER_G(L, NBLK)                          \
+  do {                                                 \
+    if (grp == 1) FEM_GATHER(L, 1, NBLK);              \
+    else if (grp == 2) FEM_GATHER(L, 2, NBLK);         \
+    else if (grp == 8) FEM_GATHER(L, 8, NBLK);         \
+    else FEM_GATHER(L, 4, NBLK);                       \
   } while (0)
-  auto gather = [&](int nblk, hipStream_t st, const int32_t* bl) {
-    if (nblk <= 0) return;
-    if (c->max_row_nnz <= 9) FEM_GATHER_G(9, nblk, st, bl);
-    else if (c->max_row_nnz <= 27) FEM_GATHER_G(27, nblk, st, bl);
-    else FEM_GATHER_G(32, nblk, st, bl);
+  auto gather = [&](int nblk) {
+    if (c->max_row_nnz <= 9) FEM_GATHER_G(9, nblk);
+    else if (c->max_row_nnz <= 27) FEM_GATHER_G(27, nblk);
+    else FEM_GATHER_G(32, nblk);
   };
   const T* qc = sizeof(T) == 8 ? (const T*)c->d_qc : (const T*)c->d_qcf;
   const T* aff = sizeof(T) == 8 ? (const T*)c->d_aff : (const T*)c->d_afff;
@@ -462,36 +407,20 @@ static int run_gather(fem_ctx* c, const T* tab, double src, bool fillzero) {
         c->d_err);
   };
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
-  if (!piped) {
-    elem(0, nc, c->stream);
-    FEM_HIP(c, hipEventRecord(c->evm, c->stream));
-    gather((int)(((c->n + kGatherBlock - 1) / kGatherBlock + 7) / 8 * 8), c->stream, nullptr);
-    c->timings.assemble_launches = 2;
-  } else {
-    // element planes of chunk k on the context stream; the gather of the column blocks that
-    // chunk completes on stream2 after it, overlapping the element kernel of chunk k + 1
-    for (int k = 0; k < nchunk; ++k) {
-      elem(c->gchunk_cell[k], c->gchunk_cell[k + 1], c->stream);
-      FEM_HIP(c, hipEventRecord(c->gchunk_ev[k], c->stream));
-      FEM_HIP(c, hipStreamWaitEvent(c->stream2, c->gchunk_ev[k], 0));
-      gather((int)(c->gchunk_blk[k + 1] - c->gchunk_blk[k]), c->stream2, c->d_gblist + c->gchunk_blk[k]);
-    }
-    FEM_HIP(c, hipEventRecord(c->gchunk_ev[nchunk], c->stream2));
-    FEM_HIP(c, hipStreamWaitEvent(c->stream, c->gchunk_ev[nchunk], 0));
-    c->timings.assemble_launches = 2 * nchunk;
-  }
+  elem(0, nc, c->stream);
+  FEM_HIP(c, hipEventRecord(c->evm, c->stream));
+  gather((int)(((c->n + kGatherBlock - 1) / kGatherBlock + 7) / 8 * 8));
+  c->timings.assemble_launches = 2;
 #undef FEM_GATHER_G
 #undef FEM_GATHER
   FEM_HIP(c, hipGetLastError());
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));
   c->timings.assemble_kernel_ms = event_ms(c);
-  if (!piped) {
-    float e1 = 0.f, e2 = 0.f;
-    FEM_HIP(c, hipEventElapsedTime(&e1, c->ev0, c->evm));
-    FEM_HIP(c, hipEventElapsedTime(&e2, c->evm, c->ev1));
-    c->timings.element_ms = e1;
-    c->timings.scatter_ms = e2;
-  }
+  float e1 = 0.f, e2 = 0.f;
+  FEM_HIP(c, hipEventElapsedTime(&e1, c->ev0, c->evm));
+  FEM_HIP(c, hipEventElapsedTime(&e2, c->evm, c->ev1));
+  c->timings.element_ms = e1;
+  c->timings.scatter_ms = e2;
   return FEM_OK;
 }
 

@@ -131,16 +131,6 @@ struct fem_ctx {
   float* d_keq32 = nullptr;
   double* d_keq64 = nullptr;
   int64_t keq32_cells = 0, keq64_cells = 0;
-  // pipelined gather: cells in nchunk contiguous chunks; column blocks listed by the chunk
-  // after which all their cells are computed (each chunk's list padded to a multiple of 8
-  // with −1); a second stream runs the gather of chunk c while the element kernel of c + 1
-  // runs on the context stream
-  int32_t* d_gblist = nullptr;
-  std::vector<int64_t> gchunk_cell;    // [nchunk + 1] first cell of each chunk
-  std::vector<int64_t> gchunk_blk;     // [nchunk + 1] first list entry of each chunk
-  int gchunks_built = 0;
-  hipStream_t stream2 = nullptr;
-  std::vector<hipEvent_t> gchunk_ev;   // one per chunk + one for the join
 
   // cellvalues
   int rdim = 0, nqp = 0, nbf = 0, ngeo = 0, vdim = 1;

@@ -363,7 +363,6 @@ int32_t fem_dofs_distribute(fem_ctx* c, int32_t order, int32_t vdim, int64_t* nd
   c->ea_kind = 0;
   c->tables_ready = false;
   c->gather_ready = false;
-  c->gchunks_built = 0;
   c->affine_mesh = -1;
   *ndofs_out = ndofs;
   *ndpc_out = ndpc;

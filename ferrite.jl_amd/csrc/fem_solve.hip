@@ -1237,150 +1237,22 @@ __global__ void k_fold(int64_t n, X* __restrict__ x, const double* __restrict__ 
 //     γ = r̂·r̂ (partials of the previous step), β = γ/γ_prev,
 //     α = γ / (δ − β γ / α_prev), p = r̂ + β p, s = w + β s, x̂ += α p, r̂ −= α s,
 //     partials of the next γ.
-// By default an iteration is three kernels: SpMV, k_cg_scal (one block reduces the partials
-// in a fixed order and writes the scalars), k_cg_move (the vectors).  In the one-kernel form
-// k_cg_step, every block reduces the partials itself (same fixed order in every block, hence
-// the same scalars bitwise).  The scalar
-// records and the vector partials are double-buffered (read one, write the other).
+// An iteration is three kernels: SpMV, k_cg_scal (one wave reduces the partials in a fixed
+// order and writes the scalars), k_cg_move (the vectors).  The scalar records and the vector
+// partials are double-buffered (read one, write the other).
 // --------------------------------------------------------------------------
-constexpr int kCgBlocks = 4096;  // most blocks of k_cg_step (its partial count)
+constexpr int kCgBlocks = 4096;  // most blocks of k_cg_move (its partial count)
 
-// γ = Σ pin[i] (i < nin) and δ = Σ ps[i] (i < na) in one round: all loads are issued before
-// any add; fixed order, the same in every block and thread.
-__device__ __forceinline__ void reduce2(const double* __restrict__ pin, int nin, const double* __restrict__ ps,
-                                        int na, double (*sh)[kBlock / 64], double& gam, double& del) {
-  constexpr int kPin = 8, kPs = 8;   // up to 2048 partials of each kind in flight
-  double a[kPin], d[kPs];
-#pragma unroll
-  for (int k = 0; k < kPin; ++k) {
-    const int i = k * kBlock + threadIdx.x;
-    a[k] = i < nin ? pin[i] : 0.0;
-  }
-#pragma unroll
-  for (int k = 0; k < kPs; ++k) {
-    const int i = k * kBlock + threadIdx.x;
-    d[k] = i < na ? ps[i] : 0.0;
-  }
-  double va = 0.0, vd = 0.0;
-#pragma unroll
-  for (int k = 0; k < kPin; ++k) va += a[k];
-#pragma unroll
-  for (int k = 0; k < kPs; ++k) vd += d[k];
-  for (int i = kPin * kBlock + threadIdx.x; i < nin; i += kBlock) va += pin[i];
-  for (int i = kPs * kBlock + threadIdx.x; i < na; i += kBlock) vd += ps[i];
-  va = wave_sum(va);
-  vd = wave_sum(vd);
-  if ((threadIdx.x & 63) == 0) {
-    sh[0][threadIdx.x >> 6] = va;
-    sh[1][threadIdx.x >> 6] = vd;
-  }
-  __syncthreads();
-  gam = del = 0.0;
-#pragma unroll
-  for (int w = 0; w < kBlock / 64; ++w) {
-    gam += sh[0][w];
-    del += sh[1][w];
-  }
-}
-
-// Vector elements per thread per pass of k_cg_step (tuning cg_elems; measured default 8).
+// Vector elements per thread per pass of k_cg_move (tuning cg_elems; measured default 8).
 static int cg_elems(const fem_ctx* c) { return c->tune.cg_elems ? c->tune.cg_elems : 8; }
 
-// Blocks of k_cg_step (and of the reseed kernels) for n unknowns: one pass over the vectors
+// Blocks of k_cg_move (and of the reseed kernels) for n unknowns: one pass over the vectors
 // when possible.
 static int cg_blocks(const fem_ctx* c, int64_t n) {
   const int64_t per = (int64_t)kBlock * cg_elems(c);
   return (int)std::max<int64_t>(1, std::min<int64_t>(kCgBlocks, (n + per - 1) / per));
 }
 
-template <typename X, int E>
-__global__ __launch_bounds__(kBlock) void k_cg_step(int64_t n, X* __restrict__ x, X* __restrict__ r,
-                                                    const X* __restrict__ w, X* __restrict__ p, X* __restrict__ sv,
-                                                    int na, const double* __restrict__ part_s,
-                                                    const double* __restrict__ part_in, double* __restrict__ part_out,
-                                                    const Scal* __restrict__ sin, Scal* __restrict__ sout) {
-  __shared__ double sh[2][kBlock / 64];
-  const Scal S = *sin;
-  if (S.done) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *sout = S;
-    return;
-  }
-  const bool fresh = S.fresh != 0;
-  const int64_t stride = (int64_t)gridDim.x * kBlock * E;
-  int64_t base = (int64_t)blockIdx.x * kBlock * E;
-  // this thread's first elements: their loads do not depend on the scalars, so they are in
-  // flight while the partials are reduced
-  X wv[E], pv[E], sw[E], xv[E], rv[E];
-  auto load = [&](int64_t b0) {
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const int64_t i = b0 + k * kBlock + threadIdx.x;
-      const bool in = i < n;
-      wv[k] = in && !fresh ? w[i] : X(0);
-      pv[k] = in ? p[i] : X(0);
-      sw[k] = in ? sv[i] : X(0);
-      xv[k] = in ? x[i] : X(0);
-      rv[k] = in ? r[i] : X(0);
-    }
-  };
-  load(base);
-  double gam = S.rz, alpha = S.alpha, beta = 0.0;
-  if (!fresh) {
-    double del;
-    reduce2(part_in, gridDim.x, part_s, na, sh, gam, del);
-    beta = gam / S.rzp;
-    const double den = del - beta * gam / S.alpha;
-    const bool stop = gam <= S.gstop;
-    const bool brk = !(S.rzp > 0.0) || !(S.alpha != 0.0) || !(den > 0.0);
-    if (stop || brk) {   // hand over to the host: x̂, r̂ of this step, p and s of the last one
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        Scal o = S;
-        o.done = 1;
-        o.rz = gam;
-        if (!stop) o.restart = 1;
-        *sout = o;
-      }
-      return;
-    }
-    alpha = gam / den;
-  }
-  const X al = (X)alpha, be = (X)beta;
-  double ga = 0.0;
-  while (true) {
-#pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const int64_t i = base + k * kBlock + threadIdx.x;
-      if (i < n) {
-        X pi = pv[k], si = sw[k];
-        if (!fresh) {
-          pi = rv[k] + be * pi;
-          si = wv[k] + be * si;
-          p[i] = pi;
-          sv[i] = si;
-        }
-        x[i] = xv[k] + al * pi;
-        const X ri = rv[k] - al * si;
-        r[i] = ri;
-        ga += (double)ri * (double)ri;
-      }
-    }
-    base += stride;
-    if (base >= n) break;
-    load(base);
-  }
-  const double b = block_sum(ga, sh[0]);
-  if (threadIdx.x == 0) part_out[blockIdx.x] = b;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    Scal o = S;
-    o.fresh = 0;
-    o.rzp = gam;
-    o.alpha = alpha;
-    o.beta = beta;
-    o.rz = gam;
-    o.iters = S.iters + 1;
-    *sout = o;
-  }
-}
 
 // Σ a[0..m) and Σ b[0..mb) by one wave in a fixed order: lane l takes i ≡ l (mod 64) into
 // eight accumulators (i / 64 mod 8), which are added pairwise, then a butterfly across lanes.
@@ -1405,10 +1277,11 @@ __device__ __forceinline__ void wave_sum2(const double* __restrict__ a, int m, c
   }
 }
 
-// Split form of the step (the default; tuning cg_split): k_cg_scal reduces the partials once
-// (one wave, fixed order) and writes the step's scalars; k_cg_move then updates the vectors
-// with many small blocks (E elements per thread), which lets later blocks load while earlier
-// ones store.  Same recurrence as k_cg_step.
+// The step after the SpMV: k_cg_scal reduces the partials once (one wave, fixed order) and
+// writes the step's scalars; k_cg_move then updates the vectors with many small blocks (E
+// elements per thread), which lets later blocks load while earlier ones store.  (One kernel
+// whose every block reduced the partials itself measured slower at 128³: mixed loop 34.9 vs
+// 29.7 ms.)
 __global__ __launch_bounds__(64) void k_cg_scal(int nin, const double* __restrict__ part_in, int na,
                                                const double* __restrict__ part_s, const Scal* __restrict__ sin,
                                                Scal* __restrict__ sout) {
@@ -1567,7 +1440,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_pfix(int64_t n, const X* __restri
   if (threadIdx.x == 0) part[2 * blockIdx.x + 1] = 0.0;
 }
 
-// Reseed, step 4 (scalars): α = γ / p·s; the next k_cg_step only moves x̂ and r̂.
+// Reseed, step 4 (scalars): α = γ / p·s; the next step only moves x̂ and r̂.
 __global__ __launch_bounds__(kBlock) void k_cg_alpha(int m, const double* __restrict__ part, Scal* __restrict__ s) {
   __shared__ double sh[kBlock / 64];
   const double ps = sum_parts(part, m, 2, sh);
@@ -1878,7 +1751,7 @@ static void cell_shape(const fem_ctx* c, const LatOp& l, CellOp* cl, int bytes) 
   const int pi = bytes == 4 ? 0 : 1;
   int& grid = bytes == 4 ? cl->grid32 : cl->grid64;
   cl->zn[pi] = 0;
-  if (cl->dim == 3 && c->tune.cell_march != -1) {
+  if (cl->dim == 3) {
     // z-marching: rows per thread (two only in FP32: the FP64 windows would not fit in LDS),
     // then the window entries per thread (ZN, KN) of an instantiated kernel
     int rpt = bytes == 4 ? (c->tune.cell_rows ? c->tune.cell_rows : 2) : 1;
@@ -1905,7 +1778,13 @@ static void cell_shape(const fem_ctx* c, const LatOp& l, CellOp* cl, int bytes) 
     grid = (int)std::max<int64_t>(8, std::min<int64_t>(want, (nitems + 7) / 8 * 8));
     const int64_t words = 3 * (T + 2 * l.px + 2) + 2 * (T + l.px + 1) + (lz + D - 1) / D * D * T;
     (bytes == 4 ? cl->ldsz32 : cl->ldsz64) = (int)(bytes * words);
-  } else {
+    // the windows and the staged rows must fit one block's LDS; wider lattices take the tiles
+    int maxlds = 0;
+    if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
+      maxlds = 65536;
+    if (bytes * words > (int64_t)maxlds) cl->zn[pi] = 0;
+  }
+  if (cl->zn[pi] == 0) {
     const int64_t ntiles = (l.nrows + kBlock - 1) / kBlock;
     int64_t gr = std::min<int64_t>(want, ntiles);
     if (gr >= 8) gr -= gr % 8;
@@ -2330,30 +2209,14 @@ static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int
   if (timed) (void)hipEventRecord(w->tev[2 * k], c->stream);
   const int na = spmv_hat<X>(c, w, v.r, v.w, sin);
   if (timed) (void)hipEventRecord(w->tev[2 * k + 1], c->stream);
-  // default: scalars once, then the vector move (measured at 128³: mixed loop 29.7 vs 34.9 ms,
-  // FP64 loop 89.7 vs 92.4 ms for the one-kernel step)
-  if (c->tune.cg_split != -1) {
-    const int mb = cg_blocks(c, v.n);
-    k_cg_scal<<<1, 64, 0, c->stream>>>(mb, pin, na, w->part_s, sin, sout);
-    switch (cg_elems(c)) {
-      case 2: k_cg_move<X, 2><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
-      case 4: k_cg_move<X, 4><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
-      case 16: k_cg_move<X, 16><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
-      default: k_cg_move<X, 8><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
-    }
-    return;
-  }
-  const int nb = cg_blocks(c, v.n);
-#define FEM_STEP(EE)                                                                                          \
-  k_cg_step<X, EE><<<nb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, na, w->part_s, pin, pout, sin, \
-                                                 sout)
+  const int mb = cg_blocks(c, v.n);
+  k_cg_scal<<<1, 64, 0, c->stream>>>(mb, pin, na, w->part_s, sin, sout);
   switch (cg_elems(c)) {
-    case 2: FEM_STEP(2); break;
-    case 4: FEM_STEP(4); break;
-    case 16: FEM_STEP(16); break;
-    default: FEM_STEP(8); break;
+    case 2: k_cg_move<X, 2><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
+    case 4: k_cg_move<X, 4><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
+    case 16: k_cg_move<X, 16><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
+    default: k_cg_move<X, 8><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
   }
-#undef FEM_STEP
 }
 
 // The captured batch of kBatch iterations (rebuilt only when the buffers change).
@@ -2623,16 +2486,13 @@ extern "C" int32_t fem_set_tuning(fem_ctx* c, const fem_tuning* t) {
       !(t->gather_group == 0 || t->gather_group == 1 || t->gather_group == 2 || t->gather_group == 4 ||
         t->gather_group == 8) ||
       !(t->cg_elems == 0 || t->cg_elems == 2 || t->cg_elems == 4 || t->cg_elems == 8 || t->cg_elems == 16) ||
-      !(t->gather_chunks == 0 || t->gather_chunks == 1 || t->gather_chunks == 2 || t->gather_chunks == 4 ||
-        t->gather_chunks == 8 || t->gather_chunks == 16 || t->gather_chunks == 32) ||
       !(t->sell_nt == 0 || t->sell_nt == 1 || t->sell_nt == -1) || !(t->affine == 0 || t->affine == -1) ||
       !(t->lattice == 0 || t->lattice == 1 || t->lattice == -1) ||
       !(t->lattice_blocks == 0 || (t->lattice_blocks >= 8 && t->lattice_blocks <= kMaxVecBlocks &&
                                    t->lattice_blocks % 8 == 0)) ||
-      !(t->cg_split == 0 || t->cg_split == 1 || t->cg_split == -1) ||
       !(t->cell_operator == 0 || t->cell_operator == 1 || t->cell_operator == -1) ||
       !(t->cell_blocks == 0 || (t->cell_blocks >= 8 && t->cell_blocks <= kMaxVecBlocks && t->cell_blocks % 8 == 0)) ||
-      !(t->cell_march == 0 || t->cell_march == -1) || !(t->cell_rows >= 0 && t->cell_rows <= 2) ||
+      !(t->cell_rows >= 0 && t->cell_rows <= 2) ||
       !(t->elem_operator >= -1 && t->elem_operator <= 1))
     return set_error(c, FEM_ERR_ARG, "unsupported tuning value");
   (void)hipStreamSynchronize(c->stream);

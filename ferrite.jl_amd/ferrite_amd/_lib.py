@@ -59,10 +59,10 @@ class Tuning(C.Structure):
     _fields_ = [("spmv_cap64", C.c_int32), ("spmv_lanes64", C.c_int32), ("sell_compress", C.c_int32),
                 ("asm_block", C.c_int32), ("asm_mode", C.c_int32), ("time_kernels", C.c_int32),
                 ("sell_unroll", C.c_int32), ("sell_map", C.c_int32), ("gather_group", C.c_int32),
-                ("cg_elems", C.c_int32), ("gather_chunks", C.c_int32), ("sell_nt", C.c_int32),
+                ("cg_elems", C.c_int32), ("sell_nt", C.c_int32),
                 ("affine", C.c_int32), ("lattice", C.c_int32), ("lattice_blocks", C.c_int32),
-                ("cg_split", C.c_int32), ("cell_operator", C.c_int32), ("cell_blocks", C.c_int32),
-                ("cell_march", C.c_int32), ("cell_rows", C.c_int32), ("elem_operator", C.c_int32)]
+                ("cell_operator", C.c_int32), ("cell_blocks", C.c_int32),
+                ("cell_rows", C.c_int32), ("elem_operator", C.c_int32)]
 
 
 class Timings(C.Structure):

@@ -212,9 +212,6 @@ typedef struct {
                                         lane loads before adding them, 1 / 2 / 4 / 8 (0 = default 4) */
   int32_t cg_elems;                  /* CG vector step: elements per thread per pass, 2 / 4 / 8 / 16
                                         (0 = default 8) */
-  int32_t gather_chunks;             /* owner-computes assembly: cell chunks whose element kernel
-                                        overlaps the previous chunk's gather on a second stream,
-                                        1 (no overlap) / 2 / 4 / 8 / 16 / 32 (0 = default 1) */
   int32_t sell_nt;                   /* sliced-ELL SpMV: non-temporal loads of the operator's values
                                         and offsets (0 = default, 1) or ordinary loads (-1) */
   int32_t affine;                    /* owner-computes assembly on trilinear hexes: constant-Jacobian
@@ -227,9 +224,6 @@ typedef struct {
                                         -1: never, the sliced ELL copy); the cell form
                                         (cell_operator) is preferred where it applies */
   int32_t lattice_blocks;            /* blocks of the lattice SpMV, a multiple of 8 (0 = default) */
-  int32_t cg_split;                  /* CG step after the SpMV: 0 (default) or 1 = a one-block scalar
-                                        kernel, then the vector move; -1 = one kernel whose every
-                                        block reduces the partials itself */
   int32_t cell_operator;             /* solver operator on a lattice of congruent cells whose K is
                                         the sum of k_e times cell 0's unit-coefficient element
                                         matrix (checked entry by entry against K): applied from
@@ -237,9 +231,6 @@ typedef struct {
                                         planes (0 = default: mixed solver only; 1: FP64 solver too;
                                         -1: never) */
   int32_t cell_blocks;               /* blocks of that SpMV, a multiple of 8 (0 = default 768) */
-  int32_t cell_march;                /* 3D cell form: 0 (default) = each block marches a tile up
-                                        through a chunk of planes, loading one plane per step;
-                                        -1 = independent tiles of rows (three planes per tile) */
   int32_t cell_rows;                 /* 3D cell form, FP32: rows per thread of the marching SpMV,
                                         1 or 2 (0 = default 2; FP64 always 1) */
   int32_t elem_operator;             /* solver operator from the element matrices the last owner-

@@ -210,37 +210,6 @@ def test_hex_geometry_paths():
     _check_K(K2, f2, OK2, of2, 1e-12)
 
 
-def test_pipelined_gather_bitwise():
-    """Element kernel of chunk k+1 overlapping the gather of chunk k on a second stream: every
-    entry still sums its cells in ascending order, so nzval and f are bitwise those of the
-    single-launch gather, for FP64 and FP32 element math and start_assemble(fillzero=false)."""
-    print("case pipelined gather", flush=True)
-    nel = (40, 36, 30)   # 43,200 cells: pipelined for up to 8 chunks (>= 4096 cells each)
-    k = poisson_conductivity(nel, seed=12)
-    g = fa.generate_grid("Hexahedron", nel, (0, 0, 0), (1, 1, 1))
-    ip = fa.Lagrange("RefHexahedron", 1)
-    cv = fa.CellValues(fa.QuadratureRule("RefHexahedron", 2), ip)
-    dh = fa.DofHandler(g).add("u", ip).close()
-    K = fa.allocate_matrix(fa.MI355XCSC, dh)
-    f = fa.allocate_vector(K)
-    colors = fa.create_coloring(g)
-    el = fa.PoissonElement(k=k)
-    for precision in ("fp64", "mixed"):
-        fa.set_tuning(K, asm_mode=ASM["gather"], gather_chunks=1)
-        fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=precision)
-        fa.assemble_global(fa.start_assemble(K, f, fillzero=False), cv, colors, el, precision=precision)
-        ref_a, ref_f = K.nzval(), f.to_numpy()
-        assert fa.timings(K)["assemble_launches"] == 2
-        for chunks in (2, 8):
-            fa.set_tuning(K, asm_mode=ASM["gather"], gather_chunks=chunks)
-            fa.assemble_global(fa.start_assemble(K, f), cv, colors, el, precision=precision)
-            fa.assemble_global(fa.start_assemble(K, f, fillzero=False), cv, colors, el, precision=precision)
-            assert fa.timings(K)["assemble_launches"] == 2 * chunks
-            assert np.array_equal(K.nzval(), ref_a), (precision, chunks)
-            assert np.array_equal(f.to_numpy(), ref_f), (precision, chunks)
-
-
-@pytest.mark.parametrize("asm", ["gather", "scatter"])
 def test_inverted_cell_raises_detj(asm):
     """det(J) <= 0 -> ArgumentError (common_values.jl:5), reported with the cell id."""
     print("case detJ error", asm, flush=True)
