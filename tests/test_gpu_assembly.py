@@ -210,6 +210,7 @@ def test_hex_geometry_paths():
     _check_K(K2, f2, OK2, of2, 1e-12)
 
 
+@pytest.mark.parametrize("asm", ["gather", "scatter"])
 def test_inverted_cell_raises_detj(asm):
     """det(J) <= 0 -> ArgumentError (common_values.jl:5), reported with the cell id."""
     print("case detJ error", asm, flush=True)
