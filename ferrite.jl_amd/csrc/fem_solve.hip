@@ -1023,20 +1023,46 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv(int64_t r_lo, int64_t nrow
   if (threadIdx.x == 0) part[blockIdx.x] = b;
 }
 
-// 3D cell form marching in z: work item (j, c) = in-plane tile j (kBlock consecutive rows of
-// a plane, in-plane offsets j kBlock ..) over the planes [z0, z1) of chunk c.  The block keeps
-// a ring of three t·x windows (planes z − 1, z, z + 1; kBlock + 2 px + 2 entries each, ZN per
-// thread) and two coefficient windows (cell layers z, z − 1; kBlock + px + 1 entries, KN per
-// thread) in LDS and loads one new window of each per plane: step p brings the t·x window of
-// plane p, the coefficients of layer p − 1 and t, x of the rows of plane p − 1.  D register
-// stages hold the loads of D steps, issued D planes ahead of their use.  The step loop is
-// straight-line code (every load issued, out-of-range rows read as padding row 0, which
-// holds zeros; the rows of a plane go to an LDS buffer and are written once per item; the
-// planes per chunk are rounded up to a multiple of D, the extra steps' rows discarded) and
-// its barriers order LDS only, so the loads of later steps stay in flight across them.
+// Raw buffer resource over `bytes` bytes from `base`: a load at a byte offset at or beyond
+// `bytes` (the offset is unsigned, so a negative one too) returns 0 without touching memory.
+template <typename X>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const X* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<X*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename X>
+__device__ __forceinline__ X buf_load(__amdgpu_buffer_rsrc_t r, uint32_t boff) {
+  if constexpr (sizeof(X) == 4) return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)boff, 0, 0));
+  else return __longlong_as_double((long long)__builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, 0, 0));
+}
+
+// The resource for the rows [row0, npad) of a vector (none when row0 is outside [0, npad)).
+// 32-bit row arithmetic (npad · sizeof(X) < 2³¹), which stays on the scalar unit.
+template <typename X>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_from(const X* v, int32_t row0, int32_t npad) {
+  const bool in = row0 >= 0 && row0 < npad;
+  return buf_rsrc(in ? v + row0 : v, in ? (uint32_t)(npad - row0) * (uint32_t)sizeof(X) : 0u);
+}
+
+// 3D cell form marching in z: work item (j, c) = in-plane tile j (T = RPT kBlock consecutive
+// rows of a plane) over the planes [z0, z1) of chunk c.  The block keeps a ring of three t·x
+// windows (planes z − 1, z, z + 1; T + 2 px + 2 entries each, ZN per thread) and two
+// coefficient windows (cell layers z, z − 1; T + px + 1 entries, KN per thread) in LDS and
+// loads one new window of each per plane: step p brings the t·x window of plane p, the
+// coefficients of layer p − 1 and t, x of the rows of plane p − 1.  D register stages hold
+// the loads of D steps, issued D planes ahead of their use.  Every load goes through a buffer
+// resource made per step from the step's first row (scalar work): a thread's offsets into it
+// are fixed for the whole launch, and rows past the vector's end (steps past the last plane)
+// read 0 — the padding's value — with no compare or select.  A thread computes RPT rows
+// kBlock apart (lanes read consecutive LDS words: no bank conflicts).  The LDS windows are
+// padded to ZN (KN) kBlock entries, so every thread stores all its entries.  The step loop is
+// straight-line code (the rows of a plane go to an LDS buffer and are written once per item;
+// the planes per chunk are rounded up to a multiple of D, the extra steps' rows discarded)
+// and its barriers order LDS only, so the loads of later steps stay in flight across them.
 // Padding rows of the marched planes are computed too (t = 0 there, so y = x = 0).  Items are
 // ordered chunk-major; MAP 1: the blocks of one XCD sweep one contiguous eighth of the items,
 // so neighbouring tiles (which share their halo rows) run on the same XCD at the same time.
+// The host keeps npad · sizeof(X) below 2³¹ (the resources' 32-bit extent).
 template <typename X, int MAP, int ZN, int KN, int D, int RPT>
 __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px, int32_t pxy, int32_t pz,
                                                         int32_t ntx, int32_t lz, int64_t nitems,
@@ -1048,28 +1074,26 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
   extern __shared__ __align__(16) unsigned char cell_lds[];
   __shared__ double sh[kBlock / 64];
   constexpr int T = RPT * kBlock;            // rows of a tile: RPT per thread, kBlock apart
-  const int W = T + 2 * px + 2, WK = T + px + 1;   // W ≤ ZN kBlock, WK ≤ KN kBlock (host)
+  constexpr int WZ = ZN * kBlock, WKP = KN * kBlock;   // padded window lengths (≥ the used ones)
+  const int W = T + 2 * px + 2, WK = T + px + 1;      // W ≤ WZ, WK ≤ WKP (host)
   const int nsteps = (lz + D - 1) / D * D;
-  X* zw = reinterpret_cast<X*>(cell_lds);   // [3][W]
-  X* kw = zw + 3 * W;                        // [2][WK]
-  X* yb = kw + 2 * WK;                       // [nsteps][T] rows of the item's planes
-  // this thread's window entries: offsets from the tile's first row of plane p (t·x) and of
-  // layer p − 1 (coefficients); entries past a window read row 0
-  int32_t relz[ZN], relk[KN];
-  bool inz[ZN], ink[KN];
+  X* zw = reinterpret_cast<X*>(cell_lds);   // [3][WZ]
+  X* kw = zw + 3 * WZ;                       // [2][WKP]
+  X* yb = kw + 2 * WKP;                      // [nsteps][T] rows of the item's planes
+  // byte offsets of this thread's window entries from the window's first row (entries past a
+  // window: an offset no resource reaches)
+  uint32_t oz[ZN], ok[KN];
 #pragma unroll
   for (int k = 0; k < ZN; ++k) {
     const int idx = threadIdx.x + k * kBlock;
-    inz[k] = idx < W;
-    relz[k] = idx - px - 1;
+    oz[k] = idx < W ? (uint32_t)idx * (uint32_t)sizeof(X) : 0x80000000u;
   }
 #pragma unroll
   for (int k = 0; k < KN; ++k) {
     const int idx = threadIdx.x + k * kBlock;
-    ink[k] = idx < WK;
-    relk[k] = idx - px - 1 - pxy;
+    ok[k] = idx < WK ? (uint32_t)idx * (uint32_t)sizeof(X) : 0x80000000u;
   }
-  const uint32_t np32 = (uint32_t)npad;
+  const uint32_t orow = threadIdx.x * (uint32_t)sizeof(X);   // this thread's first row
   int64_t i0, i1, istep;
   if (MAP == 1) {
     const int64_t xcd = blockIdx.x % 8;
@@ -1082,39 +1106,35 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
     istep = gridDim.x;
   }
   X vt[D][ZN], vx[D][ZN], vk[D][KN], tn[D][RPT], xn[D][RPT];
-  // step p of tile offset o0 into stage u (q wraps below 0 to beyond npad: row 0)
+  // step p of tile offset o0 into stage u: the t·x window of plane p, the coefficients and
+  // this thread's t, x of plane p − 1.  A window starts px + 1 rows before its tile; it can
+  // start before row 0 only in plane 0 (t·x) or layer 0 (coefficients), where every entry a
+  // computed row reads is padding, i.e. 0, and then the whole window reads 0.
   auto fetch = [&](int u, int32_t p, int32_t o0) {
-    const uint32_t base = (uint32_t)p * (uint32_t)pxy + (uint32_t)o0;
+    const int32_t b0 = p * pxy + o0, b1 = b0 - pxy, np = (int32_t)npad;
+    const __amdgpu_buffer_rsrc_t rt = rows_from(t, b0 - px - 1, np), rx = rows_from(x, b0 - px - 1, np);
+    const __amdgpu_buffer_rsrc_t rk = rows_from(kc, b1 - px - 1, np);
+    const __amdgpu_buffer_rsrc_t rt1 = rows_from(t, b1, np), rx1 = rows_from(x, b1, np);
 #pragma unroll
     for (int k = 0; k < ZN; ++k) {
-      const uint32_t q = base + (uint32_t)relz[k];
-      const uint32_t qq = (inz[k] && q < np32) ? q : 0u;
-      vt[u][k] = at_byte(t, qq * (uint32_t)sizeof(X));
-      vx[u][k] = at_byte(x, qq * (uint32_t)sizeof(X));
+      vt[u][k] = buf_load<X>(rt, oz[k]);
+      vx[u][k] = buf_load<X>(rx, oz[k]);
     }
 #pragma unroll
-    for (int k = 0; k < KN; ++k) {
-      const uint32_t q = base + (uint32_t)relk[k];
-      const uint32_t qq = (ink[k] && q < np32) ? q : 0u;
-      vk[u][k] = at_byte(kc, qq * (uint32_t)sizeof(X));
-    }
+    for (int k = 0; k < KN; ++k) vk[u][k] = buf_load<X>(rk, ok[k]);
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-      const uint32_t P = base - (uint32_t)pxy + (uint32_t)(r * kBlock) + threadIdx.x;
-      const uint32_t PP = P < np32 ? P : 0u;
-      tn[u][r] = at_byte(t, PP * (uint32_t)sizeof(X));
-      xn[u][r] = at_byte(x, PP * (uint32_t)sizeof(X));
+      tn[u][r] = buf_load<X>(rt1, orow + r * (uint32_t)(kBlock * sizeof(X)));
+      xn[u][r] = buf_load<X>(rx1, orow + r * (uint32_t)(kBlock * sizeof(X)));
     }
   };
   auto store = [&](int u, int32_t p) {   // z slot p mod 3, coefficient slot (p − 1) mod 2
-    X* zs = zw + (p % 3) * W;
-    X* ks = kw + ((p - 1) & 1) * WK;
+    X* zs = zw + (p % 3) * WZ;
+    X* ks = kw + ((p - 1) & 1) * WKP;
 #pragma unroll
-    for (int k = 0; k < ZN; ++k)
-      if (inz[k]) zs[threadIdx.x + k * kBlock] = vt[u][k] * vx[u][k];
+    for (int k = 0; k < ZN; ++k) zs[threadIdx.x + k * kBlock] = vt[u][k] * vx[u][k];
 #pragma unroll
-    for (int k = 0; k < KN; ++k)
-      if (ink[k]) ks[threadIdx.x + k * kBlock] = vk[u][k];
+    for (int k = 0; k < KN; ++k) ks[threadIdx.x + k * kBlock] = vk[u][k];
   };
   double d = 0.0;
   for (int64_t item = i0; item < i1; item += istep) {
@@ -1141,11 +1161,12 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
         store(u, zp + 1);
         lds_barrier();
         fetch(u, zp + 1 + D, o0);
-        const X* zm = zw + ((zp + 2) % 3) * W;   // plane zp − 1
-        const X* z0w = zw + (zp % 3) * W;
-        const X* zpw = zw + ((zp + 1) % 3) * W;
-        const X* k0 = kw + (zp & 1) * WK;         // layer zp
-        const X* k1 = kw + ((zp - 1) & 1) * WK;   // layer zp − 1
+        const X* zm = zw + ((zp + 2) % 3) * WZ;   // plane zp − 1
+        const X* z0w = zw + (zp % 3) * WZ;
+        const X* zpw = zw + ((zp + 1) % 3) * WZ;
+        const X* k0 = kw + (zp & 1) * WKP;         // layer zp
+        const X* k1 = kw + ((zp - 1) & 1) * WKP;   // layer zp − 1
+        const bool live = zp < z1;
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
           const int l = r * kBlock + threadIdx.x + px + 1;   // the row in the windows
@@ -1166,7 +1187,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
           const X yv = tc[r] != X(0) ? tc[r] * acc : xc[r];
           yb[(sb + u) * T + r * kBlock + threadIdx.x] = yv;
           const bool col = o0 + r * kBlock + (int32_t)threadIdx.x < pxy;
-          d += (col && zp < z1) ? (double)xc[r] * (double)yv : 0.0;
+          d += (col && live) ? (double)(xc[r] * yv) : 0.0;
         }
         lds_barrier();     // this plane's reads are done before the next store
       }
@@ -1776,13 +1797,14 @@ static void cell_shape(const fem_ctx* c, const LatOp& l, CellOp* cl, int bytes) 
     lz = (int)std::min<int64_t>(16, std::max<int64_t>(2, (cl->ntx[pi] * (pz - 2) + want - 1) / want));
     nitems = (int64_t)cl->ntx[pi] * ((pz - 2 + lz - 1) / lz);
     grid = (int)std::max<int64_t>(8, std::min<int64_t>(want, (nitems + 7) / 8 * 8));
-    const int64_t words = 3 * (T + 2 * l.px + 2) + 2 * (T + l.px + 1) + (lz + D - 1) / D * D * T;
+    const int64_t words = (3 * cl->zn[pi] + 2 * cl->kn[pi]) * (int64_t)kBlock + (lz + D - 1) / D * D * T;
     (bytes == 4 ? cl->ldsz32 : cl->ldsz64) = (int)(bytes * words);
-    // the windows and the staged rows must fit one block's LDS; wider lattices take the tiles
+    // the (padded) windows and the staged rows must fit one block's LDS, and the vectors the
+    // 32-bit extent of a buffer resource; otherwise the tiles
     int maxlds = 0;
     if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess)
       maxlds = 65536;
-    if (bytes * words > (int64_t)maxlds) cl->zn[pi] = 0;
+    if (bytes * words > (int64_t)maxlds || l.npad * bytes >= (int64_t)1 << 31) cl->zn[pi] = 0;
   }
   if (cl->zn[pi] == 0) {
     const int64_t ntiles = (l.nrows + kBlock - 1) / kBlock;
