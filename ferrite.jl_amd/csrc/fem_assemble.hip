@@ -32,14 +32,74 @@ __global__ void k_reset_err(DevError* e) {
   e->pattern_entry = -1;
 }
 
-// Scalar Poisson/heat element, Q1 geometry == function interpolation.
-template <typename T, int DIM, int NG, int NB, int NQ, int BLOCK>
+// The constraints handed to the element kernel for apply_local!: per-dof flag, the sorted
+// prescribed dofs and their inhomogeneities (0-based), apply_zero.
+struct LocalBC {
+  const uint8_t* isc;
+  const int32_t* pres;
+  const double* g;
+  int64_t np;
+  int zero;
+};
+
+// apply_local!(Ke, fe, dofs, ch) (ConstraintHandler.jl:1763-1820) on one cell's symmetric Ke
+// (upper triangle, tri_idx) and fe, Dirichlet constraints: fe −= Ke[:, a] g_a for each
+// constrained local dof a in order (not with apply_zero), m = meandiag(Ke) before any change,
+// then each constrained row and column zeroed, Ke[a, a] = m, fe[a] = g_a m (0: apply_zero).
+// Products and sums unfused, in the reference's order.
+template <typename T, int NB>
+__device__ __forceinline__ void apply_local_cell(T* ke, T* fe, const int32_t* dofs, const LocalBC& bc) {
+#pragma clang fp contract(off)
+  bool con[NB];
+  T gv[NB];
+  bool any = false;
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    const int32_t d = dofs[a];
+    con[a] = bc.isc[d] != 0;
+    gv[a] = T(0);
+    if (con[a]) {
+      int64_t lo = 0, hi = bc.np;   // d is in pres[0, np)
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (bc.pres[mid] < d) lo = mid + 1; else hi = mid;
+      }
+      gv[a] = (T)bc.g[lo];
+      any = true;
+    }
+  }
+  if (!any) return;
+  if (!bc.zero) {
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+      if (con[a] && gv[a] != T(0)) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) fe[j] = fe[j] - gv[a] * ke[tri_idx(j < a ? j : a, j < a ? a : j, NB)];
+      }
+  }
+  T m = T(0);
+#pragma unroll
+  for (int i = 0; i < NB; ++i) m = m + fabs(ke[tri_idx(i, i, NB)]);
+  m = m / (T)NB;
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+    if (con[a]) {
+#pragma unroll
+      for (int j = 0; j < NB; ++j) ke[tri_idx(j < a ? j : a, j < a ? a : j, NB)] = T(0);
+      ke[tri_idx(a, a, NB)] = m;
+      fe[a] = bc.zero ? T(0) : gv[a] * m;
+    }
+}
+
+// Scalar Poisson/heat element, Q1 geometry == function interpolation.  LOCAL: apply_local!
+// with `bc` on every cell before it is added.
+template <typename T, int DIM, int NG, int NB, int NQ, int BLOCK, bool LOCAL>
 __global__ __launch_bounds__(BLOCK) void k_poisson(
     int64_t s0, int64_t s1, const double* __restrict__ X0, const double* __restrict__ X1,
     const double* __restrict__ X2, const int32_t* __restrict__ cnodes, const int32_t* __restrict__ cdofs,
     const int32_t* __restrict__ cnz, const int32_t* __restrict__ ccell, const double* __restrict__ cparam,
     const T* __restrict__ tab, T src, double* __restrict__ nzval, double* __restrict__ f,
-    DevError* __restrict__ err) {
+    DevError* __restrict__ err, LocalBC bc) {
   constexpr int TRI = NB * (NB + 1) / 2;
   __shared__ T s_ke[BLOCK * TRI];
   __shared__ T s_fe[BLOCK * NB];
@@ -64,6 +124,7 @@ __global__ __launch_bounds__(BLOCK) void k_poisson(
     T fe[NB];
     const bool bad = !poisson_cell<T, DIM, NG, NB, NQ>(x, (T)cparam[s], src, stab, ke, fe);
     if (bad) atomicMin(&err->detj_cell, ccell[s]);
+    if constexpr (LOCAL) apply_local_cell<T, NB>(ke, fe, cdofs + s * NB, bc);
 #pragma unroll
     for (int i = 0; i < TRI; ++i) s_ke[lc * TRI + i] = ke[i];
 #pragma unroll
@@ -222,7 +283,7 @@ static int upload_tables(fem_ctx* c) {
 }
 
 template <typename T, int DIM, int NG, int NB, int NQ, int BLOCK>
-static int launch_poisson(fem_ctx* c, double src) {
+static int launch_poisson(fem_ctx* c, double src, int local) {
   int rc = upload_tables<DIM, NG, NB, NQ>(c);
   if (rc) return rc;
   const T* tab = nullptr;
@@ -232,9 +293,15 @@ static int launch_poisson(fem_ctx* c, double src) {
     const int64_t s0 = c->color_ptr[col], s1 = c->color_ptr[col + 1];
     if (s1 <= s0) continue;
     const int64_t nb = (s1 - s0 + BLOCK - 1) / BLOCK;
-    k_poisson<T, DIM, NG, NB, NQ, BLOCK><<<dim3((unsigned)nb), dim3(BLOCK), 0, c->stream>>>(
-        s0, s1, c->d_x[0], c->d_x[1], c->d_x[2], c->d_cnodes, c->d_cdofs, c->d_cnz, c->d_ccell,
-        c->d_cparam, tab, (T)src, c->d_nzval, c->d_f, c->d_err);
+    const LocalBC bc{c->d_isc, c->d_pres, c->d_g, c->np, local == 2 ? 1 : 0};
+    if (local)
+      k_poisson<T, DIM, NG, NB, NQ, BLOCK, true><<<dim3((unsigned)nb), dim3(BLOCK), 0, c->stream>>>(
+          s0, s1, c->d_x[0], c->d_x[1], c->d_x[2], c->d_cnodes, c->d_cdofs, c->d_cnz, c->d_ccell,
+          c->d_cparam, tab, (T)src, c->d_nzval, c->d_f, c->d_err, bc);
+    else
+      k_poisson<T, DIM, NG, NB, NQ, BLOCK, false><<<dim3((unsigned)nb), dim3(BLOCK), 0, c->stream>>>(
+          s0, s1, c->d_x[0], c->d_x[1], c->d_x[2], c->d_cnodes, c->d_cdofs, c->d_cnz, c->d_ccell,
+          c->d_cparam, tab, (T)src, c->d_nzval, c->d_f, c->d_err, bc);
     c->timings.assemble_launches++;
   }
   FEM_HIP(c, hipGetLastError());
@@ -298,9 +365,18 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
   c->timings.element_ms = c->timings.scatter_ms = 0.0;
   k_reset_err<<<1, 1, 0, c->stream>>>(c->d_err);
   const bool mixed = cfg->precision == FEM_MIXED;
+  const int local = cfg->local_constraints;
+  if (local) {   // apply_local! on every cell: the colored scatter with the uploaded constraints
+    if (!(local == 1 || local == 2)) return set_error(c, FEM_ERR_ARG, "local_constraints must be 0, 1 or 2");
+    if (cfg->kernel != FEM_KERNEL_POISSON)
+      return set_error(c, FEM_ERR_ARG, "apply_local! during assembly is provided for the Poisson kernel");
+    if (!c->d_isc) return set_error(c, FEM_ERR_STATE, "upload constraints first (fem_constraints_upload)");
+    if (c->naff > 0)
+      return set_error(c, FEM_ERR_ARG, "can not condense constraint without the global matrix and vector");
+  }
   // Scalar Q1 Poisson: owner-computes gather (each nzval/f entry written once, no fill
   // needed) unless the colored scatter is asked for.
-  if (cfg->kernel == FEM_KERNEL_POISSON && c->tune.asm_mode != 1) {
+  if (cfg->kernel == FEM_KERNEL_POISSON && c->tune.asm_mode != 1 && !local) {
     if (c->vdim != 1 || c->ndpc != c->nbf) return set_error(c, FEM_ERR_ARG, "Poisson kernel needs a scalar field");
     if (gather_supported(c)) {
       int rg = c->rdim == 3 ? upload_tables<3, 8, 8, 8>(c) : upload_tables<2, 4, 4, 4>(c);
@@ -342,17 +418,17 @@ extern "C" int32_t fem_assemble(fem_ctx* c, const fem_assemble_cfg* cfg) {
     const int bs = c->tune.asm_block;
     if (c->rdim == 3 && c->ngeo == 8 && c->nbf == 8 && c->nqp == 8 && c->npc == 8 && c->sdim == 3) {
       if (mixed) {
-        rc = bs == 64 ? launch_poisson<float, 3, 8, 8, 8, 64>(c, cfg->source)
-           : bs == 256 ? launch_poisson<float, 3, 8, 8, 8, 256>(c, cfg->source)
-                       : launch_poisson<float, 3, 8, 8, 8, 128>(c, cfg->source);
+        rc = bs == 64 ? launch_poisson<float, 3, 8, 8, 8, 64>(c, cfg->source, local)
+           : bs == 256 ? launch_poisson<float, 3, 8, 8, 8, 256>(c, cfg->source, local)
+                       : launch_poisson<float, 3, 8, 8, 8, 128>(c, cfg->source, local);
       } else {
-        rc = bs == 128 ? launch_poisson<double, 3, 8, 8, 8, 128>(c, cfg->source)
-           : bs == 256 ? launch_poisson<double, 3, 8, 8, 8, 256>(c, cfg->source)
-                       : launch_poisson<double, 3, 8, 8, 8, 64>(c, cfg->source);
+        rc = bs == 128 ? launch_poisson<double, 3, 8, 8, 8, 128>(c, cfg->source, local)
+           : bs == 256 ? launch_poisson<double, 3, 8, 8, 8, 256>(c, cfg->source, local)
+                       : launch_poisson<double, 3, 8, 8, 8, 64>(c, cfg->source, local);
       }
     } else if (c->rdim == 2 && c->ngeo == 4 && c->nbf == 4 && c->nqp == 4 && c->npc == 4 && c->sdim == 2) {
-      rc = mixed ? launch_poisson<float, 2, 4, 4, 4, 128>(c, cfg->source)
-                 : launch_poisson<double, 2, 4, 4, 4, 128>(c, cfg->source);
+      rc = mixed ? launch_poisson<float, 2, 4, 4, 4, 128>(c, cfg->source, local)
+                 : launch_poisson<double, 2, 4, 4, 4, 128>(c, cfg->source, local);
     } else {
       (void)key;
       return set_error(c, FEM_ERR_ARG, "no Poisson kernel for this element/quadrature combination "

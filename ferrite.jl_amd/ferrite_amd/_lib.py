@@ -42,7 +42,7 @@ class FemConvergenceError(FemError):
 
 class AssembleCfg(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("precision", C.c_int32), ("fillzero", C.c_int32),
-                ("reserved", C.c_int32), ("source", C.c_double), ("nu", C.c_double)]
+                ("local_constraints", C.c_int32), ("source", C.c_double), ("nu", C.c_double)]
 
 
 class SolveCfg(C.Structure):

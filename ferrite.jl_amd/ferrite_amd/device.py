@@ -209,17 +209,28 @@ def set_cell_parameters(K, values):
     K._param = values
 
 
-def assemble_global(assembler, cv, colors, element, precision="fp64"):
-    """The fused colored cell loop: K[dofs, dofs] += Ke, f[dofs] += fe for every cell."""
+def assemble_global(assembler, cv, colors, element, precision="fp64", ch=None, apply_zero=False):
+    """The fused colored cell loop: K[dofs, dofs] += Ke, f[dofs] += fe for every cell.  With a
+    ConstraintHandler `ch`, every cell's Ke and fe first go through apply_local!(Ke, fe, dofs,
+    ch; apply_zero) (ConstraintHandler.jl:1750-1820; Dirichlet constraints, Poisson kernel), so
+    K and f come out constrained and no apply! follows."""
     K = assembler.K
     _upload_colors(K, colors)
     _upload_cellvalues(K, cv)
     vals = element.param(K.dh.grid.ncells)
     if K._param is None or K._param is not vals and not np.array_equal(np.asarray(K._param), vals):
         set_cell_parameters(K, vals)
+    local = 0
+    if ch is not None:
+        if getattr(ch, "has_affine", False):
+            raise L.FemArgumentError(L.FEM_ERR_ARG, "can not condense constraint without the global matrix and vector")
+        p = L.as_i64(ch.prescribed_dofs)
+        g = L.as_f64(ch.inhomogeneities)
+        K.ctx.call("fem_constraints_upload", len(p), L.ptr(p, C.c_int64), L.ptr(g, C.c_double))
+        local = 2 if apply_zero else 1
     cfg = L.AssembleCfg(kernel=element.kernel,
                         precision=L.FEM_MIXED if precision == "mixed" else L.FEM_FP64,
-                        fillzero=1 if assembler.fillzero else 0, reserved=0,
+                        fillzero=1 if assembler.fillzero else 0, local_constraints=local,
                         source=float(getattr(element, "source", 0.0)), nu=float(getattr(element, "nu", 0.0)))
     K.ctx.call("fem_assemble", C.byref(cfg))
     assembler.fillzero = False

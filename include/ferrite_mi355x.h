@@ -102,7 +102,10 @@ typedef struct {
   int32_t kernel;      /* fem_kernel */
   int32_t precision;   /* fem_precision */
   int32_t fillzero;    /* start_assemble(K, f; fillzero) */
-  int32_t reserved;
+  int32_t local_constraints; /* apply_local!(Ke, fe, dofs, ch) on every cell before it is added
+                                (ConstraintHandler.jl:1750-1820) with the constraints of the last
+                                fem_constraints_upload: 0 none, 1 apply_local!, 2 apply_zero;
+                                scalar Poisson (colored scatter), Dirichlet constraints only */
   double source;       /* Poisson: constant body source s */
   double nu;           /* elasticity: Poisson's ratio */
 } fem_assemble_cfg;

@@ -8,6 +8,9 @@ Restates src/Dofs/ConstraintHandler.jl:
   * apply!: :710-747 (m = meandiag(K) before zeroing, f -= K g, zero prescribed
     columns and rows, K[d,d] = m, f[d] = g_d m), meandiag :952-958,
     add_inhomogeneities_csc! :755-789, zero_out_columns!/rows! :931-950
+  * apply_local!: :1750-1820 (on one element matrix and vector before assembly, Dirichlet
+    constraints: fe -= Ke[:, a] g_a per constrained local dof a, m = meandiag(Ke) before
+    any change, constrained rows and columns zeroed, Ke[a, a] = m, fe[a] = g_a m)
 """
 import numpy as np
 
@@ -99,6 +102,37 @@ def apply(K, f, ch, applyzero=False):
             vz = 0.0 if applyzero else ch.inhomogeneities[i]
             f[d - 1] = vz * m
     return m
+
+
+def apply_local(Ke, fe, dofs, ch, applyzero=False):
+    """apply_local!(Ke, fe, global_dofs, ch; apply_zero) in place (dense Ke, 1-based dofs);
+    Dirichlet constraints only (affine ones need the global arrays, _condense_local!)."""
+    idx = {int(d): i for i, d in enumerate(ch.prescribed_dofs)}
+    n = len(dofs)
+    has = False
+    for a in range(n):
+        i = idx.get(int(dofs[a]))
+        if i is None:
+            continue
+        has = True
+        v = ch.inhomogeneities[i]
+        if not applyzero and v != 0:
+            for j in range(n):
+                fe[j] -= v * Ke[j, a]
+    if not has:
+        return
+    m = 0.0
+    for i in range(n):
+        m += abs(Ke[i, i])
+    m = m / n
+    for a in range(n):
+        i = idx.get(int(dofs[a]))
+        if i is None:
+            continue
+        Ke[:, a] = 0.0
+        Ke[a, :] = 0.0
+        Ke[a, a] = m
+        fe[a] = 0.0 if applyzero else ch.inhomogeneities[i] * m
 
 
 def solve(K, f):

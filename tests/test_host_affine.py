@@ -162,3 +162,37 @@ def test_nested_affine_rejected():
     ch.add(fa.AffineConstraint(1, [(2, 1.0)], 0.0)).add(fa.AffineConstraint(2, [(3, 1.0)], 0.0))
     with pytest.raises(ValueError, match="nested"):
         ch.close()
+
+
+def test_periodic_map_of_the_reference_local_bc_case():
+    """The periodic constraints of the reference's "local application of bc" test (5×5 quads on
+    [-1, 1]², PeriodicDirichlet mirror "bottom" / image "top", then Dirichlet on "left" and
+    "right"), whose solution norm the oracle reproduces (tests/test_oracle_local.py): the
+    product's constraints equal the oracle's dof map built from the reference's facet-pair
+    and facet-dof order, minus the dofs the Dirichlet sets take over."""
+    from oracle.affine import periodic_dof_map
+    from oracle.refel import reference_facets
+    g = fa.generate_grid("Quadrilateral", (5, 5))
+    dh = fa.DofHandler(g).add("u", fa.Lagrange(g.refshape, 1)).close()
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.PeriodicDirichlet("u", fa.collect_periodic_facets(g, "bottom", "top")))
+    ch.add(fa.Dirichlet("u", g.facetsets["left"], lambda x, t: 0.0))
+    ch.add(fa.Dirichlet("u", g.facetsets["right"], lambda x, t: 1.0))
+    ch.close()
+    X, _ = dof_coords(dh)
+
+    def fdofs(fc):
+        c, f = fc
+        return [int(dh.cell_dofs[c - 1][a - 1]) for a in reference_facets("RefQuadrilateral")[f - 1]]
+    pairs = []
+    for mf in g.facetsets["bottom"]:
+        mx = sorted(X[d - 1][0] for d in fdofs(mf))
+        imf = [fc for fc in g.facetsets["top"] if np.allclose(sorted(X[d - 1][0] for d in fdofs(fc)), mx)]
+        for idof in fdofs(imf[0]):
+            pairs.append(([d for d in fdofs(mf) if abs(X[d - 1][0] - X[idof - 1][0]) < 1e-12][0], idof))
+    dmap = periodic_dof_map(pairs)
+    dirichlet = set(int(d) for d in ch.prescribed_dofs) - set(int(d) for d in ch.affine_dofs)
+    expect = {k: v for k, v in dmap.items() if k not in dirichlet}
+    dofs, ptr, cd, cv, b = ch.affine_arrays()
+    assert dict(zip(dofs.tolist(), cd.tolist())) == expect
+    assert np.all(cv == 1.0) and np.all(b == 0.0)
