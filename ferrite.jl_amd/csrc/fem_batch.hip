@@ -403,10 +403,14 @@ struct BScal {
 // Per-problem reductions in two levels: every segment (≤ kSeg consecutive rows of one
 // problem) is one block and writes its partials; the problem's blocks (or a per-problem
 // kernel) sum them in segment order.  Fixed order throughout (deterministic).
-constexpr int kSeg = 256;
-constexpr int kSegThreads = 256;     // vector kernels: one thread per row of a segment
-constexpr int kSpmvThreads = 1024;   // FP64 residual kernel: 4 lanes per row of a segment
-static_assert(kSpmvThreads == 4 * kSeg, "one residual pass covers a segment");
+// 1024-row segments: at 256 the latency-bound tail (a few large 2D problems) ran several
+// rounds of small blocks per launch, each block paying the scalar record, its problem's
+// partials and the block sums for 256 rows.
+constexpr int kSeg = 1024;
+constexpr int kSegThreads = 256;     // vector kernels: kSegR rows per thread, kSegThreads apart
+constexpr int kSegR = kSeg / kSegThreads;
+constexpr int kSpmvThreads = 1024;   // FP64 residual kernel: 4 lanes per row, kSpmvThreads / 4 rows a pass
+static_assert(kSeg % kSegThreads == 0 && kSeg % (kSpmvThreads / 4) == 0, "segment tiling");
 
 // Sum over the kSpmvThreads threads of a block, valid in every thread; `sh` holds 16.
 __device__ __forceinline__ double spmv_block_sum(double v, double* sh) {
@@ -434,7 +438,7 @@ __device__ __forceinline__ double seg_block_sum(double v, double* sh) {
 // δ = r̂·w.  list/nlist: the segments to run; blocks past the list's length return.
 // Problems held or right after a reseed (fresh) are skipped.
 template <bool COMP>
-__global__ __launch_bounds__(kSegThreads) void k_bc_sell(const int32_t* __restrict__ list,
+__global__ __launch_bounds__(kSpmvThreads) void k_bc_sell(const int32_t* __restrict__ list,
                                                          const int32_t* __restrict__ nlist, int which,
                                                          const int32_t* __restrict__ sprob,
                                                          const int64_t* __restrict__ srow,
@@ -447,15 +451,15 @@ __global__ __launch_bounds__(kSegThreads) void k_bc_sell(const int32_t* __restri
                                                          const float* __restrict__ val, const float* __restrict__ r,
                                                          float* __restrict__ w, const BScal* __restrict__ sin,
                                                          double* __restrict__ part_d) {
-  __shared__ double sh[4];
+  __shared__ double sh[kSpmvThreads / 64];
   if ((int)blockIdx.x >= nlist[which]) return;
   const int seg = list[blockIdx.x];
   const BScal& S = sin[sprob[seg]];
   if (S.done || S.hold || S.fresh) return;
-  const int sl = bsl[seg] + (int)(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double d = 0.0;
-  if (sl < bsl[seg + 1]) {
+  // one wave per slice (a segment has at most kSpmvThreads / 64 of them)
+  for (int sl = bsl[seg] + (int)(threadIdx.x >> 6); sl < bsl[seg + 1]; sl += kSpmvThreads / 64) {
     const int64_t row = (int64_t)slrow[sl] + lane;
     const int32_t o0 = boff[sl];
     const int W = (boff[sl + 1] - o0) >> 6;
@@ -486,10 +490,10 @@ __global__ __launch_bounds__(kSegThreads) void k_bc_sell(const int32_t* __restri
     }
     if (row < srow[seg + 1]) {
       w[row] = acc;
-      d = (double)r[row] * (double)acc;
+      d += (double)r[row] * (double)acc;
     }
   }
-  const double t = seg_block_sum(d, sh);
+  const double t = spmv_block_sum(d, sh);
   if (threadIdx.x == 0) part_d[seg] = t;
 }
 
@@ -576,22 +580,37 @@ __global__ __launch_bounds__(kSegThreads) void k_bc_step(const int32_t* __restri
   const float al = (float)alpha, be = (float)beta;
   const int64_t j0 = srow[seg], j1 = srow[seg + 1];
   double ga = 0.0, gr = 0.0;
-  for (int64_t j = j0 + threadIdx.x; j < j1; j += kSegThreads) {
-    float pj = p[j], sj = sv[j];
-    const float rj0 = r[j];
-    const float dj = dsq[j];
+  // the segment's kSegR rows of this thread: every load issued before any use
+  float pv[kSegR], sw[kSegR], rv[kSegR], dv[kSegR], wv[kSegR], xv[kSegR];
+#pragma unroll
+  for (int k = 0; k < kSegR; ++k) {
+    const int64_t j = j0 + threadIdx.x + k * kSegThreads;
+    const bool in = j < j1;
+    pv[k] = in ? p[j] : 0.f;
+    sw[k] = in ? sv[j] : 0.f;
+    rv[k] = in ? r[j] : 0.f;
+    dv[k] = in ? dsq[j] : 0.f;
+    wv[k] = in && !fresh ? w[j] : 0.f;
+    xv[k] = in ? x[j] : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < kSegR; ++k) {
+    const int64_t j = j0 + threadIdx.x + k * kSegThreads;
+    if (j >= j1) continue;
+    float pj = pv[k], sj = sw[k];
+    const float rj0 = rv[k];
     if (!fresh) {
       pj = rj0 + be * pj;
-      sj = w[j] + be * sj;
+      sj = wv[k] + be * sj;
       p[j] = pj;
       sv[j] = sj;
     }
-    x[j] += al * pj;
+    x[j] = xv[k] + al * pj;
     const float rj = rj0 - al * sj;
     r[j] = rj;
     const double rd = (double)rj * (double)rj;
     ga += rd;
-    gr += rd * (double)dj;
+    gr += rd * (double)dv[k];
   }
   const double t = seg_block_sum(ga, sh);
   const double t2 = seg_block_sum(gr, sh);
@@ -625,18 +644,20 @@ __global__ __launch_bounds__(kSpmvThreads) void k_bc_resid(const int32_t* __rest
   __shared__ double sh[kSpmvThreads / 64];
   if ((int)blockIdx.x >= nlist[1]) return;
   const int seg = list[blockIdx.x];
-  const int64_t j = srow[seg] + (threadIdx.x >> 2);
   const int sub = threadIdx.x & 3;
-  const bool in = j < srow[seg + 1];
-  double acc = 0.0, d = 0.0;
-  if (in)
-    for (int32_t q = colptr[j] + sub; q < colptr[j + 1]; q += 4) acc += val[q] * u[rowval[q]];
-  acc += __shfl_down(acc, 2, 4);
-  acc += __shfl_down(acc, 1, 4);
-  if (in && sub == 0) {
-    const double rj = f[j] - acc;
-    res[j] = rj;
-    d = rj * rj;
+  double d = 0.0;
+  for (int64_t j = srow[seg] + (threadIdx.x >> 2); j - (threadIdx.x >> 2) < srow[seg + 1]; j += kSpmvThreads / 4) {
+    const bool in = j < srow[seg + 1];
+    double acc = 0.0;
+    if (in)
+      for (int32_t q = colptr[j] + sub; q < colptr[j + 1]; q += 4) acc += val[q] * u[rowval[q]];
+    acc += __shfl_down(acc, 2, 4);
+    acc += __shfl_down(acc, 1, 4);
+    if (in && sub == 0) {
+      const double rj = f[j] - acc;
+      res[j] = rj;
+      d += rj * rj;
+    }
   }
   const double t = spmv_block_sum(d, sh);
   if (threadIdx.x == 0) part[seg] = t;
@@ -960,19 +981,21 @@ int timed_assemble(fem_batch* B, int cpw, int chunk, bool mixed, float* ms) {
 // One batch of kCgBatch iterations, captured once per batch object: SpMV + fused step per
 // iteration over the list of recurring segments (its length is read on the device, so the
 // same graph serves every active set).  kCgBatch is even: a replay starts and ends on
-// scalar record 0 and partial buffer 0.
-constexpr int kCgBatch = 16;
+// scalar record 0 and partial buffer 0.  32 (measured on the 64-problem batch: total 81.7 ms
+// against 84.1 with 16; a held problem waits longer for its hand-over, p50 35.7 vs 32.3 ms,
+// but the host round trips halve).
+constexpr int kCgBatch = 32;
 
 // The sliced SpMV over `blocks` entries of a segment list (which = 0: recurrence list,
 // 1: hand-over list).
 void launch_bc_sell(fem_batch* B, int blocks, const int32_t* list, int which, const BScal* sin, double* part) {
   fem_ctx* c = B->c;
   if (B->d_broff)
-    k_bc_sell<true><<<blocks, kSegThreads, 0, c->stream>>>(list, B->d_nlist, which, B->d_sprob, B->d_srow, B->d_bsl,
+    k_bc_sell<true><<<blocks, kSpmvThreads, 0, c->stream>>>(list, B->d_nlist, which, B->d_sprob, B->d_srow, B->d_bsl,
                                                            B->d_slrow, B->d_boff, B->d_bbase, B->d_broff, nullptr,
                                                            B->d_bval, B->d_r32, B->d_w32, sin, part);
   else
-    k_bc_sell<false><<<blocks, kSegThreads, 0, c->stream>>>(list, B->d_nlist, which, B->d_sprob, B->d_srow, B->d_bsl,
+    k_bc_sell<false><<<blocks, kSpmvThreads, 0, c->stream>>>(list, B->d_nlist, which, B->d_sprob, B->d_srow, B->d_bsl,
                                                             B->d_slrow, B->d_boff, B->d_bbase, nullptr, B->d_bcol,
                                                             B->d_bval, B->d_r32, B->d_w32, sin, part);
 }
@@ -1354,7 +1377,11 @@ int32_t fem_batch_autotune(fem_batch* B, int32_t precision, int32_t* cpw_out, in
   if (!B->finalized) return set_error(c, FEM_ERR_STATE, "fem_batch_autotune: finalize first");
   FEM_HIP(c, hipSetDevice(c->device));
   const bool mixed = precision == FEM_MIXED;
-  const int cpws[3] = {64, 128, 256};
+  // 64-cell workgroups are not tried: each work item ends with an agent-scope release fence
+  // (its nzval writes must reach the next colour's items on other XCDs), which writes back the
+  // XCD's L2, and 64-cell items made thousands of them per launch (0.1-0.34 s per assembly,
+  // against 6.6 ms for the chosen shape)
+  const int cpws[2] = {128, 256};
   const int chunks[5] = {256, 512, 1024, 2048, 4096};
   float best = 1e30f;
   int bc = 128, bk = 512;
