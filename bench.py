@@ -146,7 +146,7 @@ def main():
         fa.set_tuning(K, time_kernels=0)
         spmv_ms, spmv_n = ts["spmv_ms"], ts["spmv_launches"]
         results[name] = dict(ms=dt * 1e3, cells_per_s=g.ncells / dt, iters=iters, u=u, operator=res.operator,
-                             unit_diagonal=res.unit_diagonal,
+                             unit_diagonal=res.unit_diagonal, cell_shape=res.cell_shape,
                              asm_kernel_ms=float(np.median(asm_ms)), spmv_avg_ms=spmv_ms / max(spmv_n, 1),
                              spmv_launches=spmv_n)
         log(f"{name}: {dt * 1e3:.1f} ms/step, iters {iters[-1]}, assembly kernels {np.median(asm_ms):.3f} ms, "
@@ -217,8 +217,12 @@ def main():
         return kern, nbytes, gbs
 
     spmv_ms = results["mixed"]["spmv_avg_ms"]               # inside the CG loop of the timed steps
-    if results["mixed"]["operator"] == "cells":
-        SPMV32_KERNEL = "k_cell_spmv_z<float, 1, 4, 3, 2, 2>"   # two rows per thread
+    if results["mixed"]["operator"] == "cells" and results["mixed"]["cell_shape"]:
+        # the z-marching template the solver reports (MAP, ZN, KN, stages, rows per thread)
+        SPMV32_KERNEL = "k_cell_spmv_z<float, {}, {}, {}, {}, {}>".format(*results["mixed"]["cell_shape"])
+        spmv_bytes = 3 * 4 * npad + 4 * (px - 2) * px * px
+    elif results["mixed"]["operator"] == "cells":
+        SPMV32_KERNEL = "k_cell_spmv<float, 3, 1, 4>"   # tile form (lattice too wide for the march)
         spmv_bytes = 3 * 4 * npad + 4 * (px - 2) * px * px
     elif results["mixed"]["operator"] == "lattice":
         SPMV32_KERNEL, spmv_bytes, _ = lattice_roofline(results["mixed"]["unit_diagonal"], spmv_ms)

@@ -2451,6 +2451,14 @@ extern "C" int32_t fem_solve(fem_ctx* c, const fem_solve_cfg* cfg, fem_solve_inf
   out.outer_iterations = handovers;
   out.operator_format = w->use_elem ? 4 : w->use_cells ? 3 : !w->use_lat ? 0
                         : (mixed ? w->lat.unit32 : w->lat.unit64) ? 2 : 1;
+  if (w->use_cells && w->cells.dim == 3) {   // the z-marching template that ran
+    const int pi = mixed ? 0 : 1;
+    const CellOp& cl = w->cells;
+    const int grid = mixed ? cl.grid32 : cl.grid64;
+    if (cl.zn[pi] > 0)
+      out.operator_shape = (c->tune.sell_map != 1 && grid % 8 == 0 ? 10000 : 0) + 1000 * cl.zn[pi] +
+                           100 * cl.kn[pi] + 10 * cell_stages(mixed ? 4 : 8, cl.rpt[pi]) + cl.rpt[pi];
+  }
   if ((rc = read_scal(c, w, &h))) return rc;   // s->rr: the true residual of the returned u
   out.relres = std::sqrt(h.rr / bnorm2);
   c->timings.solve_ms = cg_ms;

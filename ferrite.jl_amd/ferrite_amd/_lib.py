@@ -52,7 +52,7 @@ class SolveCfg(C.Structure):
 
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("outer_iterations", C.c_int32), ("relres", C.c_double),
-                ("operator_format", C.c_int32), ("reserved", C.c_int32)]
+                ("operator_format", C.c_int32), ("operator_shape", C.c_int32)]
 
 
 class Tuning(C.Structure):

@@ -278,6 +278,9 @@ class SolveResult:
         # or "cells" (per-cell coefficients times cell 0's unit-coefficient element matrix)
         self.operator = {1: "lattice", 2: "lattice", 3: "cells", 4: "elements"}.get(info.operator_format, "sell")
         self.unit_diagonal = info.operator_format == 2
+        # the 3D cell form's SpMV template (MAP, ZN, KN, stages, rows per thread), or None
+        sh = info.operator_shape
+        self.cell_shape = None if sh <= 0 else (sh // 10000, sh // 1000 % 10, sh // 100 % 10, sh // 10 % 10, sh % 10)
         self.u = u
 
 

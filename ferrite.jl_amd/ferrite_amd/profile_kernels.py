@@ -31,6 +31,17 @@ def _copy_calibration(mib):
     b.copy_(a)
     torch.cuda.synchronize()
     print(f"phase copy: {nbytes} bytes read + {nbytes} bytes written", flush=True)
+    # the measured copy rate (read + written bytes / time), the practical HBM peak
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(f"phase copy timing: {ms:.3f} ms per copy = {2 * nbytes / (ms * 1e-3) / 1e9:.0f} GB/s "
+          f"(read + written)", flush=True)
 
 
 def poisson(args):

@@ -175,7 +175,9 @@ typedef struct {
                                 SpMV does not read the centre plane, 3 cell coefficients
                                 (tuning cell_operator), 4 the element matrices of the last
                                 assembly (tuning elem_operator) */
-  int32_t reserved;
+  int32_t operator_shape;    /* the z-marching cell-form SpMV's template (format 3, 3D):
+                                MAP·10⁴ + ZN·10³ + KN·10² + stages·10 + rows per thread of
+                                k_cell_spmv_z<X, MAP, ZN, KN, stages, rows>; 0 otherwise */
 } fem_solve_info;
 
 int32_t fem_solve(fem_ctx* ctx, const fem_solve_cfg* cfg, fem_solve_info* info);

@@ -296,3 +296,34 @@ def test_cell_operator_new_parameters_without_reassembly():
     fa.set_cell_parameters(K, k)                  # parameters that match K again
     back = fa.solve(K, f, mode="mixed", rtol=1e-12)
     assert back.operator == "cells" and np.array_equal(back.u, ref.u)
+
+
+def test_cell_operator_after_coordinate_reupload():
+    """Coordinates re-uploaded (the mesh scaled by 2, same cells, dofs and pattern), K
+    re-assembled and constrained: the solver rebuilds the cell form (cell 0's corners, K¹) and
+    uses it again, and the result matches the FP64 solve on the sliced copy of the new K."""
+    import ctypes as C
+    from ferrite_amd import _lib as L
+    print("case cell operator after re-upload", flush=True)
+    nel = (12, 10, 8)
+    k = poisson_conductivity(nel, seed=23)
+    g, dh, K, f = _system("Hexahedron", nel, (1.2, 1.0, 0.8), k, cell_operator=0)
+    r1 = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert r1.operator == "cells"
+    xyz = L.as_f64(g.nodes * 2.0)
+    cells = L.as_i64(g.cells)
+    K.ctx.call("fem_mesh_upload", 3, g.nnodes, L.ptr(xyz, C.c_double), g.ncells, 8, L.ptr(cells, C.c_int64))
+    ip = fa.Lagrange(g.refshape, 1)
+    cv = fa.CellValues(fa.QuadratureRule(g.refshape, 2), ip)
+    fa.assemble_global(fa.start_assemble(K, f), cv, fa.create_coloring(g), fa.PoissonElement(k=k, source=1.0))
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.Dirichlet("u", g.facetsets["left"] + g.facetsets["right"], lambda x, t: 0.0)).close()
+    fa.apply(K, f, ch)
+    r2 = fa.solve(K, f, mode="mixed", rtol=1e-12)
+    assert r2.operator == "cells" and r2.cell_shape is not None
+    fa.set_tuning(K, lattice=-1, cell_operator=-1)
+    ref = fa.solve(K, f, mode="cg", rtol=1e-12)
+    assert ref.operator == "sell"
+    assert np.linalg.norm(r2.u - ref.u) / np.linalg.norm(ref.u) < 1e-8
+    # the scaled problem's solution is a different one (u scales with h²)
+    assert np.linalg.norm(r2.u - r1.u) > 1e-3 * np.linalg.norm(r1.u)
