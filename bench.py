@@ -21,6 +21,12 @@ assembly variant is timed separately with its K and u errors (an FP32 Ke moves u
 k_e = exp(g_e), g_e ~ N(0, 0.5²) i.i.d. from the seed (synthetic data).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu-baseline]
+       python bench.py --config heat|elasticity|cantilever|batch [--no-cpu-baseline]
+
+`--config` runs one of the other workloads instead (ferrite_amd.profile_configs: the 100x100 quad
+heat equation, Q2 hex elasticity at 48³, the 512x256 cantilever compliance with its gradients,
+the batch of 64 Poisson problems) and adds the timed CPU restatement of the same workload as
+`cpu_baseline`; it prints that workload's JSON record, not the headline line.
 """
 import argparse
 import json
@@ -51,6 +57,8 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--config", choices=("heat", "elasticity", "cantilever", "batch"), default=None,
+                    help="run another workload with its CPU baseline instead of the headline")
     return ap.parse_args()
 
 
@@ -77,6 +85,9 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU (a multi-GPU run would partition the "
                                    "cells into slabs with a one-plane halo; not built); run with --gpus 1"}))
         sys.exit(2)
+    if args.config:
+        run_config(args)
+        return
     import numpy as np
     import torch
 
@@ -384,6 +395,46 @@ def cpu_baseline(args, g, dh, k, colors, K, ch):
     except Exception as e:  # the oracle's C library is built by __graft_entry__.build()
         log(f"cpu baseline unavailable: {e}")
         return None
+
+
+def config_cpu_leg(args):
+    """The CPU baseline of a `--config` workload: the oracle's C restatement of the same workload
+    on the host cores, called by the workload after its device measurements."""
+    if args.no_cpu_baseline:
+        return None
+    from oracle import cbaseline as CB
+    nt = args.cpu_threads
+
+    def leg(**kw):
+        try:
+            if args.config == "heat":
+                return CB.run_heat(kw["g"], kw["dh"], kw["colors"], kw["colptr"], kw["rowval"], kw["prescribed"],
+                                   threads=nt, log=log)
+            if args.config == "elasticity":
+                return CB.run_elasticity(kw["g"], kw["dh"], kw["E"], kw["nu"], kw["colors"], kw["colptr"],
+                                         kw["rowval"], threads=nt, log=log)
+            if args.config == "cantilever":
+                return CB.run_cantilever(kw["g"], kw["dh"], kw["theta"], kw["colors"], kw["colptr"], kw["rowval"],
+                                         kw["prescribed"], kw["ydof"], threads=nt, rtol=kw["rtol"], log=log)
+            import ferrite_amd as fa
+            from ferrite_amd.batch import poisson_problem
+            probs = []
+            for celltype, nel, k in kw["setups"]:
+                g, dh, colors, ch, k = poisson_problem(celltype, nel, k)
+                cp, rv = fa.create_sparsity_pattern(dh)
+                probs.append((g, dh, colors, cp, rv, ch.prescribed_dofs, k))
+            return CB.run_batch(probs, threads=nt, log=log)
+        except Exception as e:  # the oracle's C library is built by __graft_entry__.build()
+            log(f"cpu baseline unavailable: {e}")
+            return None
+    return leg
+
+
+def run_config(args):
+    from ferrite_amd import profile_configs as PC
+    pc_args = PC.default_args(args.config)
+    out = PC.RUNS[args.config](pc_args, cpu=config_cpu_leg(args))
+    print(json.dumps({"config": args.config, **out}), flush=True)
 
 
 if __name__ == "__main__":

@@ -1,9 +1,13 @@
-"""Full-size runs of the elasticity, differentiation and batching workloads (one JSON line each).
+"""Full-size runs of the heat, elasticity, differentiation and batching workloads (one JSON line each).
 
+    python -m ferrite_amd.profile_configs heat         [--nx 100 --ny 100]
     python -m ferrite_amd.profile_configs elasticity   [--n 48]
     python -m ferrite_amd.profile_configs cantilever   [--nx 512 --ny 256 --fd 64]
     python -m ferrite_amd.profile_configs batch        [--problems 64]
 
+heat        The heat equation of heat_equation.jl on generate_grid(Quadrilateral, (nx, ny)) over
+            [-1,1]^2, Q1, 2x2 Gauss, k = 1, f = 1, u = 0 on all four boundary facet sets: assembly
+            kernel time (best of `reps`), apply! and the FP64 CG solve to 1e-10.
 elasticity  Q2 hex^3 small-strain elasticity on generate_grid(Hexahedron, (n, n, n)) over [0,1]^3,
             3x3x3 Gauss, nu = 0.3, E_e ~ U[0.5, 2.0] (seed 0), clamped 'left', traction (0,0,-1)
             on 'right' (3x3 facet rule).  Times the FP64 element path and the FP32 MFMA element
@@ -24,6 +28,10 @@ cantilever  Plane-strain Q1^2 cantilever on (nx, ny) cells over [0,2]x[0,1], E =
 batch       `problems` seeded scalar Poisson problems (2D/3D, log-uniform cell counts in
             [2e3, 2e5]) assembled and solved (mixed CG to 1e-8) in one batched context after
             the launch-shape autotuner; total cells/s and p50 / p99 completion times.
+
+Each function takes an optional `cpu` callable; when given, it is called once after the device
+measurements with the host-side inputs of the workload (keyword arguments) and its result is
+stored as `cpu_baseline` (the bench passes the timed CPU restatement of the same workload).
 """
 import argparse
 import json
@@ -54,7 +62,7 @@ def q2_gather_bytes(ncells, nnode, nnz, ke_bytes):
     return ncells * (27 * 252 * ke_bytes + 27 * 27 + 27 * 4) + nnode * (8 + 4) + nnz * 8
 
 
-def elasticity(args):
+def elasticity(args, cpu=None):
     import numpy as np
     import scipy.sparse as sp
     import ferrite_amd as fa
@@ -161,6 +169,12 @@ def elasticity(args):
     log(f"u error mixed vs fp64: {out[f'u_rel_err_mixed_vs_fp64_rtol_{args.rtol:g}']:.2e} at rtol {args.rtol:g}, "
         f"{out[f'u_rel_err_mixed_vs_fp64_rtol_{TIGHT_RTOL:g}']:.2e} at {TIGHT_RTOL:g}; backward error of the FP32 Ke "
         f"{out['backward_err_fp32_Ke_at_u64']:.2e}")
+    if cpu is not None:
+        del D, kc, sols, nzs
+        out["cpu_baseline"] = cpu(g=g, dh=dh, E=E, nu=0.3, colors=colors, colptr=K.colptr, rowval=K.rowval)
+        cb = out["cpu_baseline"]
+        if cb:
+            out["fp64_gather"]["speedup_vs_cpu_assembly"] = out["fp64_gather"]["cells_per_s"] / cb["value"]
     return out
 
 
@@ -322,7 +336,7 @@ def exact_central_differences(prob, theta, cells, coords, hs, chunk=64):
     return u, refine_res, out
 
 
-def cantilever(args):
+def cantilever(args, cpu=None):
     import numpy as np
     import ferrite_amd as fa
     from ferrite_amd import _lib as L
@@ -394,10 +408,24 @@ def cantilever(args):
     trunc = (x2 - x1) / 3
     out["dJdX_h1e-6_error_vs_truncation_estimate"] = float(np.max(np.abs((x1 - ad_x) - trunc) / np.abs(trunc)))
     log(f"Richardson h = 1e-6: dJ/dX max rel {out['richardson_h1e-6']['dJdX']['max_entry_rel_err']:.2e}")
+    if cpu is not None:
+        cb = cpu(g=g, dh=dh, theta=theta, colors=colors, colptr=K.colptr, rowval=K.rowval,
+                 prescribed=ch.prescribed_dofs, ydof=ydof, rtol=args.rtol)
+        if cb:
+            # the CPU restatement's J and gradients against the device's (both CG to the same rtol)
+            cb["J_rel_diff_vs_device"] = abs(cb.pop("J") - J) / abs(J)
+            cb["dJdtheta_rel_diff_vs_device"] = float(np.linalg.norm(cb.pop("dJdtheta") - dth_cg) / np.linalg.norm(dth_cg))
+            cb["dJdX_rel_diff_vs_device"] = float(np.linalg.norm(cb.pop("dJdX") - dx_cg) / np.linalg.norm(dx_cg))
+            cb.pop("u", None)
+            cpu_s = cb["assembly_s"] + cb["apply_s"] + cb["solve_s"] + cb["J_and_gradients_s"]
+            cb["value"] = cpu_s
+            out["gpu_J_and_gradients_s"] = tj
+            out["speedup_vs_cpu"] = cpu_s / tj if cb["solve_complete"] else None
+        out["cpu_baseline"] = cb
     return out
 
 
-def batch(args):
+def batch(args, cpu=None):
     import numpy as np
     import ferrite_amd as fa
     t0 = time.perf_counter()
@@ -417,26 +445,82 @@ def batch(args):
                      "all_total_ms": [x.total_ms for x in runs]}
         log(f"{prec}: total {r.total_ms:.2f} ms (assemble {r.assemble_ms:.2f}, solve {r.solve_ms:.2f}), "
             f"p50 {r.percentile(50):.2f} p99 {r.percentile(99):.2f} ms, converged {r.converged}")
+    if cpu is not None:
+        b.close()
+        out["cpu_baseline"] = cpu(setups=specs)
+        if out["cpu_baseline"]:
+            out["speedup_vs_cpu"] = out["mixed"]["cells_per_s"] / out["cpu_baseline"]["value"]
     return out
+
+
+def heat(args, cpu=None):
+    import numpy as np
+    import ferrite_amd as fa
+    g = fa.generate_grid("Quadrilateral", (args.nx, args.ny))
+    ip = fa.Lagrange("RefQuadrilateral", 1)
+    cv = fa.CellValues(fa.QuadratureRule("RefQuadrilateral", 2), ip)
+    dh = fa.DofHandler(g).add("u", ip).close()
+    K = fa.allocate_matrix(fa.MI355XCSC, dh)
+    f = fa.allocate_vector(K)
+    colors = fa.create_coloring(g)
+    ch = fa.ConstraintHandler(dh)
+    facets = [fc for name in ("left", "right", "bottom", "top") for fc in g.facetsets[name]]
+    ch.add(fa.Dirichlet("u", facets, lambda x, t: 0.0)).close()
+    el = fa.PoissonElement()
+    kms = []
+    for _ in range(max(args.reps, 10)):
+        fa.assemble_global(fa.start_assemble(K, f), cv, colors, el)
+        kms.append(fa.timings(K)["assemble_kernel_ms"])
+    t0 = time.perf_counter()
+    fa.apply(K, f, ch)
+    res = fa.solve(K, f, mode="cg", rtol=1e-10)
+    t_solve = time.perf_counter() - t0
+    ms = min(kms)
+    out = {"workload": f"heat equation Q1 quad {args.nx}x{args.ny} on [-1,1]^2, 2x2 Gauss, u = 0 on the boundary",
+           "cells": g.ncells, "dofs": dh.ndofs, "nnz": K.nnz, "colors": len(colors),
+           "assembly_kernel_ms": ms, "assembly_kernel_ms_all": kms, "assembly_cells_per_s": g.ncells / (ms * 1e-3),
+           "apply_solve_s": t_solve, "cg_iterations": res.iterations, "relres": res.relres,
+           "u_norm": float(np.linalg.norm(res.u))}
+    log(f"heat {args.nx}x{args.ny}: assembly {ms * 1e3:.1f} us, apply + CG {t_solve * 1e3:.2f} ms "
+        f"({res.iterations} its), |u| = {out['u_norm']:.12f}")
+    if cpu is not None:
+        out["cpu_baseline"] = cpu(g=g, dh=dh, colors=colors, colptr=K.colptr, rowval=K.rowval,
+                                  prescribed=ch.prescribed_dofs)
+        if out["cpu_baseline"]:
+            out["assembly_speedup_vs_cpu"] = out["assembly_cells_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
+DEFAULTS = {"heat": dict(nx=100, ny=100, rtol=1e-10), "elasticity": dict(n=48, rtol=1e-8),
+            "cantilever": dict(nx=512, ny=256, rtol=2e-9), "batch": dict(problems=64, rtol=1e-8)}
+
+
+def default_args(config, **over):
+    """The argument namespace of `config` with its defaults (and `over` on top)."""
+    ns = argparse.Namespace(n=48, nx=512, ny=256, fd=64, problems=64, reps=3, seed=0, rtol=0.0)
+    for k, v in {**DEFAULTS[config], **over}.items():
+        setattr(ns, k, v)
+    return ns
+
+
+RUNS = {"heat": heat, "elasticity": elasticity, "cantilever": cantilever, "batch": batch}
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("config", choices=("elasticity", "cantilever", "batch"))
-    ap.add_argument("--n", type=int, default=48)
-    ap.add_argument("--nx", type=int, default=512)
-    ap.add_argument("--ny", type=int, default=256)
-    ap.add_argument("--fd", type=int, default=64)
-    ap.add_argument("--problems", type=int, default=64)
-    ap.add_argument("--reps", type=int, default=3)
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--rtol", type=float, default=0.0,
-                    help="CG relative residual (default 1e-8 elasticity, 2e-9 cantilever)")
-    args = ap.parse_args()
-    fn = {"elasticity": elasticity, "cantilever": cantilever, "batch": batch}[args.config]
-    if args.rtol == 0.0:
-        args.rtol = {"elasticity": 1e-8, "cantilever": 2e-9, "batch": 1e-8}[args.config]
-    print(json.dumps({"config": args.config, **fn(args)}), flush=True)
+    ap.add_argument("config", choices=tuple(RUNS))
+    ap.add_argument("--n", type=int)
+    ap.add_argument("--nx", type=int)
+    ap.add_argument("--ny", type=int)
+    ap.add_argument("--fd", type=int)
+    ap.add_argument("--problems", type=int)
+    ap.add_argument("--reps", type=int)
+    ap.add_argument("--seed", type=int)
+    ap.add_argument("--rtol", type=float,
+                    help="CG relative residual (default 1e-10 heat, 1e-8 elasticity and batch, 2e-9 cantilever)")
+    a = ap.parse_args()
+    args = default_args(a.config, **{k: v for k, v in vars(a).items() if v is not None and k != "config"})
+    print(json.dumps({"config": a.config, **RUNS[a.config](args)}), flush=True)
 
 
 if __name__ == "__main__":
