@@ -560,3 +560,13 @@ double cb_dot(int64_t n, const double* a, const double* b) {
   for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }
+
+/* Starts the OpenMP pool with nthreads threads (a parallel sum), so that no timed call pays
+ * for the threads' creation. */
+double cb_warm(int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int i = 0; i < 1 << 16; ++i) s += (double)i;
+  return s;
+}

@@ -45,6 +45,8 @@ def lib():
         h.cb_compliance_adjoint.argtypes = [C.c_int, C.c_int, C.c_int, DP, DP, C.c_int64, C.c_int64, DP, I32P, I32P,
                                             DP, C.c_double, C.c_double, C.c_double, DP, C.c_int, I64P, I32P,
                                             C.c_int, DP, DP]
+        h.cb_warm.restype = C.c_double
+        h.cb_warm.argtypes = [C.c_int]
         h.cb_dot.restype = C.c_double
         h.cb_dot.argtypes = [C.c_int64, DP, DP]
         h.cb_wtime.restype = C.c_double
@@ -73,8 +75,10 @@ class System:
         self.cptr[1:] = np.cumsum(sizes)
         self.ccells = np.ascontiguousarray(np.concatenate([np.asarray(c) - 1 for c in colors]), dtype=np.int32)
         self.ncolors = len(colors)
-        self.nzval = np.zeros(len(self.rowval))
-        self.f = np.zeros(self.n)
+        # written, not only allocated: the pages are mapped before any timed call, as K and f
+        # of an allocated system are
+        self.nzval = np.full(len(self.rowval), 0.0)
+        self.f = np.full(self.n, 0.0)
 
 
 _ERR = {-1: "det(J) is not positive", -2: "missing sparsity pattern entry", -3: "tables exceed the C limits"}
@@ -183,7 +187,7 @@ def run(g, dh, k, colors, colptr, rowval, prescribed, threads=0, rtol=1e-10, max
     of CG: if the solve would exceed it, the CG part is measured on its first iterations and
     extrapolated to the iteration count the same algorithm needs, stated in `sample`)."""
     h = lib()
-    nt = threads or min(16, os.cpu_count() or 1)
+    nt = _threads(threads)
     s = System(g.nodes, g.cells, dh.cell_dofs, colptr, rowval, colors)
     t0 = h.cb_wtime()
     assemble(s, k, 1.0, nt)
@@ -217,7 +221,10 @@ def run(g, dh, k, colors, colptr, rowval, prescribed, threads=0, rtol=1e-10, max
 
 
 def _threads(threads):
-    return threads or min(16, os.cpu_count() or 1)
+    """Thread count (16 at most by default), with the OpenMP pool started before any timing."""
+    nt = threads or min(16, os.cpu_count() or 1)
+    lib().cb_warm(nt)
+    return nt
 
 
 def _timed_pcg(h, s, rtol, nt, max_seconds):
@@ -332,20 +339,21 @@ def run_batch(problems, threads=0, rtol=1e-8, log=print):
     nt = _threads(threads)
     done, ncells, iters = [], 0, 0
     tabs = {}
+    systems = [(System(g.nodes, g.cells, dh.cell_dofs, colptr, rowval, colors),
+                tabs.setdefault(g.refshape, Tables(g.refshape, 1, 2)), g.ncells, prescribed, k)
+               for (g, dh, colors, colptr, rowval, prescribed, k) in problems]
     t0 = h.cb_wtime()
-    for (g, dh, colors, colptr, rowval, prescribed, k) in problems:
-        s = System(g.nodes, g.cells, dh.cell_dofs, colptr, rowval, colors)
-        t = tabs.setdefault(g.refshape, Tables(g.refshape, 1, 2))
+    for s, t, nc, prescribed, k in systems:
         assemble_poisson(s, t, k, 1.0, nt)
         apply_zero(s, prescribed, nt)
         _, it, rel = pcg(s, rtol, maxit=100000, threads=nt)
         iters += it
-        ncells += g.ncells
+        ncells += nc
         done.append((h.cb_wtime() - t0) * 1e3)
     total = done[-1] * 1e-3
     log(f"cpu baseline (batch): {len(problems)} problems, {ncells} cells in {total:.2f} s on {nt} threads")
     return {"value": ncells / total, "unit": "cells/s", "cores": nt, "kind": "port",
-            "sample": f"all {len(problems)} problems one after another (table set-up of each included), each "
+            "sample": f"all {len(problems)} problems one after another (systems allocated beforehand), each "
                       f"assembled in FP64, apply!, Jacobi-PCG to {rtol:g}; Ferrite-algorithm CPU restatement in "
                       "C/OpenMP",
             "total_ms": total * 1e3, "p50_ms": float(np.percentile(done, 50)),
