@@ -241,6 +241,7 @@ def main():
         SPMV32_KERNEL = "k_sell<float, float, 0, true, 8, false, true>"   # sliced ELL, u16 offsets, NT loads
         spmv_bytes = nnz * (4 + 4) + 4 * (nd + 1) + 4 * nd + 4 * nd
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    stream = same_size_stream(torch, npad)
     lat_kernel, lat_bytes, lat_gbs = lattice_roofline(mixed_lattice["unit_diagonal"], mixed_lattice["spmv_avg_ms"])
     spmv64_iso_ms = fa.spmv_bench(K, mixed=False, reps=50)
     spmv64_ms = results["fp64"]["spmv_avg_ms"]
@@ -303,7 +304,8 @@ def main():
                      "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(SPMV32_KERNEL, n),
                      "bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_ms,
-                     "launches": results["mixed"]["spmv_launches"], "isolated_avg_launch_ms": spmv_iso_ms},
+                     "launches": results["mixed"]["spmv_launches"], "isolated_avg_launch_ms": spmv_iso_ms,
+                     "same_size_stream": stream},
         # the same workload's SpMV on the lattice value planes (the mixed_lattice run)
         "roofline_lattice": {"bound": "hbm", "kernel": lat_kernel, "achieved": lat_gbs, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": lat_gbs / HBM_PEAK_GBS,
@@ -383,6 +385,34 @@ def distorted_run(args, fa, np, torch, g0, ip, cv, dh0, colors, k):
                               "frac": b32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": b32,
                               "avg_launch_ms": ms32}
     del K, f
+    return out
+
+
+def same_size_stream(torch, npad, reps=50):
+    """What a plain streaming kernel of the cell-form SpMV's size achieves on this GPU: torch's
+    addcmul y = a + b·c over FP32 vectors of npad entries (three read, one written: the SpMV's
+    t, x, k_e in and y out).  Timed with events over `reps` launches, once on one set of
+    vectors (back to back: they stay in the 256 MB Infinity Cache, like the isolated SpMV
+    timing) and once rotating over eight sets (288 MB, more than that cache holds)."""
+    sets = [[torch.rand(npad, device="cuda", dtype=torch.float32) for _ in range(4)] for _ in range(8)]
+    nbytes = 16 * npad
+    out = {"kernel": "torch.addcmul(a, b, c, out=y), FP32", "bytes_per_launch": nbytes}
+    for name, pick in (("same_vectors", lambda i: sets[0]), ("rotating_8_sets", lambda i: sets[i % 8])):
+        for i in range(8):
+            a, b, c, y = pick(i)
+            torch.addcmul(a, b, c, out=y)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for i in range(reps):
+            a, b, c, y = pick(i)
+            torch.addcmul(a, b, c, out=y)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"avg_launch_ms": ms, "GBs": nbytes / (ms * 1e-3) / 1e9}
+    del sets
+    torch.cuda.empty_cache()
     return out
 
 
