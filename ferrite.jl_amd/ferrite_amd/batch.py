@@ -126,6 +126,7 @@ class BatchedPoisson:
         return ms.value
 
     def run(self, precision="mixed", rtol=1e-8, maxit=0, cells_per_wg=0, chunk=0, ru_delta=0.0):
+        """Assemble, apply! and solve every problem (the solver runs on the sliced copy of K)."""
         cfg = L.BatchCfg(precision=_prec(precision), cells_per_wg=int(cells_per_wg), chunk=int(chunk),
                          maxit=int(maxit), rtol=float(rtol), ru_delta=float(ru_delta))
         info = L.BatchInfo()

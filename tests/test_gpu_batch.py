@@ -91,6 +91,31 @@ def test_small_batch_apply_and_solve(precision, tol):
     b.close()
 
 
+def test_batch_with_a_distorted_problem():
+    """One problem with a moved interior node (a general quadrilateral mesh) in the batch:
+    every problem still meets rtol on its own downloaded system."""
+    print("case batch-fallback", flush=True)
+    import scipy.sparse as sp
+    rng = np.random.default_rng(3)
+    b = BatchedPoisson(0)
+    for j, (ct, nel) in enumerate(SMALL[:3]):
+        k = conductivity(int(np.prod(nel)), rng)
+        g, dh, colors, ch, k = poisson_problem(ct, nel, k)
+        if j == 2:
+            inner = np.nonzero(np.all(np.abs(g.nodes) < 0.99, axis=1))[0][0]
+            g.nodes[inner] += 0.01
+        b.add(g, dh, colors, ch, k)
+    b.finalize()
+    res = b.run("mixed", rtol=1e-10)
+    assert res.converged == 3
+    for i in range(3):
+        p = b.problems[i]
+        nz, f, u = b.download(i)
+        A = sp.csc_matrix((nz, p["rowval"] - 1, p["colptr"] - 1), shape=(p["ndofs"], p["ndofs"]))
+        assert np.linalg.norm(f - A @ u) <= 1e-10 * np.linalg.norm(f) * 1.0001
+    b.close()
+
+
 def test_small_batch_error_paths():
     print("case batch-errors", flush=True)
     import ferrite_amd as fa
@@ -146,4 +171,21 @@ def test_full_batch_converges_and_is_reproducible():
         nz3, _, u3 = b.download(i)
         assert np.linalg.norm(nz3 - nz) <= 1e-5 * np.linalg.norm(nz3), i
         assert np.linalg.norm(u3 - u) <= 1e-4 * np.linalg.norm(u3), i   # FP32 element rounding
+    b.close()
+
+
+@pytest.mark.skip(reason="the batched solver's per-problem cell-coefficient operator (k_bc_cell) is "
+                         "not built: under rocprofv3's kernel trace its first solve ended with an "
+                         "unspecified launch failure, twice, with no cause found in its indexing "
+                         "(checked on the CPU over the 64-problem batch); it was removed")
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-6)])
+def test_small_batch_cell_operator(precision, tol):
+    """The per-problem cell coefficients (every problem a box lattice) against the oracle."""
+    b, ks = _small_batch()
+    res = b.run(precision, rtol=1e-11, cells_per_wg=64, chunk=128, operator="cells")
+    assert res.operator == "cells" and res.converged == len(SMALL)
+    for i, (ct, nel) in enumerate(SMALL):
+        *_, u_ref, _ = _oracle_problem(i, ct, nel, ks[i])
+        _, _, u = b.download(i)
+        assert _relerr(u, u_ref) <= (1e-8 if precision == "fp64" else 2e-5), i
     b.close()
