@@ -7,8 +7,12 @@ constraint on a dof replaces the earlier one), `PeriodicDirichlet(field, facet_p
 components)` (:1060-1294: u_mirror = u_image as affine constraints), `close!` (:303-366:
 prescribed dofs sorted, nested affine constraints rejected), `update!` (:504-583: g evaluated
 at the dof's spatial coordinate; affine constraints get the effective inhomogeneity
-b + Σ a g over Dirichlet masters).  `apply!` runs on the device (device.py).
+b + Σ a g over Dirichlet masters), `ConformityConstraint(field)` (Adaptivity/constraints.jl:
+15-101: on a NonConformingGrid, every hanging vertex dof the mean of its masters' dofs, per
+component).  `apply!` runs on the device (device.py).
 """
+import warnings
+
 import numpy as np
 
 from .refel import Lagrange, base_interpolation, n_components
@@ -29,6 +33,14 @@ class AffineConstraint:
         self.constrained_dof = int(constrained_dof)
         self.entries = [(int(j), float(a)) for j, a in entries]
         self.b = float(b)
+
+
+class ConformityConstraint:
+    """ConformityConstraint(field) (Adaptivity/constraints.jl:15-18): the affine constraints that
+    make a linear Lagrange field (scalar or vectorized) conforming on a NonConformingGrid."""
+
+    def __init__(self, field):
+        self.field = field
 
 
 class PeriodicDirichlet:
@@ -92,6 +104,8 @@ class ConstraintHandler:
             return self
         if isinstance(dbc, PeriodicDirichlet):
             return self._add_periodic(dbc)
+        if isinstance(dbc, ConformityConstraint):
+            return self._add_conformity(dbc)
         dh = self.dh
         fi = dh.field_names.index(dbc.field)
         ip = dh.field_ips[fi]
@@ -184,6 +198,38 @@ class ConstraintHandler:
             return v[comp - 1] if v.size > 1 else v[0]
         for k, v in dof_map.items():
             self.add(AffineConstraint(k, [(v, 1.0)], 0.0 if pdbc.f is None else -g(v) + g(k)))
+        return self
+
+    def _add_conformity(self, cc):
+        """add!(ch, ConformityConstraint(field)) (Adaptivity/constraints.jl:20-101): skipped with a
+        warning on a conforming grid; on a NonConformingGrid one AffineConstraint per hanging
+        node and component, u_h = Σ u_m / (number of masters), b = 0.  Only linear Lagrange
+        fields (and their vectorizations) are supported, as in the reference."""
+        from ._lib import FEM_ERR_ARG, FemArgumentError
+        dh = self.dh
+        info = getattr(dh.grid, "conformity_info", None)
+        if info is None:
+            warnings.warn(f"Trying to add conformity constraint to {cc.field} on a conforming grid. Skipping.")
+            return self
+        if cc.field not in dh.field_names:
+            raise FemArgumentError(FEM_ERR_ARG, f"Field {cc.field} not found in provided dof handler. "
+                                                f"Available fields are {dh.field_names}.")
+        fi = dh.field_names.index(cc.field)
+        ip = dh.field_ips[fi]
+        base = base_interpolation(ip)
+        if not (isinstance(base, Lagrange) and base.order == 1):
+            raise FemArgumentError(FEM_ERR_ARG, "ConformityConstraint supports only linear Lagrange interpolations "
+                                                f"(and their vectorizations), got {ip}.")
+        vdim = n_components(ip)
+        # first dof of every vertex of the field (its components follow)
+        vdof = np.zeros(dh.grid.nnodes + 1, dtype=np.int64)
+        off = dh.field_offsets[fi]
+        for a in range(dh.grid.cells.shape[1]):
+            vdof[dh.grid.cells[:, a]] = dh.cell_dofs[:, off + a * vdim]
+        for h, masters in info.items():
+            w = 1.0 / len(masters)
+            for c in range(vdim):
+                self.add(AffineConstraint(int(vdof[h]) + c, [(int(vdof[m]) + c, w) for m in masters], 0.0))
         return self
 
     def close(self):

@@ -67,6 +67,23 @@ class Grid:
         return out
 
 
+class NonConformingGrid(Grid):
+    """NonConformingGrid(cells, nodes; facetsets, conformity_info) (Adaptivity/ncgrid.jl:24-50):
+    a grid with hanging vertices, each lying on an edge (or, in 3D, a face) of a neighbouring
+    cell that does not have it as a vertex.  conformity_info maps every hanging node to its 2
+    (edge midpoint) or 4 (face centre) master nodes (1-based ids)."""
+
+    def __init__(self, celltype, cells, nodes, conformity_info, facetsets=None, nodesets=None):
+        super().__init__(celltype, cells, nodes, facetsets, nodesets)
+        info = {}
+        for h, masters in dict(conformity_info).items():
+            ms = [int(m) for m in masters]
+            if not 1 <= int(h) <= self.nnodes or len(ms) not in (2, 4) or not all(1 <= m <= self.nnodes for m in ms):
+                raise ValueError(f"conformity_info entry {h} => {masters}: a hanging node and 2 or 4 master nodes")
+            info[int(h)] = ms
+        self.conformity_info = info
+
+
 def _extrema_to_corners(rdim, left, right):
     """grid_generators.jl:565-578: corner = left + (Δx/Δξ)(ξ - ξ_left) per axis."""
     from .refel import REF_COORDS

@@ -9,7 +9,9 @@ the oracle (oracle/affine.py) and against properties the mathematics fixes.
   * periodic Poisson: u_mirror == u_image exactly, and shifting a random per-cell conductivity
     cyclically along a periodic axis shifts the solution the same way (within 1e-9), which
     homogeneous Neumann sides would not do;
-  * a pattern without the constraint entries is rejected with the reference's message.
+  * a pattern without the constraint entries is rejected with the reference's message;
+  * ConformityConstraint on a grid with hanging nodes: apply! and the solve against the oracle,
+    and the linear patch test through the hanging nodes.
 """
 import numpy as np
 import pytest
@@ -176,3 +178,42 @@ def test_pattern_without_constraint_entries_rejected():
     K, f = _assemble(g, dh, ch, None, pattern_ch=False)
     with pytest.raises(fa.FemPatternError, match=r"allocate_matrix\(dh, ch\)"):
         fa.apply(K, f, ch)
+
+
+def test_conformity_constraints_on_a_nonconforming_grid():
+    """Hanging nodes (ConformityConstraint on a 6x5 grid with three cells split in four): the
+    device apply! against the oracle's condensation of the same K, f; and the patch test a
+    conforming space passes exactly: with u = 1 + x + 2y on the boundary, no source and a
+    constant conductivity, the discrete solution is that linear function at every node,
+    hanging nodes included (each is the mean of its edge's two ends), to the solver's
+    tolerance.  Without the constraints the same solve misses the linear function by 0.25 at
+    the hanging nodes (the oracle's dense solve), so the test sees a dropped constraint."""
+    from ncgrid_util import refine_quads
+    print("case conformity constraints", flush=True)
+    g = refine_quads(fa.generate_grid("Quadrilateral", (6, 5)), [8, 10, 21])
+    assert len(g.conformity_info) == 12
+    dh = fa.DofHandler(g).add("u", fa.Lagrange(g.refshape, 1)).close()
+    bnd = g.facetsets["left"] + g.facetsets["right"] + g.facetsets["bottom"] + g.facetsets["top"]
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.Dirichlet("u", bnd, lambda x, t: 1.0 + x[0] + 2.0 * x[1]))
+    ch.add(fa.ConformityConstraint("u"))
+    ch.close()
+    # random conductivity, unit source: device apply! and solve against the oracle
+    k = np.random.default_rng(9).uniform(0.5, 2.0, g.ncells)
+    K, f = _assemble(g, dh, ch, k)
+    OK, of, oh = _check_apply(K, f, ch)
+    _solve_and_check(K, f, ch, OK, of, oh)
+    # the patch test
+    cv = fa.CellValues(fa.QuadratureRule(g.refshape, 2), dh.field_ips[0])
+    K2 = fa.allocate_matrix(fa.MI355XCSC, dh, ch)
+    f2 = fa.allocate_vector(K2)
+    fa.assemble_global(fa.start_assemble(K2, f2), cv, fa.create_coloring(g), fa.PoissonElement(source=0.0))
+    fa.apply(K2, f2, ch)
+    fa.solve(K2, f2, mode="cg", rtol=1e-14, maxit=20000)
+    u = fa.apply_solution(K2, ch)
+    nd = _node_dofs(g, dh)
+    exact = 1.0 + g.nodes[:, 0] + 2.0 * g.nodes[:, 1]
+    assert np.max(np.abs(u[nd - 1] - exact)) <= 1e-10 * np.max(np.abs(exact))
+    for h, ms in g.conformity_info.items():
+        mean = sum(u[nd[m - 1] - 1] for m in ms) / len(ms)
+        assert abs(u[nd[h - 1] - 1] - mean) <= 1e-14 * abs(mean)

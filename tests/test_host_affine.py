@@ -196,3 +196,42 @@ def test_periodic_map_of_the_reference_local_bc_case():
     dofs, ptr, cd, cv, b = ch.affine_arrays()
     assert dict(zip(dofs.tolist(), cd.tolist())) == expect
     assert np.all(cv == 1.0) and np.all(b == 0.0)
+
+
+def test_conformity_constraint_host():
+    """ConformityConstraint (Adaptivity/constraints.jl), as the reference's tests check it
+    (test/test_constraints.jl:1752-1795): a warning and nothing added on a conforming grid; on
+    a 2x2 grid with its first cell split (two hanging nodes) one constraint per hanging node and
+    component, each the mean of the two edge ends (coefficients 0.5, right-hand side 0), the
+    sibling component constrained too; only linear Lagrange fields are accepted."""
+    from ncgrid_util import refine_quads
+    g0 = fa.generate_grid("Quadrilateral", (2, 2))
+    dh = fa.DofHandler(g0).add("u", fa.Lagrange(g0.refshape, 1)).close()
+    ch = fa.ConstraintHandler(dh)
+    with pytest.warns(UserWarning, match="conforming grid"):
+        ch.add(fa.ConformityConstraint("u"))
+    ch.close()
+    assert len(ch.prescribed_dofs) == 0
+    ncg = refine_quads(g0, [1])
+    assert len(ncg.conformity_info) == 2
+    for h, ms in ncg.conformity_info.items():   # hanging nodes are edge midpoints
+        assert np.allclose(ncg.nodes[h - 1], ncg.nodes[np.array(ms) - 1].mean(axis=0))
+    dh2 = fa.DofHandler(ncg).add("u", fa.Lagrange(ncg.refshape, 2)).close()
+    with pytest.raises(fa.FemArgumentError, match="only linear Lagrange"):
+        fa.ConstraintHandler(dh2).add(fa.ConformityConstraint("u"))
+    dh1 = fa.DofHandler(ncg).add("u", fa.Lagrange(ncg.refshape, 1)).close()
+    with pytest.raises(fa.FemArgumentError, match="not found"):
+        fa.ConstraintHandler(dh1).add(fa.ConformityConstraint("v"))
+    chs = fa.ConstraintHandler(dh1).add(fa.ConformityConstraint("u")).close()
+    assert len(chs.prescribed_dofs) == 2
+    dhv = fa.DofHandler(ncg).add("u", fa.Lagrange(ncg.refshape, 1) ** 2).close()
+    chv = fa.ConstraintHandler(dhv).add(fa.ConformityConstraint("u")).close()
+    assert len(chv.prescribed_dofs) == 4
+    pdofs = set(int(d) for d in chv.prescribed_dofs)
+    dofs, ptr, cd, cv, b = chv.affine_arrays()
+    for i, d in enumerate(dofs):
+        assert ptr[i + 1] - ptr[i] == 2 and np.all(cv[ptr[i]:ptr[i + 1]] == 0.5) and b[i] == 0.0
+        partner = d + 1 if d % 2 == 1 else d - 1
+        assert partner in pdofs
+        # masters are the same component of the edge ends
+        assert np.all((cd[ptr[i]:ptr[i + 1]] - d) % 2 == 0)
