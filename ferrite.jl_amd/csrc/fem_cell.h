@@ -75,6 +75,14 @@ __device__ __forceinline__ X buf_load(__amdgpu_buffer_rsrc_t r, uint32_t boff) {
   else return __longlong_as_double((long long)__builtin_amdgcn_raw_buffer_load_b64(r, (int)boff, 0, 0));
 }
 
+// Store of one value at a byte offset; at or beyond the resource's extent it is dropped.
+template <typename X>
+__device__ __forceinline__ void buf_store(X v, __amdgpu_buffer_rsrc_t r, uint32_t boff) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  if constexpr (sizeof(X) == 4) __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)boff, 0, 0);
+  else __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)boff, 0, 0);
+}
+
 // The resource for the rows [row0, npad) of a vector (none when row0 is outside [0, npad)).
 // 32-bit row arithmetic (npad · sizeof(X) < 2³¹), which stays on the scalar unit.
 template <typename X>

@@ -970,24 +970,32 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv(int64_t r_lo, int64_t nrow
 }
 
 // 3D cell form marching in z: work item (j, c) = in-plane tile j (T = RPT kBlock consecutive
-// rows of a plane) over the planes [z0, z1) of chunk c.  The block keeps a ring of three t·x
-// windows (planes z − 1, z, z + 1; T + 2 px + 2 entries each, ZN per thread) and two
-// coefficient windows (cell layers z, z − 1; T + px + 1 entries, KN per thread) in LDS and
-// loads one new window of each per plane: step p brings the t·x window of plane p, the
-// coefficients of layer p − 1 and t, x of the rows of plane p − 1.  D register stages hold
-// the loads of D steps, issued D planes ahead of their use.  Every load goes through a buffer
-// resource made per step from the step's first row (scalar work): a thread's offsets into it
-// are fixed for the whole launch, and rows past the vector's end (steps past the last plane)
-// read 0 — the padding's value — with no compare or select.  A thread computes RPT rows
-// kBlock apart (lanes read consecutive LDS words: no bank conflicts).  The LDS windows are
-// padded to ZN (KN) kBlock entries, so every thread stores all its entries.  The step loop is
-// straight-line code (the rows of a plane go to an LDS buffer and are written once per item;
-// the planes per chunk are rounded up to a multiple of D, the extra steps' rows discarded)
-// and its barriers order LDS only, so the loads of later steps stay in flight across them.
-// Padding rows of the marched planes are computed too (t = 0 there, so y = x = 0).  Items are
-// ordered chunk-major; MAP 1: the blocks of one XCD sweep one contiguous eighth of the items,
-// so neighbouring tiles (which share their halo rows) run on the same XCD at the same time.
-// The host keeps npad · sizeof(X) below 2³¹ (the resources' 32-bit extent).
+// rows of a plane) over the planes [z0, z1) of chunk c.  Step p (p = z0 .. z1 − 1) computes
+// the rows of plane p; it brings the t·x window of plane p + 1 (T + 2 px + 2 entries, ZN per
+// thread) and the coefficient window of cell layer p (T + px + 1 entries, KN per thread)
+// into LDS, and each thread reads from them only what is new to its rows: the 9 in-plane
+// neighbours in plane p + 1 and the 4 cells around it in layer p.  The neighbours in planes
+// p − 1 and p and the cells of layer p − 1 are still in its registers from the two steps
+// before (a thread keeps the same rows of every plane), so a row reads 13 LDS words per
+// step instead of 35.  Register planes rotate with the step's index mod 3 (layers mod 2)
+// over an unrolled group of six steps, so no values move; the LDS windows are double
+// buffered (slot = plane − z0 + 1 mod 2, fixed per unrolled step): one barrier per step
+// orders a window's stores after the reads of the window two steps back.  D register
+// stages hold the global loads of D steps, issued D planes ahead of their use; an item
+// issues its first windows and its first D steps together.  A step's loads go through
+// three buffer resources (t, x, coefficients) made from one row, the window's first
+// coefficient row: a thread's byte offsets into them are fixed for the whole launch, rows
+// outside the vector read 0 — the padding's value — with no compare or select, and a step
+// past the item's last plane gets empty resources (its loads read 0 and move no data).
+// Rows are stored through a resource that ends at the plane's end, so a tile's rows past it
+// are dropped.  A thread computes RPT rows kBlock apart (lanes read consecutive LDS words: no
+// bank conflicts).  The LDS windows are padded to ZN (KN) kBlock entries, so every thread
+// stores all its entries; the barriers order LDS only, so the loads of later steps stay in
+// flight across them.  Padding rows of the marched planes are computed too (t = 0 there, so
+// y = x = 0).  Items are ordered chunk-major; MAP 1: the blocks of one XCD sweep one
+// contiguous eighth of the items, so neighbouring tiles (which share their halo rows) run on
+// the same XCD at the same time.  The host keeps npad · sizeof(X) below 2³¹ (the resources'
+// 32-bit extent).
 template <typename X, int MAP, int ZN, int KN, int D, int RPT>
 __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px, int32_t pxy, int32_t pz,
                                                         int32_t ntx, int32_t lz, int64_t nitems,
@@ -995,30 +1003,33 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
                                                         const X* __restrict__ x, X* __restrict__ y,
                                                         double* __restrict__ part, const Scal* __restrict__ s,
                                                         CellKrefT<X> kr) {
+  static_assert(6 % D == 0, "the unrolled group of six steps holds whole stage cycles");
   if (s->done || s->fresh) return;
   extern __shared__ __align__(16) unsigned char cell_lds[];
   __shared__ double sh[kBlock / 64];
   constexpr int T = RPT * kBlock;            // rows of a tile: RPT per thread, kBlock apart
   constexpr int WZ = ZN * kBlock, WKP = KN * kBlock;   // padded window lengths (≥ the used ones)
   const int W = T + 2 * px + 2, WK = T + px + 1;      // W ≤ WZ, WK ≤ WKP (host)
-  const int nsteps = (lz + D - 1) / D * D;
-  X* zw = reinterpret_cast<X*>(cell_lds);   // [3][WZ]
-  X* kw = zw + 3 * WZ;                       // [2][WKP]
-  X* yb = kw + 2 * WKP;                      // [nsteps][T] rows of the item's planes
-  // byte offsets of this thread's window entries from the window's first row (entries past a
-  // window: an offset no resource reaches)
-  uint32_t oz[ZN], ok[KN];
+  X* zw = reinterpret_cast<X*>(cell_lds);   // [2][WZ]
+  X* kw = zw + 2 * WZ;                       // [2][WKP]
+  const int32_t np = (int32_t)npad;
+  // byte offsets from a step's base row B = (p − 1) pxy + o0 − px − 1 (the first row of the
+  // coefficient window of layer p − 1): the t·x window of plane p starts pxy rows after it,
+  // this thread's rows of plane p − 1 px + 1 rows after it (entries past a window: an offset
+  // no resource reaches)
+  uint32_t oz[ZN], ok[KN], orw[RPT];
 #pragma unroll
   for (int k = 0; k < ZN; ++k) {
     const int idx = threadIdx.x + k * kBlock;
-    oz[k] = idx < W ? (uint32_t)idx * (uint32_t)sizeof(X) : 0x80000000u;
+    oz[k] = idx < W ? (uint32_t)(pxy + idx) * (uint32_t)sizeof(X) : 0x80000000u;
   }
 #pragma unroll
   for (int k = 0; k < KN; ++k) {
     const int idx = threadIdx.x + k * kBlock;
     ok[k] = idx < WK ? (uint32_t)idx * (uint32_t)sizeof(X) : 0x80000000u;
   }
-  const uint32_t orow = threadIdx.x * (uint32_t)sizeof(X);   // this thread's first row
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) orw[r] = (uint32_t)(px + 1 + r * kBlock + threadIdx.x) * (uint32_t)sizeof(X);
   int64_t i0, i1, istep;
   if (MAP == 1) {
     const int64_t xcd = blockIdx.x % 8;
@@ -1031,15 +1042,14 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
     istep = gridDim.x;
   }
   X vt[D][ZN], vx[D][ZN], vk[D][KN], tn[D][RPT], xn[D][RPT];
-  // step p of tile offset o0 into stage u: the t·x window of plane p, the coefficients and
-  // this thread's t, x of plane p − 1.  A window starts px + 1 rows before its tile; it can
-  // start before row 0 only in plane 0 (t·x) or layer 0 (coefficients), where every entry a
-  // computed row reads is padding, i.e. 0, and then the whole window reads 0.
-  auto fetch = [&](int u, int32_t p, int32_t o0) {
-    const int32_t b0 = p * pxy + o0, b1 = b0 - pxy, np = (int32_t)npad;
-    const __amdgpu_buffer_rsrc_t rt = rows_from(t, b0 - px - 1, np), rx = rows_from(x, b0 - px - 1, np);
-    const __amdgpu_buffer_rsrc_t rk = rows_from(kc, b1 - px - 1, np);
-    const __amdgpu_buffer_rsrc_t rt1 = rows_from(t, b1, np), rx1 = rows_from(x, b1, np);
+  // step p − 1 of tile offset o0 into stage u: the t·x window of plane p, the coefficients of
+  // layer p − 1 and this thread's t, x of plane p − 1; nothing is read for p > z1 (the last
+  // plane an item's rows read).  For 2 ≤ p ≤ pz − 1, 0 ≤ B < npad.
+  auto fetch = [&](int u, int32_t p, int32_t o0, int32_t z1) {
+    const bool on = p <= z1;
+    const int32_t B = on ? (p - 1) * pxy + o0 - px - 1 : 0;
+    const uint32_t ext = on ? (uint32_t)(np - B) * (uint32_t)sizeof(X) : 0u;
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(t + B, ext), rx = buf_rsrc(x + B, ext), rk = buf_rsrc(kc + B, ext);
 #pragma unroll
     for (int k = 0; k < ZN; ++k) {
       vt[u][k] = buf_load<X>(rt, oz[k]);
@@ -1049,82 +1059,119 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
     for (int k = 0; k < KN; ++k) vk[u][k] = buf_load<X>(rk, ok[k]);
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-      tn[u][r] = buf_load<X>(rt1, orow + r * (uint32_t)(kBlock * sizeof(X)));
-      xn[u][r] = buf_load<X>(rx1, orow + r * (uint32_t)(kBlock * sizeof(X)));
+      tn[u][r] = buf_load<X>(rt, orw[r]);
+      xn[u][r] = buf_load<X>(rx, orw[r]);
     }
   };
-  auto store = [&](int u, int32_t p) {   // z slot p mod 3, coefficient slot (p − 1) mod 2
-    X* zs = zw + (p % 3) * WZ;
-    X* ks = kw + ((p - 1) & 1) * WKP;
+  // a row's 9 in-plane neighbours (dy, dx row-major) from a t·x window; the 4 cells around it
+  // (origin P − e, e = ex + 2 ey) from a coefficient window; l = the row's index in the windows
+  auto plane9 = [&](X* v, const X* zs, int l) {
 #pragma unroll
-    for (int k = 0; k < ZN; ++k) zs[threadIdx.x + k * kBlock] = vt[u][k] * vx[u][k];
+    for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-    for (int k = 0; k < KN; ++k) ks[threadIdx.x + k * kBlock] = vk[u][k];
+      for (int dx = 0; dx < 3; ++dx) v[dy * 3 + dx] = zs[l + (dx - 1) + px * (dy - 1)];
+  };
+  auto layer4 = [&](X* v, const X* ks, int l) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = ks[l - (e & 1) - px * ((e >> 1) & 1)];
   };
   double d = 0.0;
   for (int64_t item = i0; item < i1; item += istep) {
     const int32_t c = (int32_t)(item / ntx), j = (int32_t)(item - (int64_t)c * ntx);
-    const int32_t z0 = 1 + c * lz, z1 = min(z0 + lz, pz - 1);
+    const int32_t z0 = 1 + c * lz, z1 = min(z0 + lz, pz - 1), nl = z1 - z0;
     const int32_t o0 = j * T;
-    fetch(0, z0 - 1, o0);
+    // the item's first windows (t·x of planes z0 − 1 and z0, coefficients of layer z0 − 1)
+    // and its first D steps are issued together, so the item waits on one load latency
+    // before its first plane.  Plane z0 − 1 may be plane 0, whose window can start before
+    // row 0: every entry a computed row reads there is padding, and the whole window reads 0.
+    X wt0[ZN], wx0[ZN], wt1[ZN], wx1[ZN], wk1[KN];
+    {
+      const int32_t b0 = (z0 - 1) * pxy + o0 - px - 1;
+      const __amdgpu_buffer_rsrc_t rt0 = rows_from(t, b0, np), rx0 = rows_from(x, b0, np);
+      const __amdgpu_buffer_rsrc_t rt1 = rows_from(t, b0 + pxy, np), rx1 = rows_from(x, b0 + pxy, np);
+      const __amdgpu_buffer_rsrc_t rk1 = rows_from(kc, b0, np);
+#pragma unroll
+      for (int k = 0; k < ZN; ++k) {
+        const int idx = threadIdx.x + k * kBlock;   // from the window's first row
+        const uint32_t o = idx < W ? (uint32_t)idx * (uint32_t)sizeof(X) : 0x80000000u;
+        wt0[k] = buf_load<X>(rt0, o);
+        wx0[k] = buf_load<X>(rx0, o);
+        wt1[k] = buf_load<X>(rt1, o);
+        wx1[k] = buf_load<X>(rx1, o);
+      }
+#pragma unroll
+      for (int k = 0; k < KN; ++k) wk1[k] = buf_load<X>(rk1, ok[k]);
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) fetch(u, z0 + 1 + u, o0, z1);
     lds_barrier();     // the previous item's LDS reads are done
-    store(0, z0 - 1);
-    fetch(0, z0, o0);
-    store(0, z0);
 #pragma unroll
-    for (int u = 0; u < D; ++u) fetch(u, z0 + 1 + u, o0);
-    for (int32_t sb = 0; sb < nsteps; sb += D) {
+    for (int k = 0; k < ZN; ++k) {
+      zw[threadIdx.x + k * kBlock] = wt0[k] * wx0[k];        // slot 0: plane z0 − 1
+      zw[WZ + threadIdx.x + k * kBlock] = wt1[k] * wx1[k];   // slot 1: plane z0
+    }
 #pragma unroll
-      for (int u = 0; u < D; ++u) {
+    for (int k = 0; k < KN; ++k) kw[threadIdx.x + k * kBlock] = wk1[k];   // slot 0: layer z0 − 1
+    lds_barrier();
+    // register planes: slot (p − z0 + 1) mod 3 holds plane p, layer slot (q − z0 + 1) mod 2 layer q
+    X zr[3][RPT][9], kq[2][RPT][4];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int l = r * kBlock + threadIdx.x + px + 1;   // the row in the windows
+      plane9(zr[0][r], zw, l);
+      plane9(zr[1][r], zw + WZ, l);
+      layer4(kq[0][r], kw, l);
+    }
+    bool col[RPT];     // the row lies in the plane (the last tile of a plane is cut)
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) col[r] = o0 + r * kBlock + (int32_t)threadIdx.x < pxy;
+    const uint32_t yext = (uint32_t)(pxy - o0) * (uint32_t)sizeof(X);
+    lds_barrier();     // the first step's store overwrites the window of plane z0 − 1
+    for (int32_t sb = 0; sb < nl; sb += 6) {
+#pragma unroll
+      for (int u = 0; u < 6; ++u) {
+        if (sb + u >= nl) break;
+        const int su = u % D;
         const int32_t zp = z0 + sb + u;
         X tc[RPT], xc[RPT];   // rows of plane zp
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
-          tc[r] = tn[u][r];
-          xc[r] = xn[u][r];
+          tc[r] = tn[su][r];
+          xc[r] = xn[su][r];
         }
-        store(u, zp + 1);
+        X* zs = zw + (u & 1) * WZ;              // plane zp + 1: slot (sb + u + 2) mod 2
+        X* ks = kw + ((u + 1) & 1) * WKP;       // layer zp: slot (sb + u + 1) mod 2
+#pragma unroll
+        for (int k = 0; k < ZN; ++k) zs[threadIdx.x + k * kBlock] = vt[su][k] * vx[su][k];
+#pragma unroll
+        for (int k = 0; k < KN; ++k) ks[threadIdx.x + k * kBlock] = vk[su][k];
         lds_barrier();
-        fetch(u, zp + 1 + D, o0);
-        const X* zm = zw + ((zp + 2) % 3) * WZ;   // plane zp − 1
-        const X* z0w = zw + (zp % 3) * WZ;
-        const X* zpw = zw + ((zp + 1) % 3) * WZ;
-        const X* k0 = kw + (zp & 1) * WKP;         // layer zp
-        const X* k1 = kw + ((zp - 1) & 1) * WKP;   // layer zp − 1
-        const bool live = zp < z1;
+        fetch(su, zp + 1 + D, o0, z1);
+        const __amdgpu_buffer_rsrc_t ry = buf_rsrc(y + (zp * pxy + o0), yext);
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
-          const int l = r * kBlock + threadIdx.x + px + 1;   // the row in the windows
-          X zn[27];
+          const int l = r * kBlock + threadIdx.x + px + 1;
+          plane9(zr[(u + 2) % 3][r], zs, l);
+          layer4(kq[(u + 1) % 2][r], ks, l);
+          X zn[27], kn[8];   // (dz, dy, dx) row-major; cell with origin P − e: P is its corner e
 #pragma unroll
-          for (int dy = 0; dy < 3; ++dy)
+          for (int i = 0; i < 9; ++i) {
+            zn[i] = zr[u % 3][r][i];
+            zn[9 + i] = zr[(u + 1) % 3][r][i];
+            zn[18 + i] = zr[(u + 2) % 3][r][i];
+          }
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-              const int o = l + (dx - 1) + px * (dy - 1);
-              zn[dy * 3 + dx] = zm[o];
-              zn[9 + dy * 3 + dx] = z0w[o];
-              zn[18 + dy * 3 + dx] = zpw[o];
-            }
-          X kn[8];                                  // cell with origin P − e: P is its corner e
-#pragma unroll
-          for (int e = 0; e < 8; ++e) kn[e] = (e >> 2 ? k1 : k0)[l - (e & 1) - px * ((e >> 1) & 1)];
+          for (int e = 0; e < 4; ++e) {
+            kn[e] = kq[(u + 1) % 2][r][e];   // layer zp
+            kn[4 + e] = kq[u % 2][r][e];     // layer zp − 1
+          }
           const X acc = cell_row<X, 3>(zn, kn, kr.f);
           const X yv = tc[r] != X(0) ? tc[r] * acc : xc[r];
-          yb[(sb + u) * T + r * kBlock + threadIdx.x] = yv;
-          const bool col = o0 + r * kBlock + (int32_t)threadIdx.x < pxy;
-          d += (col && live) ? (double)(xc[r] * yv) : 0.0;
+          buf_store<X>(yv, ry, (uint32_t)(r * kBlock + threadIdx.x) * (uint32_t)sizeof(X));
+          d += col[r] ? (double)(xc[r] * yv) : 0.0;
         }
-        lds_barrier();     // this plane's reads are done before the next store
       }
     }
-#pragma unroll
-    for (int r = 0; r < RPT; ++r)
-      if (o0 + r * kBlock + (int32_t)threadIdx.x < pxy)
-        for (int32_t pl = 0; pl < z1 - z0; ++pl) {
-          const uint32_t P = (uint32_t)(z0 + pl) * (uint32_t)pxy + (uint32_t)(o0 + r * kBlock + threadIdx.x);
-          *reinterpret_cast<X*>(reinterpret_cast<char*>(y) + P * (uint32_t)sizeof(X)) =
-              yb[pl * T + r * kBlock + threadIdx.x];
-        }
   }
   const double b = block_sum(d, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
@@ -1684,14 +1731,14 @@ static bool cell_kref(const fem_ctx* c, const double (*x0)[3], int dim, CellKref
 }
 
 // Steps in flight (register stages) of the z-marching cell SpMV: three for FP32 with one row
-// per thread, otherwise two (register pressure).
+// per thread, otherwise two (register pressure; at 128³ three or more stages measured no
+// faster with two rows per thread: the loads of two steps ahead already arrive in time).
 constexpr int cell_stages(int bytes, int rpt) { return bytes == 4 && rpt == 1 ? 3 : 2; }
 
 // Launch shape of the cell-form SpMV for values of `bytes` bytes.  Measured at 128³ (in the
-// solve, z-marching, XCD ranges): FP32 two rows per thread 512 blocks 20.6 µs, 768 17.7, 1024
-// 23.8, 1536 22.9; one row per thread 512 32.2, 768 19.5, 1024 22.1, 1536 20.8; FP64 (one row)
-// 512 37.5, 768 32.3, 1024 33.9, 1536 34.0.  Z-marching: about one work item per block (the
-// planes per chunk follow from the block count).
+// solve, z-marching, XCD ranges): FP32 two rows per thread 768 blocks 12.2 µs, 544 12.3; one
+// row per thread 768 12.7, 1024 13.0.  Z-marching: about one work item per block (the planes
+// per chunk follow from the block count).
 static void cell_shape(const fem_ctx* c, const LatOp& l, CellOp* cl, int bytes) {
   const int64_t want = c->tune.cell_blocks ? c->tune.cell_blocks : 768;
   const int pi = bytes == 4 ? 0 : 1;
@@ -1715,14 +1762,12 @@ static void cell_shape(const fem_ctx* c, const LatOp& l, CellOp* cl, int bytes) 
   }
   if (cl->zn[pi] > 0) {
     const int64_t pz = l.npad / l.pxy;
-    const int D = cell_stages(bytes, cl->rpt[pi]);   // register stages of the kernel
-    const int64_t T = cl->rpt[pi] * kBlock;
     int& lz = bytes == 4 ? cl->lz32 : cl->lz64;
     int64_t& nitems = bytes == 4 ? cl->nitems32 : cl->nitems64;
     lz = (int)std::min<int64_t>(16, std::max<int64_t>(2, (cl->ntx[pi] * (pz - 2) + want - 1) / want));
     nitems = (int64_t)cl->ntx[pi] * ((pz - 2 + lz - 1) / lz);
     grid = (int)std::max<int64_t>(8, std::min<int64_t>(want, (nitems + 7) / 8 * 8));
-    const int64_t words = (3 * cl->zn[pi] + 2 * cl->kn[pi]) * (int64_t)kBlock + (lz + D - 1) / D * D * T;
+    const int64_t words = (2 * cl->zn[pi] + 2 * cl->kn[pi]) * (int64_t)kBlock;   // the windows
     (bytes == 4 ? cl->ldsz32 : cl->ldsz64) = (int)(bytes * words);
     // the (padded) windows and the staged rows must fit one block's LDS, and the vectors the
     // 32-bit extent of a buffer resource; otherwise the tiles
