@@ -10,8 +10,9 @@ the oracle (oracle/affine.py) and against properties the mathematics fixes.
     cyclically along a periodic axis shifts the solution the same way (within 1e-9), which
     homogeneous Neumann sides would not do;
   * a pattern without the constraint entries is rejected with the reference's message;
-  * ConformityConstraint on a grid with hanging nodes: apply! and the solve against the oracle,
-    and the linear patch test through the hanging nodes.
+  * ConformityConstraint on quadrilateral and hexahedral grids with hanging nodes (2 and 4
+    masters): apply! and the solve against the oracle, and the linear patch test through the
+    hanging nodes.
 """
 import numpy as np
 import pytest
@@ -213,6 +214,42 @@ def test_conformity_constraints_on_a_nonconforming_grid():
     u = fa.apply_solution(K2, ch)
     nd = _node_dofs(g, dh)
     exact = 1.0 + g.nodes[:, 0] + 2.0 * g.nodes[:, 1]
+    assert np.max(np.abs(u[nd - 1] - exact)) <= 1e-10 * np.max(np.abs(exact))
+    for h, ms in g.conformity_info.items():
+        mean = sum(u[nd[m - 1] - 1] for m in ms) / len(ms)
+        assert abs(u[nd[h - 1] - 1] - mean) <= 1e-14 * abs(mean)
+
+
+def test_conformity_constraints_hexahedra():
+    """Hanging nodes in 3D: a 5x4x4 hex grid with three interior cells split in eight (two of
+    them face neighbours, so their shared face is regular); edge midpoints take 2 masters and
+    face centres 4.  apply! and the solve against the oracle, and the linear patch test
+    u = 1 + x + 2y + 3z through the hanging nodes (without the constraints the oracle's dense
+    solve misses it by 1.2)."""
+    from ncgrid_util import refine_hexes
+    print("case conformity constraints, hexahedra", flush=True)
+    g = refine_hexes(fa.generate_grid("Hexahedron", (5, 4, 4)), [27, 28, 54])
+    nm = [len(v) for v in g.conformity_info.values()]
+    assert len(nm) == 48 and nm.count(4) == 16
+    dh = fa.DofHandler(g).add("u", fa.Lagrange(g.refshape, 1)).close()
+    bnd = [fc for n in ("left", "right", "front", "back", "bottom", "top") for fc in g.facetsets[n]]
+    ch = fa.ConstraintHandler(dh)
+    ch.add(fa.Dirichlet("u", bnd, lambda x, t: 1.0 + x[0] + 2.0 * x[1] + 3.0 * x[2]))
+    ch.add(fa.ConformityConstraint("u"))
+    ch.close()
+    k = np.random.default_rng(11).uniform(0.5, 2.0, g.ncells)
+    K, f = _assemble(g, dh, ch, k)
+    OK, of, oh = _check_apply(K, f, ch)
+    _solve_and_check(K, f, ch, OK, of, oh)
+    cv = fa.CellValues(fa.QuadratureRule(g.refshape, 2), dh.field_ips[0])
+    K2 = fa.allocate_matrix(fa.MI355XCSC, dh, ch)
+    f2 = fa.allocate_vector(K2)
+    fa.assemble_global(fa.start_assemble(K2, f2), cv, fa.create_coloring(g), fa.PoissonElement(source=0.0))
+    fa.apply(K2, f2, ch)
+    fa.solve(K2, f2, mode="cg", rtol=1e-14, maxit=20000)
+    u = fa.apply_solution(K2, ch)
+    nd = _node_dofs(g, dh)
+    exact = 1.0 + g.nodes[:, 0] + 2.0 * g.nodes[:, 1] + 3.0 * g.nodes[:, 2]
     assert np.max(np.abs(u[nd - 1] - exact)) <= 1e-10 * np.max(np.abs(exact))
     for h, ms in g.conformity_info.items():
         mean = sum(u[nd[m - 1] - 1] for m in ms) / len(ms)
