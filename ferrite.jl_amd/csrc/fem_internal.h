@@ -60,6 +60,10 @@ struct Scal {
   int fresh;      // p, s = Â p and α were just set by a reseed: the next step only moves x̂, r̂
   int sfresh;     // (record written by k_cg_scal) the step k_cg_move runs now is such a step
   int pad;
+  // x̂ moves every other step: k_cg_move adds xa0 p_old + xa1 p_new (both 0: x̂ is not touched
+  // and α p of this step is pending); pend = the α whose step's p is not yet in x̂ (k_fold
+  // adds it)
+  double xa0, xa1, pend;
 };
 
 }  // namespace fem
