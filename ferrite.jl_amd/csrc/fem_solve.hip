@@ -784,7 +784,9 @@ __device__ __forceinline__ void lds_barrier() {
 // than tol, cnt[1] lattice-neighbour entries in the pattern, cnt[2] lattice neighbours of
 // the rows (the cell form couples every two corners of a cell, so each must be a pattern
 // entry), cnt[3] entries that differ by more than tolr (the residual's bar); other entries
-// must hold zero.
+// must hold zero.  It runs once K¹ has passed cell_classes, so an entry's expected value is
+// f[class] times the sum of the coefficients of the cells that hold both corners (selects,
+// no table lookups).
 constexpr int kCheckLanes = 8;
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nrows, const int32_t* __restrict__ dofp,
@@ -802,6 +804,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nro
   const int64_t groups = (int64_t)gridDim.x * (kBlock / kCheckLanes);
   const int64_t px = g.px, pxy = g.pxy;
   double bad = 0.0, badr = 0.0, seen = 0.0, want = 0.0;
+  // K¹ is in its class form (K¹[a][b] = f[a ⊕ b], set by cell_classes): f = the row of corner 0
+  double f8[NB];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) f8[m] = kr.k[cell_lid(m & 1, (m >> 1) & 1, m >> 2)];
   for (int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kCheckLanes; q < nrows; q += groups) {
     const int64_t P = r_lo + q;
     const int32_t j = dofp[P];
@@ -844,14 +850,20 @@ __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nro
         if (pj || (isc && isc[i])) {
           m = i == j ? 1.0 : 0.0;
         } else {
+          // the cells with origin P − e that hold both corners (e_a = 0 where d_a = +1, 1 where
+          // d_a = −1), all with the class value f[|dx| + 2 |dy| + 4 |dz|]
+          double ks = 0.0;
 #pragma unroll
-          for (int e = 0; e < NB; ++e) {   // the cells with origin P − e that hold both corners
+          for (int e = 0; e < NB; ++e) {
             const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
-            const int bx = ex + dx, by = ey + dy, bz = ez + dz;
-            const bool both = bx >= 0 && bx <= 1 && by >= 0 && by <= 1 && bz >= 0 && bz <= 1;
-            m += both ? k8[e] * kr.k[cell_lid(ex, ey, ez) * NB + cell_lid(bx & 1, by & 1, bz & 1)] : 0.0;
+            const bool both = (dx == 0 || ex == (dx < 0)) && (dy == 0 || ey == (dy < 0)) && (dz == 0 || ez == (dz < 0));
+            ks += both ? k8[e] : 0.0;
           }
-          m *= sj * sc[i];
+          const int cls = (dx != 0) + 2 * (dy != 0) + 4 * (dz != 0);
+          double fc = f8[0];
+#pragma unroll
+          for (int c = 1; c < NB; ++c) fc = cls == c ? f8[c] : fc;
+          m = fc * ks * (sj * sc[i]);
         }
         if (!(fabs(av - m) <= tol)) bad += 1.0;
         if (!(fabs(av - m) <= tolr)) badr += 1.0;
