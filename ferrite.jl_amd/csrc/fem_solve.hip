@@ -177,6 +177,7 @@ struct SolverWork {
   double* part_v = nullptr;    // vector-kernel partials, two buffers of 2 per block
   Scal* s = nullptr;           // two scalar records (read one, write the other)
   hipEvent_t tev[2 * kBatch] = {};   // SpMV start/end events (time_kernels)
+  Scal* hscal = nullptr;       // pinned host copy of scalar record 0 (read_scal)
   GraphCache g64, g32;
 };
 
@@ -192,6 +193,7 @@ void free_solver(fem_ctx* c) {
   destroy_graph(w->g32);
   for (hipEvent_t& e : w->tev)
     if (e) (void)hipEventDestroy(e);
+  if (w->hscal) (void)hipHostFree(w->hscal);
   dev_free(&w->x); dev_free(&w->r); dev_free(&w->wv); dev_free(&w->p); dev_free(&w->sv);
   dev_free(&w->x32); dev_free(&w->r32); dev_free(&w->wv32); dev_free(&w->p32); dev_free(&w->sv32);
   dev_free(&w->sc); dev_free(&w->res);
@@ -1476,6 +1478,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(int m, const double* __rest
   }
 }
 
+// After a fold: the pending x̂ term is in u now (the reseed clears it too; a solve that stops
+// after a hand-over folds once more).
+__global__ void k_cg_unpend(Scal* __restrict__ s) { s->pend = 0.0; }
+
 // Both scalar records cleared; record 0 starts from ||f||² as its (true) residual.
 __global__ void k_cg_init_scal(Scal* __restrict__ s, double rr0) {
   Scal z{};
@@ -1556,6 +1562,7 @@ static int ensure_work(fem_ctx* c, bool mixed) {
         (rc = dev_alloc(c, &w->part_s, std::max(w->sh64.blocks, kMaxVecBlocks))) ||
         (rc = dev_alloc(c, &w->part_v, 4 * std::max(kMaxVecBlocks, kCgBlocks))) || (rc = dev_alloc(c, &w->s, 2)))
       return rc;
+    if (!w->hscal) FEM_HIP(c, hipHostMalloc((void**)&w->hscal, sizeof(Scal), hipHostMallocDefault));
     w->n = n;
   }
   (void)mixed;
@@ -2204,8 +2211,9 @@ static void add_spmv_times(fem_ctx* c, SolverWork* w, int first, int count) {
 }
 
 static int read_scal(fem_ctx* c, SolverWork* w, Scal* h) {   // scalar record 0 (the current one)
-  FEM_HIP(c, hipMemcpyAsync(h, w->s, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
+  FEM_HIP(c, hipMemcpyAsync(w->hscal, w->s, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
   FEM_HIP(c, hipStreamSynchronize(c->stream));
+  *h = *w->hscal;
   return FEM_OK;
 }
 
@@ -2374,6 +2382,7 @@ static int cg_run(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double bnorm2, 
     fresh_start = true;
     ++nho;
     k_fold<X><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, v.x, v.p, w->s, w->sc, c->d_u, v.map);
+    k_cg_unpend<<<1, 1, 0, c->stream>>>(w->s);
     residual(c, w);
     Scal ht{};
     if ((rc = read_scal(c, w, &ht))) return rc;
@@ -2385,6 +2394,7 @@ static int cg_run(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double bnorm2, 
   }
   if (!converged) {   // fold what the recurrence holds and leave the true residual in s->rr
     k_fold<X><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, v.x, v.p, w->s, w->sc, c->d_u, v.map);
+    k_cg_unpend<<<1, 1, 0, c->stream>>>(w->s);
     residual(c, w);
   }
   FEM_HIP(c, hipEventRecord(c->ev1, c->stream));

@@ -327,3 +327,31 @@ def test_cell_operator_after_coordinate_reupload():
     assert np.linalg.norm(r2.u - ref.u) / np.linalg.norm(ref.u) < 1e-8
     # the scaled problem's solution is a different one (u scales with h²)
     assert np.linalg.norm(r2.u - r1.u) > 1e-3 * np.linalg.norm(r1.u)
+
+
+def test_graph_replays_match_direct_launches():
+    """The solver replays captured graphs of its iterations; with timed SpMVs it launches the
+    same kernels directly.  Both must give the same iterates bit for bit, through the
+    hand-overs of the mixed solver (and of the FP64 solver on the sliced copy).  A solve asked
+    for more than the arithmetic attains stops on its stall rule with the u it reached: the
+    step left pending at the last hand-over is folded into u once, not again at the exit."""
+    print("case graph replays vs direct launches 20x16x12", flush=True)
+    nel = (20, 16, 12)
+    k = poisson_conductivity(nel, seed=3)
+    g, dh, K, f = _system("Hexahedron", nel, (1.0, 0.8, 0.6), k, cell_operator=0)
+    for mode in ("mixed", "cg"):
+        out = []
+        for timed in (0, 1):
+            fa.set_tuning(K, time_kernels=timed)
+            r = fa.solve(K, f, mode=mode, rtol=1e-12)
+            out.append(r)
+        fa.set_tuning(K, time_kernels=0)
+        a, b = out
+        assert a.relres <= 1e-12 and a.iterations == b.iterations, mode
+        assert mode == "cg" or a.outer_iterations > 1
+        assert np.array_equal(a.u, b.u), mode
+    ref = fa.solve(K, f, mode="cg", rtol=1e-13).u
+    with pytest.raises(fa.FemConvergenceError):
+        fa.solve(K, f, mode="mixed", rtol=1e-18)
+    u = fa.device.DeviceVector(K, 2).to_numpy()
+    assert _rel(u, ref) < 1e-10
