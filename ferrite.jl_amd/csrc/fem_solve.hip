@@ -1021,26 +1021,31 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
   if (s->done || s->fresh) return;
   extern __shared__ __align__(16) unsigned char cell_lds[];
   __shared__ double sh[kBlock / 64];
+  typedef unsigned int U4 __attribute__((ext_vector_type(4)));
+  constexpr int VW = 16 / (int)sizeof(X);    // window entries per 16-byte load
+  constexpr int ZC = (ZN + VW - 1) / VW, KC = (KN + VW - 1) / VW;   // 16-byte loads per thread
   constexpr int T = RPT * kBlock;            // rows of a tile: RPT per thread, kBlock apart
-  constexpr int WZ = ZN * kBlock, WKP = KN * kBlock;   // padded window lengths (≥ the used ones)
-  const int W = T + 2 * px + 2, WK = T + px + 1;      // W ≤ WZ, WK ≤ WKP (host)
+  constexpr int WZ = ZC * VW * kBlock, WKP = KC * VW * kBlock;   // padded window lengths
+  const int W = T + 2 * px + 2, WK = T + px + 1;      // W ≤ ZN kBlock ≤ WZ, WK ≤ WKP (host)
   X* zw = reinterpret_cast<X*>(cell_lds);   // [2][WZ]
   X* kw = zw + 2 * WZ;                       // [2][WKP]
   const int32_t np = (int32_t)npad;
-  // byte offsets from a step's base row B = (p − 1) pxy + o0 − px − 1 (the first row of the
-  // coefficient window of layer p − 1): the t·x window of plane p starts pxy rows after it,
-  // this thread's rows of plane p − 1 px + 1 rows after it (entries past a window: an offset
-  // no resource reaches)
-  uint32_t oz[ZN], ok[KN], orw[RPT];
+  // Window entries are loaded VW at a time: chunk c of this thread holds the entries
+  // (c kBlock + tid) VW .. + VW − 1.  Byte offsets from a step's base row B = (p − 1) pxy + o0
+  // − px − 1 (the first row of the coefficient window of layer p − 1): the t·x window of plane
+  // p starts pxy rows after it, this thread's rows of plane p − 1 px + 1 rows after it (a
+  // chunk that starts past its window: an offset no resource reaches; one that reaches past
+  // it loads rows no computed row reads)
+  uint32_t oz[ZC], ok[KC], orw[RPT];
 #pragma unroll
-  for (int k = 0; k < ZN; ++k) {
-    const int idx = threadIdx.x + k * kBlock;
-    oz[k] = idx < W ? (uint32_t)(pxy + idx) * (uint32_t)sizeof(X) : 0x80000000u;
+  for (int c = 0; c < ZC; ++c) {
+    const int idx = (c * kBlock + (int)threadIdx.x) * VW;
+    oz[c] = idx < W ? (uint32_t)(pxy + idx) * (uint32_t)sizeof(X) : 0x80000000u;
   }
 #pragma unroll
-  for (int k = 0; k < KN; ++k) {
-    const int idx = threadIdx.x + k * kBlock;
-    ok[k] = idx < WK ? (uint32_t)idx * (uint32_t)sizeof(X) : 0x80000000u;
+  for (int c = 0; c < KC; ++c) {
+    const int idx = (c * kBlock + (int)threadIdx.x) * VW;
+    ok[c] = idx < WK ? (uint32_t)idx * (uint32_t)sizeof(X) : 0x80000000u;
   }
 #pragma unroll
   for (int r = 0; r < RPT; ++r) orw[r] = (uint32_t)(px + 1 + r * kBlock + threadIdx.x) * (uint32_t)sizeof(X);
@@ -1055,7 +1060,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
     i1 = nitems;
     istep = gridDim.x;
   }
-  X vt[D][ZN], vx[D][ZN], vk[D][KN], tn[D][RPT], xn[D][RPT];
+  U4 vt[D][ZC], vx[D][ZC], vk[D][KC];
+  X tn[D][RPT], xn[D][RPT];
   // step p − 1 of tile offset o0 into stage u: the t·x window of plane p, the coefficients of
   // layer p − 1 and this thread's t, x of plane p − 1; nothing is read for p > z1 (the last
   // plane an item's rows read).  For 2 ≤ p ≤ pz − 1, 0 ≤ B < npad.
@@ -1065,17 +1071,29 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
     const uint32_t ext = on ? (uint32_t)(np - B) * (uint32_t)sizeof(X) : 0u;
     const __amdgpu_buffer_rsrc_t rt = buf_rsrc(t + B, ext), rx = buf_rsrc(x + B, ext), rk = buf_rsrc(kc + B, ext);
 #pragma unroll
-    for (int k = 0; k < ZN; ++k) {
-      vt[u][k] = buf_load<X>(rt, oz[k]);
-      vx[u][k] = buf_load<X>(rx, oz[k]);
+    for (int c = 0; c < ZC; ++c) {
+      vt[u][c] = __builtin_amdgcn_raw_buffer_load_b128(rt, (int)oz[c], 0, 0);
+      vx[u][c] = __builtin_amdgcn_raw_buffer_load_b128(rx, (int)oz[c], 0, 0);
     }
 #pragma unroll
-    for (int k = 0; k < KN; ++k) vk[u][k] = buf_load<X>(rk, ok[k]);
+    for (int c = 0; c < KC; ++c) vk[u][c] = __builtin_amdgcn_raw_buffer_load_b128(rk, (int)ok[c], 0, 0);
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
       tn[u][r] = buf_load<X>(rt, orw[r]);
       xn[u][r] = buf_load<X>(rx, orw[r]);
     }
+  };
+  // the products t·x of a chunk into LDS, and a chunk of coefficients
+  auto put_tx = [&](X* zs, int c, const U4& tv, const U4& xv) {
+    struct A { X v[VW]; };
+    const A ta = __builtin_bit_cast(A, tv), xa = __builtin_bit_cast(A, xv);
+    A pa;
+#pragma unroll
+    for (int j = 0; j < VW; ++j) pa.v[j] = ta.v[j] * xa.v[j];
+    *reinterpret_cast<U4*>(zs + (c * kBlock + (int)threadIdx.x) * VW) = __builtin_bit_cast(U4, pa);
+  };
+  auto put_k = [&](X* ks, int c, const U4& kv) {
+    *reinterpret_cast<U4*>(ks + (c * kBlock + (int)threadIdx.x) * VW) = kv;
   };
   // a row's 9 in-plane neighbours (dy, dx row-major) from a t·x window; the 4 cells around it
   // (origin P − e, e = ex + 2 ey) from a coefficient window; l = the row's index in the windows
@@ -1098,34 +1116,34 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
     // and its first D steps are issued together, so the item waits on one load latency
     // before its first plane.  Plane z0 − 1 may be plane 0, whose window can start before
     // row 0: every entry a computed row reads there is padding, and the whole window reads 0.
-    X wt0[ZN], wx0[ZN], wt1[ZN], wx1[ZN], wk1[KN];
+    U4 wt0[ZC], wx0[ZC], wt1[ZC], wx1[ZC], wk1[KC];
     {
       const int32_t b0 = (z0 - 1) * pxy + o0 - px - 1;
       const __amdgpu_buffer_rsrc_t rt0 = rows_from(t, b0, np), rx0 = rows_from(x, b0, np);
       const __amdgpu_buffer_rsrc_t rt1 = rows_from(t, b0 + pxy, np), rx1 = rows_from(x, b0 + pxy, np);
       const __amdgpu_buffer_rsrc_t rk1 = rows_from(kc, b0, np);
 #pragma unroll
-      for (int k = 0; k < ZN; ++k) {
-        const int idx = threadIdx.x + k * kBlock;   // from the window's first row
-        const uint32_t o = idx < W ? (uint32_t)idx * (uint32_t)sizeof(X) : 0x80000000u;
-        wt0[k] = buf_load<X>(rt0, o);
-        wx0[k] = buf_load<X>(rx0, o);
-        wt1[k] = buf_load<X>(rt1, o);
-        wx1[k] = buf_load<X>(rx1, o);
+      for (int c = 0; c < ZC; ++c) {
+        const int idx = (c * kBlock + (int)threadIdx.x) * VW;   // from the window's first row
+        const int o = idx < W ? idx * (int)sizeof(X) : (int)0x80000000u;
+        wt0[c] = __builtin_amdgcn_raw_buffer_load_b128(rt0, o, 0, 0);
+        wx0[c] = __builtin_amdgcn_raw_buffer_load_b128(rx0, o, 0, 0);
+        wt1[c] = __builtin_amdgcn_raw_buffer_load_b128(rt1, o, 0, 0);
+        wx1[c] = __builtin_amdgcn_raw_buffer_load_b128(rx1, o, 0, 0);
       }
 #pragma unroll
-      for (int k = 0; k < KN; ++k) wk1[k] = buf_load<X>(rk1, ok[k]);
+      for (int c = 0; c < KC; ++c) wk1[c] = __builtin_amdgcn_raw_buffer_load_b128(rk1, (int)ok[c], 0, 0);
     }
 #pragma unroll
     for (int u = 0; u < D; ++u) fetch(u, z0 + 1 + u, o0, z1);
     lds_barrier();     // the previous item's LDS reads are done
 #pragma unroll
-    for (int k = 0; k < ZN; ++k) {
-      zw[threadIdx.x + k * kBlock] = wt0[k] * wx0[k];        // slot 0: plane z0 − 1
-      zw[WZ + threadIdx.x + k * kBlock] = wt1[k] * wx1[k];   // slot 1: plane z0
+    for (int c = 0; c < ZC; ++c) {
+      put_tx(zw, c, wt0[c], wx0[c]);        // slot 0: plane z0 − 1
+      put_tx(zw + WZ, c, wt1[c], wx1[c]);   // slot 1: plane z0
     }
 #pragma unroll
-    for (int k = 0; k < KN; ++k) kw[threadIdx.x + k * kBlock] = wk1[k];   // slot 0: layer z0 − 1
+    for (int c = 0; c < KC; ++c) put_k(kw, c, wk1[c]);   // slot 0: layer z0 − 1
     lds_barrier();
     // register planes: slot (p − z0 + 1) mod 3 holds plane p, layer slot (q − z0 + 1) mod 2 layer q
     X zr[3][RPT][9], kq[2][RPT][4];
@@ -1156,9 +1174,9 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
         X* zs = zw + (u & 1) * WZ;              // plane zp + 1: slot (sb + u + 2) mod 2
         X* ks = kw + ((u + 1) & 1) * WKP;       // layer zp: slot (sb + u + 1) mod 2
 #pragma unroll
-        for (int k = 0; k < ZN; ++k) zs[threadIdx.x + k * kBlock] = vt[su][k] * vx[su][k];
+        for (int c = 0; c < ZC; ++c) put_tx(zs, c, vt[su][c], vx[su][c]);
 #pragma unroll
-        for (int k = 0; k < KN; ++k) ks[threadIdx.x + k * kBlock] = vk[su][k];
+        for (int c = 0; c < KC; ++c) put_k(ks, c, vk[su][c]);
         lds_barrier();
         fetch(su, zp + 1 + D, o0, z1);
         const __amdgpu_buffer_rsrc_t ry = buf_rsrc(y + (zp * pxy + o0), yext);
@@ -1801,7 +1819,9 @@ static void cell_shape(const fem_ctx* c, const LatOp& l, CellOp* cl, int bytes) 
     lz = (int)std::min<int64_t>(16, std::max<int64_t>(2, (cl->ntx[pi] * (pz - 2) + want - 1) / want));
     nitems = (int64_t)cl->ntx[pi] * ((pz - 2 + lz - 1) / lz);
     grid = (int)std::max<int64_t>(8, std::min<int64_t>(want, (nitems + 7) / 8 * 8));
-    const int64_t words = (2 * cl->zn[pi] + 2 * cl->kn[pi]) * (int64_t)kBlock;   // the windows
+    const int vw = 16 / bytes;   // the kernel's 16-byte chunks per window entry group
+    const int64_t words = (2 * ((cl->zn[pi] + vw - 1) / vw) + 2 * ((cl->kn[pi] + vw - 1) / vw)) * vw *
+                          (int64_t)kBlock;   // the windows
     (bytes == 4 ? cl->ldsz32 : cl->ldsz64) = (int)(bytes * words);
     // the (padded) windows and the staged rows must fit one block's LDS, and the vectors the
     // 32-bit extent of a buffer resource; otherwise the tiles
