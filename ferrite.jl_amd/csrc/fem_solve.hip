@@ -1062,14 +1062,20 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
   }
   U4 vt[D][ZC], vx[D][ZC], vk[D][KC];
   X tn[D][RPT], xn[D][RPT];
-  // step p − 1 of tile offset o0 into stage u: the t·x window of plane p, the coefficients of
-  // layer p − 1 and this thread's t, x of plane p − 1; nothing is read for p > z1 (the last
-  // plane an item's rows read).  For 2 ≤ p ≤ pz − 1, 0 ≤ B < npad.
-  auto fetch = [&](int u, int32_t p, int32_t o0, int32_t z1) {
-    const bool on = p <= z1;
-    const int32_t B = on ? (p - 1) * pxy + o0 - px - 1 : 0;
-    const uint32_t ext = on ? (uint32_t)(np - B) * (uint32_t)sizeof(X) : 0u;
-    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(t + B, ext), rx = buf_rsrc(x + B, ext), rk = buf_rsrc(kc + B, ext);
+  // The item's steps fetch planes p = z0 + 1, z0 + 2, … in order: fetch(u) brings into stage u
+  // the t·x window of plane p, the coefficients of layer p − 1 and this thread's t, x of plane
+  // p − 1, from the base row B = (p − 1) pxy + o0 − px − 1 (fB: its byte offset, advanced one
+  // plane per fetch; 0 ≤ B < npad for 2 ≤ p ≤ pz − 1); nothing is read for p > z1 (the last
+  // plane an item's rows read).
+  uint32_t fB = 0;
+  int32_t fp = 0, fz1 = 0;
+  const uint32_t plane_bytes = (uint32_t)pxy * (uint32_t)sizeof(X), end_bytes = (uint32_t)np * (uint32_t)sizeof(X);
+  auto fetch = [&](int u) {
+    const bool on = fp <= fz1;
+    const uint32_t off = on ? fB : 0u, ext = on ? end_bytes - fB : 0u;
+    const __amdgpu_buffer_rsrc_t rt = buf_rsrc(reinterpret_cast<const char*>(t) + off, ext);
+    const __amdgpu_buffer_rsrc_t rx = buf_rsrc(reinterpret_cast<const char*>(x) + off, ext);
+    const __amdgpu_buffer_rsrc_t rk = buf_rsrc(reinterpret_cast<const char*>(kc) + off, ext);
 #pragma unroll
     for (int c = 0; c < ZC; ++c) {
       vt[u][c] = __builtin_amdgcn_raw_buffer_load_b128(rt, (int)oz[c], 0, 0);
@@ -1082,6 +1088,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
       tn[u][r] = buf_load<X>(rt, orw[r]);
       xn[u][r] = buf_load<X>(rx, orw[r]);
     }
+    fB += plane_bytes;
+    ++fp;
   };
   // the products t·x of a chunk into LDS, and a chunk of coefficients
   auto put_tx = [&](X* zs, int c, const U4& tv, const U4& xv) {
@@ -1134,8 +1142,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
 #pragma unroll
       for (int c = 0; c < KC; ++c) wk1[c] = __builtin_amdgcn_raw_buffer_load_b128(rk1, (int)ok[c], 0, 0);
     }
+    fp = z0 + 1;
+    fz1 = z1;
+    fB = (uint32_t)(z0 * pxy + o0 - px - 1) * (uint32_t)sizeof(X);
 #pragma unroll
-    for (int u = 0; u < D; ++u) fetch(u, z0 + 1 + u, o0, z1);
+    for (int u = 0; u < D; ++u) fetch(u);
     lds_barrier();     // the previous item's LDS reads are done
 #pragma unroll
     for (int c = 0; c < ZC; ++c) {
@@ -1178,7 +1189,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv_z(int64_t npad, int32_t px
 #pragma unroll
         for (int c = 0; c < KC; ++c) put_k(ks, c, vk[su][c]);
         lds_barrier();
-        fetch(su, zp + 1 + D, o0, z1);
+        fetch(su);   // plane zp + 1 + D
         const __amdgpu_buffer_rsrc_t ry = buf_rsrc(y + (zp * pxy + o0), yext);
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
