@@ -998,14 +998,16 @@ __global__ __launch_bounds__(kBlock) void k_cell_spmv(int64_t r_lo, int64_t nrow
 // stages hold the global loads of D steps, issued D planes ahead of their use; an item
 // issues its first windows and its first D steps together.  A step's loads go through
 // three buffer resources (t, x, coefficients) made from one row, the window's first
-// coefficient row: a thread's byte offsets into them are fixed for the whole launch, rows
+// coefficient row (a byte offset advanced one plane per step): a thread's byte offsets
+// into them are fixed for the whole launch, rows
 // outside the vector read 0 — the padding's value — with no compare or select, and a step
 // past the item's last plane gets empty resources (its loads read 0 and move no data).
 // Rows are stored through a resource that ends at the plane's end, so a tile's rows past it
 // are dropped.  A thread computes RPT rows kBlock apart (lanes read consecutive LDS words: no
-// bank conflicts).  The LDS windows are padded to ZN (KN) kBlock entries, so every thread
-// stores all its entries; the barriers order LDS only, so the loads of later steps stay in
-// flight across them.  Padding rows of the marched planes are computed too (t = 0 there, so
+// bank conflicts).  Window entries move 16 bytes at a time (one buffer load, one LDS write
+// of 16 / sizeof(X) consecutive entries), and the LDS windows are padded to whole such
+// chunks per thread, so every thread stores all its chunks; the barriers order LDS only, so
+// the loads of later steps stay in flight across them.  Padding rows of the marched planes are computed too (t = 0 there, so
 // y = x = 0).  Items are ordered chunk-major; MAP 1: the blocks of one XCD sweep one
 // contiguous eighth of the items, so neighbouring tiles (which share their halo rows) run on
 // the same XCD at the same time.  The host keeps npad · sizeof(X) below 2³¹ (the resources'
