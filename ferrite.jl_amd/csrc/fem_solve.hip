@@ -1282,8 +1282,9 @@ __global__ void k_fold(int64_t n, X* __restrict__ x, const X* __restrict__ p, co
 //   next adds α_prev p_prev + α p, p_prev being read anyway to form p; a hand-over folds the
 //   pending term into u (8 instead of 9 vector passes per step on average).
 // An iteration is three kernels: SpMV, k_cg_scal (one wave reduces the partials in a fixed
-// order and writes the scalars), k_cg_move (the vectors).  The scalar records and the vector
-// partials are double-buffered (read one, write the other).
+// order and writes the scalars), k_cg_move (the vectors); for small systems two, the vector
+// kernel's blocks each doing the scalar step themselves (k_cg_move_scal).  The scalar records
+// and the vector partials are double-buffered (read one, write the other).
 // --------------------------------------------------------------------------
 constexpr int kCgBlocks = 4096;  // most blocks of k_cg_move (its partial count)
 
@@ -1326,12 +1327,13 @@ __device__ __forceinline__ void wave_sum2(const double* __restrict__ a, int m, c
 // elements per thread), which lets later blocks load while earlier ones store.  (One kernel
 // whose every block reduced the partials itself measured slower at 128³: mixed loop 34.9 vs
 // 29.7 ms.)
-__global__ __launch_bounds__(64) void k_cg_scal(int nin, const double* __restrict__ part_in, int na,
-                                               const double* __restrict__ part_s, const Scal* __restrict__ sin,
-                                               Scal* __restrict__ sout) {
-  const Scal S = *sin;
+// The scalar step: the record after this step from the record before it and the partials of
+// γ (vector step) and δ (SpMV), reduced by the calling wave (threads 0-63) in a fixed order;
+// thread 0 writes it to *out.
+__device__ __forceinline__ void cg_scalars(int nin, const double* __restrict__ part_in, int na,
+                                           const double* __restrict__ part_s, const Scal& S, Scal* out) {
   if (S.done) {
-    if (threadIdx.x == 0) *sout = S;
+    if (threadIdx.x == 0) *out = S;
     return;
   }
   if (S.fresh) {
@@ -1345,7 +1347,7 @@ __global__ __launch_bounds__(64) void k_cg_scal(int nin, const double* __restric
       o.xa0 = 0.0;       // x̂ += α p of the reseed
       o.xa1 = S.alpha;
       o.pend = 0.0;
-      *sout = o;
+      *out = o;
     }
     return;
   }
@@ -1374,17 +1376,23 @@ __global__ __launch_bounds__(64) void k_cg_scal(int nin, const double* __restric
       o.xa1 = S.pend != 0.0 ? o.alpha : 0.0;
       o.pend = S.pend != 0.0 ? 0.0 : o.alpha;
     }
-    *sout = o;
+    *out = o;
   }
 }
 
+__global__ __launch_bounds__(64) void k_cg_scal(int nin, const double* __restrict__ part_in, int na,
+                                               const double* __restrict__ part_s, const Scal* __restrict__ sin,
+                                               Scal* __restrict__ sout) {
+  const Scal S = *sin;
+  cg_scalars(nin, part_in, na, part_s, S, sout);
+}
+
+// The vector step of the record S: p = r̂ + βp, s = w + βs, x̂ moved (every other step), r̂ −=
+// αs, the block's partial of the next γ.
 template <typename X, int E>
-__global__ __launch_bounds__(kBlock) void k_cg_move(int64_t n, X* __restrict__ x, X* __restrict__ r,
-                                                    const X* __restrict__ w, X* __restrict__ p, X* __restrict__ sv,
-                                                    double* __restrict__ part_out, const Scal* __restrict__ sc) {
-  __shared__ double sh[kBlock / 64];
-  const Scal S = *sc;
-  if (S.done) return;
+__device__ __forceinline__ void cg_move(const Scal& S, int64_t n, X* __restrict__ x, X* __restrict__ r,
+                                        const X* __restrict__ w, X* __restrict__ p, X* __restrict__ sv,
+                                        double* __restrict__ part_out, double* sh) {
   const bool fresh = S.sfresh != 0;
   const X al = (X)S.alpha, be = (X)S.beta, xa0 = (X)S.xa0, xa1 = (X)S.xa1;
   const bool xs = S.xa0 != 0.0 || S.xa1 != 0.0;   // this step moves x̂
@@ -1421,6 +1429,39 @@ __global__ __launch_bounds__(kBlock) void k_cg_move(int64_t n, X* __restrict__ x
   }
   const double b = block_sum(ga, sh);
   if (threadIdx.x == 0) part_out[blockIdx.x] = b;
+}
+
+template <typename X, int E>
+__global__ __launch_bounds__(kBlock) void k_cg_move(int64_t n, X* __restrict__ x, X* __restrict__ r,
+                                                    const X* __restrict__ w, X* __restrict__ p, X* __restrict__ sv,
+                                                    double* __restrict__ part_out, const Scal* __restrict__ sc) {
+  __shared__ double sh[kBlock / 64];
+  const Scal S = *sc;
+  if (S.done) return;
+  cg_move<X, E>(S, n, x, r, w, p, sv, part_out, sh);
+}
+
+// Small systems (at most kCgFuseBlocks vector blocks): the scalar step folded into the vector
+// step.  The first wave of every block reduces the partials itself (in k_cg_scal's order, so
+// every block holds the same record) and block 0 writes the record for the next step: one
+// launch per iteration fewer, which is what a small iteration costs most.
+constexpr int kCgFuseBlocks = 512;
+template <typename X, int E>
+__global__ __launch_bounds__(kBlock) void k_cg_move_scal(int64_t n, X* __restrict__ x, X* __restrict__ r,
+                                                         const X* __restrict__ w, X* __restrict__ p,
+                                                         X* __restrict__ sv, double* __restrict__ part_out, int nin,
+                                                         const double* __restrict__ part_in, int na,
+                                                         const double* __restrict__ part_s,
+                                                         const Scal* __restrict__ sin, Scal* __restrict__ sout) {
+  __shared__ double sh[kBlock / 64];
+  __shared__ Scal so;
+  const Scal S0 = *sin;
+  if (threadIdx.x < 64) cg_scalars(nin, part_in, na, part_s, S0, &so);
+  __syncthreads();
+  const Scal S = so;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *sout = S;
+  if (S.done) return;
+  cg_move<X, E>(S, n, x, r, w, p, sv, part_out, sh);
 }
 
 // Reseed, step 1: r̂ = S res (rounded to X); partials r̂·r̂ and r̂·p.
@@ -2270,6 +2311,15 @@ static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int
   const int na = spmv_hat<X>(c, w, v.r, v.w, sin);
   if (timed) (void)hipEventRecord(w->tev[2 * k + 1], c->stream);
   const int mb = cg_blocks(c, v.n);
+  if (mb <= kCgFuseBlocks) {   // small system: the vector step does the scalar step too
+    switch (cg_elems(c)) {
+      case 2: k_cg_move_scal<X, 2><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, mb, pin, na, w->part_s, sin, sout); break;
+      case 4: k_cg_move_scal<X, 4><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, mb, pin, na, w->part_s, sin, sout); break;
+      case 16: k_cg_move_scal<X, 16><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, mb, pin, na, w->part_s, sin, sout); break;
+      default: k_cg_move_scal<X, 8><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, mb, pin, na, w->part_s, sin, sout); break;
+    }
+    return;
+  }
   k_cg_scal<<<1, 64, 0, c->stream>>>(mb, pin, na, w->part_s, sin, sout);
   switch (cg_elems(c)) {
     case 2: k_cg_move<X, 2><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, sout); break;
