@@ -1441,11 +1441,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_move(int64_t n, X* __restrict__ x
   cg_move<X, E>(S, n, x, r, w, p, sv, part_out, sh);
 }
 
-// Small systems (at most kCgFuseBlocks vector blocks): the scalar step folded into the vector
+// Systems of at most kCgFuseBlocks vector blocks: the scalar step folded into the vector
 // step.  The first wave of every block reduces the partials itself (in k_cg_scal's order, so
 // every block holds the same record) and block 0 writes the record for the next step: one
-// launch per iteration fewer, which is what a small iteration costs most.
-constexpr int kCgFuseBlocks = 512;
+// launch per iteration fewer.  Measured (mixed, cell form): faster at 1,099 blocks (128³:
+// 16.9 against 17.4 ms per step), slower at 2,048 (160³: 35.6 against 34.2 ms), where every
+// block's sum over ~2,800 partials costs more than the launch it saves.
+constexpr int kCgFuseBlocks = 1280;
 template <typename X, int E>
 __global__ __launch_bounds__(kBlock) void k_cg_move_scal(int64_t n, X* __restrict__ x, X* __restrict__ r,
                                                          const X* __restrict__ w, X* __restrict__ p,
