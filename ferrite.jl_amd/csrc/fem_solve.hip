@@ -1388,15 +1388,18 @@ __global__ __launch_bounds__(64) void k_cg_scal(int nin, const double* __restric
 }
 
 // The vector step of the record S: p = r̂ + βp, s = w + βs, x̂ moved (every other step), r̂ −=
-// αs, the block's partial of the next γ.
-template <typename X, int E>
+// αs, the block's partial of the next γ.  With PRE, p, s and r̂ of the block's first pass
+// were loaded before S was known (pv0, sw0, rv0).
+template <typename X, int E, bool PRE = false>
 __device__ __forceinline__ void cg_move(const Scal& S, int64_t n, X* __restrict__ x, X* __restrict__ r,
                                         const X* __restrict__ w, X* __restrict__ p, X* __restrict__ sv,
-                                        double* __restrict__ part_out, double* sh) {
+                                        double* __restrict__ part_out, double* sh, const X* pv0 = nullptr,
+                                        const X* sw0 = nullptr, const X* rv0 = nullptr) {
   const bool fresh = S.sfresh != 0;
   const X al = (X)S.alpha, be = (X)S.beta, xa0 = (X)S.xa0, xa1 = (X)S.xa1;
   const bool xs = S.xa0 != 0.0 || S.xa1 != 0.0;   // this step moves x̂
   double ga = 0.0;
+  bool first = true;
   for (int64_t base = (int64_t)blockIdx.x * kBlock * E; base < n; base += (int64_t)gridDim.x * kBlock * E) {
     X wv[E], pv[E], sw[E], xv[E], rv[E];
 #pragma unroll
@@ -1404,11 +1407,18 @@ __device__ __forceinline__ void cg_move(const Scal& S, int64_t n, X* __restrict_
       const int64_t i = base + k * kBlock + threadIdx.x;
       const bool in = i < n;
       wv[k] = in && !fresh ? w[i] : X(0);
-      pv[k] = in ? p[i] : X(0);
-      sw[k] = in ? sv[i] : X(0);
+      if (PRE && first) {
+        pv[k] = pv0[k];
+        sw[k] = sw0[k];
+        rv[k] = rv0[k];
+      } else {
+        pv[k] = in ? p[i] : X(0);
+        sw[k] = in ? sv[i] : X(0);
+        rv[k] = in ? r[i] : X(0);
+      }
       xv[k] = in && xs ? x[i] : X(0);
-      rv[k] = in ? r[i] : X(0);
     }
+    first = false;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const int64_t i = base + k * kBlock + threadIdx.x;
@@ -1448,7 +1458,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_move(int64_t n, X* __restrict__ x
 // 16.9 against 17.4 ms per step), slower at 2,048 (160³: 35.6 against 34.2 ms), where every
 // block's sum over ~2,800 partials costs more than the launch it saves.
 constexpr int kCgFuseBlocks = 1280;
-template <typename X, int E>
+constexpr int kCgPreBlocks = 512;   // and up to this many, with the vector loads issued first
+template <typename X, int E, bool PRE>
 __global__ __launch_bounds__(kBlock) void k_cg_move_scal(int64_t n, X* __restrict__ x, X* __restrict__ r,
                                                          const X* __restrict__ w, X* __restrict__ p,
                                                          X* __restrict__ sv, double* __restrict__ part_out, int nin,
@@ -1457,13 +1468,29 @@ __global__ __launch_bounds__(kBlock) void k_cg_move_scal(int64_t n, X* __restric
                                                          const Scal* __restrict__ sin, Scal* __restrict__ sout) {
   __shared__ double sh[kBlock / 64];
   __shared__ Scal so;
+  // PRE: p, s and r̂ of the first pass (needed whatever the record says) are loaded while the
+  // first wave sums the partials.  It pays where few blocks each move little (up to
+  // kCgPreBlocks); with more, the first wave's partial loads queue behind its vector loads
+  // (loads complete in order) and the step got slower (128³: 18.8 against 16.9 ms).
+  X pv0[E], sw0[E], rv0[E];
+  if (PRE) {
+    const int64_t base0 = (int64_t)blockIdx.x * kBlock * E;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int64_t i = base0 + k * kBlock + threadIdx.x;
+      const bool in = i < n;
+      pv0[k] = in ? p[i] : X(0);
+      sw0[k] = in ? sv[i] : X(0);
+      rv0[k] = in ? r[i] : X(0);
+    }
+  }
   const Scal S0 = *sin;
   if (threadIdx.x < 64) cg_scalars(nin, part_in, na, part_s, S0, &so);
   __syncthreads();
   const Scal S = so;
   if (blockIdx.x == 0 && threadIdx.x == 0) *sout = S;
   if (S.done) return;
-  cg_move<X, E>(S, n, x, r, w, p, sv, part_out, sh);
+  cg_move<X, E, PRE>(S, n, x, r, w, p, sv, part_out, sh, pv0, sw0, rv0);
 }
 
 // Reseed, step 1: r̂ = S res (rounded to X); partials r̂·r̂ and r̂·p.
@@ -2313,13 +2340,24 @@ static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int
   const int na = spmv_hat<X>(c, w, v.r, v.w, sin);
   if (timed) (void)hipEventRecord(w->tev[2 * k + 1], c->stream);
   const int mb = cg_blocks(c, v.n);
-  if (mb <= kCgFuseBlocks) {   // small system: the vector step does the scalar step too
+  if (mb <= kCgFuseBlocks) {   // the vector step does the scalar step too
+    const bool pre = mb <= kCgPreBlocks;
+#define FEM_MOVE_SCAL(E_)                                                                                      \
+  do {                                                                                                          \
+    if (pre)                                                                                                    \
+      k_cg_move_scal<X, E_, true><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, mb, pin, na, \
+                                                                w->part_s, sin, sout);                          \
+    else                                                                                                        \
+      k_cg_move_scal<X, E_, false><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, mb, pin,    \
+                                                                 na, w->part_s, sin, sout);                     \
+  } while (0)
     switch (cg_elems(c)) {
-      case 2: k_cg_move_scal<X, 2><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, mb, pin, na, w->part_s, sin, sout); break;
-      case 4: k_cg_move_scal<X, 4><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, mb, pin, na, w->part_s, sin, sout); break;
-      case 16: k_cg_move_scal<X, 16><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, mb, pin, na, w->part_s, sin, sout); break;
-      default: k_cg_move_scal<X, 8><<<mb, kBlock, 0, c->stream>>>(v.n, v.x, v.r, v.w, v.p, v.sv, pout, mb, pin, na, w->part_s, sin, sout); break;
+      case 2: FEM_MOVE_SCAL(2); break;
+      case 4: FEM_MOVE_SCAL(4); break;
+      case 16: FEM_MOVE_SCAL(16); break;
+      default: FEM_MOVE_SCAL(8); break;
     }
+#undef FEM_MOVE_SCAL
     return;
   }
   k_cg_scal<<<1, 64, 0, c->stream>>>(mb, pin, na, w->part_s, sin, sout);
