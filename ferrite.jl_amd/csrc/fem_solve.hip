@@ -1871,9 +1871,9 @@ static bool cell_kref(const fem_ctx* c, const double (*x0)[3], int dim, CellKref
 constexpr int cell_stages(int bytes, int rpt) { return bytes == 4 && rpt == 1 ? 3 : 2; }
 
 // Launch shape of the cell-form SpMV for values of `bytes` bytes.  Measured at 128³ (in the
-// solve, z-marching, XCD ranges): FP32 two rows per thread 768 blocks 12.2 µs, 544 12.3; one
-// row per thread 768 12.7, 1024 13.0.  Z-marching: about one work item per block (the planes
-// per chunk follow from the block count).
+// solve, z-marching, XCD ranges): FP32 two rows per thread 768 blocks 12.1 µs, 512 12.4, 640
+// 15.5, 896 and 1024 14.5; one row per thread 768 12.9, 1024 13.3.  Z-marching: about one
+// work item per block (the planes per chunk follow from the block count).
 static void cell_shape(const fem_ctx* c, const LatOp& l, CellOp* cl, int bytes) {
   const int64_t want = c->tune.cell_blocks ? c->tune.cell_blocks : 768;
   const int pi = bytes == 4 ? 0 : 1;
