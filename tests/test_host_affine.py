@@ -235,3 +235,31 @@ def test_conformity_constraint_host():
         assert partner in pdofs
         # masters are the same component of the edge ends
         assert np.all((cd[ptr[i]:ptr[i + 1]] - d) % 2 == 0)
+
+
+def test_conformity_constraint_hexahedra_host():
+    """3D hanging nodes: a 3x2x2 hex grid with cell 1 split in eight.  Its three faces shared
+    with unsplit cells get a hanging centre with the face's four corners as masters
+    (coefficients 0.25), and its edges that unsplit cells also have get a hanging midpoint
+    with the edge's two ends (0.5); a vector field constrains every component to the same
+    component of its masters, right-hand side 0."""
+    from ncgrid_util import refine_hexes
+    ncg = refine_hexes(fa.generate_grid("Hexahedron", (3, 2, 2)), [1])
+    info = ncg.conformity_info
+    for h, ms in info.items():
+        assert np.allclose(ncg.nodes[h - 1], ncg.nodes[np.array(ms) - 1].mean(axis=0))
+    # cell 1 touches the x = 0, y = 0 and z = 0 boundaries: three inner faces, and its edges
+    # off those boundaries (all but the three on the x, y, z axes through its first corner)
+    faces = [h for h, ms in info.items() if len(ms) == 4]
+    edges = [h for h, ms in info.items() if len(ms) == 2]
+    assert len(faces) == 3 and len(edges) == 9
+    dhv = fa.DofHandler(ncg).add("u", fa.Lagrange(ncg.refshape, 1) ** 3).close()
+    chv = fa.ConstraintHandler(dhv).add(fa.ConformityConstraint("u")).close()
+    assert len(chv.prescribed_dofs) == 3 * len(info)
+    dofs, ptr, cd, cv, b = chv.affine_arrays()
+    counts = sorted(int(ptr[i + 1] - ptr[i]) for i in range(len(dofs)))
+    assert counts == [2] * 27 + [4] * 9
+    for i, d in enumerate(dofs):
+        k = ptr[i + 1] - ptr[i]
+        assert np.all(cv[ptr[i]:ptr[i + 1]] == 1.0 / k) and b[i] == 0.0
+        assert np.all((cd[ptr[i]:ptr[i + 1]] - d) % 3 == 0)
