@@ -1459,6 +1459,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_move(int64_t n, X* __restrict__ x
 // block's sum over ~2,800 partials costs more than the launch it saves.
 constexpr int kCgFuseBlocks = 1280;
 constexpr int kCgPreBlocks = 512;   // and up to this many, with the vector loads issued first
+// PRE: p, s and r̂ of the first pass (needed whatever the record says) are loaded while the
+// first wave sums the partials.  It pays where few blocks each move little (up to
+// kCgPreBlocks).  At 128³ (1,099 blocks) it measured level with the plain form (16.9 ms per
+// step at rtol 1e-8) once the preload was issued after the record's load (96 VGPRs, five
+// waves per SIMD: every block resident); issued first it took 108 (four waves per SIMD: 1,024
+// resident, 75 blocks in a second round) and the step 18.8 ms.  Preloading in every wave but
+// the first (68 VGPRs, the summing wave loading its vectors afterwards) measured 17.6 ms.
 template <typename X, int E, bool PRE>
 __global__ __launch_bounds__(kBlock) void k_cg_move_scal(int64_t n, X* __restrict__ x, X* __restrict__ r,
                                                          const X* __restrict__ w, X* __restrict__ p,
@@ -1468,12 +1475,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_move_scal(int64_t n, X* __restric
                                                          const Scal* __restrict__ sin, Scal* __restrict__ sout) {
   __shared__ double sh[kBlock / 64];
   __shared__ Scal so;
-  // PRE: p, s and r̂ of the first pass (needed whatever the record says) are loaded while the
-  // first wave sums the partials.  It pays where few blocks each move little (up to
-  // kCgPreBlocks); with more, the first wave's partial loads queue behind its vector loads
-  // (loads complete in order) and the step got slower (128³: 18.8 against 16.9 ms).
   X pv0[E], sw0[E], rv0[E];
-  if (PRE) {
+  auto preload = [&]() {
     const int64_t base0 = (int64_t)blockIdx.x * kBlock * E;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -1483,8 +1486,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_move_scal(int64_t n, X* __restric
       sw0[k] = in ? sv[i] : X(0);
       rv0[k] = in ? r[i] : X(0);
     }
-  }
+  };
   const Scal S0 = *sin;
+  if (PRE) preload();
   if (threadIdx.x < 64) cg_scalars(nin, part_in, na, part_s, S0, &so);
   __syncthreads();
   const Scal S = so;
