@@ -780,18 +780,23 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // Entry-by-entry check of the scaled K against the cell form, in FP64, kCheckLanes lanes per
-// padded row (column j of the CSC arrays), each lane up to four entries whose loads are issued
-// together (a lattice column has at most 27), and the row's eight cell coefficients loaded
-// once.  Â's diagonal is 1, so tol is relative to it.  cnt[0] entries that differ by more
+// column j of the CSC arrays (padded row P = map[j], loaded together with the column's
+// bounds), each lane up to 32 / kCheckLanes entries whose loads are issued together (a
+// lattice column has at most 27), and the row's eight cell coefficients loaded once.  Â's diagonal is 1, so tol is relative to it.  cnt[0] entries that differ by more
 // than tol, cnt[1] lattice-neighbour entries in the pattern, cnt[2] lattice neighbours of
 // the rows (the cell form couples every two corners of a cell, so each must be a pattern
 // entry), cnt[3] entries that differ by more than tolr (the residual's bar); other entries
 // must hold zero.  It runs once K¹ has passed cell_classes, so an entry's expected value is
 // f[class] times the sum of the coefficients of the cells that hold both corners (selects,
 // no table lookups).
-constexpr int kCheckLanes = 8;
+// Measured at 128³ (kernel trace): 526 µs with 8 lanes per padded row (its dof looked up
+// first), 488 µs with 4, 643 µs with 16; 476 µs with 4 per column.  The per-row loads (the
+// coefficients, the neighbour flags, the column bounds) are paid once per lane group, so
+// fewer lanes per row amortise them over more rows per wave; summing the coefficients over
+// x, y, z instead of the eight-way select measured 491 µs.
+constexpr int kCheckLanes = 4;
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nrows, const int32_t* __restrict__ dofp,
+__global__ __launch_bounds__(kBlock) void k_cell_check(int64_t n, const int32_t* __restrict__ dofp,
                                                        const int32_t* __restrict__ colptr,
                                                        const int32_t* __restrict__ rowval,
                                                        const double* __restrict__ nzval, const double* __restrict__ sc,
@@ -801,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nro
                                                        unsigned long long* __restrict__ cnt) {
   __shared__ double sh[kBlock / 64];
   constexpr int NB = 1 << DIM;
-  constexpr int U = 4;   // entries per lane per pass
+  constexpr int U = 32 / kCheckLanes;   // entries per lane per pass (a pass covers 32)
   const int lane = threadIdx.x % kCheckLanes;
   const int64_t groups = (int64_t)gridDim.x * (kBlock / kCheckLanes);
   const int64_t px = g.px, pxy = g.pxy;
@@ -810,10 +815,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nro
   double f8[NB];
 #pragma unroll
   for (int m = 0; m < NB; ++m) f8[m] = kr.k[cell_lid(m & 1, (m >> 1) & 1, m >> 2)];
-  for (int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kCheckLanes; q < nrows; q += groups) {
-    const int64_t P = r_lo + q;
-    const int32_t j = dofp[P];
-    if (j < 0) continue;
+  for (int64_t q = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / kCheckLanes; q < n; q += groups) {
+    const int32_t j = (int32_t)q;
+    const int64_t P = map[j];
+    const int32_t p0 = colptr[j], p1 = colptr[j + 1];
     if (lane == 0) {   // the lattice fills its box: the neighbours are a product over the axes
       const double nx = 1.0 + (dofp[P - 1] >= 0) + (dofp[P + 1] >= 0);
       const double ny = 1.0 + (dofp[P - px] >= 0) + (dofp[P + px] >= 0);
@@ -825,7 +830,6 @@ __global__ __launch_bounds__(kBlock) void k_cell_check(int64_t r_lo, int64_t nro
     for (int e = 0; e < NB; ++e) k8[e] = kc[P - (e & 1) - px * ((e >> 1) & 1) - pxy * (e >> 2)];
     const double sj = sc[j];
     const bool pj = isc && isc[j];
-    const int32_t p0 = colptr[j], p1 = colptr[j + 1];
     for (int32_t pb = p0; pb < p1; pb += U * kCheckLanes) {
       int32_t iv[U];
       double a[U];
@@ -2000,16 +2004,16 @@ static int ensure_cells(fem_ctx* c, SolverWork* w, int jacobi, bool* ok) {
       k_cell_coef<<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->ncells, cl.corg, c->d_param_cell, cl.kc64, cl.kc32);
       const LatGeom g{(uint32_t)l.px, (uint32_t)l.pxy, (uint32_t)(1 + l.px + l.pxy), l.ns, l.npad};
       const int64_t rows_per_block = kBlock / kCheckLanes;
-      const int fb = (int)std::min<int64_t>(16384, (l.nrows + rows_per_block - 1) / rows_per_block);
+      const int fb = (int)std::min<int64_t>(16384, (c->n + rows_per_block - 1) / rows_per_block);
       const double tol = sizeof(X) == 4 ? 1.0 / (1 << 22) : 1e-12;
       const uint8_t* isc = c->np > 0 ? c->d_isc : nullptr;
       FEM_HIP(c, hipMemsetAsync(cl.cnt, 0, 4 * sizeof(unsigned long long), c->stream));
       if (cl.dim == 3)
-        k_cell_check<3><<<fb, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.dofp, c->d_colptr, c->d_rowval, c->d_nzval,
+        k_cell_check<3><<<fb, kBlock, 0, c->stream>>>(c->n, l.dofp, c->d_colptr, c->d_rowval, c->d_nzval,
                                                       w->sc, isc, l.map, g, cl.kc64, cl.kref, tol, kCellTolR,
                                                       cl.cnt);
       else
-        k_cell_check<2><<<fb, kBlock, 0, c->stream>>>(l.r_lo, l.nrows, l.dofp, c->d_colptr, c->d_rowval, c->d_nzval,
+        k_cell_check<2><<<fb, kBlock, 0, c->stream>>>(c->n, l.dofp, c->d_colptr, c->d_rowval, c->d_nzval,
                                                       w->sc, isc, l.map, g, cl.kc64, cl.kref, tol, kCellTolR,
                                                       cl.cnt);
       unsigned long long h[4] = {};
