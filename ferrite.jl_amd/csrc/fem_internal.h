@@ -64,6 +64,7 @@ struct Scal {
   // and α p of this step is pending); pend = the α whose step's p is not yet in x̂ (k_fold
   // adds it)
   double xa0, xa1, pend;
+  double rz0;     // γ at the last reseed (the host's batch-length choice)
 };
 
 }  // namespace fem

@@ -40,7 +40,8 @@ namespace fem {
 
 constexpr int kBlock = 256;
 constexpr int kMaxVecBlocks = 2048;
-constexpr int kBatch = 16;           // CG iterations per graph replay
+constexpr int kBatch = 16;           // CG iterations per batch: the first after a reseed, and direct launches
+constexpr int kBatchMin = 8;         // captured batches: kBatchMin << i iterations, i = 0, 1, 2
 
 
 struct SpmvShape {
@@ -178,7 +179,7 @@ struct SolverWork {
   Scal* s = nullptr;           // two scalar records (read one, write the other)
   hipEvent_t tev[2 * kBatch] = {};   // SpMV start/end events (time_kernels)
   Scal* hscal = nullptr;       // pinned host copy of scalar record 0 (read_scal)
-  GraphCache g64, g32;
+  GraphCache g64[3], g32[3];   // per batch length kBatchMin << i
 };
 
 static void destroy_graph(GraphCache& g) {
@@ -189,8 +190,8 @@ static void destroy_graph(GraphCache& g) {
 void free_solver(fem_ctx* c) {
   if (!c->solver) return;
   SolverWork* w = c->solver;
-  destroy_graph(w->g64);
-  destroy_graph(w->g32);
+  for (GraphCache& g : w->g64) destroy_graph(g);
+  for (GraphCache& g : w->g32) destroy_graph(g);
   for (hipEvent_t& e : w->tev)
     if (e) (void)hipEventDestroy(e);
   if (w->hscal) (void)hipHostFree(w->hscal);
@@ -1538,6 +1539,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_reseed_scal(int m, const double* 
     s->gam = ru > 0.0 ? rp / ru : 0.0;
     s->beta = fresh ? 0.0 : ru / s->rzp;
     s->rz = ru;
+    s->rz0 = ru;
     const double conv = s->rr > 0.0 ? tol2 / s->rr : 0.0;
     s->gstop = ru * (hold_factor2 > conv ? hold_factor2 : conv);
     s->done = 0;
@@ -1664,8 +1666,8 @@ static int ensure_work(fem_ctx* c, bool mixed) {
   const int64_t n = c->n;
   int rc;
   if (w->n != n || !w->res) {
-    destroy_graph(w->g64);
-    destroy_graph(w->g32);
+    for (GraphCache& g : w->g64) destroy_graph(g);
+    for (GraphCache& g : w->g32) destroy_graph(g);
     w->sh64 = spmv_shape(c, c->tune.spmv_cap64 ? c->tune.spmv_cap64 : 2048, c->tune.spmv_lanes64 ? c->tune.spmv_lanes64 : 2);
     if ((rc = dev_alloc(c, &w->sc, n)) || (rc = dev_alloc(c, &w->res, n)) ||
         (rc = dev_alloc(c, &w->part_s, std::max(w->sh64.blocks, kMaxVecBlocks))) ||
@@ -2377,11 +2379,11 @@ static void enqueue_iteration(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, int
   }
 }
 
-// The captured batch of kBatch iterations (rebuilt only when the buffers change).
-// kBatch is even, so the batch starts and ends on scalar record 0 and partial buffer 0.
+// The captured batch of len iterations (rebuilt only when the buffers change).  len is even,
+// so the batch starts and ends on scalar record 0 and partial buffer 0.
 template <typename X>
-static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache& gc) {
-  static_assert(kBatch % 2 == 0, "batch must end on scalar record 0");
+static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache& gc, int len) {
+  static_assert(kBatch % 2 == 0 && kBatchMin % 2 == 0, "batch must end on scalar record 0");
   const void* val = w->use_elem ? elem_op_key(c, w->eop, sizeof(X) == 4)
                     : w->use_cells ? (sizeof(X) == 4 ? (const void*)w->cells.kc32 : (const void*)w->cells.kc64)
                     : w->use_lat ? (sizeof(X) == 4 ? (const void*)w->lat.val32 : (const void*)w->lat.val64)
@@ -2419,7 +2421,7 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache
   destroy_graph(gc);
   hipGraph_t graph = nullptr;
   FEM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < kBatch; ++k) enqueue_iteration<X>(c, w, v, k, false);
+  for (int k = 0; k < len; ++k) enqueue_iteration<X>(c, w, v, k, false);
   FEM_HIP(c, hipStreamEndCapture(c->stream, &graph));
   hipError_t e = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -2428,8 +2430,8 @@ static int batch_graph(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache
   return FEM_OK;
 }
 
-// One batch of kBatch iterations: the captured graph, or (time_kernels) direct launches with
-// every SpMV bracketed by HIP events.
+// One batch: the captured graph, or (time_kernels) kBatch direct launches with every SpMV
+// bracketed by HIP events.
 template <typename X>
 static int launch_batch(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, GraphCache& gc) {
   if (c->tune.time_kernels) {
@@ -2482,24 +2484,36 @@ static void residual(fem_ctx* c, SolverWork* w) {
 // (measured on a plane-strain cantilever, cond(K) ~ 1e6): six hand-overs in a row, not two.
 constexpr int kMaxStall = 6;
 
+// Index of the next batch's length (kBatchMin << i): the recurrence's remaining iterations
+// predicted from γ's geometric rate since the reseed (rz0 → rz in `it` iterations, hand-over
+// at gstop), so a batch seldom runs past the hand-over, where its launches return at once.
+static int batch_index(const Scal& h, int it) {
+  if (it <= 0 || !(h.rz > 0.0) || !(h.rz < h.rz0) || !(h.gstop > 0.0) || !(h.gstop < h.rz)) return 1;
+  const double m = it * std::log(h.gstop / h.rz) / std::log(h.rz / h.rz0);
+  return m >= 5.0 * kBatchMin ? 2 : m >= 2.5 * kBatchMin ? 1 : 0;
+}
+
 template <typename X>
 static int cg_run(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double bnorm2, double tol2, double hold_factor2,
-                  int maxit, GraphCache& gc, int* iters, int* handovers, double* ms) {
+                  int maxit, GraphCache* gcs, int* iters, int* handovers, double* ms) {
   zero_vec<double>(c, c->d_u, c->n);
   // all five: the lattice numbering's padding entries must start (and so stay) zero
   for (X* a : {v.x, v.r, v.w, v.p, v.sv}) zero_vec<X>(c, a, v.n);
   k_cg_init_scal<<<1, 1, 0, c->stream>>>(w->s, bnorm2);
   reseed<X>(c, w, v, c->d_f, hold_factor2, tol2, 1);
   FEM_HIP(c, hipGetLastError());
-  int rc = batch_graph<X>(c, w, v, gc);
-  if (rc) return rc;
-  int launched = 0, nho = 0, stall = 0, iters_before = 0;
+  int rc;
+  int launched = 0, nho = 0, stall = 0, iters_before = 0, iters_seed = 0;
+  int li = 1;   // the next batch is kBatchMin << li iterations (kBatch with time_kernels)
   bool converged = false, fresh_start = true;
   double prev_true = bnorm2;
   FEM_HIP(c, hipEventRecord(c->ev0, c->stream));
   while (launched < maxit) {
+    if (c->tune.time_kernels) li = 1;
+    GraphCache& gc = gcs[li];
+    if ((rc = batch_graph<X>(c, w, v, gc, kBatchMin << li))) return rc;   // at once when captured
     if ((rc = launch_batch<X>(c, w, v, gc))) return rc;
-    launched += kBatch;
+    launched += kBatchMin << li;
     Scal h{};
     if ((rc = read_scal(c, w, &h))) return rc;
     // SpMVs that did work: every executed step's except a fresh one's (it skips the SpMV),
@@ -2509,8 +2523,13 @@ static int cg_run(fem_ctx* c, SolverWork* w, const CgVecs<X>& v, double bnorm2, 
     add_spmv_times(c, w, first, std::max(0, ran - first));
     iters_before = h.iters;
     fresh_start = false;
-    if (!h.done) continue;
+    if (!h.done) {
+      li = batch_index(h, h.iters - iters_seed);
+      continue;
+    }
     fresh_start = true;
+    li = 1;
+    iters_seed = h.iters;
     ++nho;
     k_fold<X><<<kMaxVecBlocks, kBlock, 0, c->stream>>>(c->n, v.x, v.p, w->s, w->sc, c->d_u, v.map);
     k_cg_unpend<<<1, 1, 0, c->stream>>>(w->s);
